@@ -1,0 +1,94 @@
+/*
+ * merbit_amd.h -- C ABI of the MI355X (gfx950) merge-path tile SpMV.
+ *
+ * Plain C: device pointers, sizes and an opaque plan; `stream` is a
+ * hipStream_t passed as void* (NULL = the null stream).  All calls are
+ * stream-ordered and never synchronise the host; spmv calls allocate nothing.
+ * Every function returns MBX_OK (0) or a negative status (mbx_status_str).
+ *
+ * Each declaration names the reference interface it replaces (paths relative
+ * to the reference repository root).  The reference is a C++ template API
+ * without an FFI; the C++ host mirror in merbit_amd/host/merbit.hpp keeps its
+ * names (Fun_Merbit, TILE, GRAPH, PARAMETER, ENV, Timer, Fun_Check) on top of
+ * these entry points.
+ *
+ * Matrix layout (as the reference kernel consumes it): 0-based CSR with
+ * int32 row_ptr[nrow+1], int32 col[nnz] and values[nnz]; nrow + nnz must stay
+ * below 2^31 minus one tile.  The caller owns the CSR arrays, x and y.
+ */
+#ifndef MERBIT_AMD_H
+#define MERBIT_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  MBX_OK = 0,
+  MBX_EINVAL = -1,       /* bad argument (null pointer, negative size, ...) */
+  MBX_EOVERFLOW = -2,    /* nrow + nnz (+ one tile) does not fit int32 */
+  MBX_EUNSUPPORTED = -3, /* items_per_thread / wg_size / k not instantiated */
+  MBX_ENOMEM = -4,       /* device allocation failed */
+  MBX_EHIP = -5          /* a HIP runtime call or launch failed */
+};
+
+typedef struct mbx_plan mbx_plan;
+
+/* Replaces TILE<BLOCK,SIGMA>::TILE(offset, nrow, nnz), include/tile.cuh:106-122
+ * (with Tile_Kernel, tile.cuh:28-104, and merge_search, utils.cuh:5-29).
+ * Runs the merge-path partition search over (row_ptr, nnz index) on `stream`
+ * and keeps one (row, nnz) coordinate per tile of wg_size*items_per_thread
+ * merge items plus the per-tile carry slots.  items_per_thread plays the
+ * reference's SIGMA and wg_size its BLOCK; 0 selects the defaults (8, 256).
+ * d_row_ptr must stay valid until the partition kernel has run. */
+int mbx_plan_create(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol,
+                    int32_t nnz, int32_t items_per_thread, int32_t wg_size,
+                    void* stream, mbx_plan** out);
+
+/* Replaces ~TILE, include/tile.cuh:19-24.  Waits for the device. */
+void mbx_plan_destroy(mbx_plan* plan);
+
+/* Plan geometry: number of tiles, merge items per tile, workgroup size. */
+int mbx_plan_info(const mbx_plan* plan, int32_t* ntiles, int32_t* tile_items,
+                  int32_t* wg_size);
+
+/* Copies the ntiles+1 tile start coordinates (rows consumed, nnz consumed)
+ * to host arrays (synchronous; for tests and diagnostics, like TILE::show,
+ * include/tile.cuh:124-175). */
+int mbx_plan_partition(const mbx_plan* plan, int32_t* h_rows, int32_t* h_nnz,
+                       int32_t count);
+
+/* Optional timing hook: when ev_begin/ev_end (hipEvent_t) are non-NULL, the
+ * next spmv call on this plan records them immediately before and after its
+ * tile kernel on the call's stream, then clears them. */
+int mbx_plan_set_events(mbx_plan* plan, void* ev_begin, void* ev_end);
+
+/* Replaces merbit_kernel<double,BLOCK,SIGMA>, include/tile.cuh:476-561, and the
+ * block-boundary atomics of load_mem, tile.cuh:354-375: y = alpha*A*x +
+ * beta*y in FP64 (beta == 0: y is not read, so it need not be initialised).
+ * Deterministic: the cross-tile carry fixup sums in a fixed order. */
+int mbx_spmv_f64(const mbx_plan* plan, const int32_t* d_row_ptr,
+                 const int32_t* d_col, const double* d_val, double alpha,
+                 const double* d_x, double beta, double* d_y, void* stream);
+
+/* Mixed precision forms of mbx_spmv_f64 (no reference counterpart: the
+ * reference instantiates float and double only, src/class.cu:427-430).
+ * Values FP32 (f32) or bfloat16 bit patterns (bf16), x and y FP32; products
+ * are split error-free (TwoProduct) and rows summed with TwoSum compensation
+ * in FP32; carries of rows that cross tiles are fixed up in FP64. */
+int mbx_spmv_f32(const mbx_plan* plan, const int32_t* d_row_ptr,
+                 const int32_t* d_col, const float* d_val, float alpha,
+                 const float* d_x, float beta, float* d_y, void* stream);
+int mbx_spmv_bf16(const mbx_plan* plan, const int32_t* d_row_ptr,
+                  const int32_t* d_col, const uint16_t* d_val, float alpha,
+                  const float* d_x, float beta, float* d_y, void* stream);
+
+/* Status text, like the CSR5 error codes (include/csr5/detail/common.h:13-22). */
+const char* mbx_status_str(int status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MERBIT_AMD_H */

@@ -1,0 +1,85 @@
+"""ctypes binding of libmerbit_amd.so (include/merbit_amd.h, include/merbit_gen.h).
+
+The library is the product: there is no Python or CPU fallback for any
+device entry point.  If the shared library is missing this module raises on
+first use, naming the build command.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libmerbit_amd.so")
+
+# Every symbol include/merbit_amd.h and include/merbit_gen.h declare.
+ABI_SYMBOLS = (
+    "mbx_plan_create", "mbx_plan_destroy", "mbx_plan_info", "mbx_plan_partition",
+    "mbx_plan_set_events", "mbx_spmv_f64", "mbx_spmv_f32", "mbx_spmv_bf16",
+    "mbx_status_str",
+)
+GEN_SYMBOLS = (
+    "mbg_fill_uniform_f64", "mbg_fill_uniform_f32", "mbg_fill_normal_f64",
+    "mbg_fill_normal_f32", "mbg_f32_to_bf16", "mbg_laplacian5_nnz", "mbg_laplacian5",
+    "mbg_rmat", "mbg_banded_nnz", "mbg_banded", "mbg_poisson_rows",
+)
+
+MBX_OK = 0
+
+_lib = None
+
+
+class MerbitError(RuntimeError):
+    """A negative MBX_* status returned by the C ABI."""
+
+    def __init__(self, status: int, what: str):
+        self.status = status
+        super().__init__(f"{what}: {status_str(status)} (status {status})")
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built; run `python -m merbit_amd.build` "
+            "(the HIP library is required: there is no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32, i64, u64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64
+    f64, f32 = ctypes.c_double, ctypes.c_float
+    L.mbx_plan_create.argtypes = [vp, i32, i32, i32, i32, i32, vp, ctypes.POINTER(vp)]
+    L.mbx_plan_destroy.argtypes = [vp]
+    L.mbx_plan_destroy.restype = None
+    L.mbx_plan_info.argtypes = [vp, vp, vp, vp]
+    L.mbx_plan_partition.argtypes = [vp, vp, vp, i32]
+    L.mbx_plan_set_events.argtypes = [vp, vp, vp]
+    L.mbx_spmv_f64.argtypes = [vp, vp, vp, vp, f64, vp, f64, vp, vp]
+    L.mbx_spmv_f32.argtypes = [vp, vp, vp, vp, f32, vp, f32, vp, vp]
+    L.mbx_spmv_bf16.argtypes = [vp, vp, vp, vp, f32, vp, f32, vp, vp]
+    L.mbx_status_str.argtypes = [ctypes.c_int]
+    L.mbx_status_str.restype = ctypes.c_char_p
+    L.mbg_fill_uniform_f64.argtypes = [vp, i64, f64, f64, u64]
+    L.mbg_fill_uniform_f32.argtypes = [vp, i64, f32, f32, u64]
+    L.mbg_fill_normal_f64.argtypes = [vp, i64, u64]
+    L.mbg_fill_normal_f32.argtypes = [vp, i64, u64]
+    L.mbg_f32_to_bf16.argtypes = [vp, vp, i64]
+    L.mbg_laplacian5_nnz.argtypes = [i32]
+    L.mbg_laplacian5_nnz.restype = i64
+    L.mbg_laplacian5.argtypes = [i32, vp, vp, vp]
+    L.mbg_rmat.argtypes = [i32, i64, f64, f64, f64, u64, vp, vp]
+    L.mbg_banded_nnz.argtypes = [i32, i32]
+    L.mbg_banded_nnz.restype = i64
+    L.mbg_banded.argtypes = [i32, i32, vp, vp]
+    L.mbg_poisson_rows.argtypes = [i32, i32, f64, u64, vp, vp]
+    _lib = L
+    return L
+
+
+def status_str(status: int) -> str:
+    return lib().mbx_status_str(int(status)).decode()
+
+
+def check(status: int, what: str) -> None:
+    if status != MBX_OK:
+        raise MerbitError(status, what)

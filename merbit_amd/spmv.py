@@ -1,0 +1,139 @@
+"""Python mirror of the merge-path SpMV host API over the C ABI.
+
+``MerbitPlan`` plays the reference's ``TILE<BLOCK,SIGMA>`` plan object
+(include/tile.cuh:6-25,106-122) and ``MerbitPlan.spmv`` its
+``merbit_kernel`` launch (tile.cuh:476-561, host loop 596-603).  Tensors are
+torch device tensors; the call runs on torch's current stream.  There is no
+CPU path: every call goes through libmerbit_amd.so.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from ._lib import check, lib
+
+_VALUE_KINDS = {torch.float64: "f64", torch.float32: "f32", torch.uint16: "bf16",
+                torch.bfloat16: "bf16"}
+
+
+def _stream_handle(stream) -> int:
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return int(stream.cuda_stream)
+
+
+def _dptr(t: torch.Tensor, dtype, name: str) -> int:
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a device tensor")
+    if t.dtype != dtype:
+        raise TypeError(f"{name}: expected {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    return t.data_ptr()
+
+
+class MerbitPlan:
+    """Merge-path partition of one CSR pattern (``TILE`` of the reference)."""
+
+    def __init__(self, row_ptr: torch.Tensor, ncol: int, nnz: int | None = None,
+                 items_per_thread: int = 8, wg_size: int = 256, stream=None):
+        self.nrow = int(row_ptr.numel()) - 1
+        self.ncol = int(ncol)
+        self.nnz = int(row_ptr[-1].item()) if nnz is None else int(nnz)
+        h = ctypes.c_void_p()
+        check(lib().mbx_plan_create(_dptr(row_ptr, torch.int32, "row_ptr"), self.nrow,
+                                    self.ncol, self.nnz, items_per_thread, wg_size,
+                                    _stream_handle(stream), ctypes.byref(h)),
+              "mbx_plan_create")
+        self._h = h
+        nt, ti, wg = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        check(lib().mbx_plan_info(h, ctypes.byref(nt), ctypes.byref(ti), ctypes.byref(wg)),
+              "mbx_plan_info")
+        self.ntiles, self.tile_items, self.wg_size = nt.value, ti.value, wg.value
+        self.items_per_thread = self.tile_items // self.wg_size
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        if self._h is None:
+            raise RuntimeError("plan is closed")
+        return self._h
+
+    def partition(self):
+        """Tile start coordinates (rows consumed, nnz consumed), ntiles+1 each."""
+        n = self.ntiles + 1
+        rows = np.empty(n, np.int32)
+        nnz = np.empty(n, np.int32)
+        check(lib().mbx_plan_partition(self.handle, rows.ctypes.data, nnz.ctypes.data, n),
+              "mbx_plan_partition")
+        return rows, nnz
+
+    def set_events(self, begin, end) -> None:
+        """Record torch.cuda.Event pair around the next call's tile kernel."""
+        check(lib().mbx_plan_set_events(self.handle, begin.cuda_event, end.cuda_event),
+              "mbx_plan_set_events")
+
+    def spmv(self, row_ptr: torch.Tensor, col: torch.Tensor, val: torch.Tensor,
+             x: torch.Tensor, y: torch.Tensor | None = None, alpha: float = 1.0,
+             beta: float = 0.0, stream=None) -> torch.Tensor:
+        """y = alpha*A*x + beta*y; FP64, FP32 or bfloat16 values by val.dtype."""
+        kind = _VALUE_KINDS.get(val.dtype)
+        if kind is None:
+            raise TypeError(f"unsupported value dtype {val.dtype}")
+        if row_ptr.numel() != self.nrow + 1 or col.numel() != self.nnz or \
+                val.numel() != self.nnz:
+            raise ValueError("CSR arrays do not match the plan")
+        xdt = torch.float64 if kind == "f64" else torch.float32
+        if x.numel() != self.ncol:
+            raise ValueError("x has the wrong length")
+        if y is None:
+            y = torch.empty(self.nrow, dtype=xdt, device=x.device)
+            beta = 0.0
+        if y.numel() != self.nrow:
+            raise ValueError("y has the wrong length")
+        args = (self.handle, _dptr(row_ptr, torch.int32, "row_ptr"),
+                _dptr(col, torch.int32, "col"))
+        s = _stream_handle(stream)
+        if kind == "f64":
+            st = lib().mbx_spmv_f64(*args, _dptr(val, torch.float64, "val"), alpha,
+                                    _dptr(x, xdt, "x"), beta, _dptr(y, xdt, "y"), s)
+        elif kind == "f32":
+            st = lib().mbx_spmv_f32(*args, _dptr(val, torch.float32, "val"), alpha,
+                                    _dptr(x, xdt, "x"), beta, _dptr(y, xdt, "y"), s)
+        else:
+            v = val.view(torch.uint16) if val.dtype == torch.bfloat16 else val
+            st = lib().mbx_spmv_bf16(*args, _dptr(v, torch.uint16, "val"), alpha,
+                                     _dptr(x, xdt, "x"), beta, _dptr(y, xdt, "y"), s)
+        check(st, f"mbx_spmv_{kind}")
+        return y
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None:
+            lib().mbx_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceCsr:
+    """A CSR matrix resident in HBM with its plan (GRAPH + TILE together)."""
+
+    def __init__(self, A, device="cuda", value_dtype=None, items_per_thread: int = 8,
+                 wg_size: int = 256):
+        self.nrow, self.ncol, self.nnz = A.nrow, A.ncol, A.nnz
+        self.row_ptr = torch.from_numpy(np.ascontiguousarray(A.row_ptr)).to(device)
+        self.col = torch.from_numpy(np.ascontiguousarray(A.col)).to(device)
+        val = np.ascontiguousarray(A.val)
+        self.val = torch.from_numpy(val).to(device)
+        if value_dtype is not None and self.val.dtype != value_dtype:
+            self.val = self.val.to(value_dtype)
+        self.plan = MerbitPlan(self.row_ptr, self.ncol, self.nnz, items_per_thread, wg_size)
+
+    def spmv(self, x, y=None, alpha=1.0, beta=0.0, stream=None):
+        return self.plan.spmv(self.row_ptr, self.col, self.val, x, y, alpha, beta, stream)

@@ -1,0 +1,148 @@
+"""GPU parity of the merge-path SpMV (libmerbit_amd.so through the C ABI)
+against the CPU oracle.  FP64 bar: condition-scaled error <= 1e-12 against
+the Dot2-accurate oracle and against the reference-order emulator; mixed
+precision bar: <= 1e-6 against the FP64 oracle on the same stored values."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import cond_err, edge_case_patterns, norm_err, random_csr
+
+pytestmark = pytest.mark.gpu
+
+F64_TOL = 1e-12
+MIXED_TOL = 1e-6
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def dev(a, device):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device)
+
+
+def run_f64(rp, col, val, x, device, ipt=8, alpha=1.0, beta=0.0, y0=None):
+    from merbit_amd.spmv import MerbitPlan
+    d_rp, d_col, d_val, d_x = (dev(rp, device), dev(col, device), dev(val, device),
+                               dev(x, device))
+    plan = MerbitPlan(d_rp, len(x), int(rp[-1]), items_per_thread=ipt)
+    y = None if y0 is None else dev(y0, device)
+    out = plan.spmv(d_rp, d_col, d_val, d_x, y, alpha, beta)
+    torch.cuda.synchronize()
+    res = out.cpu().numpy()
+    plan.close()
+    return res
+
+
+@pytest.mark.parametrize("ipt", [4, 8, 16])
+def test_partition_bit_exact(orc, device, ipt):
+    from merbit_amd.spmv import MerbitPlan
+    rng = np.random.default_rng(ipt)
+    for name, lens, ncol in edge_case_patterns(256 * ipt) + [
+            ("random", rng.poisson(6, 5000), 5000)]:
+        say(f"partition {name} ipt={ipt}")
+        rp, _ = random_csr(rng, len(lens), ncol, lens=lens)
+        plan = MerbitPlan(dev(rp, device), ncol, int(rp[-1]), items_per_thread=ipt)
+        rows, nnz = plan.partition()
+        T = plan.tile_items
+        path = len(rp) - 1 + int(rp[-1])
+        for t in range(plan.ntiles + 1):
+            assert (nnz[t], rows[t]) == orc.merge_search(rp, int(rp[-1]), min(t * T, path))
+        plan.close()
+
+
+@pytest.mark.parametrize("ipt", [4, 6, 8, 12, 16])
+def test_f64_edge_cases(orc, device, ipt):
+    rng = np.random.default_rng(1000 + ipt)
+    for name, lens, ncol in edge_case_patterns(256 * ipt):
+        say(f"f64 edge {name} ipt={ipt}")
+        rp, col = random_csr(rng, len(lens), ncol, lens=lens)
+        val = rng.standard_normal(int(rp[-1]))
+        x = rng.uniform(-1, 1, ncol)
+        y = run_f64(rp, col, val, x, device, ipt)
+        ya, absr = orc.spmv_accurate(rp, col, val, x)
+        assert cond_err(y, ya, absr) <= F64_TOL, name
+        if ncol == len(lens):
+            ye = orc.merbit_emulate(rp, col, val, x, sigma=14)
+            assert cond_err(y, ye, absr) <= F64_TOL, name
+
+
+def test_f64_laplacian_closed_form(orc, device):
+    from merbit_amd import workloads
+    say("laplacian 256 x=1 closed form")
+    A, x = workloads.laplacian(256)
+    g = 256
+    y = run_f64(A.row_ptr, A.col, A.val, np.ones(A.nrow), device)
+    r, c = np.divmod(np.arange(g * g), g)
+    want = (r == 0).astype(float) + (r == g - 1) + (c == 0) + (c == g - 1)
+    assert np.array_equal(y, want)
+    say("laplacian 256 x~U[-1,1)")
+    y = run_f64(A.row_ptr, A.col, A.val, x, device)
+    ya, absr = orc.spmv_accurate(A.row_ptr, A.col, A.val, x)
+    assert cond_err(y, ya, absr) <= F64_TOL
+    ye = orc.merbit_emulate(A.row_ptr, A.col, A.val, x, sigma=14)
+    assert cond_err(y, ye, absr) <= F64_TOL
+
+
+def test_f64_alpha_beta(orc, device):
+    rng = np.random.default_rng(77)
+    rp, col = random_csr(rng, 20000, 20000, lens=(rng.pareto(1.1, 20000) * 4).astype(int))
+    val = rng.standard_normal(int(rp[-1]))
+    x = rng.uniform(-1, 1, 20000)
+    y0 = rng.uniform(-1, 1, 20000)
+    say("alpha/beta")
+    y = run_f64(rp, col, val, x, device, alpha=-1.5, beta=0.25, y0=y0)
+    ya, absr = orc.spmv_accurate(rp, col, val, x)
+    want = -1.5 * ya + 0.25 * y0
+    assert np.max(np.abs(y - want) / (1.5 * absr + 0.25 * np.abs(y0) + 1e-300)) <= F64_TOL
+    # beta == 0 must not read y: NaN garbage in y is overwritten
+    y = run_f64(rp, col, val, x, device, alpha=2.0, beta=0.0,
+                y0=np.full(20000, np.nan))
+    assert cond_err(y, 2.0 * ya, 2.0 * absr) <= F64_TOL
+
+
+@pytest.mark.slow
+def test_f64_rmat_full(orc, device):
+    from merbit_amd import workloads
+    from merbit_amd.spmv import DeviceCsr
+    say("rmat scale 20, 2^24 entries, FP64")
+    A, x = workloads.rmat()
+    M = DeviceCsr(A, device)
+    d_x = dev(x, device)
+    y1 = M.spmv(d_x).cpu().numpy()
+    y2 = M.spmv(d_x).cpu().numpy()
+    assert np.array_equal(y1, y2), "result must be deterministic"
+    ya, absr = orc.spmv_accurate(A.row_ptr, A.col, A.val, x)
+    assert cond_err(y1, ya, absr) <= F64_TOL
+    ye = orc.merbit_emulate(A.row_ptr, A.col, A.val, x, sigma=14)
+    assert cond_err(y1, ye, absr) <= F64_TOL
+    M.plan.close()
+
+
+@pytest.mark.parametrize("kind", ["f32", "bf16"])
+def test_mixed_precision_small(orc, device, kind):
+    from merbit_amd import gen
+    from merbit_amd.spmv import MerbitPlan
+    rng = np.random.default_rng(5)
+    for name, lens, ncol in edge_case_patterns(2048) + [
+            ("pareto", (rng.pareto(1.1, 30000) * 5).astype(int), 30000)]:
+        say(f"{kind} {name}")
+        rp, col = random_csr(rng, len(lens), ncol, lens=lens)
+        v32 = rng.uniform(-1, 1, int(rp[-1])).astype(np.float32)
+        x = rng.uniform(-1, 1, ncol).astype(np.float32)
+        stored = v32 if kind == "f32" else gen.to_bf16(v32)
+        d_rp = dev(rp, device)
+        plan = MerbitPlan(d_rp, ncol, int(rp[-1]))
+        y = plan.spmv(d_rp, dev(col, device), dev(stored, device), dev(x, device))
+        y = y.cpu().numpy().astype(np.float64)
+        plan.close()
+        ya, absr = orc.spmv_accurate(rp, col, stored, x)
+        # FP32 output: one rounding of the result plus compensated-sum error
+        assert cond_err(y, ya, absr) <= MIXED_TOL, name
+        big = np.abs(ya) > 1e-3 * absr
+        if np.any(big):
+            rel = np.max(np.abs(y[big] - ya[big]) / np.abs(ya[big]))
+            assert rel <= MIXED_TOL, name
