@@ -52,11 +52,39 @@ def _run(cmd, verbose):
         print(out)
 
 
+def _fatbin_targets(so_path: str) -> list[str]:
+    """Offload targets embedded in a built shared library's .hip_fatbin."""
+    import tempfile
+    bundler = os.path.join(os.path.dirname(os.path.dirname(HIPCC)), "llvm", "bin",
+                           "clang-offload-bundler")
+    with tempfile.TemporaryDirectory() as td:
+        fb = os.path.join(td, "fatbin")
+        # an explicit output file: without one objcopy rewrites its input
+        subprocess.run(["objcopy", f"--dump-section=.hip_fatbin={fb}", so_path,
+                        os.path.join(td, "copy.so")],
+                       check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        r = subprocess.run([bundler, "--list", "--type=o", f"--input={fb}"],
+                           check=True, stdout=subprocess.PIPE, text=True)
+    return [l.strip() for l in r.stdout.splitlines() if l.strip()]
+
+
 def build_library(verbose: bool = False) -> str:
     inc = os.path.join(ROOT, "include")
-    cmd = [HIPCC, *HIP_FLAGS, "-shared", f"-I{inc}", "-x", "hip",
-           LIB_SOURCES[0], "-x", "c++", LIB_SOURCES[1], "-o", LIB + ".tmp"]
-    _run(cmd, verbose)
+    obj_dir = os.path.join(ROOT, "_build")
+    os.makedirs(obj_dir, exist_ok=True)
+    objs = []
+    # One hipcc call per source: mixing -x hip and -x c++ inputs in one call
+    # drops the requested --offload-arch.
+    for src in LIB_SOURCES:
+        obj = os.path.join(obj_dir, os.path.basename(src) + ".o")
+        lang = "hip" if src.endswith(".hip") else "c++"
+        flags = HIP_FLAGS if lang == "hip" else ["-O3", "-std=c++17", "-fPIC", "-Wall"]
+        _run([HIPCC, *flags, f"-I{inc}", "-x", lang, "-c", src, "-o", obj], verbose)
+        objs.append(obj)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", *objs, "-o", LIB + ".tmp"], verbose)
+    targets = _fatbin_targets(LIB + ".tmp")
+    if not any(t.endswith("--" + ARCH) for t in targets):
+        raise RuntimeError(f"built library carries {targets}, not {ARCH}")
     os.replace(LIB + ".tmp", LIB)
     return LIB
 

@@ -1,0 +1,115 @@
+"""C-ABI library checks that need no GPU: it loads, exports every symbol the
+headers declare, carries gfx950 code, and rejects bad arguments on the host
+before touching the device."""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in ("merbit_amd.h", "merbit_gen.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(mb[xg]_[a-z0-9_]+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    from merbit_amd import _lib
+    L = _lib.lib()
+    decl = declared_functions()
+    assert decl == set(_lib.ABI_SYMBOLS) | set(_lib.GEN_SYMBOLS)
+    for name in sorted(decl):
+        assert hasattr(L, name), name
+
+
+def test_library_carries_gfx950():
+    from merbit_amd.build import LIB, _fatbin_targets
+    assert "hipv4-amdgcn-amd-amdhsa--gfx950" in _fatbin_targets(LIB)
+
+
+def test_host_side_argument_checks():
+    from merbit_amd import _lib
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    assert L.mbx_plan_create(None, 0, 0, 0, 8, 256, None, None) == -1
+    assert L.mbx_plan_create(None, -1, 0, 0, 8, 256, None, ctypes.byref(h)) == -1
+    assert L.mbx_plan_create(None, 5, 5, 0, 8, 256, None, ctypes.byref(h)) == -1
+    assert L.mbx_plan_create(None, 0, 0, 0, 7, 256, None, ctypes.byref(h)) == -3
+    assert L.mbx_plan_create(None, 0, 0, 0, 8, 100, None, ctypes.byref(h)) == -3
+    big = ctypes.c_void_p(16)  # never dereferenced: the size check comes first
+    assert L.mbx_plan_create(big, 2**30, 2**30, 2**30, 8, 256, None, ctypes.byref(h)) == -2
+    assert L.mbx_spmv_f64(None, None, None, None, 1.0, None, 0.0, None, None) == -1
+    assert L.mbx_spmv_f32(None, None, None, None, 1.0, None, 0.0, None, None) == -1
+    assert L.mbx_spmv_bf16(None, None, None, None, 1.0, None, 0.0, None, None) == -1
+    assert L.mbx_plan_info(None, None, None, None) == -1
+    assert _lib.status_str(0) == "ok"
+    assert "overflow" not in _lib.status_str(-1)
+    assert "int32" in _lib.status_str(-2)
+
+
+def test_generators_laplacian_matches_scipy():
+    import scipy.sparse as sp
+    from merbit_amd import gen
+    g = 9
+    A = gen.laplacian5(g)
+    I = sp.identity(g)
+    T = sp.diags([-1, 2, -1], [-1, 0, 1], shape=(g, g))
+    B = (sp.kron(I, T) + sp.kron(T, I)).tocsr()
+    B.sort_indices()
+    assert np.array_equal(A.row_ptr, B.indptr) and np.array_equal(A.col, B.indices)
+    assert np.array_equal(A.val, B.data)
+    assert A.nnz == 5 * g * g - 4 * g
+
+
+def test_generators_rmat_properties():
+    from merbit_amd import gen
+    A = gen.rmat(12, 16 << 12, 0.57, 0.19, 0.19, seed=1)
+    lens = np.diff(A.row_ptr)
+    assert A.nnz == 16 << 12 and A.nrow == 1 << 12
+    assert all(np.all(np.diff(A.col[A.row_ptr[r]:A.row_ptr[r + 1]]) >= 0)
+               for r in range(A.nrow))
+    # row 0 takes the top-left quadrant (a+b = 0.76) at every level
+    assert abs(lens[0] / A.nnz - 0.76 ** 12) < 0.3 * 0.76 ** 12
+    assert 0.2 < np.mean(lens == 0) < 0.6
+    B = gen.rmat(12, 16 << 12, 0.57, 0.19, 0.19, seed=1)
+    assert np.array_equal(A.col, B.col)
+
+
+def test_generators_banded_and_poisson():
+    from merbit_amd import gen
+    A = gen.banded(1000, 13)
+    assert A.nnz == 27 * 1000 - 13 * 14
+    r = 500
+    assert A.col[A.row_ptr[r]:A.row_ptr[r + 1]].tolist() == list(range(487, 514))
+    P = gen.poisson_rows(20000, 20000, 25.0, seed=3)
+    lens = np.diff(P.row_ptr)
+    assert abs(lens.mean() - 25.0) < 0.2 and abs(lens.var() - 25.0) < 1.5
+    for rr in range(0, 20000, 997):
+        c = P.col[P.row_ptr[rr]:P.row_ptr[rr + 1]]
+        assert np.all(np.diff(c) > 0)  # distinct and sorted
+
+
+def test_bf16_round_to_nearest_even():
+    from merbit_amd import gen
+    rng = np.random.default_rng(0)
+    f = np.concatenate([rng.standard_normal(20000).astype(np.float32),
+                        np.array([1.0, 1.00390625, 1.01171875, -0.0, 3.0e38], np.float32)])
+    h = gen.to_bf16(f)
+    back = gen.bf16_to_f32(h).astype(np.float64)
+    # the nearest bf16 neighbours of each value, found independently by ulp steps
+    ulp = np.float64(2.0) ** (np.floor(np.log2(np.abs(f.astype(np.float64)) + 1e-300)) - 7)
+    err = np.abs(back - f.astype(np.float64))
+    assert np.all(err <= ulp / 2 + 1e-300)
+    ties = err == ulp / 2
+    assert np.all((h[ties] & 1) == 0)
+    # above the largest bf16 plus half an ulp the value rounds to infinity
+    assert gen.to_bf16(np.array([3.4e38, -3.4e38], np.float32)).tolist() == [0x7F80, 0xFF80]
