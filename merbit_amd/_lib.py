@@ -23,6 +23,7 @@ GEN_SYMBOLS = (
     "mbg_fill_normal_f32", "mbg_f32_to_bf16", "mbg_laplacian5_nnz", "mbg_laplacian5",
     "mbg_rmat", "mbg_banded_nnz", "mbg_banded", "mbg_poisson_rows",
 )
+HOST_SYMBOLS = ("mbh_config_lookup", "mbh_mtx_read", "mbh_ref_ingest")
 
 MBX_OK = 0
 
@@ -72,6 +73,10 @@ def lib() -> ctypes.CDLL:
     L.mbg_banded_nnz.restype = i64
     L.mbg_banded.argtypes = [i32, i32, vp, vp]
     L.mbg_poisson_rows.argtypes = [i32, i32, f64, u64, vp, vp]
+    L.mbh_config_lookup.argtypes = [ctypes.c_char_p, i32, ctypes.c_char_p, i32]
+    L.mbh_mtx_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(i32), ctypes.POINTER(i64),
+                               vp, vp, vp]
+    L.mbh_ref_ingest.argtypes = [i32, i64, vp, vp, vp, vp, vp]
     _lib = L
     return L
 

@@ -36,6 +36,7 @@ HIP_FLAGS = [
 LIB_SOURCES = [
     os.path.join(CSRC, "merbit_spmv.hip"),
     os.path.join(CSRC, "merbit_gen.cpp"),
+    os.path.join(CSRC, "merbit_host.cpp"),
 ]
 
 
@@ -95,10 +96,13 @@ def build_driver(verbose: bool = False) -> str | None:
         return None
     os.makedirs(os.path.dirname(BIN), exist_ok=True)
     inc = os.path.join(ROOT, "include")
-    cmd = [HIPCC, "-O2", "-std=c++17", f"-I{inc}", f"-I{HOST}",
-           main, os.path.join(HOST, "graph.cpp"),
-           f"-L{PKG}", "-lmerbit_amd", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/..",
-           "-o", BIN]
+    # host-only C++ over the C ABI; the rpath lets the binary find the library
+    # in tree wherever the tree is copied
+    rocm = os.path.dirname(os.path.dirname(HIPCC))
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__",
+           f"-I{inc}", f"-I{HOST}", f"-I{rocm}/include", main,
+           f"-L{PKG}", "-lmerbit_amd", f"-L{rocm}/lib", "-lamdhip64",
+           "-Wl,-rpath,$ORIGIN/..", "-o", BIN]
     _run(cmd, verbose)
     return BIN
 

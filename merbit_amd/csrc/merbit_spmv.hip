@@ -181,22 +181,30 @@ __global__ __launch_bounds__(WG) void tile_spmv_kernel(
 
   // 1. Stage: row ends of the rows ending here, then products.  The open
   //    row at the tile end never ends inside the tile: its end is +inf.
-  for (int i = tid; i < rows_t; i += WG) s_re[i] = row_ptr[r0 + 1 + i] - k0;
-  if (tid == 0) s_re[rows_t] = INT_MAX;
+  //    All global loads of the tile are issued before the first wait.
   {
+    int re[IPT];
     int ci[IPT];
     typename P::val_t vv[IPT];
     typename P::x_t xv[IPT];
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
       const int i = tid + j * WG;
+      re[j] = INT_MAX;
       ci[j] = 0;
       vv[j] = typename P::val_t(0);
+      if (i < rows_t) re[j] = row_ptr[r0 + 1 + i];
       if (i < nnz_t) {
         ci[j] = col[k0 + i];
         vv[j] = val[k0 + i];
       }
     }
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const int i = tid + j * WG;
+      if (i < rows_t) s_re[i] = re[j] - k0;
+    }
+    if (tid == 0) s_re[rows_t] = INT_MAX;
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
       const int i = tid + j * WG;

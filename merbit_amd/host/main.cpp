@@ -1,0 +1,110 @@
+// main.cpp -- config-ID driver: ./merbit_main <ID> (src/main.cu:3-84).
+//
+//   merbit_main <ID> [--config FILE] [--log FILE] [--device N] [--type f64|f32]
+//               [--sigma 4|6|8|12|16] [--perrnd N] [--maxiter N] [--check]
+//   merbit_main --mtx FILE [...]
+//
+// Loads graph <ID> from the registry (lines "ID path [nrow ncol nnz]"), builds
+// the operated matrix with the reference's semantics, runs Fun_Merbit
+// (maxiter*perrnd SpMVs, default 20*20 = 400 as src/main.cu:32) and appends
+// " gid&MERBIT&sigma&tim1&tim2" to the log.  --check verifies y against a
+// sequential FP64 CSR product on the host (Fun_Check).
+
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+#include <string>
+
+#include "merbit.hpp"
+
+namespace {
+
+struct Args {
+  int gid = -1;
+  std::string config = "config/config.txt";
+  std::string log;
+  std::string mtx;
+  std::string type = "f64";
+  int device = 0, sigma = 8, perrnd = 20, maxiter = 20;
+  bool check = false;
+};
+
+int usage() {
+  std::cerr << "usage: merbit_main <ID> [--config FILE] [--log FILE] [--device N] "
+               "[--type f64|f32] [--sigma 4|6|8|12|16] [--perrnd N] [--maxiter N] [--check]\n"
+               "       merbit_main --mtx FILE [same options]\n";
+  return 2;
+}
+
+template <typename V, unsigned S>
+int run_sigma(const ENV& env, GRAPH<V>& graph, const Args& a) {
+  PARAMETER<V> para(&graph, 0.85, a.perrnd, a.maxiter);
+  const MerbitTimes t = Fun_Merbit<V, 256, S>(env, para);
+  const double bytes = 4.0 * (graph.nrow + 1) + graph.nnz * (4.0 + sizeof(V)) +
+                       2.0 * graph.nrow * sizeof(V);
+  std::cout << "graph = " << graph.graph_id << ", block = 256, sigma = " << S
+            << ", nrow = " << graph.nrow << ", nnz = " << graph.nnz << "\n"
+            << "tim1 = " << t.plan_ms << " ms, tim2 = " << t.spmv_ms << " ms/SpMV, "
+            << bytes / (t.spmv_ms * 1e-3) / 1e9 << " GB/s\n";
+  if (!a.check) return 0;
+  std::vector<V> y((size_t)graph.nrow), ref((size_t)graph.nrow);
+  merbit::hip_check(hipMemcpy(y.data(), para.pagerank_real, sizeof(V) * y.size(),
+                              hipMemcpyDeviceToHost),
+                    "hipMemcpy");
+  for (int r = 0; r < graph.nrow; ++r) {  // x = 1
+    double s = 0.0;
+    for (int k = graph.h_offset[r]; k < graph.h_offset[r + 1]; ++k) s += (double)graph.h_values[k];
+    ref[(size_t)r] = (V)s;
+  }
+  const double rtol = std::is_same<V, double>::value ? 1e-12 : 1e-5;
+  const int bad = Fun_Check<V>(ref.data(), y.data(), graph.nrow, rtol);
+  std::cout << (bad ? "CHECK FAILED" : "check ok") << " (" << bad << " rows)\n";
+  return bad ? 1 : 0;
+}
+
+template <typename V>
+int run(const Args& a) {
+  ENV env(a.device, a.config, a.log);
+  GRAPH<V> graph = a.mtx.empty() ? GRAPH<V>(a.config, a.gid) : GRAPH<V>::from_mtx(a.mtx, a.gid);
+  switch (a.sigma) {
+    case 4: return run_sigma<V, 4>(env, graph, a);
+    case 6: return run_sigma<V, 6>(env, graph, a);
+    case 8: return run_sigma<V, 8>(env, graph, a);
+    case 12: return run_sigma<V, 12>(env, graph, a);
+    case 16: return run_sigma<V, 16>(env, graph, a);
+    default: std::cerr << "unsupported sigma " << a.sigma << "\n"; return 2;
+  }
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  Args a;
+  for (int i = 1; i < argc; ++i) {
+    const std::string s = argv[i];
+    auto next = [&](void) -> const char* { return i + 1 < argc ? argv[++i] : nullptr; };
+    const char* v = nullptr;
+    if (s == "--config" && (v = next())) a.config = v;
+    else if (s == "--log" && (v = next())) a.log = v;
+    else if (s == "--mtx" && (v = next())) a.mtx = v;
+    else if (s == "--type" && (v = next())) a.type = v;
+    else if (s == "--device" && (v = next())) a.device = std::atoi(v);
+    else if (s == "--sigma" && (v = next())) a.sigma = std::atoi(v);
+    else if (s == "--perrnd" && (v = next())) a.perrnd = std::atoi(v);
+    else if (s == "--maxiter" && (v = next())) a.maxiter = std::atoi(v);
+    else if (s == "--check") a.check = true;
+    else if (!s.empty() && s[0] != '-' && a.gid < 0) a.gid = std::atoi(s.c_str());
+    else return usage();
+  }
+  if (a.gid < 0 && a.mtx.empty()) return usage();
+  if (!a.log.empty()) FSTR.open(a.log, std::ios_base::app);
+  int rc = 1;
+  try {  // TRYCATCH, include/inc.cuh:124-132
+    rc = a.type == "f32" ? run<float>(a) : run<double>(a);
+  } catch (const std::exception& e) {
+    std::cerr << "Exception caught: " << e.what() << std::endl;
+    rc = 1;
+  }
+  if (FSTR.is_open()) FSTR.close();
+  return rc;
+}

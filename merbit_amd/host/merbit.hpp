@@ -1,0 +1,305 @@
+// merbit.hpp -- C++ host mirror of the reference's SpMV host API.
+//
+// Same names and roles as the reference (include/class.cuh, include/tile.cuh,
+// include/utils.cuh), rebuilt over the C ABI of libmerbit_amd.so:
+//
+//   Timer       include/class.cuh:5-31    HIP events; start() syncs the device
+//   ENV         include/class.cuh:36-59   device + stream (no cuSPARSE/cuBLAS)
+//   GRAPH<V>    include/class.cuh:62-81   the operated CSR (A^T of the loaded
+//               src/class.cu:60-177       graph, values 1/out-degree), on device
+//   PARAMETER   include/class.cuh:119-146 x (= 1) and y on device
+//   TILE        include/tile.cuh:6-25     the merge-path plan (mbx_plan)
+//   Fun_Merbit  include/tile.cuh:563-615  plan + maxiter*perrnd timed SpMVs
+//   Fun_Check   include/utils.cuh:116-132 per-row relative check
+//   FSTR        include/inc.cuh:70        the global result log stream
+//
+// Errors throw std::runtime_error (the reference's TRYCATCH, inc.cuh:124-132,
+// catches them in the driver).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <fstream>
+#include <iostream>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "merbit_amd.h"
+#include "merbit_host.h"
+
+inline std::fstream FSTR;
+
+namespace merbit {
+
+inline void check(int st, const char* what) {
+  if (st != MBX_OK)
+    throw std::runtime_error(std::string(what) + ": " + mbx_status_str(st));
+}
+
+inline void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T>
+T* device_copy(const std::vector<T>& h) {
+  T* d = nullptr;
+  const size_t bytes = sizeof(T) * (h.empty() ? 1 : h.size());
+  hip_check(hipMalloc((void**)&d, bytes), "hipMalloc");
+  if (!h.empty()) hip_check(hipMemcpy(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice),
+                            "hipMemcpy");
+  return d;
+}
+
+template <class T>
+void device_free(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+}  // namespace merbit
+
+struct Timer {
+  hipEvent_t start_event, stop_event;
+  Timer() {
+    merbit::hip_check(hipEventCreate(&start_event), "hipEventCreate");
+    merbit::hip_check(hipEventCreate(&stop_event), "hipEventCreate");
+  }
+  void start(hipStream_t stream) {
+    merbit::hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    merbit::hip_check(hipEventRecord(start_event, stream), "hipEventRecord");
+  }
+  float stop(hipStream_t stream) {
+    merbit::hip_check(hipEventRecord(stop_event, stream), "hipEventRecord");
+    merbit::hip_check(hipEventSynchronize(stop_event), "hipEventSynchronize");
+    float ms = 0.f;
+    merbit::hip_check(hipEventElapsedTime(&ms, start_event, stop_event), "hipEventElapsedTime");
+    return ms;
+  }
+  ~Timer() {
+    (void)hipEventDestroy(start_event);
+    (void)hipEventDestroy(stop_event);
+  }
+  Timer(const Timer&) = delete;
+  Timer& operator=(const Timer&) = delete;
+};
+
+struct ENV {
+  const int device_id = 0;
+  const std::string config;
+  const std::string log;
+  hipStream_t stream = nullptr;
+  ENV(int id, const std::string& config, const std::string& log)
+      : device_id(id), config(config), log(log) {
+    merbit::hip_check(hipSetDevice(device_id), "hipSetDevice");
+    merbit::hip_check(hipStreamCreate(&stream), "hipStreamCreate");
+  }
+  ~ENV() { (void)hipStreamDestroy(stream); }
+  ENV(const ENV&) = delete;
+  ENV& operator=(const ENV&) = delete;
+};
+
+// The operated matrix in HBM: CSR (offset, col, values) of A^T of the loaded
+// graph with values 1/out-degree, i.e. what the reference holds after
+// Fun_Values, GRAPH::sort(false) and cusparseXcoo2csr.  Host copies are kept
+// for verification.
+template <typename V>
+struct GRAPH {
+  int graph_id = -1;
+  int nrow = 0;
+  int nnz = 0;
+  bool owner = true;
+  int* offset = nullptr;
+  int* col = nullptr;
+  V* values = nullptr;
+  std::vector<int32_t> h_offset, h_col;
+  std::vector<V> h_values;
+
+  GRAPH() = default;
+
+  // src/class.cu:60-141: config ID -> MTX path -> COO -> reference ingest
+  GRAPH(const std::string& config, int gid) : graph_id(gid) {
+    char path[4096];
+    merbit::check(mbh_config_lookup(config.c_str(), gid, path, sizeof path),
+                  ("config ID " + std::to_string(gid) + " in " + config).c_str());
+    load_mtx(path);
+  }
+
+  static GRAPH from_mtx(const std::string& path, int gid = -1) {
+    GRAPH g;
+    g.graph_id = gid;
+    g.load_mtx(path);
+    return g;
+  }
+
+  // A matrix given directly as host CSR (synthetic workloads).
+  GRAPH(int gid, std::vector<int32_t> offset_h, std::vector<int32_t> col_h, std::vector<V> val_h)
+      : graph_id(gid), h_offset(std::move(offset_h)), h_col(std::move(col_h)),
+        h_values(std::move(val_h)) {
+    nrow = (int)h_offset.size() - 1;
+    nnz = (int)h_col.size();
+    upload();
+  }
+
+  GRAPH(GRAPH&& o) noexcept { *this = std::move(o); }
+  GRAPH& operator=(GRAPH&& o) noexcept {
+    std::swap(graph_id, o.graph_id);
+    std::swap(nrow, o.nrow);
+    std::swap(nnz, o.nnz);
+    std::swap(owner, o.owner);
+    std::swap(offset, o.offset);
+    std::swap(col, o.col);
+    std::swap(values, o.values);
+    h_offset.swap(o.h_offset);
+    h_col.swap(o.h_col);
+    h_values.swap(o.h_values);
+    return *this;
+  }
+  GRAPH(const GRAPH&) = delete;
+  GRAPH& operator=(const GRAPH&) = delete;
+
+  ~GRAPH() {
+    if (owner) {
+      merbit::device_free(offset);
+      merbit::device_free(col);
+      merbit::device_free(values);
+    }
+  }
+
+ private:
+  void load_mtx(const std::string& path) {
+    int32_t n = 0;
+    int64_t m = 0;
+    merbit::check(mbh_mtx_read(path.c_str(), &n, &m, nullptr, nullptr, nullptr),
+                  ("read " + path).c_str());
+    std::vector<int32_t> ri((size_t)m), ci((size_t)m);
+    merbit::check(mbh_mtx_read(path.c_str(), &n, &m, ri.data(), ci.data(), nullptr),
+                  ("read " + path).c_str());
+    nrow = n;
+    nnz = (int)m;
+    h_offset.assign((size_t)n + 1, 0);
+    h_col.assign((size_t)m, 0);
+    std::vector<double> v((size_t)m);
+    merbit::check(mbh_ref_ingest(n, m, ri.data(), ci.data(), h_offset.data(), h_col.data(),
+                                 v.data()),
+                  "ingest");
+    h_values.assign(v.begin(), v.end());
+    upload();
+  }
+
+  void upload() {
+    offset = merbit::device_copy(h_offset);
+    col = merbit::device_copy(h_col);
+    values = merbit::device_copy(h_values);
+  }
+};
+
+template <typename V>
+struct PARAMETER {
+  GRAPH<V>* graph;
+  double damp;
+  int perrnd;
+  int maxiter;
+  int iter = 0;
+  V* person = nullptr;         // x, filled with 1 (class.cuh:137-139)
+  V* pagerank_real = nullptr;  // y
+  PARAMETER(GRAPH<V>* g, double damp, int perrnd, int maxiter)
+      : graph(g), damp(damp), perrnd(perrnd), maxiter(maxiter) {
+    std::vector<V> ones((size_t)g->nrow, V(1));
+    person = merbit::device_copy(ones);
+    merbit::hip_check(hipMalloc((void**)&pagerank_real, sizeof(V) * (g->nrow > 0 ? g->nrow : 1)),
+                      "hipMalloc");
+  }
+  ~PARAMETER() {
+    merbit::device_free(person);
+    merbit::device_free(pagerank_real);
+  }
+  PARAMETER(const PARAMETER&) = delete;
+  PARAMETER& operator=(const PARAMETER&) = delete;
+};
+
+// include/tile.cuh:6-25: the plan.  SIGMA = items per thread, BLOCK = threads.
+template <unsigned BLOCK, unsigned SIGMA>
+struct TILE {
+  mbx_plan* plan = nullptr;
+  int tile_num = 0;
+  int tile_items = 0;
+  TILE(const int* offset, int nrow, int nnz, hipStream_t stream = nullptr) {
+    merbit::check(mbx_plan_create(offset, nrow, nrow, nnz, (int32_t)SIGMA, (int32_t)BLOCK,
+                                  stream, &plan),
+                  "mbx_plan_create");
+    int32_t nt = 0, ti = 0, wg = 0;
+    merbit::check(mbx_plan_info(plan, &nt, &ti, &wg), "mbx_plan_info");
+    tile_num = nt;
+    tile_items = ti;
+  }
+  ~TILE() { mbx_plan_destroy(plan); }
+  TILE(const TILE&) = delete;
+  TILE& operator=(const TILE&) = delete;
+};
+
+namespace merbit {
+
+inline int spmv(const mbx_plan* p, const GRAPH<double>& g, double alpha, const double* x,
+                double beta, double* y, hipStream_t s) {
+  return mbx_spmv_f64(p, g.offset, g.col, g.values, alpha, x, beta, y, s);
+}
+
+inline int spmv(const mbx_plan* p, const GRAPH<float>& g, float alpha, const float* x,
+                float beta, float* y, hipStream_t s) {
+  return mbx_spmv_f32(p, g.offset, g.col, g.values, alpha, x, beta, y, s);
+}
+
+}  // namespace merbit
+
+struct MerbitTimes {
+  float plan_ms = 0.f;  // tim1
+  float spmv_ms = 0.f;  // tim2 / (maxiter * perrnd)
+};
+
+// include/tile.cuh:563-615.  Builds the plan (tim1), runs maxiter*perrnd
+// SpMVs y = A*x on env.stream (tim2), leaves y in para.pagerank_real and
+// appends " gid&MERBIT&SIGMA&tim1&tim2/iters" to FSTR when it is open.
+template <typename V, unsigned BLOCK, unsigned SIGMA>
+MerbitTimes Fun_Merbit(const ENV& env, const PARAMETER<V>& para) {
+  static_assert(std::is_same<V, double>::value || std::is_same<V, float>::value,
+                "Fun_Merbit is instantiated for float and double");
+  Timer timer;
+  MerbitTimes t;
+  const GRAPH<V>& g = *para.graph;
+  timer.start(env.stream);
+  TILE<BLOCK, SIGMA> tile(g.offset, g.nrow, g.nnz, env.stream);
+  t.plan_ms = timer.stop(env.stream);
+  const int iters = para.maxiter * para.perrnd;
+  timer.start(env.stream);
+  for (int rnd = 0; rnd < iters; ++rnd)
+    merbit::check(merbit::spmv(tile.plan, g, V(1), para.person, V(0), para.pagerank_real,
+                               env.stream),
+                  "spmv");
+  const float tim2 = timer.stop(env.stream);
+  t.spmv_ms = iters > 0 ? tim2 / (float)iters : 0.f;
+  if (FSTR.is_open())
+    FSTR << " " << g.graph_id << "&MERBIT&" << SIGMA << "&" << t.plan_ms << "&" << t.spmv_ms
+         << "\n";
+  return t;
+}
+
+// include/utils.cuh:116-132: flags rows with |X-Y| > rtol*|X|, prints the
+// first few and the total flagged error; returns the number of flagged rows.
+template <typename V>
+int Fun_Check(const V* X, const V* Y, int nrow, double rtol = 1e-4, std::ostream& os = std::cout) {
+  double err = 0.0;
+  int bad = 0;
+  for (int i = 0; i < nrow; ++i) {
+    const double d = std::fabs((double)X[i] - (double)Y[i]);
+    if (d > std::fabs((double)X[i]) * rtol || std::isnan(d)) {
+      err += d;
+      if (bad++ < 10) os << "X[" << i << "] = " << X[i] << ", Y[" << i << "] = " << Y[i] << "\n";
+    }
+  }
+  os << ", total error = " << err << std::endl;
+  return bad;
+}

@@ -15,10 +15,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_functions():
     names = set()
-    for h in ("merbit_amd.h", "merbit_gen.h"):
+    for h in ("merbit_amd.h", "merbit_gen.h", "merbit_host.h"):
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-        names |= set(re.findall(r"\b(mb[xg]_[a-z0-9_]+)\s*\(", text))
+        names |= set(re.findall(r"\b(mb[xgh]_[a-z0-9_]+)\s*\(", text))
     return names
 
 
@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
     from merbit_amd import _lib
     L = _lib.lib()
     decl = declared_functions()
-    assert decl == set(_lib.ABI_SYMBOLS) | set(_lib.GEN_SYMBOLS)
+    assert decl == set(_lib.ABI_SYMBOLS) | set(_lib.GEN_SYMBOLS) | set(_lib.HOST_SYMBOLS)
     for name in sorted(decl):
         assert hasattr(L, name), name
 
