@@ -170,6 +170,7 @@ __global__ __launch_bounds__(WG) void tile_spmv_kernel(
   __shared__ lds_t s_v[T];      // products [0,nnz_t), row sums after them
   __shared__ acc_t s_wtail[NW];
   __shared__ int s_wflag[NW];
+  __shared__ double s_tail;
 
   const int tile = xcd_tile(blockIdx.x, ntiles);
   const int tid = threadIdx.x;
@@ -293,24 +294,29 @@ __global__ __launch_bounds__(WG) void tile_spmv_kernel(
   if (tid == WG - 1) {
     acc_t t = s;
     if (fmask == 0ull) t = P::combine(win, t);
-    carry[2 * tile] = P::to_f64(t);  // tail slot: open row at tile end
+    s_tail = P::to_f64(t);
   }
   __syncthreads();
 
-  // 4. Coalesced write-out of the rows that end in this tile.
+  // 4. Coalesced write-out of the rows that end in this tile, and the carry
+  //    slots: tail = partial of the row open at the tile end, head = partial
+  //    of a row that began in an earlier tile (completed by the fixup).
   const bool head_multi = rows_t > 0 && row_ptr[r0] < k0;
   for (int i = tid; i < rows_t; i += WG) {
     const lds_t v = s_y[i];
     if (i == 0 && head_multi)
-      carry[2 * tile + 1] = P::to_f64(v);  // head slot: row began earlier
+      carry[2 * tile + 1] = P::to_f64(v);
     else
       P::store(y + r0 + i, alpha, v, beta, beta_zero != 0);
   }
+  if (tid == 0) carry[2 * tile] = s_tail;
 }
 
-// One wave per tile t >= 1: when t's first row began in an earlier tile ts
-// and ends in t, y[row] = alpha*(tail[ts] + ... + tail[t-1] + head[t]) +
-// beta*y[row], summed in a fixed order.
+// Carry fixup, one wave per tile t >= 1: when t's first row began in an
+// earlier tile ts and ends in t, y[row] = alpha*(tail[ts] + ... + tail[t-1]
+// + head[t]) + beta*y[row].  Lane l sums tails ts+l, ts+l+64, ... in order,
+// then a fixed xor butterfly adds the lanes: the result does not depend on
+// scheduling, and rows spanning thousands of tiles stay 64-way parallel.
 template <class P>
 __global__ __launch_bounds__(256) void carry_fixup_kernel(
     const int32_t* __restrict__ row_ptr, const int2* __restrict__ coord,
@@ -318,7 +324,7 @@ __global__ __launch_bounds__(256) void carry_fixup_kernel(
     int tile_items, typename P::scal_t alpha, typename P::scal_t beta, int beta_zero) {
   const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (t < 1 || t >= ntiles) return;
+  if (t < 1 || t >= ntiles) return;  // uniform per wave
   const int2 c0 = coord[t];
   const int2 c1 = coord[t + 1];
   const int r0 = c0.x, k0 = c0.y;
@@ -426,9 +432,9 @@ int mbx_plan_create(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol,
   p->d_coord = nullptr;
   p->d_carry = nullptr;
   p->ev_begin = p->ev_end = nullptr;
-  if (hipMalloc((void**)&p->d_coord, sizeof(int2) * (size_t)(p->ntiles + 1)) != hipSuccess ||
-      hipMalloc((void**)&p->d_carry, sizeof(double) * 2 * (size_t)(p->ntiles > 0 ? p->ntiles : 1)) !=
-          hipSuccess) {
+  const size_t nt1 = (size_t)p->ntiles + 1;
+  if (hipMalloc((void**)&p->d_coord, sizeof(int2) * nt1) != hipSuccess ||
+      hipMalloc((void**)&p->d_carry, sizeof(double) * 2 * nt1) != hipSuccess) {
     (void)hipGetLastError();
     mbx_plan_destroy(p);
     return MBX_ENOMEM;

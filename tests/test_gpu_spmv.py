@@ -104,6 +104,35 @@ def test_f64_alpha_beta(orc, device):
     assert cond_err(y, 2.0 * ya, 2.0 * absr) <= F64_TOL
 
 
+@pytest.mark.parametrize("kind", ["f64", "f32", "bf16"])
+def test_repeat_calls_bit_identical(device, kind):
+    """The carry completion sums in a fixed order: repeated calls agree bit
+    for bit, including rows that span thousands of tiles."""
+    from merbit_amd import gen
+    from merbit_amd.spmv import MerbitPlan
+    rng = np.random.default_rng(31)
+    cases = edge_case_patterns(2048) + [
+        ("pareto", (rng.pareto(0.9, 40000) * 6).astype(int), 40000)]
+    for name, lens, ncol in cases:
+        say(f"repeat {kind} {name}")
+        rp, col = random_csr(rng, len(lens), ncol, lens=lens)
+        v = rng.standard_normal(int(rp[-1]))
+        x = rng.uniform(-1, 1, ncol)
+        if kind == "f64":
+            vals, xs = v, x
+        else:
+            v32 = v.astype(np.float32)
+            vals = v32 if kind == "f32" else gen.to_bf16(v32)
+            xs = x.astype(np.float32)
+        d_rp, d_col, d_val, d_x = (dev(rp, device), dev(col, device), dev(vals, device),
+                                   dev(xs, device))
+        plan = MerbitPlan(d_rp, ncol, int(rp[-1]))
+        outs = [plan.spmv(d_rp, d_col, d_val, d_x).cpu().numpy() for _ in range(4)]
+        plan.close()
+        for o in outs[1:]:
+            assert np.array_equal(o.view(np.uint8), outs[0].view(np.uint8)), name
+
+
 @pytest.mark.slow
 def test_f64_rmat_full(orc, device):
     from merbit_amd import workloads
