@@ -19,6 +19,7 @@
 #ifndef MERBIT_AMD_H
 #define MERBIT_AMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -84,6 +85,24 @@ int mbx_spmv_f32(const mbx_plan* plan, const int32_t* d_row_ptr,
 int mbx_spmv_bf16(const mbx_plan* plan, const int32_t* d_row_ptr,
                   const int32_t* d_col, const uint16_t* d_val, float alpha,
                   const float* d_x, float beta, float* d_y, void* stream);
+
+/* Dense operand layouts for SpMM: element (i, j) of an n x k matrix with
+ * leading dimension ld is at [i + j*ld] (column-major) or [i*ld + j]. */
+enum { MBX_COL_MAJOR = 0, MBX_ROW_MAJOR = 1 };
+
+/* Batched right-hand sides (no reference counterpart; the reference has SpMV
+ * only): Y = alpha*A*X + beta*Y with X ncol x k and Y nrow x k in FP32, FP32
+ * values, FP32 accumulation, FP64 carries for rows that cross tiles.  Uses a
+ * plan made with items_per_thread 2 or 4 (MBX_EUNSUPPORTED otherwise).  The
+ * caller provides a 256-byte aligned device workspace of at least
+ * mbx_spmm_workspace_bytes(plan, k, X, ldx, x_layout) bytes; a column-major
+ * (or unaligned row-major) X is repacked into it as rows once per call. */
+size_t mbx_spmm_workspace_bytes(const mbx_plan* plan, int32_t k, const float* d_x,
+                                int64_t ldx, int32_t x_layout);
+int mbx_spmm_f32(const mbx_plan* plan, const int32_t* d_row_ptr, const int32_t* d_col,
+                 const float* d_val, int32_t k, float alpha, const float* d_x, int64_t ldx,
+                 int32_t x_layout, float beta, float* d_y, int64_t ldy, int32_t y_layout,
+                 void* d_workspace, size_t workspace_bytes, void* stream);
 
 /* Status text, like the CSR5 error codes (include/csr5/detail/common.h:13-22). */
 const char* mbx_status_str(int status);

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(_PKG, "libmerbit_amd.so")
 ABI_SYMBOLS = (
     "mbx_plan_create", "mbx_plan_destroy", "mbx_plan_info", "mbx_plan_partition",
     "mbx_plan_set_events", "mbx_spmv_f64", "mbx_spmv_f32", "mbx_spmv_bf16",
+    "mbx_spmm_workspace_bytes", "mbx_spmm_f32",
     "mbx_status_str",
 )
 GEN_SYMBOLS = (
@@ -26,6 +27,8 @@ GEN_SYMBOLS = (
 HOST_SYMBOLS = ("mbh_config_lookup", "mbh_mtx_read", "mbh_ref_ingest")
 
 MBX_OK = 0
+MBX_COL_MAJOR = 0
+MBX_ROW_MAJOR = 1
 
 _lib = None
 
@@ -58,6 +61,10 @@ def lib() -> ctypes.CDLL:
     L.mbx_spmv_f64.argtypes = [vp, vp, vp, vp, f64, vp, f64, vp, vp]
     L.mbx_spmv_f32.argtypes = [vp, vp, vp, vp, f32, vp, f32, vp, vp]
     L.mbx_spmv_bf16.argtypes = [vp, vp, vp, vp, f32, vp, f32, vp, vp]
+    L.mbx_spmm_workspace_bytes.argtypes = [vp, i32, vp, i64, i32]
+    L.mbx_spmm_workspace_bytes.restype = ctypes.c_size_t
+    L.mbx_spmm_f32.argtypes = [vp, vp, vp, vp, i32, f32, vp, i64, i32, f32, vp, i64, i32,
+                               vp, ctypes.c_size_t, vp]
     L.mbx_status_str.argtypes = [ctypes.c_int]
     L.mbx_status_str.restype = ctypes.c_char_p
     L.mbg_fill_uniform_f64.argtypes = [vp, i64, f64, f64, u64]

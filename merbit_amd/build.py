@@ -35,6 +35,7 @@ HIP_FLAGS = [
 
 LIB_SOURCES = [
     os.path.join(CSRC, "merbit_spmv.hip"),
+    os.path.join(CSRC, "merbit_spmm.hip"),
     os.path.join(CSRC, "merbit_gen.cpp"),
     os.path.join(CSRC, "merbit_host.cpp"),
 ]
@@ -77,6 +78,7 @@ def build_library(verbose: bool = False) -> str:
     # One hipcc call per source: mixing -x hip and -x c++ inputs in one call
     # drops the requested --offload-arch.
     for src in LIB_SOURCES:
+        # every .hip is a HIP translation unit; .cpp files are host C++
         obj = os.path.join(obj_dir, os.path.basename(src) + ".o")
         lang = "hip" if src.endswith(".hip") else "c++"
         flags = HIP_FLAGS if lang == "hip" else ["-O3", "-std=c++17", "-fPIC", "-Wall"]

@@ -1,0 +1,58 @@
+// merbit_internal.hpp -- pieces shared by the HIP translation units of
+// libmerbit_amd.so (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "merbit_amd.h"
+
+// The merge-path plan: one (rows consumed, nnz consumed) coordinate per tile
+// boundary of wg*ipt merge items, plus FP64 carry slots (tail, head) per tile
+// used by the SpMV carry fixup.
+struct mbx_plan {
+  int32_t nrow, ncol, nnz;
+  int32_t wg, ipt, tile_items, ntiles;
+  int2* d_coord;
+  double* d_carry;
+  mutable hipEvent_t ev_begin, ev_end;
+};
+
+namespace mbx_detail {
+
+constexpr int kXcds = 8;
+
+// Workgroups are dispatched round-robin over the 8 XCDs; give XCD j the
+// contiguous tile range starting at j*q+min(j,r) so neighbouring tiles (which
+// share x lines and the cache lines at tile boundaries) meet in one L2.
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  const int q = n / kXcds, r = n % kXcds;
+  const int j = b % kXcds, i = b / kXcds;
+  return j * q + min(j, r) + i;
+}
+
+// Merge-path search in LDS: first row r in [max(0,d-nnz_t), min(d,rows_t)]
+// whose local end s_re[r] exceeds d-r-1 (an nnz index is consumed before an
+// equal row end), include/utils.cuh:5-29.
+__device__ __forceinline__ int lds_merge_search(const int* s_re, int d, int nnz_t, int rows_t) {
+  int lo = max(0, d - nnz_t), hi = min(d, rows_t);
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (s_re[mid] <= d - mid - 1)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Launch-side validation shared by every entry point taking a plan.
+inline bool plan_csr_args_ok(const mbx_plan* p, const void* rp, const void* col,
+                             const void* val) {
+  if (!p) return false;
+  if (p->nrow > 0 && !rp) return false;
+  if (p->nnz > 0 && (!col || !val)) return false;
+  return true;
+}
+
+}  // namespace mbx_detail

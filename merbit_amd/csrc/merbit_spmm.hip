@@ -1,0 +1,388 @@
+// merbit_spmm.hip -- merge-path tile SpMM Y = alpha*A*X + beta*Y (FP32).
+//
+// The SpMV tile kernel generalised to KC-wide vectors: a tile's col/val stay
+// in registers, and for each chunk of KC right-hand sides the products
+// val*X[col, chunk] are staged in LDS as KC-vectors, walked per thread along
+// the merge path, combined by a per-component segmented scan, and rows ending
+// in the tile are written with coalesced stores.  X is read as rows (the KC
+// values of one column index are contiguous), so each nonzero gathers one
+// KC*4-byte segment instead of KC scattered words; a column-major X is
+// repacked once per call into the caller's workspace.  Carries of rows that
+// cross tiles (FP64, per right-hand side) are completed by a fixup kernel in a
+// fixed order, as for SpMV.
+
+#include <hip/hip_runtime.h>
+#include <limits.h>
+#include <stdint.h>
+
+#include "merbit_amd.h"
+#include "merbit_internal.hpp"
+
+using mbx_detail::lds_merge_search;
+using mbx_detail::xcd_tile;
+
+namespace {
+
+constexpr int kTransRows = 128;
+
+// Column-major X (ncol x k, leading dimension ldx) -> row-major Xr (ld ldr):
+// coalesced reads down each column, coalesced writes along each row.
+__global__ __launch_bounds__(256) void repack_rows_kernel(const float* __restrict__ X,
+                                                          long long ldx, int x_row_major,
+                                                          int ncol, int k,
+                                                          float* __restrict__ Xr, int ldr) {
+  __shared__ float t[64][kTransRows + 1];
+  const int c0 = blockIdx.x * kTransRows;
+  const int j0 = blockIdx.y * 64;
+  const int kk = min(64, k - j0);
+  for (int e = threadIdx.x; e < kk * kTransRows; e += 256) {
+    int j, cc;
+    if (x_row_major) {
+      cc = e / kk;
+      j = e % kk;
+    } else {
+      j = e / kTransRows;
+      cc = e % kTransRows;
+    }
+    const int c = c0 + cc;
+    if (c < ncol)
+      t[j][cc] = x_row_major ? X[(long long)c * ldx + j0 + j] : X[c + (long long)(j0 + j) * ldx];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < kk * kTransRows; e += 256) {
+    const int cc = e / kk, j = e % kk;
+    const int c = c0 + cc;
+    if (c < ncol) Xr[(long long)c * ldr + j0 + j] = t[j][cc];
+  }
+}
+
+template <int KC>
+struct Vec {
+  float v[KC];
+};
+
+template <int KC>
+__device__ __forceinline__ void vzero(Vec<KC>& a) {
+#pragma unroll
+  for (int q = 0; q < KC; ++q) a.v[q] = 0.f;
+}
+
+template <int WG, int IPT, int KC>
+__global__ __launch_bounds__(WG) void tile_spmm_kernel(
+    const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
+    const float* __restrict__ val, const float* __restrict__ Xr, int ldr, int k,
+    float* __restrict__ Y, long long ldy, int y_row_major, const int2* __restrict__ coord,
+    double* __restrict__ carry, int ntiles, float alpha, float beta, int beta_zero) {
+  constexpr int T = WG * IPT;
+  constexpr int NW = WG / 64;
+  constexpr int KG = KC / 4;  // float4 groups per chunk
+  __shared__ int s_re[T + 1];
+  __shared__ float4 s_v[T * KG];  // product vectors [0,nnz_t), row sums after
+  __shared__ float s_wtail[NW][KC];
+  __shared__ int s_wflag[NW];
+  __shared__ float s_tail[KC];
+
+  const int tile = xcd_tile(blockIdx.x, ntiles);
+  const int tid = threadIdx.x;
+  const int2 c0 = coord[tile];
+  const int2 c1 = coord[tile + 1];
+  const int r0 = c0.x, k0 = c0.y;
+  const int rows_t = c1.x - r0;
+  const int nnz_t = c1.y - k0;
+
+  int ci[IPT];
+  float vv[IPT];
+  {
+    int re[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const int i = tid + j * WG;
+      re[j] = INT_MAX;
+      ci[j] = 0;
+      vv[j] = 0.f;
+      if (i < rows_t) re[j] = row_ptr[r0 + 1 + i];
+      if (i < nnz_t) {
+        ci[j] = col[k0 + i];
+        vv[j] = val[k0 + i];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const int i = tid + j * WG;
+      if (i < rows_t) s_re[i] = re[j] - k0;
+    }
+    if (tid == 0) s_re[rows_t] = INT_MAX;
+  }
+  __syncthreads();
+
+  const int d0 = tid * IPT;
+  const int items = min(max(nnz_t + rows_t - d0, 0), IPT);
+  const int r_start = items > 0 ? lds_merge_search(s_re, d0, nnz_t, rows_t) : 0;
+  const int k_start = d0 - r_start;
+  const bool head_multi = rows_t > 0 && row_ptr[r0] < k0;
+  const int lane = tid & 63, wave = tid >> 6;
+  float4* s_y = s_v + nnz_t * KG;
+
+  for (int cb = 0; cb < k; cb += KC) {
+    const int kc = min(KC, k - cb);
+    // 1. products for this chunk of right-hand sides
+    {
+      float4 xv[IPT][KG];
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        const int i = tid + j * WG;
+#pragma unroll
+        for (int g = 0; g < KG; ++g) {
+          xv[j][g] = make_float4(0.f, 0.f, 0.f, 0.f);
+          if (i < nnz_t && 4 * g < kc)
+            xv[j][g] = *reinterpret_cast<const float4*>(Xr + (long long)ci[j] * ldr + cb + 4 * g);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        const int i = tid + j * WG;
+        if (i < nnz_t) {
+#pragma unroll
+          for (int g = 0; g < KG; ++g) {
+            const float4 a = xv[j][g];
+            s_v[i * KG + g] = make_float4(vv[j] * a.x, vv[j] * a.y, vv[j] * a.z, vv[j] * a.w);
+          }
+        }
+      }
+    }
+    __syncthreads();
+
+    // 2. walk
+    Vec<KC> sum, head;
+    vzero(sum);
+    vzero(head);
+    int head_row = 0;
+    bool flagged = false;
+    if (items > 0) {
+      int r = r_start, kk = k_start;
+      int re = s_re[r];
+#pragma unroll
+      for (int i = 0; i < IPT; ++i) {
+        if (i < items) {
+          if (kk < re) {
+#pragma unroll
+            for (int g = 0; g < KG; ++g) {
+              const float4 p = s_v[kk * KG + g];
+              sum.v[4 * g] += p.x;
+              sum.v[4 * g + 1] += p.y;
+              sum.v[4 * g + 2] += p.z;
+              sum.v[4 * g + 3] += p.w;
+            }
+            ++kk;
+          } else {
+            if (flagged) {
+#pragma unroll
+              for (int g = 0; g < KG; ++g)
+                s_y[r * KG + g] = make_float4(sum.v[4 * g], sum.v[4 * g + 1],
+                                              sum.v[4 * g + 2], sum.v[4 * g + 3]);
+            } else {
+              head = sum;
+              head_row = r;
+              flagged = true;
+            }
+            vzero(sum);
+            ++r;
+            re = s_re[r];
+          }
+        }
+      }
+    }
+
+    // 3. per-component segmented scan across the tile
+    const unsigned long long fmask = __ballot(flagged);
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    const unsigned long long fb = fmask & upto;
+    const int seg = fb ? 63 - __clzll(fb) : -1;
+    Vec<KC> s = sum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const bool take = lane >= d && lane - d >= seg;
+#pragma unroll
+      for (int q = 0; q < KC; ++q) {
+        const float o = __shfl_up(s.v[q], d);
+        if (take) s.v[q] = o + s.v[q];
+      }
+    }
+    Vec<KC> cin;
+#pragma unroll
+    for (int q = 0; q < KC; ++q) cin.v[q] = __shfl_up(s.v[q], 1);
+    if (lane == 63) {
+#pragma unroll
+      for (int q = 0; q < KC; ++q) s_wtail[wave][q] = s.v[q];
+      s_wflag[wave] = fmask != 0ull;
+    }
+    __syncthreads();
+    Vec<KC> win;
+    vzero(win);
+#pragma unroll
+    for (int w = 0; w < NW - 1; ++w)
+      if (w < wave) {
+        const bool f = s_wflag[w];
+#pragma unroll
+        for (int q = 0; q < KC; ++q) win.v[q] = f ? s_wtail[w][q] : win.v[q] + s_wtail[w][q];
+      }
+    const bool ext = (fmask & ((1ull << lane) - 1ull)) == 0ull;
+    if (flagged) {
+      Vec<KC> c;
+#pragma unroll
+      for (int q = 0; q < KC; ++q) {
+        c.v[q] = (lane == 0) ? 0.f : cin.v[q];
+        if (ext) c.v[q] = win.v[q] + c.v[q];
+      }
+#pragma unroll
+      for (int g = 0; g < KG; ++g)
+        s_y[head_row * KG + g] =
+            make_float4(c.v[4 * g] + head.v[4 * g], c.v[4 * g + 1] + head.v[4 * g + 1],
+                        c.v[4 * g + 2] + head.v[4 * g + 2], c.v[4 * g + 3] + head.v[4 * g + 3]);
+    }
+    if (tid == WG - 1) {
+#pragma unroll
+      for (int q = 0; q < KC; ++q) s_tail[q] = (fmask == 0ull) ? win.v[q] + s.v[q] : s.v[q];
+    }
+    __syncthreads();
+
+    // 4. write-out: coalesced along rows (column-major Y) or along columns
+    const float* sy = reinterpret_cast<const float*>(s_y);
+    const int n_out = rows_t * kc;
+    for (int e = tid; e < n_out; e += WG) {
+      int i, q;
+      if (y_row_major) {
+        i = e / kc;
+        q = e - i * kc;
+      } else {
+        q = e / rows_t;
+        i = e - q * rows_t;
+      }
+      const float v = sy[i * KC + q];
+      const int jcol = cb + q;
+      if (i == 0 && head_multi) {
+        carry[(2ll * tile + 1) * k + jcol] = (double)v;
+      } else {
+        float* yp = y_row_major ? Y + (long long)(r0 + i) * ldy + jcol
+                                : Y + (r0 + i) + (long long)jcol * ldy;
+        *yp = beta_zero ? alpha * v : alpha * v + beta * *yp;
+      }
+    }
+    if (tid < kc) carry[(2ll * tile) * k + cb + tid] = (double)s_tail[tid];
+    __syncthreads();
+  }
+}
+
+// One wave per tile t >= 1 whose first row began in an earlier tile ts and
+// ends in t; lane q completes right-hand side q (q += 64) summing
+// tail[ts..t-1] then head[t] in order.
+__global__ __launch_bounds__(256) void spmm_fixup_kernel(
+    const int32_t* __restrict__ row_ptr, const int2* __restrict__ coord,
+    const double* __restrict__ carry, int k, float* __restrict__ Y, long long ldy,
+    int y_row_major, int ntiles, int tile_items, float alpha, float beta, int beta_zero) {
+  const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (t < 1 || t >= ntiles) return;
+  const int2 c0 = coord[t];
+  const int2 c1 = coord[t + 1];
+  const int r0 = c0.x, k0 = c0.y;
+  if (c1.x <= r0) return;
+  const int rs = row_ptr[r0];
+  if (rs >= k0) return;
+  const int ts = (int)(((long long)r0 + rs) / tile_items);
+  for (int q = lane; q < k; q += 64) {
+    double acc = 0.0;
+    for (int s = ts; s < t; ++s) acc += carry[(2ll * s) * k + q];
+    acc += carry[(2ll * t + 1) * k + q];
+    float* yp = y_row_major ? Y + (long long)r0 * ldy + q : Y + r0 + (long long)q * ldy;
+    *yp = beta_zero ? (float)((double)alpha * acc)
+                    : (float)((double)alpha * acc + (double)beta * (double)*yp);
+  }
+}
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Row-major X usable in place: 16-byte aligned rows holding whole float4
+// groups (the kernel reads KC-wide chunks as float4).
+bool x_direct(const float* X, long long ldx, int x_layout, int k) {
+  return x_layout == MBX_ROW_MAJOR && k % 4 == 0 && ldx % 4 == 0 &&
+         (reinterpret_cast<uintptr_t>(X) & 15u) == 0;
+}
+
+size_t workspace_bytes(const mbx_plan* p, int k, bool repack) {
+  const int ldr = (k + 3) / 4 * 4;
+  size_t b = align_up(sizeof(double) * 2 * ((size_t)p->ntiles + 1) * (size_t)k, 256);
+  if (repack) b += align_up(sizeof(float) * (size_t)p->ncol * (size_t)ldr, 256);
+  return b;
+}
+
+template <int IPT, int KC>
+void launch_spmm(const mbx_plan* p, hipStream_t s, const int32_t* rp, const int32_t* col,
+                 const float* val, const float* Xr, int ldr, int k, float* Y, long long ldy,
+                 int yrm, double* carry, float alpha, float beta) {
+  const int bz = beta == 0.f;
+  if (p->ev_begin) (void)hipEventRecord(p->ev_begin, s);
+  tile_spmm_kernel<256, IPT, KC><<<p->ntiles, 256, 0, s>>>(
+      rp, col, val, Xr, ldr, k, Y, ldy, yrm, p->d_coord, carry, p->ntiles, alpha, beta, bz);
+  if (p->ev_end) (void)hipEventRecord(p->ev_end, s);
+  p->ev_begin = p->ev_end = nullptr;
+  if (p->ntiles > 1)
+    spmm_fixup_kernel<<<(p->ntiles + 3) / 4, 256, 0, s>>>(
+        rp, p->d_coord, carry, k, Y, ldy, yrm, p->ntiles, p->tile_items, alpha, beta, bz);
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t mbx_spmm_workspace_bytes(const mbx_plan* p, int32_t k, const float* d_x, int64_t ldx,
+                                int32_t x_layout) {
+  if (!p || k <= 0) return 0;
+  return workspace_bytes(p, k, !x_direct(d_x, ldx, x_layout, k));
+}
+
+int mbx_spmm_f32(const mbx_plan* p, const int32_t* rp, const int32_t* col, const float* val,
+                 int32_t k, float alpha, const float* X, int64_t ldx, int32_t x_layout,
+                 float beta, float* Y, int64_t ldy, int32_t y_layout, void* workspace,
+                 size_t workspace_size, void* stream) {
+  if (!mbx_detail::plan_csr_args_ok(p, rp, col, val) || k <= 0) return MBX_EINVAL;
+  if (x_layout != MBX_COL_MAJOR && x_layout != MBX_ROW_MAJOR) return MBX_EINVAL;
+  if (y_layout != MBX_COL_MAJOR && y_layout != MBX_ROW_MAJOR) return MBX_EINVAL;
+  if ((p->ncol > 0 && !X) || (p->nrow > 0 && !Y)) return MBX_EINVAL;
+  if (x_layout == MBX_COL_MAJOR ? ldx < p->ncol : ldx < k) return MBX_EINVAL;
+  if (y_layout == MBX_COL_MAJOR ? ldy < p->nrow : ldy < k) return MBX_EINVAL;
+  if (p->wg != 256 || (p->ipt != 2 && p->ipt != 4)) return MBX_EUNSUPPORTED;
+  if (p->ntiles == 0) return MBX_OK;
+  const bool repack = !x_direct(X, ldx, x_layout, k);
+  const size_t need = workspace_bytes(p, k, repack);
+  if (!workspace || workspace_size < need) return MBX_EINVAL;
+  if ((reinterpret_cast<uintptr_t>(workspace) & 255u) != 0) return MBX_EINVAL;
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = static_cast<char*>(workspace);
+  double* carry = reinterpret_cast<double*>(ws);
+  const float* Xr = X;
+  int ldr = (int)ldx;
+  if (repack) {
+    ldr = (k + 3) / 4 * 4;
+    float* xr = reinterpret_cast<float*>(
+        ws + align_up(sizeof(double) * 2 * ((size_t)p->ntiles + 1) * (size_t)k, 256));
+    if (p->ncol > 0) {
+      dim3 grid((p->ncol + kTransRows - 1) / kTransRows, (k + 63) / 64);
+      repack_rows_kernel<<<grid, 256, 0, s>>>(X, ldx, x_layout == MBX_ROW_MAJOR, p->ncol, k, xr,
+                                              ldr);
+    }
+    Xr = xr;
+  }
+  const int yrm = y_layout == MBX_ROW_MAJOR;
+  const bool wide = k > 4;
+  if (p->ipt == 2) {
+    if (wide) launch_spmm<2, 8>(p, s, rp, col, val, Xr, ldr, k, Y, ldy, yrm, carry, alpha, beta);
+    else launch_spmm<2, 4>(p, s, rp, col, val, Xr, ldr, k, Y, ldy, yrm, carry, alpha, beta);
+  } else {
+    if (wide) launch_spmm<4, 8>(p, s, rp, col, val, Xr, ldr, k, Y, ldy, yrm, carry, alpha, beta);
+    else launch_spmm<4, 4>(p, s, rp, col, val, Xr, ldr, k, Y, ldy, yrm, carry, alpha, beta);
+  }
+  return hipGetLastError() == hipSuccess ? MBX_OK : MBX_EHIP;
+}
+
+}  // extern "C"

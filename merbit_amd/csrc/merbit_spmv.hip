@@ -36,6 +36,10 @@
 #include <stdint.h>
 
 #include "merbit_amd.h"
+#include "merbit_internal.hpp"
+
+using mbx_detail::lds_merge_search;
+using mbx_detail::xcd_tile;
 
 namespace {
 
@@ -123,17 +127,6 @@ using BF16C = CompF32<uint16_t>;
 // ---------------------------------------------------------------------------
 // Kernels
 // ---------------------------------------------------------------------------
-
-constexpr int kXcds = 8;
-
-// Workgroups are dispatched round-robin over the 8 XCDs; give XCD j the
-// contiguous tile range j*q+min(j,r) .. so neighbouring tiles (which share
-// x lines and the cache lines at tile boundaries) meet in one L2.
-__device__ __forceinline__ int xcd_tile(int b, int n) {
-  const int q = n / kXcds, r = n % kXcds;
-  const int j = b % kXcds, i = b / kXcds;
-  return j * q + min(j, r) + i;
-}
 
 // merge_search over the global row_ptr (include/utils.cuh:5-29).
 __global__ void partition_kernel(const int32_t* __restrict__ row_ptr, int nrow,
@@ -229,15 +222,8 @@ __global__ __launch_bounds__(WG) void tile_spmv_kernel(
   int head_row = 0;
   bool flagged = false;
   if (items > 0) {
-    int lo = max(0, d0 - nnz_t), hi = min(d0, rows_t);
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (s_re[mid] <= d0 - mid - 1)
-        lo = mid + 1;
-      else
-        hi = mid;
-    }
-    int r = lo, k = d0 - lo;
+    int r = lds_merge_search(s_re, d0, nnz_t, rows_t);
+    int k = d0 - r;
     int re = s_re[r];
 #pragma unroll
     for (int i = 0; i < IPT; ++i) {
@@ -345,20 +331,12 @@ __global__ __launch_bounds__(256) void carry_fixup_kernel(
 // Plan and C ABI
 // ---------------------------------------------------------------------------
 
-struct mbx_plan {
-  int32_t nrow, ncol, nnz;
-  int32_t wg, ipt, tile_items, ntiles;
-  int2* d_coord;
-  double* d_carry;
-  mutable hipEvent_t ev_begin, ev_end;
-};
-
 namespace {
 
 bool supported_shape(int wg, int ipt) {
   if (wg != 256) return false;
   switch (ipt) {
-    case 4: case 6: case 8: case 12: case 16: return true;
+    case 2: case 4: case 6: case 8: case 12: case 16: return true;
     default: return false;
   }
 }
@@ -394,6 +372,7 @@ int run_spmv(const mbx_plan* p, const int32_t* rp, const int32_t* col,
   hipStream_t s = (hipStream_t)stream;
   hipError_t e;
   switch (p->ipt) {
+    case 2: e = launch_tile<P, 2>(p, s, rp, col, val, x, y, alpha, beta); break;
     case 4: e = launch_tile<P, 4>(p, s, rp, col, val, x, y, alpha, beta); break;
     case 6: e = launch_tile<P, 6>(p, s, rp, col, val, x, y, alpha, beta); break;
     case 8: e = launch_tile<P, 8>(p, s, rp, col, val, x, y, alpha, beta); break;

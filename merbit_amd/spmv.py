@@ -109,6 +109,54 @@ class MerbitPlan:
         check(st, f"mbx_spmv_{kind}")
         return y
 
+    @staticmethod
+    def _layout(t: torch.Tensor, name: str):
+        """(layout, leading dimension) of a 2-D tensor viewed as n x k."""
+        from ._lib import MBX_COL_MAJOR, MBX_ROW_MAJOR
+        if t.dim() != 2:
+            raise ValueError(f"{name} must be 2-D")
+        st = t.stride()
+        if st[1] == 1 and (t.shape[0] <= 1 or st[0] >= t.shape[1]):
+            return MBX_ROW_MAJOR, st[0] if t.shape[0] > 1 else t.shape[1]
+        if st[0] == 1 and (t.shape[1] <= 1 or st[1] >= t.shape[0]):
+            return MBX_COL_MAJOR, st[1] if t.shape[1] > 1 else t.shape[0]
+        raise ValueError(f"{name} must be row- or column-major")
+
+    def spmm(self, row_ptr: torch.Tensor, col: torch.Tensor, val: torch.Tensor,
+             X: torch.Tensor, Y: torch.Tensor | None = None, alpha: float = 1.0,
+             beta: float = 0.0, workspace: torch.Tensor | None = None,
+             stream=None) -> torch.Tensor:
+        """Y = alpha*A*X + beta*Y for FP32 X (ncol x k) and Y (nrow x k), each
+        row- or column-major (read from the strides).  Y defaults to a new
+        column-major tensor."""
+        if X.dtype != torch.float32 or val.dtype != torch.float32:
+            raise TypeError("spmm takes FP32 values and right-hand sides")
+        if X.shape[0] != self.ncol:
+            raise ValueError("X has the wrong number of rows")
+        k = int(X.shape[1])
+        if Y is None:
+            Y = torch.empty((k, self.nrow), dtype=torch.float32, device=X.device).t()
+            beta = 0.0
+        if tuple(Y.shape) != (self.nrow, k):
+            raise ValueError("Y has the wrong shape")
+        xl, ldx = self._layout(X, "X")
+        yl, ldy = self._layout(Y, "Y")
+        need = lib().mbx_spmm_workspace_bytes(self.handle, k, X.data_ptr(), ldx, xl)
+        if workspace is None or workspace.numel() < need:
+            workspace = torch.empty(max(need, 1), dtype=torch.uint8, device=X.device)
+        check(lib().mbx_spmm_f32(self.handle, _dptr(row_ptr, torch.int32, "row_ptr"),
+                                 _dptr(col, torch.int32, "col"),
+                                 _dptr(val, torch.float32, "val"), k, alpha, X.data_ptr(),
+                                 ldx, xl, beta, Y.data_ptr(), ldy, yl,
+                                 workspace.data_ptr(), workspace.numel(),
+                                 _stream_handle(stream)), "mbx_spmm_f32")
+        return Y
+
+    def spmm_workspace_bytes(self, X: torch.Tensor) -> int:
+        xl, ldx = self._layout(X, "X")
+        return int(lib().mbx_spmm_workspace_bytes(self.handle, int(X.shape[1]), X.data_ptr(),
+                                                  ldx, xl))
+
     def close(self) -> None:
         if getattr(self, "_h", None) is not None:
             lib().mbx_plan_destroy(self._h)
