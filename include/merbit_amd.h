@@ -104,6 +104,62 @@ int mbx_spmm_f32(const mbx_plan* plan, const int32_t* d_row_ptr, const int32_t* 
                  int32_t x_layout, float beta, float* d_y, int64_t ldy, int32_t y_layout,
                  void* d_workspace, size_t workspace_bytes, void* stream);
 
+/* Value kinds for the kind-generic entry points below. */
+enum { MBX_VALUE_F64 = 0, MBX_VALUE_F32 = 1, MBX_VALUE_BF16 = 2 };
+
+/* Many CSR matrices in one persistent launch (no reference counterpart; the
+ * reference times one graph per process, src/main.cu:25-65).  Each item
+ * names device arrays already in HBM: values of the batch's value kind
+ * (double, float or bfloat16 bits), x and y double for MBX_VALUE_F64 and
+ * float otherwise.  mbx_batch_spmv computes y_m = alpha*A_m*x_m + beta*y_m
+ * for every item with the same per-tile arithmetic as mbx_spmv_*, so the
+ * results are bit-identical to one call per matrix.  mbx_batch_create
+ * synchronises `stream` once (it uploads the descriptors). */
+typedef struct {
+  const int32_t* row_ptr;
+  const int32_t* col;
+  const void* val;
+  const void* x;
+  void* y;
+  int32_t nrow, ncol, nnz;
+} mbx_batch_item;
+typedef struct mbx_batch mbx_batch;
+int mbx_batch_create(const mbx_batch_item* items, int32_t count, int32_t value_kind,
+                     int32_t items_per_thread, void* stream, mbx_batch** out);
+void mbx_batch_destroy(mbx_batch* batch);
+int mbx_batch_info(const mbx_batch* batch, int32_t* total_tiles, int32_t* tile_items,
+                   int32_t* grid);
+int mbx_batch_spmv(const mbx_batch* batch, double alpha, double beta, void* stream);
+
+/* HIP graphs of repeated products (the reference's timed loop of
+ * maxiter*perrnd launches, include/tile.cuh:596-603): `repeats` back-to-back
+ * products captured once, replayed by mbx_graph_launch.  The graph holds the
+ * pointers given here.  graph_exec is a hipGraphExec_t. */
+int mbx_batch_graph_create(const mbx_batch* batch, double alpha, double beta, int32_t repeats,
+                           void** graph_exec);
+int mbx_spmv_graph_create(const mbx_plan* plan, int32_t value_kind, const int32_t* d_row_ptr,
+                          const int32_t* d_col, const void* d_val, double alpha,
+                          const void* d_x, double beta, void* d_y, int32_t repeats,
+                          void** graph_exec);
+int mbx_graph_launch(void* graph_exec, void* stream);
+void mbx_graph_destroy(void* graph_exec);
+
+/* Timing autotuner (no reference counterpart: SIGMA and BLOCK are
+ * compile-time in the reference, CMakeLists.txt:4-6): times every
+ * instantiated items_per_thread on the given operands (which it overwrites in
+ * y) and returns the fastest with its time per product in microseconds. */
+int mbx_autotune_spmv(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol, int32_t nnz,
+                      const int32_t* d_col, const void* d_val, int32_t value_kind,
+                      const void* d_x, void* d_y, int32_t reps, void* stream, int32_t* best_ipt,
+                      float* best_us);
+int mbx_autotune_spmm(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol, int32_t nnz,
+                      const int32_t* d_col, const float* d_val, int32_t k, const float* d_x,
+                      int64_t ldx, int32_t x_layout, float* d_y, int64_t ldy, int32_t y_layout,
+                      void* d_workspace, size_t workspace_bytes, int32_t reps, void* stream,
+                      int32_t* best_ipt, float* best_us);
+int mbx_autotune_batch(const mbx_batch_item* items, int32_t count, int32_t value_kind,
+                       int32_t reps, void* stream, int32_t* best_ipt, float* best_us);
+
 /* Status text, like the CSR5 error codes (include/csr5/detail/common.h:13-22). */
 const char* mbx_status_str(int status);
 

@@ -17,6 +17,9 @@ ABI_SYMBOLS = (
     "mbx_plan_create", "mbx_plan_destroy", "mbx_plan_info", "mbx_plan_partition",
     "mbx_plan_set_events", "mbx_spmv_f64", "mbx_spmv_f32", "mbx_spmv_bf16",
     "mbx_spmm_workspace_bytes", "mbx_spmm_f32",
+    "mbx_batch_create", "mbx_batch_destroy", "mbx_batch_info", "mbx_batch_spmv",
+    "mbx_batch_graph_create", "mbx_spmv_graph_create", "mbx_graph_launch", "mbx_graph_destroy",
+    "mbx_autotune_spmv", "mbx_autotune_spmm", "mbx_autotune_batch",
     "mbx_status_str",
 )
 GEN_SYMBOLS = (
@@ -29,6 +32,16 @@ HOST_SYMBOLS = ("mbh_config_lookup", "mbh_mtx_read", "mbh_ref_ingest")
 MBX_OK = 0
 MBX_COL_MAJOR = 0
 MBX_ROW_MAJOR = 1
+MBX_VALUE_F64 = 0
+MBX_VALUE_F32 = 1
+MBX_VALUE_BF16 = 2
+
+
+class BatchItem(ctypes.Structure):
+    """mbx_batch_item of include/merbit_amd.h."""
+    _fields_ = [("row_ptr", ctypes.c_void_p), ("col", ctypes.c_void_p),
+                ("val", ctypes.c_void_p), ("x", ctypes.c_void_p), ("y", ctypes.c_void_p),
+                ("nrow", ctypes.c_int32), ("ncol", ctypes.c_int32), ("nnz", ctypes.c_int32)]
 
 _lib = None
 
@@ -65,6 +78,24 @@ def lib() -> ctypes.CDLL:
     L.mbx_spmm_workspace_bytes.restype = ctypes.c_size_t
     L.mbx_spmm_f32.argtypes = [vp, vp, vp, vp, i32, f32, vp, i64, i32, f32, vp, i64, i32,
                                vp, ctypes.c_size_t, vp]
+    L.mbx_batch_create.argtypes = [vp, i32, i32, i32, vp, ctypes.POINTER(vp)]
+    L.mbx_batch_destroy.argtypes = [vp]
+    L.mbx_batch_destroy.restype = None
+    L.mbx_batch_info.argtypes = [vp, vp, vp, vp]
+    L.mbx_batch_spmv.argtypes = [vp, f64, f64, vp]
+    L.mbx_batch_graph_create.argtypes = [vp, f64, f64, i32, ctypes.POINTER(vp)]
+    L.mbx_spmv_graph_create.argtypes = [vp, i32, vp, vp, vp, f64, vp, f64, vp, i32,
+                                        ctypes.POINTER(vp)]
+    L.mbx_graph_launch.argtypes = [vp, vp]
+    L.mbx_graph_destroy.argtypes = [vp]
+    L.mbx_graph_destroy.restype = None
+    L.mbx_autotune_spmv.argtypes = [vp, i32, i32, i32, vp, vp, i32, vp, vp, i32, vp,
+                                    ctypes.POINTER(i32), ctypes.POINTER(f32)]
+    L.mbx_autotune_spmm.argtypes = [vp, i32, i32, i32, vp, vp, i32, vp, i64, i32, vp, i64, i32,
+                                    vp, ctypes.c_size_t, i32, vp, ctypes.POINTER(i32),
+                                    ctypes.POINTER(f32)]
+    L.mbx_autotune_batch.argtypes = [vp, i32, i32, i32, vp, ctypes.POINTER(i32),
+                                     ctypes.POINTER(f32)]
     L.mbx_status_str.argtypes = [ctypes.c_int]
     L.mbx_status_str.restype = ctypes.c_char_p
     L.mbg_fill_uniform_f64.argtypes = [vp, i64, f64, f64, u64]

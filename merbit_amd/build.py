@@ -36,6 +36,8 @@ HIP_FLAGS = [
 LIB_SOURCES = [
     os.path.join(CSRC, "merbit_spmv.hip"),
     os.path.join(CSRC, "merbit_spmm.hip"),
+    os.path.join(CSRC, "merbit_batch.hip"),
+    os.path.join(CSRC, "merbit_tune.hip"),
     os.path.join(CSRC, "merbit_gen.cpp"),
     os.path.join(CSRC, "merbit_host.cpp"),
 ]

@@ -1,0 +1,339 @@
+// merbit_batch.hip -- many uneven CSR matrices in one persistent launch,
+// HIP-graph capture of repeated products, and the timing autotuner.
+//
+// A batch concatenates the merge-path partitions of its matrices into one
+// global tile space.  A persistent kernel (one workgroup per resident slot)
+// walks that space, each XCD taking a contiguous range of global tiles; every
+// tile runs exactly the per-tile code of the single-matrix kernel, so a batch
+// product is bit-identical to calling mbx_spmv_* once per matrix.  One fixup
+// launch completes the rows that cross tiles in every matrix.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include "merbit_amd.h"
+#include "merbit_internal.hpp"
+#include "merbit_tile.hpp"
+
+namespace {
+
+using mbx_detail::BF16C;
+using mbx_detail::F32C;
+using mbx_detail::F64;
+using mbx_detail::TileSmem;
+
+struct BatchMat {
+  const int32_t* row_ptr;
+  const int32_t* col;
+  const void* val;
+  const void* x;
+  void* y;
+  int32_t nrow, nnz;
+  int32_t tile_base;   // first global tile of this matrix
+  int32_t coord_base;  // first coordinate (tile_base + matrix index)
+};
+
+__global__ void batch_partition_kernel(const BatchMat* __restrict__ mats, int count,
+                                       int tile_items, int ncoord, int2* __restrict__ coord) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ncoord) return;
+  int lo = 0, hi = count - 1;  // last matrix whose coord_base <= g
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (mats[mid].coord_base <= g) lo = mid; else hi = mid - 1;
+  }
+  const BatchMat m = mats[lo];
+  coord[g] = mbx_detail::tile_start_coord(m.row_ptr, m.nrow, m.nnz, tile_items,
+                                          g - m.coord_base);
+}
+
+__device__ __forceinline__ int batch_matrix_of(const int* __restrict__ tile_base, int count,
+                                               int g) {
+  int lo = 0, hi = count - 1;  // last matrix whose tile_base <= g
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tile_base[mid] <= g) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int WG, int IPT, class P>
+__global__ __launch_bounds__(WG) void batch_spmv_kernel(
+    const BatchMat* __restrict__ mats, const int* __restrict__ tile_base, int count,
+    int total_tiles, const int2* __restrict__ coord, double* __restrict__ carry,
+    typename P::scal_t alpha, typename P::scal_t beta, int beta_zero) {
+  __shared__ TileSmem<WG, IPT, P> sm;
+  constexpr int X = mbx_detail::kXcds;
+  const int b = blockIdx.x, nwg = gridDim.x;
+  int first = 0, len = total_tiles, w = b, stride = nwg;
+  if (nwg >= X) {  // every XCD has workgroups: each takes a contiguous range
+    const int xcd = b % X;
+    w = b / X;
+    stride = (nwg - xcd + X - 1) / X;  // workgroups on this XCD
+    const int q = total_tiles / X, r = total_tiles % X;
+    first = xcd * q + min(xcd, r);
+    len = q + (xcd < r ? 1 : 0);
+  }
+  for (int i = w; i < len; i += stride) {
+    const int g = first + i;
+    const int m = batch_matrix_of(tile_base, count, g);
+    const BatchMat M = mats[m];
+    mbx_detail::spmv_tile<WG, IPT, P>(
+        sm, g - M.tile_base, M.row_ptr, M.col, static_cast<const typename P::val_t*>(M.val),
+        static_cast<const typename P::x_t*>(M.x), static_cast<typename P::y_t*>(M.y),
+        coord + M.coord_base, carry + 2ll * M.coord_base, alpha, beta, beta_zero);
+    __syncthreads();  // the next tile reuses the LDS
+  }
+}
+
+template <class P>
+__global__ __launch_bounds__(256) void batch_fixup_kernel(
+    const BatchMat* __restrict__ mats, const int* __restrict__ tile_base, int count,
+    int total_tiles, int tile_items, const int2* __restrict__ coord,
+    const double* __restrict__ carry, typename P::scal_t alpha, typename P::scal_t beta,
+    int beta_zero) {
+  const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (g >= total_tiles) return;  // uniform per wave
+  const int m = batch_matrix_of(tile_base, count, g);
+  const BatchMat M = mats[m];
+  const int t = g - M.tile_base;
+  if (t < 1) return;
+  mbx_detail::fixup_tile<P>(t, threadIdx.x & 63, M.row_ptr, coord + M.coord_base,
+                            carry + 2ll * M.coord_base, static_cast<typename P::y_t*>(M.y),
+                            tile_items, alpha, beta, beta_zero);
+}
+
+}  // namespace
+
+struct mbx_batch {
+  int32_t count, kind, ipt, tile_items, total_tiles, ncoord, grid;
+  BatchMat* d_mats;
+  int* d_tile_base;
+  int2* d_coord;
+  double* d_carry;
+};
+
+namespace {
+
+template <class P, int IPT>
+int batch_grid(int device) {
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_spmv_kernel<256, IPT, P>, 256,
+                                                   0) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+    return 0;
+  return std::max(1, per_cu) * std::max(1, cus);
+}
+
+template <class P, int IPT>
+void batch_launch(const mbx_batch* b, hipStream_t s, typename P::scal_t alpha,
+                  typename P::scal_t beta) {
+  const int bz = beta == typename P::scal_t(0);
+  const int grid = std::min(b->grid, b->total_tiles);
+  batch_spmv_kernel<256, IPT, P><<<grid, 256, 0, s>>>(b->d_mats, b->d_tile_base, b->count,
+                                                      b->total_tiles, b->d_coord, b->d_carry,
+                                                      alpha, beta, bz);
+  batch_fixup_kernel<P><<<(b->total_tiles + 3) / 4, 256, 0, s>>>(
+      b->d_mats, b->d_tile_base, b->count, b->total_tiles, b->tile_items, b->d_coord,
+      b->d_carry, alpha, beta, bz);
+}
+
+template <class P>
+int batch_dispatch(const mbx_batch* b, hipStream_t s, double alpha, double beta, bool grid_only,
+                   int device) {
+  using S = typename P::scal_t;
+  switch (b->ipt) {
+#define MBX_CASE(I)                                                  \
+  case I:                                                           \
+    if (grid_only) return batch_grid<P, I>(device);                 \
+    batch_launch<P, I>(b, s, (S)alpha, (S)beta);                    \
+    return 0;
+    MBX_CASE(2) MBX_CASE(4) MBX_CASE(6) MBX_CASE(8) MBX_CASE(12) MBX_CASE(16)
+#undef MBX_CASE
+    default: return -1;
+  }
+}
+
+int batch_run(const mbx_batch* b, hipStream_t s, double alpha, double beta, bool grid_only,
+              int device) {
+  switch (b->kind) {
+    case MBX_VALUE_F64: return batch_dispatch<F64>(b, s, alpha, beta, grid_only, device);
+    case MBX_VALUE_F32: return batch_dispatch<F32C>(b, s, alpha, beta, grid_only, device);
+    case MBX_VALUE_BF16: return batch_dispatch<BF16C>(b, s, alpha, beta, grid_only, device);
+    default: return -1;
+  }
+}
+
+bool ipt_ok(int ipt) {
+  return ipt == 2 || ipt == 4 || ipt == 6 || ipt == 8 || ipt == 12 || ipt == 16;
+}
+
+// Capture `repeats` calls of fn(stream) into an executable graph.
+template <class F>
+int capture_graph(int repeats, F fn, void** exec_out) {
+  if (!exec_out || repeats < 1) return MBX_EINVAL;
+  *exec_out = nullptr;
+  hipStream_t cs = nullptr;
+  if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return MBX_EHIP;
+  int st = MBX_OK;
+  hipGraph_t g = nullptr;
+  if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    st = MBX_EHIP;
+  } else {
+    for (int i = 0; i < repeats && st == MBX_OK; ++i) st = fn(cs);
+    if (hipStreamEndCapture(cs, &g) != hipSuccess && st == MBX_OK) st = MBX_EHIP;
+  }
+  hipGraphExec_t exec = nullptr;
+  if (st == MBX_OK && hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) != hipSuccess)
+    st = MBX_EHIP;
+  if (g) (void)hipGraphDestroy(g);
+  (void)hipStreamDestroy(cs);
+  if (st == MBX_OK) *exec_out = exec;
+  (void)hipGetLastError();
+  return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mbx_batch_create(const mbx_batch_item* items, int32_t count, int32_t value_kind,
+                     int32_t items_per_thread, void* stream, mbx_batch** out) {
+  if (!out) return MBX_EINVAL;
+  *out = nullptr;
+  if (!items || count <= 0) return MBX_EINVAL;
+  if (value_kind != MBX_VALUE_F64 && value_kind != MBX_VALUE_F32 && value_kind != MBX_VALUE_BF16)
+    return MBX_EINVAL;
+  const int ipt = items_per_thread ? items_per_thread : 8;
+  if (!ipt_ok(ipt)) return MBX_EUNSUPPORTED;
+  const long long T = 256ll * ipt;
+  std::vector<BatchMat> h((size_t)count);
+  std::vector<int> tb((size_t)count);
+  long long tiles = 0;
+  for (int m = 0; m < count; ++m) {
+    const mbx_batch_item& it = items[m];
+    if (it.nrow < 0 || it.ncol < 0 || it.nnz < 0) return MBX_EINVAL;
+    if (it.nrow > 0 && (!it.row_ptr || !it.y)) return MBX_EINVAL;
+    if (it.nnz > 0 && (!it.col || !it.val || !it.x)) return MBX_EINVAL;
+    const long long path = (long long)it.nrow + it.nnz;
+    if (path + T >= (long long)INT32_MAX) return MBX_EOVERFLOW;
+    const long long nt = (path + T - 1) / T;
+    h[(size_t)m] = BatchMat{it.row_ptr, it.col, it.val, it.x, it.y, it.nrow, it.nnz,
+                            (int32_t)tiles, (int32_t)(tiles + m)};
+    tb[(size_t)m] = (int)tiles;
+    tiles += nt;
+    if (tiles + count >= (long long)INT32_MAX / 2) return MBX_EOVERFLOW;
+  }
+  mbx_batch* b = new (std::nothrow) mbx_batch();
+  if (!b) return MBX_ENOMEM;
+  b->count = count;
+  b->kind = value_kind;
+  b->ipt = ipt;
+  b->tile_items = (int)T;
+  b->total_tiles = (int)tiles;
+  b->ncoord = (int)tiles + count;
+  b->d_mats = nullptr;
+  b->d_tile_base = nullptr;
+  b->d_coord = nullptr;
+  b->d_carry = nullptr;
+  hipStream_t s = (hipStream_t)stream;
+  if (hipMalloc((void**)&b->d_mats, sizeof(BatchMat) * (size_t)count) != hipSuccess ||
+      hipMalloc((void**)&b->d_tile_base, sizeof(int) * (size_t)count) != hipSuccess ||
+      hipMalloc((void**)&b->d_coord, sizeof(int2) * (size_t)b->ncoord) != hipSuccess ||
+      hipMalloc((void**)&b->d_carry, sizeof(double) * 2 * (size_t)b->ncoord) != hipSuccess) {
+    (void)hipGetLastError();
+    mbx_batch_destroy(b);
+    return MBX_ENOMEM;
+  }
+  int device = 0;
+  if (hipMemcpyAsync(b->d_mats, h.data(), sizeof(BatchMat) * (size_t)count,
+                     hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(b->d_tile_base, tb.data(), sizeof(int) * (size_t)count,
+                     hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipGetDevice(&device) != hipSuccess) {
+    mbx_batch_destroy(b);
+    return MBX_EHIP;
+  }
+  batch_partition_kernel<<<(b->ncoord + 255) / 256, 256, 0, s>>>(b->d_mats, count, b->tile_items,
+                                                                  b->ncoord, b->d_coord);
+  b->grid = batch_run(b, s, 0.0, 0.0, true, device);
+  // the host vectors die here: wait for the copies that read them
+  if (hipGetLastError() != hipSuccess || b->grid <= 0 || hipStreamSynchronize(s) != hipSuccess) {
+    mbx_batch_destroy(b);
+    return MBX_EHIP;
+  }
+  *out = b;
+  return MBX_OK;
+}
+
+void mbx_batch_destroy(mbx_batch* b) {
+  if (!b) return;
+  if (b->d_mats) (void)hipFree(b->d_mats);
+  if (b->d_tile_base) (void)hipFree(b->d_tile_base);
+  if (b->d_coord) (void)hipFree(b->d_coord);
+  if (b->d_carry) (void)hipFree(b->d_carry);
+  delete b;
+}
+
+int mbx_batch_info(const mbx_batch* b, int32_t* total_tiles, int32_t* tile_items,
+                   int32_t* grid) {
+  if (!b) return MBX_EINVAL;
+  if (total_tiles) *total_tiles = b->total_tiles;
+  if (tile_items) *tile_items = b->tile_items;
+  if (grid) *grid = std::min(b->grid, b->total_tiles);
+  return MBX_OK;
+}
+
+int mbx_batch_spmv(const mbx_batch* b, double alpha, double beta, void* stream) {
+  if (!b) return MBX_EINVAL;
+  if (b->total_tiles == 0) return MBX_OK;
+  if (batch_run(b, (hipStream_t)stream, alpha, beta, false, 0) != 0) return MBX_EUNSUPPORTED;
+  return hipGetLastError() == hipSuccess ? MBX_OK : MBX_EHIP;
+}
+
+int mbx_batch_graph_create(const mbx_batch* b, double alpha, double beta, int32_t repeats,
+                           void** graph_exec) {
+  if (!b) return MBX_EINVAL;
+  return capture_graph(repeats, [&](hipStream_t s) { return mbx_batch_spmv(b, alpha, beta, s); },
+                       graph_exec);
+}
+
+int mbx_spmv_graph_create(const mbx_plan* plan, int32_t value_kind, const int32_t* d_row_ptr,
+                          const int32_t* d_col, const void* d_val, double alpha,
+                          const void* d_x, double beta, void* d_y, int32_t repeats,
+                          void** graph_exec) {
+  if (!plan) return MBX_EINVAL;
+  auto fn = [&](hipStream_t s) -> int {
+    switch (value_kind) {
+      case MBX_VALUE_F64:
+        return mbx_spmv_f64(plan, d_row_ptr, d_col, (const double*)d_val, alpha,
+                            (const double*)d_x, beta, (double*)d_y, s);
+      case MBX_VALUE_F32:
+        return mbx_spmv_f32(plan, d_row_ptr, d_col, (const float*)d_val, (float)alpha,
+                            (const float*)d_x, (float)beta, (float*)d_y, s);
+      case MBX_VALUE_BF16:
+        return mbx_spmv_bf16(plan, d_row_ptr, d_col, (const uint16_t*)d_val, (float)alpha,
+                             (const float*)d_x, (float)beta, (float*)d_y, s);
+      default:
+        return MBX_EINVAL;
+    }
+  };
+  return capture_graph(repeats, fn, graph_exec);
+}
+
+int mbx_graph_launch(void* graph_exec, void* stream) {
+  if (!graph_exec) return MBX_EINVAL;
+  return hipGraphLaunch((hipGraphExec_t)graph_exec, (hipStream_t)stream) == hipSuccess
+             ? MBX_OK
+             : MBX_EHIP;
+}
+
+void mbx_graph_destroy(void* graph_exec) {
+  if (graph_exec) (void)hipGraphExecDestroy((hipGraphExec_t)graph_exec);
+}
+
+}  // extern "C"

@@ -1,0 +1,142 @@
+"""GPU checks of the persistent multi-matrix batch, HIP-graph replay and the
+autotuner.  The batch runs the single-matrix per-tile code, so its results
+must equal one mbx_spmv_* call per matrix bit for bit; graph replays must
+equal eager products bit for bit, replay after replay."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import cond_err
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def make_mats(device, count, kind, seed, rows_hi=20000):
+    from merbit_amd import gen, workloads
+    mats, xs = workloads.uneven_batch(count, rows_lo=1000, rows_hi=rows_hi)
+    out = []
+    for A, x in zip(mats, xs):
+        if kind == "f64":
+            v, xv = A.val.astype(np.float64), x.astype(np.float64)
+        elif kind == "f32":
+            v, xv = A.val, x
+        else:
+            v, xv = gen.to_bf16(A.val), x
+        vdt = torch.float64 if kind == "f64" else torch.float32
+        out.append((torch.from_numpy(A.row_ptr).to(device), torch.from_numpy(A.col).to(device),
+                    torch.from_numpy(v).to(device), torch.from_numpy(xv).to(device),
+                    torch.full((A.nrow,), float("nan"), dtype=vdt, device=device), A.ncol))
+    return out
+
+
+@pytest.mark.parametrize("kind", ["f64", "f32", "bf16"])
+@pytest.mark.parametrize("ipt", [4, 8])
+def test_batch_matches_single_calls(device, kind, ipt):
+    from merbit_amd.batch import MerbitBatch
+    from merbit_amd.spmv import MerbitPlan
+    say(f"batch {kind} ipt={ipt}")
+    mats = make_mats(device, 12, kind, 0)
+    b = MerbitBatch(mats, items_per_thread=ipt)
+    b.spmv()
+    torch.cuda.synchronize()
+    got = [m[4].clone() for m in mats]
+    for (rp, col, val, x, y, ncol), g in zip(mats, got):
+        plan = MerbitPlan(rp, ncol, col.numel(), items_per_thread=ipt)
+        ref = plan.spmv(rp, col, val, x)
+        plan.close()
+        assert torch.equal(ref.view(torch.uint8), g.view(torch.uint8))
+    b.close()
+
+
+def test_batch_against_oracle(orc, device):
+    from merbit_amd.batch import MerbitBatch
+    say("batch f32 vs oracle")
+    mats = make_mats(device, 16, "f32", 1, rows_hi=60000)
+    b = MerbitBatch(mats)
+    b.spmv()
+    torch.cuda.synchronize()
+    for rp, col, val, x, y, ncol in mats:
+        ya, absr = orc.spmv_accurate(rp.cpu().numpy(), col.cpu().numpy(), val.cpu().numpy(),
+                                     x.cpu().numpy())
+        assert cond_err(y.cpu().numpy().astype(np.float64), ya, absr) <= 1e-6
+    b.close()
+
+
+def test_batch_graph_replays(device):
+    from merbit_amd.batch import MerbitBatch
+    say("batch graph replays")
+    mats = make_mats(device, 8, "f32", 2)
+    b = MerbitBatch(mats, items_per_thread=8)
+    b.spmv(alpha=0.5)
+    torch.cuda.synchronize()
+    eager = [m[4].clone() for m in mats]
+    g = b.graph(alpha=0.5, repeats=3)
+    for rep in range(4):
+        for m in mats:
+            m[4].fill_(float("nan"))        # eager work between replays
+        torch.cuda.synchronize()
+        g.launch()
+        torch.cuda.synchronize()
+        for m, e in zip(mats, eager):
+            assert torch.equal(m[4].view(torch.uint8), e.view(torch.uint8)), rep
+    g.close()
+    b.close()
+
+
+@pytest.mark.parametrize("kind", ["f64", "f32"])
+def test_spmv_graph_replays(device, kind):
+    from merbit_amd import workloads
+    from merbit_amd.batch import spmv_graph
+    from merbit_amd.spmv import MerbitPlan
+    say(f"spmv graph {kind}")
+    A, x = workloads.rmat(14)
+    dt = torch.float64 if kind == "f64" else torch.float32
+    rp, col = torch.from_numpy(A.row_ptr).to(device), torch.from_numpy(A.col).to(device)
+    val = torch.from_numpy(A.val).to(device).to(dt)
+    xx = torch.from_numpy(x).to(device).to(dt)
+    y = torch.empty(A.nrow, dtype=dt, device=device)
+    plan = MerbitPlan(rp, A.ncol, A.nnz)
+    ref = plan.spmv(rp, col, val, xx).clone()
+    g = spmv_graph(plan, rp, col, val, xx, y, repeats=5)
+    for rep in range(3):
+        y.zero_()
+        torch.cuda.synchronize()
+        g.launch()
+        torch.cuda.synchronize()
+        assert torch.equal(y.view(torch.uint8), ref.view(torch.uint8)), rep
+    g.close()
+    plan.close()
+
+
+def test_autotuners(orc, device):
+    from merbit_amd import workloads
+    from merbit_amd.batch import autotune_batch, autotune_spmm, autotune_spmv
+    from merbit_amd.spmv import MerbitPlan
+    say("autotune spmv")
+    A, x = workloads.rmat(15)
+    rp, col = torch.from_numpy(A.row_ptr).to(device), torch.from_numpy(A.col).to(device)
+    val = torch.from_numpy(A.val).to(device)
+    xx = torch.from_numpy(x).to(device)
+    y = torch.empty(A.nrow, dtype=torch.float64, device=device)
+    ipt, us = autotune_spmv(rp, col, val, xx, y, reps=3)
+    assert ipt in (2, 4, 6, 8, 12, 16) and us > 0
+    plan = MerbitPlan(rp, A.ncol, A.nnz, items_per_thread=ipt)
+    yy = plan.spmv(rp, col, val, xx).cpu().numpy()
+    ya, absr = orc.spmv_accurate(A.row_ptr, A.col, A.val, x)
+    assert cond_err(yy, ya, absr) <= 1e-12
+    plan.close()
+    say("autotune spmm")
+    X = torch.rand((A.ncol, 8), device=device)
+    Y = torch.empty((A.nrow, 8), device=device)
+    ipt, us = autotune_spmm(rp, col, val.float(), X, Y, reps=2)
+    assert ipt in (2, 4) and us > 0
+    say("autotune batch")
+    mats = make_mats(device, 6, "f32", 3)
+    ipt, us = autotune_batch(mats, reps=2)
+    assert ipt in (2, 4, 6, 8, 12, 16) and us > 0
