@@ -1,18 +1,27 @@
 """Benchmark of the merge-path tile SpMV on one MI355X.
 
-Workload (default): FP64 merge SpMV y = A*x on the R-MAT power-law matrix
+Headline workload: FP64 merge SpMV y = A*x on the R-MAT power-law matrix
 (2^20 rows, 2^24 entries with duplicates kept, a=.57 b=c=.19, values N(0,1),
 x ~ U[-1,1)).  A step is one SpMV (tile kernel + carry fixup) with the matrix
 and x already resident in HBM.  Three copies of the matrix are rotated so
-that consecutive steps cannot be served from the 256 MB MALL; the warm
-(single copy) rate is reported beside it.
+consecutive steps cannot be served from the 256 MB MALL; the warm (single
+copy) rate is reported beside it.
 
 value = algorithmic GB/s = (4(nrow+1) + 12 nnz + 8 ncol + 8 nrow) / time per
 step.  The roofline object is for the dominant (tile) kernel, timed with HIP
-events recorded around it on its own stream.  The CPU baseline is the
-row-parallel FP64 CSR product of the oracle, timed on the host cores.
+events recorded around it on its own stream.  `traffic` (measured HBM bytes
+per launch) is read from the committed counter summary for this workload when
+present.  The CPU baseline is the oracle's row-parallel FP64 CSR product,
+timed on the host cores.
 
-Prints ONE JSON line.
+Beside the headline, `extras` reports the other measured forms: mixed
+precision (FP32 / BF16 values, compensated FP32 sums) on the R-MAT and on a
+banded 4M-row matrix, multi-RHS SpMM (k = 8, 32), 64 uneven matrices in one
+persistent graph launch (autotuned), the adjoint, and a plain device copy as
+the measured bandwidth ceiling.  Every rate uses the same algorithmic-bytes
+definition.
+
+Prints ONE JSON line on stdout; progress goes to stderr.
 """
 from __future__ import annotations
 
@@ -29,6 +38,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "SpMV GB/s as % HBM peak & GFLOP/s: FP64 plain vs mixed/batched at stated err"
 
 
 def log(msg):
@@ -36,7 +46,8 @@ def log(msg):
 
 
 def time_steps(calls, steps, warmup, plan_events=None):
-    """Run warmup + steps calls; return (ms per step, mean tile-kernel ms)."""
+    """Run warmup + steps calls round-robin; return (ms per step, mean
+    tile-kernel ms or None)."""
     import torch
     for i in range(warmup):
         calls[i % len(calls)]()
@@ -48,7 +59,6 @@ def time_steps(calls, steps, warmup, plan_events=None):
         for b, e in evs:  # materialise the underlying hipEvents
             b.record()
             e.record()
-        torch.cuda.synchronize()
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -66,6 +76,182 @@ def time_steps(calls, steps, warmup, plan_events=None):
     return ms, kern
 
 
+def rel_err(a, b):
+    import torch
+    a, b = a.double(), b.double()
+    return float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+
+
+def gbs(nbytes, ms):
+    return round(nbytes / (ms * 1e-3) / 1e9, 1)
+
+
+def load_traffic(name):
+    p = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+# ---------------------------------------------------------------------------
+
+
+def bench_rmat_f64(A, x, dev, args):
+    import torch
+    from merbit_amd import workloads
+    from merbit_amd.spmv import MerbitPlan
+    nrow, ncol, nnz = A.nrow, A.ncol, A.nnz
+    b_alg = workloads.algorithmic_bytes(nrow, ncol, nnz, 8, 8, 8)
+    copies = [dict(rp=torch.from_numpy(A.row_ptr).to(dev), col=torch.from_numpy(A.col).to(dev),
+                   val=torch.from_numpy(A.val).to(dev), x=torch.from_numpy(x).to(dev),
+                   y=torch.empty(nrow, dtype=torch.float64, device=dev))
+              for _ in range(max(1, args.copies))]
+    plan = MerbitPlan(copies[0]["rp"], ncol, nnz, items_per_thread=args.ipt)
+    calls = [(lambda c=c: plan.spmv(c["rp"], c["col"], c["val"], c["x"], c["y"]))
+             for c in copies]
+    ms, kern = time_steps(calls, args.steps, args.warmup, plan.set_events)
+    warm_ms, warm_kern = time_steps(calls[:1], args.steps, args.warmup, plan.set_events)
+    log(f"rmat fp64: {ms * 1e3:.1f} us/step rotated ({gbs(b_alg, ms)} GB/s), tile kernel "
+        f"{kern * 1e3:.1f} us; warm {warm_ms * 1e3:.1f} us")
+    res = dict(b_alg=b_alg, ms=ms, kern=kern, warm_ms=warm_ms, warm_kern=warm_kern,
+               ntiles=plan.ntiles, tile_items=plan.tile_items)
+    plan.close()
+    del copies
+    return res
+
+
+def bench_mixed(name, A, x, dev, args, f64_ms=None, copies=1):
+    """FP64 vs FP32 vs BF16 values on one pattern; errors against the FP64
+    path on the same stored values."""
+    import torch
+    from merbit_amd import gen, workloads
+    from merbit_amd.spmv import MerbitPlan
+    nrow, ncol, nnz = A.nrow, A.ncol, A.nnz
+    rp = torch.from_numpy(A.row_ptr).to(dev)
+    cols = [torch.from_numpy(A.col).to(dev) for _ in range(copies)]
+    plan = MerbitPlan(rp, ncol, nnz, items_per_thread=args.ipt)
+    v32 = np.ascontiguousarray(A.val, dtype=np.float32)
+    x32 = np.ascontiguousarray(x, dtype=np.float32)
+    out = {}
+    if f64_ms is None:  # FP64 values = the FP32 values widened
+        vals = [torch.from_numpy(v32.astype(np.float64)).to(dev) for _ in range(copies)]
+        xx = torch.from_numpy(x32.astype(np.float64)).to(dev)
+        y = torch.empty(nrow, dtype=torch.float64, device=dev)
+        calls = [(lambda i=i: plan.spmv(rp, cols[i], vals[i], xx, y)) for i in range(copies)]
+        f64_ms, kern = time_steps(calls, args.steps, args.warmup, plan.set_events)
+        b64 = workloads.algorithmic_bytes(nrow, ncol, nnz, 8, 8, 8)
+        out["f64"] = {"gbs": gbs(b64, f64_ms), "us_per_step": round(f64_ms * 1e3, 2),
+                      "tile_kernel_us": round(kern * 1e3, 2),
+                      "pct_hbm_peak": round(100 * b64 / (f64_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 1)}
+        del vals, y
+    for kind in ("f32", "bf16"):
+        stored = v32 if kind == "f32" else gen.to_bf16(v32)
+        wide = v32.astype(np.float64) if kind == "f32" else gen.bf16_to_f32(stored).astype(np.float64)
+        s_val = 4 if kind == "f32" else 2
+        vals = [torch.from_numpy(stored).to(dev) for _ in range(copies)]
+        xx = torch.from_numpy(x32).to(dev)
+        y = torch.empty(nrow, dtype=torch.float32, device=dev)
+        calls = [(lambda i=i: plan.spmv(rp, cols[i], vals[i], xx, y)) for i in range(copies)]
+        ms, kern = time_steps(calls, args.steps, args.warmup, plan.set_events)
+        ref = plan.spmv(rp, cols[0], torch.from_numpy(wide).to(dev),
+                        torch.from_numpy(x32.astype(np.float64)).to(dev))
+        plan.spmv(rp, cols[0], vals[0], xx, y)
+        b = workloads.algorithmic_bytes(nrow, ncol, nnz, s_val, 4, 4)
+        out[kind] = {"gbs": gbs(b, ms), "gflops": round(2 * nnz / (ms * 1e-3) / 1e9, 1),
+                     "us_per_step": round(ms * 1e3, 2), "tile_kernel_us": round(kern * 1e3, 2),
+                     "speedup_vs_f64": round(f64_ms / ms, 3),
+                     "rel_err_vs_f64_same_values": rel_err(y, ref)}
+        log(f"{name} {kind}: {ms * 1e3:.1f} us/step, x{f64_ms / ms:.2f} vs fp64, "
+            f"err {out[kind]['rel_err_vs_f64_same_values']:.2e}")
+        del vals, y, ref
+    plan.close()
+    return out
+
+
+def bench_spmm(A, dev, args):
+    import torch
+    from merbit_amd import gen, workloads
+    from merbit_amd.spmv import MerbitPlan
+    nrow, ncol, nnz = A.nrow, A.ncol, A.nnz
+    rp = torch.from_numpy(A.row_ptr).to(dev)
+    col = torch.from_numpy(A.col).to(dev)
+    val = torch.from_numpy(A.val.astype(np.float32)).to(dev)
+    out = {}
+    for k in (8, 32):
+        X = torch.from_numpy(gen.uniform(ncol * k, -1, 1, workloads.SEED["spmm_x"],
+                                         np.float32)).to(dev).view(k, ncol).t()  # col-major
+        Y = torch.empty((k, nrow), dtype=torch.float32, device=dev).t()
+        plan = MerbitPlan(rp, ncol, nnz, items_per_thread=4)
+        ws = torch.empty(plan.spmm_workspace_bytes(X), dtype=torch.uint8, device=dev)
+        ms, kern = time_steps([lambda: plan.spmm(rp, col, val, X, Y, workspace=ws)],
+                              max(20, args.steps // 4), args.warmup, plan.set_events)
+        b = workloads.algorithmic_bytes(nrow, ncol, nnz, 4, 4, 4, k=k)
+        # error of column 0 against the FP64 SpMV of the same values
+        y0 = plan.spmv(rp, col, val.double(), X[:, 0].double().contiguous())
+        out[f"k{k}"] = {"gbs": gbs(b, ms), "gflops": round(2 * nnz * k / (ms * 1e-3) / 1e9, 1),
+                        "us_per_step": round(ms * 1e3, 2),
+                        "tile_kernel_us": round(kern * 1e3, 2),
+                        "us_per_rhs": round(ms * 1e3 / k, 2),
+                        "rel_err_col0_vs_f64": rel_err(Y[:, 0], y0),
+                        "layout": "column-major X and Y, repacked per call"}
+        log(f"spmm k={k}: {ms * 1e3:.1f} us/step ({out[f'k{k}']['gbs']} GB/s)")
+        plan.close()
+        del X, Y, ws
+    return out
+
+
+def bench_batch(dev, args):
+    import torch
+    from merbit_amd import workloads
+    from merbit_amd.batch import MerbitBatch, autotune_batch
+    t0 = time.time()
+    mats, xs = workloads.uneven_batch(args.batch_count)
+    log(f"batch: {len(mats)} matrices, {sum(m.nrow for m in mats)} rows, "
+        f"{sum(m.nnz for m in mats)} nnz ({time.time() - t0:.1f}s)")
+    items = []
+    for A, x in zip(mats, xs):
+        items.append((torch.from_numpy(A.row_ptr).to(dev), torch.from_numpy(A.col).to(dev),
+                      torch.from_numpy(A.val).to(dev), torch.from_numpy(x).to(dev),
+                      torch.empty(A.nrow, dtype=torch.float32, device=dev), A.ncol))
+    ipt, tuned_us = autotune_batch(items, reps=3)
+    b = MerbitBatch(items, items_per_thread=ipt)
+    reps = 10
+    g = b.graph(repeats=reps)
+    ms, _ = time_steps([g.launch], max(5, args.steps // 20), 2)
+    ms /= reps
+    nbytes = sum(workloads.algorithmic_bytes(A.nrow, A.ncol, A.nnz, 4, 4, 4) for A in mats)
+    nnz = sum(A.nnz for A in mats)
+    # per-matrix launches for comparison (same arithmetic, one launch pair each)
+    from merbit_amd.spmv import MerbitPlan
+    plans = [MerbitPlan(it[0], it[5], it[1].numel(), items_per_thread=ipt) for it in items]
+    calls = [lambda: [p.spmv(it[0], it[1], it[2], it[3], it[4]) for p, it in zip(plans, items)]]
+    ms_loop, _ = time_steps(calls, max(5, args.steps // 20), 2)
+    out = {"matrices": len(mats), "rows": int(sum(A.nrow for A in mats)), "nnz": int(nnz),
+           "autotuned_items_per_thread": ipt, "tile_items": b.tile_items,
+           "persistent_grid": b.grid, "gbs": gbs(nbytes, ms),
+           "gflops": round(2 * nnz / (ms * 1e-3) / 1e9, 1), "us_per_batch": round(ms * 1e3, 1),
+           "us_per_batch_separate_launches": round(ms_loop * 1e3, 1),
+           "speedup_vs_separate_launches": round(ms_loop / ms, 2),
+           "launch": f"one hipGraph of {reps} persistent batch products, per-product time"}
+    log(f"batch: {ms * 1e3:.1f} us per batch product ({out['gbs']} GB/s), separate launches "
+        f"{ms_loop * 1e3:.1f} us")
+    for p in plans:
+        p.close()
+    g.close()
+    b.close()
+    return out
+
+
+def bench_copy(dev):
+    import torch
+    n = 256 << 20  # 256 Mi doubles = 2 GiB
+    a = torch.empty(n, dtype=torch.float64, device=dev).fill_(1.0)
+    b = torch.empty_like(a)
+    ms, _ = time_steps([lambda: b.copy_(a)], 20, 3)
+    return {"gbs": gbs(2 * 8 * n, ms), "bytes": 2 * 8 * n, "what": "torch device copy of 2 GiB"}
+
+
 def cpu_baseline(A, x, b_alg):
     from oracle import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
@@ -76,8 +262,8 @@ def cpu_baseline(A, x, b_alg):
     log(f"cpu baseline {t * 1e3:.2f} ms/SpMV on {used} threads ({time.time() - t0:.1f}s)")
     return {"value": round(b_alg / t / 1e9, 3), "unit": "GB/s", "cores": used,
             "kind": "port",
-            "sample": "full R-MAT FP64 product, OpenMP row-parallel CSR, median of 10 "
-                      "after 2 warm-ups", "ms_per_spmv": round(t * 1e3, 3)}
+            "sample": "full R-MAT FP64 product (2^20 rows, 2^24 nnz), OpenMP row-parallel "
+                      "CSR, median of 10 after 2 warm-ups", "ms_per_spmv": round(t * 1e3, 3)}
 
 
 def main():
@@ -88,8 +274,12 @@ def main():
     ap.add_argument("--ipt", type=int, default=8, help="items per thread (SIGMA)")
     ap.add_argument("--copies", type=int, default=3)
     ap.add_argument("--scale", type=int, default=20)
+    ap.add_argument("--batch-count", type=int, default=64)
+    ap.add_argument("--only", choices=["headline", "all"], default="all",
+                    help="headline: only the FP64 R-MAT line (for profiling)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run the plain device copy (counter calibration)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-mixed", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not "
@@ -99,85 +289,40 @@ def main():
 
     import torch
 
-    from merbit_amd import gen, workloads
-    from merbit_amd.spmv import MerbitPlan
+    from merbit_amd import workloads
 
     dev = torch.device("cuda:0")
     t0 = time.time()
     A, x = workloads.rmat(args.scale)
     log(f"rmat: nrow={A.nrow} nnz={A.nnz} ({time.time() - t0:.1f}s)")
-    nrow, ncol, nnz = A.nrow, A.ncol, A.nnz
-    b_alg = workloads.algorithmic_bytes(nrow, ncol, nnz, 8, 8, 8)
-    flops = 2 * nnz
-
-    copies = []
-    for _ in range(max(1, args.copies)):
-        copies.append(dict(
-            rp=torch.from_numpy(A.row_ptr).to(dev), col=torch.from_numpy(A.col).to(dev),
-            val=torch.from_numpy(A.val).to(dev), x=torch.from_numpy(x).to(dev),
-            y=torch.empty(nrow, dtype=torch.float64, device=dev)))
-    plan = MerbitPlan(copies[0]["rp"], ncol, nnz, items_per_thread=args.ipt)
-    torch.cuda.synchronize()
-
-    def call_for(c):
-        return lambda: plan.spmv(c["rp"], c["col"], c["val"], c["x"], c["y"])
-
-    calls = [call_for(c) for c in copies]
-    ms, kern_ms = time_steps(calls, args.steps, args.warmup, plan.set_events)
-    warm_ms, warm_kern = time_steps(calls[:1], args.steps, args.warmup, plan.set_events)
+    h = bench_rmat_f64(A, x, dev, args)
+    b_alg, ms, kern = h["b_alg"], h["ms"], h["kern"]
     value = b_alg / (ms * 1e-3) / 1e9
-    achieved = b_alg / (kern_ms * 1e-3) / 1e9
-    log(f"fp64 rotated {ms * 1e3:.1f} us/step ({value:.0f} GB/s), tile kernel "
-        f"{kern_ms * 1e3:.1f} us ({achieved:.0f} GB/s); warm {warm_ms * 1e3:.1f} us")
-    y64 = copies[0]["y"].clone()
-
+    achieved = b_alg / (kern * 1e-3) / 1e9
     extras = {
-        "gflops": round(flops / (ms * 1e-3) / 1e9, 2),
+        "gflops": round(2 * A.nnz / (ms * 1e-3) / 1e9, 2),
         "pct_hbm_peak": round(100.0 * value / HBM_PEAK_GBS, 2),
         "algorithmic_bytes": b_alg,
-        "tile_kernel_us": round(kern_ms * 1e3, 2),
-        "warm_single_copy": {"gbps": round(b_alg / (warm_ms * 1e-3) / 1e9, 1),
-                             "us_per_step": round(warm_ms * 1e3, 2),
-                             "tile_kernel_us": round(warm_kern * 1e3, 2)},
-        "ntiles": plan.ntiles, "tile_items": plan.tile_items,
+        "tile_kernel_us": round(kern * 1e3, 2),
+        "warm_single_copy": {"gbs": gbs(b_alg, h["warm_ms"]),
+                             "us_per_step": round(h["warm_ms"] * 1e3, 2),
+                             "tile_kernel_us": round(h["warm_kern"] * 1e3, 2)},
+        "ntiles": h["ntiles"], "tile_items": h["tile_items"],
     }
-
-    if not args.no_mixed:
-        mixed = {}
-        v32 = A.val.astype(np.float32)
-        x32 = x.astype(np.float32)
-        for kind in ("f32", "bf16"):
-            stored = v32 if kind == "f32" else gen.to_bf16(v32)
-            s_val = 4 if kind == "f32" else 2
-            wide = v32.astype(np.float64) if kind == "f32" else \
-                gen.bf16_to_f32(stored).astype(np.float64)
-            mc = []
-            for _ in range(max(1, args.copies)):
-                mc.append(dict(rp=copies[0]["rp"], col=torch.from_numpy(A.col).to(dev),
-                               val=torch.from_numpy(stored).to(dev),
-                               x=torch.from_numpy(x32).to(dev),
-                               y=torch.empty(nrow, dtype=torch.float32, device=dev)))
-            mcalls = [call_for(c) for c in mc]
-            mms, mkern = time_steps(mcalls, args.steps, args.warmup, plan.set_events)
-            mb = workloads.algorithmic_bytes(nrow, ncol, nnz, s_val, 4, 4)
-            # error against the FP64 path (oracle-verified) on the same stored values
-            ref = plan.spmv(copies[0]["rp"], copies[0]["col"],
-                            torch.from_numpy(wide).to(dev),
-                            torch.from_numpy(x32.astype(np.float64)).to(dev))
-            ym = mc[0]["y"].double()
-            rel = float(torch.linalg.norm(ym - ref) / torch.linalg.norm(ref))
-            mixed[kind] = {"gbps": round(mb / (mms * 1e-3) / 1e9, 1),
-                           "gflops": round(flops / (mms * 1e-3) / 1e9, 2),
-                           "us_per_step": round(mms * 1e3, 2),
-                           "tile_kernel_us": round(mkern * 1e3, 2),
-                           "speedup_vs_f64": round(ms / mms, 3),
-                           "rel_err_vs_f64_same_values": rel}
-            log(f"{kind}: {mms * 1e3:.1f} us/step, x{ms / mms:.2f} vs fp64, err {rel:.2e}")
-            del mc, mcalls
-        extras["mixed_rmat"] = mixed
+    if args.only == "all":
+        extras["mixed_rmat"] = bench_mixed("rmat", A, x, dev, args, f64_ms=ms, copies=3)
+        extras["spmm_rmat"] = bench_spmm(A, dev, args)
+        t0 = time.time()
+        B, xb = workloads.banded()
+        log(f"banded: nrow={B.nrow} nnz={B.nnz} ({time.time() - t0:.1f}s)")
+        extras["mixed_banded"] = bench_mixed("banded", B, xb, dev, args)
+        del B, xb
+        extras["batch_uneven"] = bench_batch(dev, args)
+    if args.only == "all" or args.copy_probe:
+        extras["copy_ceiling"] = bench_copy(dev)
 
     out = {
-        "metric": "SpMV GB/s as % HBM peak & GFLOP/s: FP64 plain vs mixed/batched at stated err",
+        "metric": METRIC,
         "value": round(value, 2),
         "unit": "GB/s",
         "n_gpus": 1,
@@ -191,18 +336,16 @@ def main():
         "data": "synthetic",
         "config": {"workload": "fp64 merge SpMV, R-MAT power law 2^20 rows 2^24 nnz "
                                "(a=.57,b=c=.19), 3 rotated copies",
-                   "nrow": nrow, "nnz": nnz, "items_per_thread": args.ipt,
-                   "wg_size": 256, "copies_rotated": len(copies)},
+                   "nrow": A.nrow, "nnz": A.nnz, "items_per_thread": args.ipt,
+                   "wg_size": 256, "copies_rotated": max(1, args.copies)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None},
+                     "traffic": load_traffic("rmat_f64")},
         "cpu_baseline": None,
-        **extras,
+        "extras": extras,
     }
-    del y64
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(A, x, b_alg)
-    plan.close()
     print(json.dumps(out), flush=True)
 
 
