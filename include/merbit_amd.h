@@ -160,6 +160,29 @@ int mbx_autotune_spmm(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol, int3
 int mbx_autotune_batch(const mbx_batch_item* items, int32_t count, int32_t value_kind,
                        int32_t reps, void* stream, int32_t* best_ipt, float* best_us);
 
+/* Adjoint of y = A*x (no reference counterpart).  An adjoint object holds the
+ * transposed pattern (CSC of A built on the device by a stable radix sort:
+ * rows stay ascending within every column, so dx sums deterministically) and
+ * its merge-path plan.  mbx_adjoint_create synchronises `stream` once.
+ *   mbx_adjoint_dx:  dx = alpha*A^T*g + beta*dx, values of the given kind
+ *                    gathered into CSC order, then the tile SpMV on A^T.
+ *   mbx_sddmm:       dval[k] = g[row(k)] * x[col(k)] over the pattern of a
+ *                    forward plan (FP64 or FP32 g, x and dval).
+ *   mbx_adjoint_transpose: host copies of col_ptr[ncol+1], row_idx[nnz] and
+ *                    perm[nnz] (CSR position of each CSC entry); any may be NULL. */
+typedef struct mbx_adjoint mbx_adjoint;
+int mbx_adjoint_create(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow,
+                       int32_t ncol, int32_t nnz, int32_t items_per_thread, void* stream,
+                       mbx_adjoint** out);
+void mbx_adjoint_destroy(mbx_adjoint* adj);
+int mbx_adjoint_transpose(const mbx_adjoint* adj, int32_t* h_col_ptr, int32_t* h_row_idx,
+                          int32_t* h_perm);
+int mbx_adjoint_dx(const mbx_adjoint* adj, int32_t value_kind, const void* d_val, double alpha,
+                   const void* d_g, double beta, void* d_dx, void* stream);
+int mbx_sddmm(const mbx_plan* plan, int32_t value_kind, const int32_t* d_row_ptr,
+              const int32_t* d_col, const void* d_g, const void* d_x, void* d_dval,
+              void* stream);
+
 /* Status text, like the CSR5 error codes (include/csr5/detail/common.h:13-22). */
 const char* mbx_status_str(int status);
 

@@ -1,0 +1,114 @@
+"""Differentiable merge-path SpMV: y = A(val) x with a hand-written adjoint.
+
+``MerbitOperator`` holds one sparsity pattern in HBM with its forward plan and
+(lazily) its transposed plan.  ``spmv(op, val, x)`` is a torch.autograd
+function: the backward pass computes
+
+* dx   = A^T g        (tile SpMV over the transposed pattern), and
+* dval = g[row]*x[col] (SDDMM over the pattern, forward plan's tiles),
+
+only for the inputs that require gradients.  FP64 and FP32 values are
+supported (FP32 sums are TwoSum-compensated).  All device work runs in
+libmerbit_amd.so.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from ._lib import MBX_VALUE_F32, MBX_VALUE_F64, check, lib
+from .spmv import MerbitPlan, _dptr, _stream_handle
+
+_KIND = {torch.float64: MBX_VALUE_F64, torch.float32: MBX_VALUE_F32}
+
+
+class MerbitOperator:
+    def __init__(self, row_ptr: torch.Tensor, col: torch.Tensor, ncol: int,
+                 items_per_thread: int = 8):
+        self.row_ptr = row_ptr
+        self.col = col
+        self.nrow = row_ptr.numel() - 1
+        self.ncol = int(ncol)
+        self.nnz = col.numel()
+        self.items_per_thread = items_per_thread
+        self.plan = MerbitPlan(row_ptr, ncol, self.nnz, items_per_thread)
+        self._adj = None
+
+    @property
+    def adjoint_handle(self) -> ctypes.c_void_p:
+        if self._adj is None:
+            h = ctypes.c_void_p()
+            check(lib().mbx_adjoint_create(_dptr(self.row_ptr, torch.int32, "row_ptr"),
+                                           _dptr(self.col, torch.int32, "col"), self.nrow,
+                                           self.ncol, self.nnz, self.items_per_thread,
+                                           _stream_handle(None), ctypes.byref(h)),
+                  "mbx_adjoint_create")
+            self._adj = h
+        return self._adj
+
+    def transpose(self):
+        """Host copies (col_ptr, row_idx, perm) of the transposed pattern."""
+        cp = np.empty(self.ncol + 1, np.int32)
+        ri = np.empty(max(self.nnz, 1), np.int32)
+        pm = np.empty(max(self.nnz, 1), np.int32)
+        check(lib().mbx_adjoint_transpose(self.adjoint_handle, cp.ctypes.data, ri.ctypes.data,
+                                          pm.ctypes.data), "mbx_adjoint_transpose")
+        return cp, ri[:self.nnz], pm[:self.nnz]
+
+    def forward(self, val: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
+        return self.plan.spmv(self.row_ptr, self.col, val, x)
+
+    def adjoint_x(self, val: torch.Tensor, g: torch.Tensor) -> torch.Tensor:
+        kind = _KIND[val.dtype]
+        dx = torch.empty(self.ncol, dtype=val.dtype, device=val.device)
+        check(lib().mbx_adjoint_dx(self.adjoint_handle, kind, _dptr(val, val.dtype, "val"), 1.0,
+                                   _dptr(g, val.dtype, "g"), 0.0, _dptr(dx, val.dtype, "dx"),
+                                   _stream_handle(None)), "mbx_adjoint_dx")
+        return dx
+
+    def adjoint_val(self, g: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
+        kind = _KIND[x.dtype]
+        dval = torch.empty(self.nnz, dtype=x.dtype, device=x.device)
+        check(lib().mbx_sddmm(self.plan.handle, kind, _dptr(self.row_ptr, torch.int32, "row_ptr"),
+                              _dptr(self.col, torch.int32, "col"), _dptr(g, x.dtype, "g"),
+                              _dptr(x, x.dtype, "x"), _dptr(dval, x.dtype, "dval"),
+                              _stream_handle(None)), "mbx_sddmm")
+        return dval
+
+    def close(self) -> None:
+        if self._adj is not None:
+            lib().mbx_adjoint_destroy(self._adj)
+            self._adj = None
+        self.plan.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _SpMV(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, val, x, op):
+        if val.dtype not in _KIND or x.dtype != val.dtype:
+            raise TypeError("values and x must both be FP64 or both FP32")
+        ctx.op = op
+        ctx.save_for_backward(val, x)
+        return op.forward(val.contiguous(), x.contiguous())
+
+    @staticmethod
+    def backward(ctx, gy):
+        val, x = ctx.saved_tensors
+        op = ctx.op
+        gy = gy.contiguous().to(val.dtype)
+        dval = op.adjoint_val(gy, x) if ctx.needs_input_grad[0] else None
+        dx = op.adjoint_x(val, gy) if ctx.needs_input_grad[1] else None
+        return dval, dx, None
+
+
+def spmv(op: MerbitOperator, val: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
+    """Differentiable y = A x with A's pattern from `op` and values `val`."""
+    return _SpMV.apply(val, x, op)

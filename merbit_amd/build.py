@@ -38,6 +38,7 @@ LIB_SOURCES = [
     os.path.join(CSRC, "merbit_spmm.hip"),
     os.path.join(CSRC, "merbit_batch.hip"),
     os.path.join(CSRC, "merbit_tune.hip"),
+    os.path.join(CSRC, "merbit_adjoint.hip"),
     os.path.join(CSRC, "merbit_gen.cpp"),
     os.path.join(CSRC, "merbit_host.cpp"),
 ]

@@ -1,0 +1,144 @@
+"""GPU checks of the adjoint (dx = A^T g, dval = g[row]*x[col]) and the
+torch.autograd function: transposed pattern vs the oracle's transpose
+(exact), dx vs the oracle on the transpose (condition-scaled 1e-12 FP64,
+1e-6 FP32), dval bit-exact against the FP32/FP64 products, gradients against
+a dense reference and against central finite differences (FP64)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import cond_err, edge_case_patterns, random_csr
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def dev(a, device):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device)
+
+
+def make_op(rp, col, ncol, device, ipt=8):
+    from merbit_amd.autograd import MerbitOperator
+    return MerbitOperator(dev(rp, device), dev(col, device), ncol, items_per_thread=ipt)
+
+
+def test_transpose_matches_oracle(orc, device):
+    rng = np.random.default_rng(2)
+    for name, lens, ncol in edge_case_patterns(2048) + [
+            ("pareto", (rng.pareto(1.0, 20000) * 5).astype(int), 15000)]:
+        say(f"transpose {name}")
+        rp, col = random_csr(rng, len(lens), ncol, lens=lens)
+        op = make_op(rp, col, ncol, device)
+        cp, ri, pm = op.transpose()
+        ocp, ori, opm = orc.csr_transpose(rp, col, ncol)
+        assert np.array_equal(cp, ocp) and np.array_equal(ri, ori) and np.array_equal(pm, opm)
+        op.close()
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_adjoint_dx_and_dval(orc, device, dtype):
+    rng = np.random.default_rng(3)
+    npdt = np.float64 if dtype == "f64" else np.float32
+    tol = 1e-12 if dtype == "f64" else 1e-6
+    for name, lens, ncol in edge_case_patterns(2048) + [
+            ("pareto", (rng.pareto(1.0, 30000) * 5).astype(int), 25000)]:
+        say(f"adjoint {dtype} {name}")
+        rp, col = random_csr(rng, len(lens), ncol, lens=lens)
+        nrow = len(lens)
+        val = rng.uniform(0, 1, int(rp[-1])).astype(npdt)
+        x = rng.standard_normal(ncol).astype(npdt)
+        g = rng.standard_normal(nrow).astype(npdt)
+        op = make_op(rp, col, ncol, device)
+        dx = op.adjoint_x(dev(val, device), dev(g, device)).cpu().numpy()
+        cp, ri, pm = orc.csr_transpose(rp, col, ncol)
+        ya, absr = orc.spmv_accurate(cp, ri, np.ascontiguousarray(val[pm]), g)
+        assert cond_err(dx.astype(np.float64), ya, absr) <= tol, name
+        dval = op.adjoint_val(dev(g, device), dev(x, device)).cpu().numpy()
+        rows = np.repeat(np.arange(nrow), np.diff(rp))
+        want = (g[rows] * x[col]).astype(npdt)  # one rounding, as on the device
+        assert np.array_equal(dval, want), name
+        op.close()
+
+
+def test_autograd_dense_reference(device):
+    from merbit_amd.autograd import spmv
+    rng = np.random.default_rng(5)
+    n, m = 300, 250
+    rp, col = random_csr(rng, n, m, 6.0)
+    op = make_op(rp, col, m, device)
+    val = dev(rng.uniform(0, 1, int(rp[-1])), device).requires_grad_(True)
+    x = dev(rng.standard_normal(m), device).requires_grad_(True)
+    w = dev(rng.standard_normal(n), device)
+    say("autograd vs dense")
+    y = spmv(op, val, x)
+    (y * w).sum().backward()
+    # dense FP64 reference on the CPU
+    rows = torch.from_numpy(np.repeat(np.arange(n), np.diff(rp)))
+    vc = val.detach().cpu().clone().requires_grad_(True)
+    xc = x.detach().cpu().clone().requires_grad_(True)
+    A = torch.zeros((n, m), dtype=torch.float64).index_put((rows, torch.from_numpy(col).long()),
+                                                           vc, accumulate=True)
+    ((A @ xc) * w.cpu()).sum().backward()
+    assert torch.allclose(y.detach().cpu(), (A @ xc).detach(), rtol=1e-12, atol=1e-12)
+    assert torch.allclose(val.grad.cpu(), vc.grad, rtol=1e-12, atol=1e-14)
+    assert torch.allclose(x.grad.cpu(), xc.grad, rtol=1e-12, atol=1e-12)
+    op.close()
+
+
+def test_autograd_finite_differences(device):
+    from merbit_amd.autograd import spmv
+    rng = np.random.default_rng(6)
+    n = 400
+    rp, col = random_csr(rng, n, n, lens=(rng.pareto(1.2, n) * 4).astype(int))
+    op = make_op(rp, col, n, device, ipt=4)
+    val = dev(rng.uniform(0, 1, int(rp[-1])), device).requires_grad_(True)
+    x = dev(rng.standard_normal(n), device).requires_grad_(True)
+    w = dev(rng.standard_normal(n), device)
+    say("finite differences")
+    f = lambda v, xx: float((spmv(op, v, xx) * w).sum())
+    (spmv(op, val, x) * w).sum().backward()
+    for _ in range(3):
+        dv = dev(rng.standard_normal(val.numel()), device)
+        dxd = dev(rng.standard_normal(n), device)
+        h = 1e-3
+        with torch.no_grad():
+            fd_v = (f(val + h * dv, x) - f(val - h * dv, x)) / (2 * h)
+            fd_x = (f(val, x + h * dxd) - f(val, x - h * dxd)) / (2 * h)
+        an_v = float((val.grad * dv).sum())
+        an_x = float((x.grad * dxd).sum())
+        assert abs(fd_v - an_v) <= 1e-8 * (1 + abs(an_v))
+        assert abs(fd_x - an_x) <= 1e-8 * (1 + abs(an_x))
+    # torch's own checker on a tiny instance
+    rp2, col2 = random_csr(rng, 12, 10, 3.0)
+    op2 = make_op(rp2, col2, 10, device, ipt=2)
+    v2 = dev(rng.uniform(0, 1, int(rp2[-1])), device).requires_grad_(True)
+    x2 = dev(rng.standard_normal(10), device).requires_grad_(True)
+    assert torch.autograd.gradcheck(lambda a, b: spmv(op2, a, b), (v2, x2))
+    op.close()
+    op2.close()
+
+
+@pytest.mark.slow
+def test_adjoint_graph_c5(orc, device):
+    """Random graph with 2.4M nodes, Poisson(25) out-degree, FP32 values."""
+    from merbit_amd import workloads
+    say("adjoint graph 2.4M nodes")
+    A, x, g = workloads.graph()
+    op = make_op(A.row_ptr, A.col, A.ncol, device)
+    val = dev(A.val, device)
+    y = op.forward(val, dev(x, device)).cpu().numpy().astype(np.float64)
+    ya, absr = orc.spmv_accurate(A.row_ptr, A.col, A.val, x)
+    assert cond_err(y, ya, absr) <= 1e-6
+    dx = op.adjoint_x(val, dev(g, device)).cpu().numpy().astype(np.float64)
+    cp, ri, pm = orc.csr_transpose(A.row_ptr, A.col, A.ncol)
+    da, dabs = orc.spmv_accurate(cp, ri, np.ascontiguousarray(A.val[pm]), g)
+    assert cond_err(dx, da, dabs) <= 1e-6
+    dval = op.adjoint_val(dev(g, device), dev(x, device)).cpu().numpy()
+    rows = np.repeat(np.arange(A.nrow), np.diff(A.row_ptr))
+    assert np.array_equal(dval, g[rows] * x[A.col])
+    op.close()
