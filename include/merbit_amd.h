@@ -2,8 +2,10 @@
  * merbit_amd.h -- C ABI of the MI355X (gfx950) merge-path tile SpMV.
  *
  * Plain C: device pointers, sizes and an opaque plan; `stream` is a
- * hipStream_t passed as void* (NULL = the null stream).  All calls are
- * stream-ordered and never synchronise the host; spmv calls allocate nothing.
+ * hipStream_t passed as void* (NULL = the null stream).  Products (spmv,
+ * spmm, batch, adjoint, sddmm) are stream-ordered, never synchronise the host
+ * and allocate nothing; creating or destroying a plan, batch or adjoint
+ * allocates and synchronises.
  * Every function returns MBX_OK (0) or a negative status (mbx_status_str).
  *
  * Each declaration names the reference interface it replaces (paths relative
@@ -43,7 +45,8 @@ typedef struct mbx_plan mbx_plan;
  * and keeps one (row, nnz) coordinate per tile of wg_size*items_per_thread
  * merge items plus the per-tile carry slots.  items_per_thread plays the
  * reference's SIGMA and wg_size its BLOCK; 0 selects the defaults (8, 256).
- * d_row_ptr must stay valid until the partition kernel has run. */
+ * Synchronises `stream` once: the largest number of rows ending in one tile
+ * is read back to size the tile kernel's LDS. */
 int mbx_plan_create(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol,
                     int32_t nnz, int32_t items_per_thread, int32_t wg_size,
                     void* stream, mbx_plan** out);

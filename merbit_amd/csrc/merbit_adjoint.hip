@@ -67,14 +67,14 @@ template <int WG, int IPT, class T>
 __global__ __launch_bounds__(WG) void sddmm_tile_kernel(
     const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
     const T* __restrict__ g, const T* __restrict__ x, T* __restrict__ dval,
-    const int2* __restrict__ coord, int ntiles) {
+    const int4* __restrict__ coord, int ntiles) {
   constexpr int TI = WG * IPT;
   __shared__ int s_re[TI + 1];
   __shared__ int s_row[TI];
   const int tile = xcd_tile(blockIdx.x, ntiles);
   const int tid = threadIdx.x;
-  const int2 c0 = coord[tile];
-  const int2 c1 = coord[tile + 1];
+  const int4 c0 = coord[tile];
+  const int4 c1 = coord[tile + 1];
   const int r0 = c0.x, k0 = c0.y;
   const int rows_t = c1.x - r0;
   const int nnz_t = c1.y - k0;

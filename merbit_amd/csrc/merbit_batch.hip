@@ -38,7 +38,7 @@ struct BatchMat {
 };
 
 __global__ void batch_partition_kernel(const BatchMat* __restrict__ mats, int count,
-                                       int tile_items, int ncoord, int2* __restrict__ coord) {
+                                       int tile_items, int ncoord, int4* __restrict__ coord) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ncoord) return;
   int lo = 0, hi = count - 1;  // last matrix whose coord_base <= g
@@ -51,6 +51,19 @@ __global__ void batch_partition_kernel(const BatchMat* __restrict__ mats, int co
                                           g - m.coord_base);
 }
 
+__device__ __forceinline__ int batch_matrix_of_(const int* __restrict__ tile_base, int count,
+                                                int g);
+
+__global__ void batch_max_rows_kernel(const BatchMat* __restrict__ mats,
+                                      const int* __restrict__ tile_base, int count,
+                                      int total_tiles, const int4* __restrict__ coord,
+                                      int* __restrict__ out) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total_tiles) return;
+  const int m = batch_matrix_of_(tile_base, count, g);
+  mbx_detail::plan_stats_update(coord + mats[m].coord_base, g - mats[m].tile_base, out);
+}
+
 __device__ __forceinline__ int batch_matrix_of(const int* __restrict__ tile_base, int count,
                                                int g) {
   int lo = 0, hi = count - 1;  // last matrix whose tile_base <= g
@@ -61,12 +74,18 @@ __device__ __forceinline__ int batch_matrix_of(const int* __restrict__ tile_base
   return lo;
 }
 
+__device__ __forceinline__ int batch_matrix_of_(const int* __restrict__ tile_base, int count,
+                                                int g) {
+  return batch_matrix_of(tile_base, count, g);
+}
+
 template <int WG, int IPT, class P>
 __global__ __launch_bounds__(WG) void batch_spmv_kernel(
     const BatchMat* __restrict__ mats, const int* __restrict__ tile_base, int count,
-    int total_tiles, const int2* __restrict__ coord, double* __restrict__ carry,
+    int total_tiles, const int4* __restrict__ coord, double* __restrict__ carry,
     typename P::scal_t alpha, typename P::scal_t beta, int beta_zero) {
-  __shared__ TileSmem<WG, IPT, P> sm;
+  extern __shared__ __align__(16) char smem[];
+  TileSmem<WG, IPT, P> sm(smem);
   constexpr int X = mbx_detail::kXcds;
   const int b = blockIdx.x, nwg = gridDim.x;
   int first = 0, len = total_tiles, w = b, stride = nwg;
@@ -82,7 +101,7 @@ __global__ __launch_bounds__(WG) void batch_spmv_kernel(
     const int g = first + i;
     const int m = batch_matrix_of(tile_base, count, g);
     const BatchMat M = mats[m];
-    mbx_detail::spmv_tile<WG, IPT, P>(
+    mbx_detail::spmv_tile<WG, IPT, false, P>(
         sm, g - M.tile_base, M.row_ptr, M.col, static_cast<const typename P::val_t*>(M.val),
         static_cast<const typename P::x_t*>(M.x), static_cast<typename P::y_t*>(M.y),
         coord + M.coord_base, carry + 2ll * M.coord_base, alpha, beta, beta_zero);
@@ -93,37 +112,41 @@ __global__ __launch_bounds__(WG) void batch_spmv_kernel(
 template <class P>
 __global__ __launch_bounds__(256) void batch_fixup_kernel(
     const BatchMat* __restrict__ mats, const int* __restrict__ tile_base, int count,
-    int total_tiles, int tile_items, const int2* __restrict__ coord,
+    int total_tiles, int tile_items, const int4* __restrict__ coord,
     const double* __restrict__ carry, typename P::scal_t alpha, typename P::scal_t beta,
     int beta_zero) {
-  const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (g >= total_tiles) return;  // uniform per wave
-  const int m = batch_matrix_of(tile_base, count, g);
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  const bool in = g < total_tiles;
+  const int m = in ? batch_matrix_of(tile_base, count, g) : 0;
   const BatchMat M = mats[m];
-  const int t = g - M.tile_base;
-  if (t < 1) return;
-  mbx_detail::fixup_tile<P>(t, threadIdx.x & 63, M.row_ptr, coord + M.coord_base,
-                            carry + 2ll * M.coord_base, static_cast<typename P::y_t*>(M.y),
-                            tile_items, alpha, beta, beta_zero);
+  const int t = in ? g - M.tile_base : 0;
+  // every lane calls fixup_tile (it cooperates across the wave); lanes of
+  // different matrices pass their own matrix's pointers
+  mbx_detail::fixup_tile<P>(in && t >= 1, t, coord + M.coord_base, carry + 2ll * M.coord_base,
+                            static_cast<typename P::y_t*>(M.y), tile_items, alpha, beta,
+                            beta_zero);
 }
 
 }  // namespace
 
 struct mbx_batch {
   int32_t count, kind, ipt, tile_items, total_tiles, ncoord, grid;
+  int32_t max_rows;    // most rows ending in one tile over the batch
+  int32_t split_rows;  // tiles whose first row began earlier (fixup needed)
   BatchMat* d_mats;
   int* d_tile_base;
-  int2* d_coord;
+  int4* d_coord;  // per tile boundary: rows, nnz consumed, row_ptr[rows]
   double* d_carry;
 };
 
 namespace {
 
 template <class P, int IPT>
-int batch_grid(int device) {
+int batch_grid(int device, int max_rows) {
   int per_cu = 0, cus = 0;
+  const size_t lds = TileSmem<256, IPT, P>::bytes(max_rows);
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_spmv_kernel<256, IPT, P>, 256,
-                                                   0) != hipSuccess ||
+                                                   lds) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
     return 0;
   return std::max(1, per_cu) * std::max(1, cus);
@@ -134,10 +157,12 @@ void batch_launch(const mbx_batch* b, hipStream_t s, typename P::scal_t alpha,
                   typename P::scal_t beta) {
   const int bz = beta == typename P::scal_t(0);
   const int grid = std::min(b->grid, b->total_tiles);
-  batch_spmv_kernel<256, IPT, P><<<grid, 256, 0, s>>>(b->d_mats, b->d_tile_base, b->count,
+  const size_t lds = TileSmem<256, IPT, P>::bytes(b->max_rows);
+  batch_spmv_kernel<256, IPT, P><<<grid, 256, lds, s>>>(b->d_mats, b->d_tile_base, b->count,
                                                       b->total_tiles, b->d_coord, b->d_carry,
                                                       alpha, beta, bz);
-  batch_fixup_kernel<P><<<(b->total_tiles + 3) / 4, 256, 0, s>>>(
+  if (b->split_rows > 0)
+    batch_fixup_kernel<P><<<(b->total_tiles + 255) / 256, 256, 0, s>>>(
       b->d_mats, b->d_tile_base, b->count, b->total_tiles, b->tile_items, b->d_coord,
       b->d_carry, alpha, beta, bz);
 }
@@ -149,7 +174,7 @@ int batch_dispatch(const mbx_batch* b, hipStream_t s, double alpha, double beta,
   switch (b->ipt) {
 #define MBX_CASE(I)                                                  \
   case I:                                                           \
-    if (grid_only) return batch_grid<P, I>(device);                 \
+    if (grid_only) return batch_grid<P, I>(device, b->max_rows);    \
     batch_launch<P, I>(b, s, (S)alpha, (S)beta);                    \
     return 0;
     MBX_CASE(2) MBX_CASE(4) MBX_CASE(6) MBX_CASE(8) MBX_CASE(12) MBX_CASE(16)
@@ -236,6 +261,8 @@ int mbx_batch_create(const mbx_batch_item* items, int32_t count, int32_t value_k
   b->tile_items = (int)T;
   b->total_tiles = (int)tiles;
   b->ncoord = (int)tiles + count;
+  b->max_rows = b->tile_items;
+  b->split_rows = 0;
   b->d_mats = nullptr;
   b->d_tile_base = nullptr;
   b->d_coord = nullptr;
@@ -243,7 +270,7 @@ int mbx_batch_create(const mbx_batch_item* items, int32_t count, int32_t value_k
   hipStream_t s = (hipStream_t)stream;
   if (hipMalloc((void**)&b->d_mats, sizeof(BatchMat) * (size_t)count) != hipSuccess ||
       hipMalloc((void**)&b->d_tile_base, sizeof(int) * (size_t)count) != hipSuccess ||
-      hipMalloc((void**)&b->d_coord, sizeof(int2) * (size_t)b->ncoord) != hipSuccess ||
+      hipMalloc((void**)&b->d_coord, sizeof(int4) * (size_t)b->ncoord) != hipSuccess ||
       hipMalloc((void**)&b->d_carry, sizeof(double) * 2 * (size_t)b->ncoord) != hipSuccess) {
     (void)hipGetLastError();
     mbx_batch_destroy(b);
@@ -260,6 +287,23 @@ int mbx_batch_create(const mbx_batch_item* items, int32_t count, int32_t value_k
   }
   batch_partition_kernel<<<(b->ncoord + 255) / 256, 256, 0, s>>>(b->d_mats, count, b->tile_items,
                                                                   b->ncoord, b->d_coord);
+  int* d_stats = reinterpret_cast<int*>(b->d_carry);  // carries are unused until a product
+  int h_stats[2] = {0, 0};
+  if (hipMemsetAsync(d_stats, 0, 2 * sizeof(int), s) != hipSuccess) {
+    mbx_batch_destroy(b);
+    return MBX_EHIP;
+  }
+  if (b->total_tiles > 0)
+    batch_max_rows_kernel<<<(b->total_tiles + 255) / 256, 256, 0, s>>>(
+        b->d_mats, b->d_tile_base, count, b->total_tiles, b->d_coord, d_stats);
+  if (hipMemcpyAsync(h_stats, d_stats, 2 * sizeof(int), hipMemcpyDeviceToHost, s) !=
+          hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    mbx_batch_destroy(b);
+    return MBX_EHIP;
+  }
+  b->max_rows = h_stats[0];
+  b->split_rows = h_stats[1];
   b->grid = batch_run(b, s, 0.0, 0.0, true, device);
   // the host vectors die here: wait for the copies that read them
   if (hipGetLastError() != hipSuccess || b->grid <= 0 || hipStreamSynchronize(s) != hipSuccess) {

@@ -13,7 +13,9 @@
 struct mbx_plan {
   int32_t nrow, ncol, nnz;
   int32_t wg, ipt, tile_items, ntiles;
-  int2* d_coord;
+  int32_t max_rows;    // most rows ending in one tile (sizes the LDS row ends)
+  int32_t split_rows;  // tiles whose first row began earlier (fixup needed)
+  int4* d_coord;  // per tile boundary: rows, nnz consumed, row_ptr[rows]
   double* d_carry;
   mutable hipEvent_t ev_begin, ev_end;
 };
@@ -44,6 +46,15 @@ __device__ __forceinline__ int lds_merge_search(const int* s_re, int d, int nnz_
       hi = mid;
   }
   return lo;
+}
+
+// The vector staging path reads row_ptr and col in 16-byte groups, FP64 and
+// FP32 values in 16-byte and bfloat16 values in 8-byte groups.
+template <class V>
+inline bool vec_aligned(const int32_t* rp, const int32_t* col, const V* val) {
+  const uintptr_t a = reinterpret_cast<uintptr_t>(rp) | reinterpret_cast<uintptr_t>(col);
+  const uintptr_t v = reinterpret_cast<uintptr_t>(val);
+  return (a & 15u) == 0 && (v & (sizeof(V) == 2 ? 7u : 15u)) == 0;
 }
 
 // Launch-side validation shared by every entry point taking a plan.

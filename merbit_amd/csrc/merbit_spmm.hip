@@ -71,7 +71,7 @@ template <int WG, int IPT, int KC>
 __global__ __launch_bounds__(WG) void tile_spmm_kernel(
     const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
     const float* __restrict__ val, const float* __restrict__ Xr, int ldr, int k,
-    float* __restrict__ Y, long long ldy, int y_row_major, const int2* __restrict__ coord,
+    float* __restrict__ Y, long long ldy, int y_row_major, const int4* __restrict__ coord,
     double* __restrict__ carry, int ntiles, float alpha, float beta, int beta_zero) {
   constexpr int T = WG * IPT;
   constexpr int NW = WG / 64;
@@ -84,8 +84,8 @@ __global__ __launch_bounds__(WG) void tile_spmm_kernel(
 
   const int tile = xcd_tile(blockIdx.x, ntiles);
   const int tid = threadIdx.x;
-  const int2 c0 = coord[tile];
-  const int2 c1 = coord[tile + 1];
+  const int4 c0 = coord[tile];
+  const int4 c1 = coord[tile + 1];
   const int r0 = c0.x, k0 = c0.y;
   const int rows_t = c1.x - r0;
   const int nnz_t = c1.y - k0;
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(WG) void tile_spmm_kernel(
   const int items = min(max(nnz_t + rows_t - d0, 0), IPT);
   const int r_start = items > 0 ? lds_merge_search(s_re, d0, nnz_t, rows_t) : 0;
   const int k_start = d0 - r_start;
-  const bool head_multi = rows_t > 0 && row_ptr[r0] < k0;
+  const bool head_multi = rows_t > 0 && c0.z < k0;
   const int lane = tid & 63, wave = tid >> 6;
   float4* s_y = s_v + nnz_t * KG;
 
@@ -277,17 +277,17 @@ __global__ __launch_bounds__(WG) void tile_spmm_kernel(
 // ends in t; lane q completes right-hand side q (q += 64) summing
 // tail[ts..t-1] then head[t] in order.
 __global__ __launch_bounds__(256) void spmm_fixup_kernel(
-    const int32_t* __restrict__ row_ptr, const int2* __restrict__ coord,
+    const int32_t* __restrict__ row_ptr, const int4* __restrict__ coord,
     const double* __restrict__ carry, int k, float* __restrict__ Y, long long ldy,
     int y_row_major, int ntiles, int tile_items, float alpha, float beta, int beta_zero) {
   const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (t < 1 || t >= ntiles) return;
-  const int2 c0 = coord[t];
-  const int2 c1 = coord[t + 1];
+  const int4 c0 = coord[t];
+  const int4 c1 = coord[t + 1];
   const int r0 = c0.x, k0 = c0.y;
   if (c1.x <= r0) return;
-  const int rs = row_ptr[r0];
+  const int rs = c0.z;
   if (rs >= k0) return;
   const int ts = (int)(((long long)r0 + rs) / tile_items);
   for (int q = lane; q < k; q += 64) {

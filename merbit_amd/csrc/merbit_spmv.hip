@@ -49,35 +49,41 @@ using mbx_detail::F32C;
 using mbx_detail::F64;
 using mbx_detail::TileSmem;
 
+__global__ void plan_stats_kernel(const int4* __restrict__ coord, int ntiles,
+                                  int* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < ntiles) mbx_detail::plan_stats_update(coord, t, out);
+}
+
 __global__ void partition_kernel(const int32_t* __restrict__ row_ptr, int nrow, int nnz,
-                                 int tile_items, int ntiles, int2* __restrict__ coord) {
+                                 int tile_items, int ntiles, int4* __restrict__ coord) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t > ntiles) return;
   coord[t] = mbx_detail::tile_start_coord(row_ptr, nrow, nnz, tile_items, t);
 }
 
-template <int WG, int IPT, class P>
+template <int WG, int IPT, bool VEC, class P>
 __global__ __launch_bounds__(WG) void tile_spmv_kernel(
     const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
     const typename P::val_t* __restrict__ val, const typename P::x_t* __restrict__ x,
-    typename P::y_t* __restrict__ y, const int2* __restrict__ coord,
+    typename P::y_t* __restrict__ y, const int4* __restrict__ coord,
     double* __restrict__ carry, int ntiles, typename P::scal_t alpha,
     typename P::scal_t beta, int beta_zero) {
-  __shared__ TileSmem<WG, IPT, P> sm;
-  mbx_detail::spmv_tile<WG, IPT, P>(sm, xcd_tile(blockIdx.x, ntiles), row_ptr, col, val, x, y,
+  extern __shared__ __align__(16) char smem[];
+  TileSmem<WG, IPT, P> sm(smem);
+  mbx_detail::spmv_tile<WG, IPT, VEC, P>(sm, xcd_tile(blockIdx.x, ntiles), row_ptr, col, val, x, y,
                                     coord, carry, alpha, beta, beta_zero);
 }
 
-// Carry fixup, one wave per tile t >= 1 (see fixup_tile).
+// Carry fixup, one thread per tile (see fixup_tile).
 template <class P>
 __global__ __launch_bounds__(256) void carry_fixup_kernel(
-    const int32_t* __restrict__ row_ptr, const int2* __restrict__ coord,
-    const double* __restrict__ carry, typename P::y_t* __restrict__ y, int ntiles,
-    int tile_items, typename P::scal_t alpha, typename P::scal_t beta, int beta_zero) {
-  const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (t < 1 || t >= ntiles) return;  // uniform per wave
-  mbx_detail::fixup_tile<P>(t, threadIdx.x & 63, row_ptr, coord, carry, y, tile_items, alpha,
-                            beta, beta_zero);
+    const int4* __restrict__ coord, const double* __restrict__ carry,
+    typename P::y_t* __restrict__ y, int ntiles, int tile_items, typename P::scal_t alpha,
+    typename P::scal_t beta, int beta_zero) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  mbx_detail::fixup_tile<P>(t >= 1 && t < ntiles, t, coord, carry, y, tile_items, alpha, beta,
+                            beta_zero);
 }
 
 }  // namespace
@@ -102,14 +108,20 @@ hipError_t launch_tile(const mbx_plan* p, hipStream_t s, const int32_t* rp,
                        const typename P::x_t* x, typename P::y_t* y,
                        typename P::scal_t alpha, typename P::scal_t beta) {
   const int bz = beta == typename P::scal_t(0);
+  const bool vec = mbx_detail::vec_aligned(rp, col, val);
   if (p->ev_begin) (void)hipEventRecord(p->ev_begin, s);
-  tile_spmv_kernel<256, IPT, P><<<p->ntiles, 256, 0, s>>>(
-      rp, col, val, x, y, p->d_coord, p->d_carry, p->ntiles, alpha, beta, bz);
+  const size_t lds = TileSmem<256, IPT, P>::bytes(p->max_rows);
+  if (vec)
+    tile_spmv_kernel<256, IPT, true, P><<<p->ntiles, 256, lds, s>>>(
+        rp, col, val, x, y, p->d_coord, p->d_carry, p->ntiles, alpha, beta, bz);
+  else
+    tile_spmv_kernel<256, IPT, false, P><<<p->ntiles, 256, lds, s>>>(
+        rp, col, val, x, y, p->d_coord, p->d_carry, p->ntiles, alpha, beta, bz);
   if (p->ev_end) (void)hipEventRecord(p->ev_end, s);
   p->ev_begin = p->ev_end = nullptr;
-  if (p->ntiles > 1) {
-    const int grid = (p->ntiles + 3) / 4;
-    carry_fixup_kernel<P><<<grid, 256, 0, s>>>(rp, p->d_coord, p->d_carry, y, p->ntiles,
+  if (p->split_rows > 0) {  // some row crosses a tile boundary
+    const int grid = (p->ntiles + 255) / 256;
+    carry_fixup_kernel<P><<<grid, 256, 0, s>>>(p->d_coord, p->d_carry, y, p->ntiles,
                                                p->tile_items, alpha, beta, bz);
   }
   return hipGetLastError();
@@ -162,24 +174,42 @@ int mbx_plan_create(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol,
   p->wg = wg;
   p->ipt = ipt;
   p->tile_items = (int)T;
+  p->max_rows = (int)T;
+  p->split_rows = 0;
   p->ntiles = (int)((path + T - 1) / T);
   p->d_coord = nullptr;
   p->d_carry = nullptr;
   p->ev_begin = p->ev_end = nullptr;
   const size_t nt1 = (size_t)p->ntiles + 1;
-  if (hipMalloc((void**)&p->d_coord, sizeof(int2) * nt1) != hipSuccess ||
+  if (hipMalloc((void**)&p->d_coord, sizeof(int4) * nt1) != hipSuccess ||
       hipMalloc((void**)&p->d_carry, sizeof(double) * 2 * nt1) != hipSuccess) {
     (void)hipGetLastError();
     mbx_plan_destroy(p);
     return MBX_ENOMEM;
   }
+  hipStream_t st = (hipStream_t)stream;
   const int grid = (p->ntiles + 1 + 255) / 256;
-  partition_kernel<<<grid, 256, 0, (hipStream_t)stream>>>(d_row_ptr, nrow, nnz, p->tile_items,
-                                                          p->ntiles, p->d_coord);
-  if (hipGetLastError() != hipSuccess) {
+  partition_kernel<<<grid, 256, 0, st>>>(d_row_ptr, nrow, nnz, p->tile_items, p->ntiles,
+                                         p->d_coord);
+  // the largest rows-per-tile count sizes the LDS of every launch: read it
+  // back once (plan creation synchronises the stream)
+  int* d_stats = reinterpret_cast<int*>(p->d_carry);  // carries are unused until a product
+  if (hipMemsetAsync(d_stats, 0, 2 * sizeof(int), st) != hipSuccess) {
     mbx_plan_destroy(p);
     return MBX_EHIP;
   }
+  if (p->ntiles > 0)
+    plan_stats_kernel<<<(p->ntiles + 255) / 256, 256, 0, st>>>(p->d_coord, p->ntiles, d_stats);
+  int h_stats[2] = {0, 0};
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(h_stats, d_stats, 2 * sizeof(int), hipMemcpyDeviceToHost, st) !=
+          hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess) {
+    mbx_plan_destroy(p);
+    return MBX_EHIP;
+  }
+  p->max_rows = h_stats[0];
+  p->split_rows = h_stats[1];
   *out = p;
   return MBX_OK;
 }
@@ -202,9 +232,9 @@ int mbx_plan_info(const mbx_plan* p, int32_t* ntiles, int32_t* tile_items, int32
 int mbx_plan_partition(const mbx_plan* p, int32_t* h_rows, int32_t* h_nnz, int32_t count) {
   if (!p || !h_rows || !h_nnz || count < 0 || count > p->ntiles + 1) return MBX_EINVAL;
   if (count == 0) return MBX_OK;
-  int2* tmp = new (std::nothrow) int2[count];
+  int4* tmp = new (std::nothrow) int4[count];
   if (!tmp) return MBX_ENOMEM;
-  if (hipMemcpy(tmp, p->d_coord, sizeof(int2) * (size_t)count, hipMemcpyDeviceToHost) !=
+  if (hipMemcpy(tmp, p->d_coord, sizeof(int4) * (size_t)count, hipMemcpyDeviceToHost) !=
       hipSuccess) {
     delete[] tmp;
     return MBX_EHIP;

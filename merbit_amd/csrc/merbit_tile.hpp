@@ -34,6 +34,7 @@ struct F64 {
   // a precedes b in merge order
   __device__ static acc_t combine(acc_t a, acc_t b) { return a + b; }
   __device__ static acc_t shfl_up(acc_t a, int d) { return __shfl_up(a, d); }
+  __device__ static acc_t shfl_xor(acc_t a, int d) { return __shfl_xor(a, d); }
   __device__ static lds_t to_lds(acc_t a) { return a; }
   __device__ static double to_f64(lds_t a) { return a; }
   __device__ static void store(y_t* y, scal_t alpha, lds_t v, scal_t beta, bool beta_zero) {
@@ -77,6 +78,9 @@ struct CompF32 {
   __device__ static acc_t shfl_up(acc_t a, int d) {
     return make_float2(__shfl_up(a.x, d), __shfl_up(a.y, d));
   }
+  __device__ static acc_t shfl_xor(acc_t a, int d) {
+    return make_float2(__shfl_xor(a.x, d), __shfl_xor(a.y, d));
+  }
   __device__ static lds_t to_lds(acc_t a) { return a; }
   __device__ static double to_f64(lds_t a) { return (double)a.x + (double)a.y; }
   __device__ static void store(y_t* y, scal_t alpha, lds_t v, scal_t beta, bool beta_zero) {
@@ -92,9 +96,20 @@ struct CompF32 {
 using F32C = CompF32<float>;
 using BF16C = CompF32<uint16_t>;
 
+// out[0] = most rows ending in one tile, out[1] = tiles whose first row
+// began in an earlier tile (they need the carry fixup kernel).
+__device__ __forceinline__ void plan_stats_update(const int4* __restrict__ coord, int t,
+                                                  int* __restrict__ out) {
+  const int4 c0 = coord[t];
+  const int4 c1 = coord[t + 1];
+  atomicMax(out, c1.x - c0.x);
+  if (t > 0 && c1.x > c0.x && c0.z < c0.y) atomicAdd(out + 1, 1);
+}
+
 // merge_search over the global row_ptr (include/utils.cuh:5-29): the
-// (rows consumed, nnz consumed) coordinate of tile boundary t.
-__device__ __forceinline__ int2 tile_start_coord(const int32_t* __restrict__ row_ptr, int nrow,
+// (rows consumed, nnz consumed) coordinate of tile boundary t, and where the
+// row open there starts (row_ptr[rows], so tiles need not load it).
+__device__ __forceinline__ int4 tile_start_coord(const int32_t* __restrict__ row_ptr, int nrow,
                                                  int nnz, int tile_items, int t) {
   const long long path = (long long)nrow + nnz;
   const long long diag = min((long long)t * tile_items, path);
@@ -107,41 +122,255 @@ __device__ __forceinline__ int2 tile_start_coord(const int32_t* __restrict__ row
     else
       hi = mid;
   }
-  return make_int2((int)lo, (int)(diag - lo));
+  return make_int4((int)lo, (int)(diag - lo), nrow > 0 ? row_ptr[lo] : 0, 0);
 }
 
 
-// LDS of one tile: local row ends, then the product / row-sum area (products
-// of the tile's nonzeros first, sums of the rows ending in the tile after
-// them: together at most WG*IPT entries, the merge-path identity).
+// 16-byte aligned groups of 4 consecutive elements.  A group is loaded only
+// when it holds at least one element of the wanted range, so a load never
+// leaves the 16-byte block of an in-range element (and never faults).
+__device__ __forceinline__ void load_group(const int32_t* p, int g, int (&o)[4]) {
+  const int4 a = reinterpret_cast<const int4*>(p)[g];
+  o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+}
+__device__ __forceinline__ void load_group(const float* p, int g, float (&o)[4]) {
+  const float4 a = reinterpret_cast<const float4*>(p)[g];
+  o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+}
+// bfloat16 groups stay packed (two values per 32-bit word) until their use.
+struct Bf16x4 {
+  uint32_t w[2];
+  __device__ uint16_t operator[](int v) const {
+    return (uint16_t)((v & 1) ? (w[v >> 1] >> 16) : (w[v >> 1] & 0xFFFFu));
+  }
+};
+__device__ __forceinline__ void load_group(const uint16_t* p, int g, Bf16x4& o) {
+  const uint2 a = reinterpret_cast<const uint2*>(p)[g];
+  o.w[0] = a.x;
+  o.w[1] = a.y;
+}
+// FP64 groups are two 16-byte halves, each loaded only if it holds an element
+// of [lo, hi).
+__device__ __forceinline__ void load_group(const double* p, int g, double (&o)[4], int lo,
+                                           int hi) {
+  o[0] = o[1] = o[2] = o[3] = 0.0;
+  const int e = 4 * g;
+  if (e + 1 >= lo && e < hi) {
+    const double2 a = reinterpret_cast<const double2*>(p)[2 * g];
+    o[0] = a.x; o[1] = a.y;
+  }
+  if (e + 3 >= lo && e + 2 < hi) {
+    const double2 b = reinterpret_cast<const double2*>(p)[2 * g + 1];
+    o[2] = b.x; o[3] = b.y;
+  }
+}
+template <class T, class G>
+__device__ __forceinline__ void load_group(const T* p, int g, G& o, int, int) {
+  load_group(p, g, o);
+}
+
+// Value group g, every 16-byte block clamped to the blocks that hold elements
+// of [lo, hi) (the caller guarantees lo < hi): always in bounds.
+__device__ __forceinline__ void load_group_clamped(const double* p, int g, double (&o)[4], int lo,
+                                                   int hi) {
+  const int hb = lo >> 1, he = (hi - 1) >> 1;  // 16-byte blocks (2 doubles)
+  const double2 a = reinterpret_cast<const double2*>(p)[min(max(2 * g, hb), he)];
+  const double2 b = reinterpret_cast<const double2*>(p)[min(max(2 * g + 1, hb), he)];
+  o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+}
+
+// Register image of 4 consecutive values of type T.
+template <class T>
+struct ValGroup {
+  T v[4];
+  __device__ T operator[](int i) const { return v[i]; }
+};
+template <>
+struct ValGroup<uint16_t> : Bf16x4 {};
+
+__device__ __forceinline__ void load_group(const double* p, int g, ValGroup<double>& o, int lo,
+                                           int hi) {
+  load_group(p, g, o.v, lo, hi);
+}
+__device__ __forceinline__ void load_group(const float* p, int g, ValGroup<float>& o, int, int) {
+  load_group(p, g, o.v);
+}
+__device__ __forceinline__ void load_group_clamped(const double* p, int g, ValGroup<double>& o,
+                                                   int lo, int hi) {
+  load_group_clamped(p, g, o.v, lo, hi);
+}
+template <class T>
+__device__ __forceinline__ void load_group_clamped(const T* p, int g, ValGroup<T>& o, int lo,
+                                                   int hi) {
+  load_group(p, g, o, lo, hi);
+}
+
+// Scalar staging: element i of the tile is handled by thread i % WG.
+template <int WG, int IPT, class P>
+__device__ __forceinline__ void stage_scalar(
+    int* s_re, typename P::lds_t* s_v, const int32_t* __restrict__ row_ptr,
+    const int32_t* __restrict__ col, const typename P::val_t* __restrict__ val,
+    const typename P::x_t* __restrict__ x, int r0, int k0, int rows_t, int nnz_t) {
+  const int tid = threadIdx.x;
+  int re[IPT];
+  int ci[IPT];
+  typename P::val_t vv[IPT];
+  typename P::x_t xv[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const int i = tid + j * WG;
+    re[j] = INT_MAX;
+    ci[j] = 0;
+    vv[j] = typename P::val_t(0);
+    if (i < rows_t) re[j] = row_ptr[r0 + 1 + i];
+    if (i < nnz_t) {
+      ci[j] = col[k0 + i];
+      vv[j] = val[k0 + i];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const int i = tid + j * WG;
+    if (i < rows_t) s_re[i] = re[j] - k0;
+  }
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const int i = tid + j * WG;
+    xv[j] = typename P::x_t(0);
+    if (i < nnz_t) xv[j] = x[ci[j]];
+  }
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const int i = tid + j * WG;
+    if (i < nnz_t) s_v[i] = P::product(vv[j], xv[j]);
+  }
+}
+
+// Vector staging: 16-byte groups of 4 elements, group g of the tile by thread
+// g % WG (row_ptr, col and FP32 values 16-byte aligned, FP64 values 16-byte,
+// bfloat16 values 8-byte aligned).  Group indices are clamped into the tile's
+// range, so every load is in bounds and issued unconditionally (no branch
+// makes the compiler wait on a load before issuing the next); elements
+// outside the tile are masked where they are used.
+template <int WG, int IPT, class P>
+__device__ __forceinline__ void stage_vec(
+    int* s_re, typename P::lds_t* s_v, const int32_t* __restrict__ row_ptr,
+    const int32_t* __restrict__ col, const typename P::val_t* __restrict__ val,
+    const typename P::x_t* __restrict__ x, int r0, int k0, int rows_t, int nnz_t) {
+  constexpr int NG = (WG * IPT / 4 + 2 + WG - 1) / WG;  // groups per thread
+  using VT = typename P::val_t;
+  const int tid = threadIdx.x;
+  const int e0 = r0 + 1;                   // row_ptr entries [e0, e0+rows_t)
+  const int rb = e0 >> 2, rlast = (e0 + rows_t - 1) >> 2;
+  const int k1 = k0 + nnz_t;
+  const int kb = k0 >> 2, klast = (k1 - 1) >> 2;
+  // Row groups first (clamped into valid entries of row_ptr), then the
+  // nonzero groups, then the LDS writes: no load waits for another.  Group
+  // rounds that no thread needs are skipped (uniform branches).
+  const int nrg = rlast - rb + 1, nkg = klast - kb + 1;
+  const int rl = max(rlast, rb - 1 < 0 ? 0 : rb - 1);
+  int rg[NG][4];
+#pragma unroll
+  for (int j = 0; j < NG; ++j)
+    if (j == 0 || j * WG < nrg) load_group(row_ptr, min(rb + tid + j * WG, rl), rg[j]);
+  int cg[NG][4];
+  ValGroup<VT> vg[NG];
+  if (nnz_t > 0) {  // uniform per tile
+#pragma unroll
+    for (int j = 0; j < NG; ++j) {
+      if (j == 0 || j * WG < nkg) {
+        const int g = min(kb + tid + j * WG, klast);
+        load_group(col, g, cg[j]);
+        load_group_clamped(val, g, vg[j], k0, k1);
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NG; ++j) {
+    if (j == 0 || j * WG < nrg) {
+      const int gr = rb + tid + j * WG;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = 4 * gr + v - e0;
+        if (gr <= rlast && i >= 0 && i < rows_t) s_re[i] = rg[j][v] - k0;
+      }
+    }
+  }
+  if (nnz_t > 0) {
+    typename P::x_t xv[NG][4];
+#pragma unroll
+    for (int j = 0; j < NG; ++j) {
+      if (j == 0 || j * WG < nkg) {
+        const int gk = kb + tid + j * WG;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = 4 * gk + v - k0;
+          xv[j][v] = typename P::x_t(0);
+          if (gk <= klast && i >= 0 && i < nnz_t) xv[j][v] = x[cg[j][v]];
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NG; ++j) {
+      if (j == 0 || j * WG < nkg) {
+        const int gk = kb + tid + j * WG;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = 4 * gk + v - k0;
+          if (gk <= klast && i >= 0 && i < nnz_t) s_v[i] = P::product(vg[j][v], xv[j][v]);
+        }
+      }
+    }
+  }
+}
+
+// LDS of one tile, carved from dynamic shared memory: the product / row-sum
+// area (products of the tile's nonzeros first, sums of the rows ending in the
+// tile after them: together at most WG*IPT entries, the merge-path identity),
+// the cross-wave scan scratch, then the local row ends, sized by the plan's
+// largest number of rows ending in one tile (max_rows) instead of WG*IPT.
 template <int WG, int IPT, class P>
 struct TileSmem {
-  int re[WG * IPT + 1];                 // row_ptr[r0+1+i] - k0
-  typename P::lds_t v[WG * IPT];
-  typename P::acc_t wtail[WG / 64];
-  int wflag[WG / 64];
-  double tail;
+  static constexpr int NW = WG / 64;
+  static constexpr size_t kV = sizeof(typename P::lds_t) * WG * IPT;
+  static constexpr size_t kW = (sizeof(typename P::acc_t) * NW + 15) / 16 * 16;
+  static constexpr size_t kFixed = kV + kW + 16 + 16;  // + wflag[NW] + tail
+  typename P::lds_t* v;
+  typename P::acc_t* wtail;
+  int* wflag;
+  double* tail;
+  int* re;  // max_rows + 1 entries: row_ptr[r0+1+i] - k0
+  __device__ TileSmem(char* base)
+      : v(reinterpret_cast<typename P::lds_t*>(base)),
+        wtail(reinterpret_cast<typename P::acc_t*>(base + kV)),
+        wflag(reinterpret_cast<int*>(base + kV + kW)),
+        tail(reinterpret_cast<double*>(base + kV + kW + 16)),
+        re(reinterpret_cast<int*>(base + kFixed)) {}
+  static size_t bytes(int max_rows) {
+    return kFixed + (sizeof(int) * ((size_t)max_rows + 1) + 15) / 16 * 16;
+  }
 };
 
 // One merge-path tile of y = alpha*A*x + beta*y (the work of one workgroup).
 // Rows ending in the tile are written to y; carry[2*tile] receives the
 // partial of the row open at the tile end and carry[2*tile+1] the partial of
 // a first row that began in an earlier tile (completed by fixup_tile).
-template <int WG, int IPT, class P>
+template <int WG, int IPT, bool VEC, class P>
 __device__ __forceinline__ void spmv_tile(
     TileSmem<WG, IPT, P>& sm, int tile, const int32_t* __restrict__ row_ptr,
     const int32_t* __restrict__ col, const typename P::val_t* __restrict__ val,
     const typename P::x_t* __restrict__ x, typename P::y_t* __restrict__ y,
-    const int2* __restrict__ coord, double* __restrict__ carry, typename P::scal_t alpha,
+    const int4* __restrict__ coord, double* __restrict__ carry, typename P::scal_t alpha,
     typename P::scal_t beta, int beta_zero) {
   constexpr int NW = WG / 64;
   using lds_t = typename P::lds_t;
   using acc_t = typename P::acc_t;
   int* s_re = sm.re;
   lds_t* s_v = sm.v;
+  static_assert(WG % 64 == 0 && WG / 64 <= 4, "scan scratch holds up to 4 waves");
   const int tid = threadIdx.x;
-  const int2 c0 = coord[tile];
-  const int2 c1 = coord[tile + 1];
+  const int4 c0 = coord[tile];
+  const int4 c1 = coord[tile + 1];
   const int r0 = c0.x, k0 = c0.y;
   const int rows_t = c1.x - r0;
   const int nnz_t = c1.y - k0;
@@ -149,41 +378,11 @@ __device__ __forceinline__ void spmv_tile(
   // 1. Stage: row ends of the rows ending here, then products.  The open
   //    row at the tile end never ends inside the tile: its end is +inf.
   //    All global loads of the tile are issued before the first wait.
-  {
-    int re[IPT];
-    int ci[IPT];
-    typename P::val_t vv[IPT];
-    typename P::x_t xv[IPT];
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-      const int i = tid + j * WG;
-      re[j] = INT_MAX;
-      ci[j] = 0;
-      vv[j] = typename P::val_t(0);
-      if (i < rows_t) re[j] = row_ptr[r0 + 1 + i];
-      if (i < nnz_t) {
-        ci[j] = col[k0 + i];
-        vv[j] = val[k0 + i];
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-      const int i = tid + j * WG;
-      if (i < rows_t) s_re[i] = re[j] - k0;
-    }
-    if (tid == 0) s_re[rows_t] = INT_MAX;
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-      const int i = tid + j * WG;
-      xv[j] = typename P::x_t(0);
-      if (i < nnz_t) xv[j] = x[ci[j]];
-    }
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-      const int i = tid + j * WG;
-      if (i < nnz_t) s_v[i] = P::product(vv[j], xv[j]);
-    }
-  }
+  if (VEC)
+    stage_vec<WG, IPT, P>(s_re, s_v, row_ptr, col, val, x, r0, k0, rows_t, nnz_t);
+  else
+    stage_scalar<WG, IPT, P>(s_re, s_v, row_ptr, col, val, x, r0, k0, rows_t, nnz_t);
+  if (tid == 0) s_re[rows_t] = INT_MAX;
   __syncthreads();
 
   // 2. Per-thread walk over merge items [tid*IPT, tid*IPT+IPT) of the tile,
@@ -253,49 +452,67 @@ __device__ __forceinline__ void spmv_tile(
   if (tid == WG - 1) {
     acc_t t = s;
     if (fmask == 0ull) t = P::combine(win, t);
-    sm.tail = P::to_f64(t);
+    *sm.tail = P::to_f64(t);
   }
   __syncthreads();
 
   // 4. Coalesced write-out of the rows that end in this tile, and the carry
   //    slots: tail = partial of the row open at the tile end, head = partial
   //    of a row that began in an earlier tile (completed by the fixup).
-  const bool head_multi = rows_t > 0 && row_ptr[r0] < k0;
+  const bool head_multi = rows_t > 0 && c0.z < k0;  // row r0 began earlier
   for (int i = tid; i < rows_t; i += WG) {
     const lds_t v = s_y[i];
     if (i == 0 && head_multi)
-      carry[2 * tile + 1] = P::to_f64(v);
+      carry[2 * tile + 1] = P::to_f64(v);  // completed by the fixup kernel
     else
       P::store(y + r0 + i, alpha, v, beta, beta_zero != 0);
   }
-  if (tid == 0) carry[2 * tile] = sm.tail;
+  if (tid == 0) carry[2 * tile] = *sm.tail;
 }
 
-// Carry fixup, one wave per tile t >= 1: when t's first row began in an
-// earlier tile ts and ends in t, y[row] = alpha*(tail[ts] + ... + tail[t-1]
-// + head[t]) + beta*y[row].  Lane l sums tails ts+l, ts+l+64, ... in order,
-// then a fixed xor butterfly adds the lanes: the result does not depend on
-// scheduling, and rows spanning thousands of tiles stay 64-way parallel.
+// Carry fixup for tile t >= 1 whose first row began in an earlier tile ts and
+// ends in t: y[row] = alpha*(tail[ts] + ... + tail[t-1] + head[t]) +
+// beta*y[row].  One thread per tile: runs of at most 64 tails are summed by
+// their own thread in order; longer runs are summed by the whole wave (lane l
+// takes tails ts+l, ts+l+64, ... in order, then a fixed xor butterfly).  The
+// order depends only on the run length, so results are reproducible.  Every
+// lane of the wave must call this (inactive lanes pass valid = false).
 template <class P>
-__device__ __forceinline__ void fixup_tile(int t, int lane, const int32_t* __restrict__ row_ptr,
-                                           const int2* __restrict__ coord,
+__device__ __forceinline__ void fixup_tile(bool valid, int t, const int4* __restrict__ coord,
                                            const double* __restrict__ carry,
                                            typename P::y_t* __restrict__ y, int tile_items,
                                            typename P::scal_t alpha, typename P::scal_t beta,
                                            int beta_zero) {
-  const int2 c0 = coord[t];
-  const int2 c1 = coord[t + 1];
-  const int r0 = c0.x, k0 = c0.y;
-  if (c1.x <= r0) return;  // the row does not end in this tile
-  const int rs = row_ptr[r0];
-  if (rs >= k0) return;    // the row began in this tile
-  const int ts = (int)(((long long)r0 + rs) / tile_items);
-  double acc = 0.0;
-  for (int s = ts + lane; s < t; s += 64) acc += carry[2 * s];
+  constexpr int kShort = 64;
+  const int lane = threadIdx.x & 63;
+  int r0 = 0, ts = 0;
+  if (valid) {
+    const int4 c0 = coord[t];
+    const int4 c1 = coord[t + 1];
+    r0 = c0.x;
+    valid = c1.x > r0 && c0.z < c0.y;  // the row ends here and began earlier
+    ts = (int)(((long long)r0 + c0.z) / tile_items);
+  }
+  if (valid && t - ts <= kShort) {
+    double acc = 0.0;
+    for (int s = ts; s < t; ++s) acc += carry[2 * s];
+    P::store_f64(y + r0, alpha, acc + carry[2 * t + 1], beta, beta_zero != 0);
+  }
+  unsigned long long longs = __ballot(valid && t - ts > kShort);
+  while (longs) {
+    const int l = __ffsll(longs) - 1;
+    longs &= longs - 1;
+    const int lt = __shfl(t, l), lts = __shfl(ts, l), lr = __shfl(r0, l);
+    // lanes of a batch wave may belong to different matrices: take lane l's
+    const double* lc = reinterpret_cast<const double*>(__shfl((long long)carry, l));
+    typename P::y_t* ly = reinterpret_cast<typename P::y_t*>(__shfl((long long)y, l));
+    double acc = 0.0;
+    for (int s = lts + lane; s < lt; s += 64) acc += lc[2 * s];
 #pragma unroll
-  for (int d = 32; d > 0; d >>= 1) acc += __shfl_xor(acc, d);
-  if (lane == 0) P::store_f64(y + r0, alpha, acc + carry[2 * t + 1], beta, beta_zero != 0);
+    for (int d = 32; d > 0; d >>= 1) acc += __shfl_xor(acc, d);
+    if (lane == 0)
+      P::store_f64(ly + lr, alpha, acc + lc[2 * lt + 1], beta, beta_zero != 0);
+  }
 }
-
 
 }  // namespace mbx_detail

@@ -52,4 +52,10 @@ def edge_case_patterns(T: int):
         ("alternating", [0, T + 1] * 3 + [0], 200),
         ("many_short", [1] * (2 * T), 50),
         ("tail_rows", [0] * (T - 1) + [5], 10),
+        # rows crossing tile boundaries close to either side of them
+        ("boundary_mix", [255, 256, 257, 511, 1, 0, 254] * 12, 700),
+        # the fixup sums runs of up to 64 carries in one thread, longer runs
+        # across the wave: rows just under, at and over that length
+        ("run_64_edges", [1, 63 * T + 5, 0, 64 * T - 3, 2, 65 * T + 1, 3], 800),
+        ("run_long", [2, 150 * T + 17, 1], 900),
     ]

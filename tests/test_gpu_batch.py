@@ -54,6 +54,42 @@ def test_batch_matches_single_calls(device, kind, ipt):
     b.close()
 
 
+@pytest.mark.parametrize("kind", ["f64", "f32"])
+def test_batch_long_rows(orc, device, kind):
+    """Rows spanning more than 64 tiles are completed by a whole wave of the
+    fixup; in a batch that wave also holds tiles of other matrices."""
+    from helpers import random_csr
+    from merbit_amd.batch import MerbitBatch
+    from merbit_amd.spmv import MerbitPlan
+    say(f"batch long rows {kind}")
+    rng = np.random.default_rng(77)
+    T = 256 * 4
+    dt = np.float64 if kind == "f64" else np.float32
+    mats, host = [], []
+    for lens in ([1, 66 * T + 5, 2], [3] * 50, [2, 70 * T + 1], [0, 5, 0], [1, 65 * T, 1]):
+        ncol = 300
+        rp, col = random_csr(rng, len(lens), ncol, lens=lens)
+        val = rng.standard_normal(int(rp[-1])).astype(dt)
+        x = rng.uniform(-1, 1, ncol).astype(dt)
+        host.append((rp, col, val, x))
+        mats.append((torch.from_numpy(rp).to(device), torch.from_numpy(col).to(device),
+                     torch.from_numpy(val).to(device), torch.from_numpy(x).to(device),
+                     torch.full((len(lens),), float("nan"), dtype=torch.from_numpy(x).dtype,
+                                device=device), ncol))
+    b = MerbitBatch(mats, items_per_thread=4)
+    b.spmv()
+    torch.cuda.synchronize()
+    tol = 1e-12 if kind == "f64" else 1e-6
+    for (rp, col, val, x, y, ncol), (hrp, hcol, hval, hx) in zip(mats, host):
+        plan = MerbitPlan(rp, ncol, col.numel(), items_per_thread=4)
+        ref = plan.spmv(rp, col, val, x)
+        plan.close()
+        assert torch.equal(ref.view(torch.uint8), y.view(torch.uint8))
+        ya, absr = orc.spmv_accurate(hrp, hcol, hval.astype(np.float64), hx.astype(np.float64))
+        assert cond_err(y.cpu().numpy().astype(np.float64), ya, absr) <= tol
+    b.close()
+
+
 def test_batch_against_oracle(orc, device):
     from merbit_amd.batch import MerbitBatch
     say("batch f32 vs oracle")

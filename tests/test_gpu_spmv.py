@@ -175,3 +175,35 @@ def test_mixed_precision_small(orc, device, kind):
         if np.any(big):
             rel = np.max(np.abs(y[big] - ya[big]) / np.abs(ya[big]))
             assert rel <= MIXED_TOL, name
+
+
+@pytest.mark.parametrize("kind", ["f64", "f32", "bf16"])
+def test_unaligned_operands(orc, device, kind):
+    """Views one element into larger buffers are not 16-byte aligned: the
+    library stages them element by element; results match the oracle."""
+    from merbit_amd import gen
+    from merbit_amd.spmv import MerbitPlan
+    rng = np.random.default_rng(17)
+    for name, lens, ncol in edge_case_patterns(2048)[4:8]:
+        say(f"unaligned {kind} {name}")
+        rp, col = random_csr(rng, len(lens), ncol, lens=lens)
+        v = rng.uniform(-1, 1, int(rp[-1]))
+        x = rng.uniform(-1, 1, ncol)
+        if kind == "f64":
+            vals, xs, tol = v, x, F64_TOL
+        else:
+            v32 = v.astype(np.float32)
+            vals = v32 if kind == "f32" else gen.to_bf16(v32)
+            xs, tol = x.astype(np.float32), MIXED_TOL
+
+        def shifted(a):
+            t = torch.zeros(len(a) + 1, dtype=dev(a[:1], device).dtype, device=device)
+            t[1:] = dev(a, device)
+            return t[1:]
+        d_rp, d_col, d_val = shifted(rp), shifted(col), shifted(vals)
+        assert d_col.data_ptr() % 16 != 0
+        plan = MerbitPlan(d_rp, ncol, int(rp[-1]))
+        y = plan.spmv(d_rp, d_col, d_val, dev(xs, device)).cpu().numpy().astype(np.float64)
+        plan.close()
+        ya, absr = orc.spmv_accurate(rp, col, vals, xs)
+        assert cond_err(y, ya, absr) <= tol, name
