@@ -144,7 +144,7 @@ namespace {
 template <class P, int IPT>
 int batch_grid(int device, int max_rows) {
   int per_cu = 0, cus = 0;
-  const size_t lds = TileSmem<256, IPT, P>::bytes(max_rows);
+  const size_t lds = TileSmem<256, IPT, P>::bytes();
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_spmv_kernel<256, IPT, P>, 256,
                                                    lds) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
@@ -157,7 +157,7 @@ void batch_launch(const mbx_batch* b, hipStream_t s, typename P::scal_t alpha,
                   typename P::scal_t beta) {
   const int bz = beta == typename P::scal_t(0);
   const int grid = std::min(b->grid, b->total_tiles);
-  const size_t lds = TileSmem<256, IPT, P>::bytes(b->max_rows);
+  const size_t lds = TileSmem<256, IPT, P>::bytes();
   batch_spmv_kernel<256, IPT, P><<<grid, 256, lds, s>>>(b->d_mats, b->d_tile_base, b->count,
                                                       b->total_tiles, b->d_coord, b->d_carry,
                                                       alpha, beta, bz);

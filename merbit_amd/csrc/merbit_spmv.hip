@@ -39,7 +39,6 @@
 #include "merbit_internal.hpp"
 #include "merbit_tile.hpp"
 
-using mbx_detail::lds_merge_search;
 using mbx_detail::xcd_tile;
 
 namespace {
@@ -110,7 +109,7 @@ hipError_t launch_tile(const mbx_plan* p, hipStream_t s, const int32_t* rp,
   const int bz = beta == typename P::scal_t(0);
   const bool vec = mbx_detail::vec_aligned(rp, col, val);
   if (p->ev_begin) (void)hipEventRecord(p->ev_begin, s);
-  const size_t lds = TileSmem<256, IPT, P>::bytes(p->max_rows);
+  const size_t lds = TileSmem<256, IPT, P>::bytes();
   if (vec)
     tile_spmv_kernel<256, IPT, true, P><<<p->ntiles, 256, lds, s>>>(
         rp, col, val, x, y, p->d_coord, p->d_carry, p->ntiles, alpha, beta, bz);

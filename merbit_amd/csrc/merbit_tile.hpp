@@ -37,6 +37,9 @@ struct F64 {
   __device__ static acc_t shfl_xor(acc_t a, int d) { return __shfl_xor(a, d); }
   __device__ static lds_t to_lds(acc_t a) { return a; }
   __device__ static double to_f64(lds_t a) { return a; }
+  // a row completed from an incoming carry c (earlier lanes) and this
+  // lane's partial h
+  __device__ static lds_t finish(double c, acc_t h) { return c + h; }
   __device__ static void store(y_t* y, scal_t alpha, lds_t v, scal_t beta, bool beta_zero) {
     *y = beta_zero ? alpha * v : alpha * v + beta * *y;
   }
@@ -46,8 +49,9 @@ struct F64 {
 };
 
 // Compensated FP32: products split by TwoProduct (p + e == v*x exactly),
-// running sums kept as (s, c) with s updated by TwoSum and c collecting the
-// rounding errors.
+// each thread's running sums kept as (s, c) with s updated by TwoSum and c
+// collecting the rounding errors.  Partials that cross threads (the
+// wavefront scan, the cross-wave carries and the tile carries) are FP64.
 template <class V>
 struct CompF32 {
   using val_t = V;
@@ -83,6 +87,12 @@ struct CompF32 {
   }
   __device__ static lds_t to_lds(acc_t a) { return a; }
   __device__ static double to_f64(lds_t a) { return (double)a.x + (double)a.y; }
+  // the FP64 value split back into an unevaluated float pair
+  __device__ static lds_t finish(double c, acc_t h) {
+    const double d = c + ((double)h.x + (double)h.y);
+    const float hi = (float)d;
+    return make_float2(hi, (float)(d - (double)hi));
+  }
   __device__ static void store(y_t* y, scal_t alpha, lds_t v, scal_t beta, bool beta_zero) {
     const float r = v.x + v.y;
     *y = beta_zero ? alpha * r : alpha * r + beta * *y;
@@ -205,10 +215,22 @@ __device__ __forceinline__ void load_group_clamped(const T* p, int g, ValGroup<T
   load_group(p, g, o, lo, hi);
 }
 
-// Scalar staging: element i of the tile is handled by thread i % WG.
+// Row-end marks.  The end item of local row i sits at merge position
+// p = (row_ptr[r0+1+i] - k0) + i of the tile; bit p % IPT of s_mask[p / IPT]
+// marks it for the thread owning that item and s_wcnt[wave of p] counts it.
+template <int IPT>
+__device__ __forceinline__ void mark_row_end(unsigned* s_mask, int* s_wcnt, int i, int re) {
+  const int p = re + i;
+  atomicOr(s_mask + p / IPT, 1u << (p % IPT));
+  atomicAdd(s_wcnt + p / (64 * IPT), 1);
+}
+
+// Scalar staging: element i of the tile is handled by thread i % WG.  The
+// marks must be zero on entry; the function contains a workgroup barrier
+// between issuing the loads and setting the marks.
 template <int WG, int IPT, class P>
 __device__ __forceinline__ void stage_scalar(
-    int* s_re, typename P::lds_t* s_v, const int32_t* __restrict__ row_ptr,
+    unsigned* s_mask, int* s_wcnt, typename P::lds_t* s_v, const int32_t* __restrict__ row_ptr,
     const int32_t* __restrict__ col, const typename P::val_t* __restrict__ val,
     const typename P::x_t* __restrict__ x, int r0, int k0, int rows_t, int nnz_t) {
   const int tid = threadIdx.x;
@@ -231,13 +253,14 @@ __device__ __forceinline__ void stage_scalar(
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
     const int i = tid + j * WG;
-    if (i < rows_t) s_re[i] = re[j] - k0;
+    xv[j] = typename P::x_t(0);
+    if (i < nnz_t) xv[j] = x[ci[j]];
   }
+  __syncthreads();  // marks zeroed by every thread
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
     const int i = tid + j * WG;
-    xv[j] = typename P::x_t(0);
-    if (i < nnz_t) xv[j] = x[ci[j]];
+    if (i < rows_t) mark_row_end<IPT>(s_mask, s_wcnt, i, re[j] - k0);
   }
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
@@ -251,14 +274,28 @@ __device__ __forceinline__ void stage_scalar(
 // bfloat16 values 8-byte aligned).  Group indices are clamped into the tile's
 // range, so every load is in bounds and issued unconditionally (no branch
 // makes the compiler wait on a load before issuing the next); elements
-// outside the tile are masked where they are used.
+// outside the tile are masked where they are used.  The product of tile
+// element i goes to s_v[i + (k0 & 3)], so every group starts on a 4-slot
+// boundary and groups wholly inside the tile are written with vector LDS
+// stores.  The marks must be zero on entry; the function contains a
+// workgroup barrier between issuing the loads and setting the marks.
+__device__ __forceinline__ void lds_store4(double* dst, const double (&v)[4]) {
+  reinterpret_cast<double2*>(dst)[0] = make_double2(v[0], v[1]);
+  reinterpret_cast<double2*>(dst)[1] = make_double2(v[2], v[3]);
+}
+__device__ __forceinline__ void lds_store4(float2* dst, const float2 (&v)[4]) {
+  reinterpret_cast<float4*>(dst)[0] = make_float4(v[0].x, v[0].y, v[1].x, v[1].y);
+  reinterpret_cast<float4*>(dst)[1] = make_float4(v[2].x, v[2].y, v[3].x, v[3].y);
+}
+
 template <int WG, int IPT, class P>
 __device__ __forceinline__ void stage_vec(
-    int* s_re, typename P::lds_t* s_v, const int32_t* __restrict__ row_ptr,
+    unsigned* s_mask, int* s_wcnt, typename P::lds_t* s_v, const int32_t* __restrict__ row_ptr,
     const int32_t* __restrict__ col, const typename P::val_t* __restrict__ val,
     const typename P::x_t* __restrict__ x, int r0, int k0, int rows_t, int nnz_t) {
   constexpr int NG = (WG * IPT / 4 + 2 + WG - 1) / WG;  // groups per thread
   using VT = typename P::val_t;
+  using lds_t = typename P::lds_t;
   const int tid = threadIdx.x;
   const int e0 = r0 + 1;                   // row_ptr entries [e0, e0+rows_t)
   const int rb = e0 >> 2, rlast = (e0 + rows_t - 1) >> 2;
@@ -285,39 +322,45 @@ __device__ __forceinline__ void stage_vec(
       }
     }
   }
+  // x gathers: elements outside the tile read x[0] (present: the tile has
+  // nonzeros, so ncol > 0) and are dropped below.
+  typename P::x_t xv[NG][4];
+  if (nnz_t > 0) {
 #pragma unroll
-  for (int j = 0; j < NG; ++j) {
-    if (j == 0 || j * WG < nrg) {
-      const int gr = rb + tid + j * WG;
+    for (int j = 0; j < NG; ++j) {
+      if (j == 0 || j * WG < nkg) {
+        const int lo = k0 - 4 * (kb + tid + j * WG), hi = lo + nnz_t;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int i = 4 * gr + v - e0;
-        if (gr <= rlast && i >= 0 && i < rows_t) s_re[i] = rg[j][v] - k0;
+        for (int v = 0; v < 4; ++v) xv[j][v] = x[(v >= lo && v < hi) ? cg[j][v] : 0];
       }
     }
   }
-  if (nnz_t > 0) {
-    typename P::x_t xv[NG][4];
+  __syncthreads();  // marks zeroed by every thread
+  // row-end marks; element v of row group rb + q is local row v - lo
 #pragma unroll
-    for (int j = 0; j < NG; ++j) {
-      if (j == 0 || j * WG < nkg) {
-        const int gk = kb + tid + j * WG;
+  for (int j = 0; j < NG; ++j) {
+    if (j == 0 || j * WG < nrg) {
+      const int lo = e0 - 4 * (rb + tid + j * WG), hi = lo + rows_t;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int i = 4 * gk + v - k0;
-          xv[j][v] = typename P::x_t(0);
-          if (gk <= klast && i >= 0 && i < nnz_t) xv[j][v] = x[cg[j][v]];
-        }
-      }
+      for (int v = 0; v < 4; ++v)
+        if (v >= lo && v < hi) mark_row_end<IPT>(s_mask, s_wcnt, v - lo, rg[j][v] - k0);
     }
+  }
+  if (nnz_t > 0) {
 #pragma unroll
     for (int j = 0; j < NG; ++j) {
       if (j == 0 || j * WG < nkg) {
-        const int gk = kb + tid + j * WG;
+        const int q = tid + j * WG;  // group kb + q
+        const int lo = k0 - 4 * (kb + q), hi = lo + nnz_t;
+        lds_t pr[4];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int i = 4 * gk + v - k0;
-          if (gk <= klast && i >= 0 && i < nnz_t) s_v[i] = P::product(vg[j][v], xv[j][v]);
+        for (int v = 0; v < 4; ++v) pr[v] = P::product(vg[j][v], xv[j][v]);
+        if (lo <= 0 && hi >= 4) {
+          lds_store4(s_v + 4 * q, pr);
+        } else {
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            if (v >= lo && v < hi) s_v[4 * q + v] = pr[v];
         }
       }
     }
@@ -326,30 +369,56 @@ __device__ __forceinline__ void stage_vec(
 
 // LDS of one tile, carved from dynamic shared memory: the product / row-sum
 // area (products of the tile's nonzeros first, sums of the rows ending in the
-// tile after them: together at most WG*IPT entries, the merge-path identity),
-// the cross-wave scan scratch, then the local row ends, sized by the plan's
-// largest number of rows ending in one tile (max_rows) instead of WG*IPT.
+// tile after them: together at most WG*IPT entries, the merge-path identity,
+// plus up to 3 slots of vector-path offset), the cross-wave scan scratch and
+// the row-end marks (one word per thread, one count per wave).
 template <int WG, int IPT, class P>
 struct TileSmem {
   static constexpr int NW = WG / 64;
-  static constexpr size_t kV = sizeof(typename P::lds_t) * WG * IPT;
-  static constexpr size_t kW = (sizeof(typename P::acc_t) * NW + 15) / 16 * 16;
-  static constexpr size_t kFixed = kV + kW + 16 + 16;  // + wflag[NW] + tail
+  static constexpr size_t kV = sizeof(typename P::lds_t) * (WG * IPT + 4);
+  static constexpr size_t kW = (sizeof(double) * NW + 15) / 16 * 16;
+  static constexpr size_t kM = sizeof(unsigned) * WG;
   typename P::lds_t* v;
-  typename P::acc_t* wtail;
-  int* wflag;
+  double* wtail;
+  int* wflag;   // NW
   double* tail;
-  int* re;  // max_rows + 1 entries: row_ptr[r0+1+i] - k0
+  unsigned* mask;
+  int* wcnt;    // NW
   __device__ TileSmem(char* base)
       : v(reinterpret_cast<typename P::lds_t*>(base)),
-        wtail(reinterpret_cast<typename P::acc_t*>(base + kV)),
+        wtail(reinterpret_cast<double*>(base + kV)),
         wflag(reinterpret_cast<int*>(base + kV + kW)),
         tail(reinterpret_cast<double*>(base + kV + kW + 16)),
-        re(reinterpret_cast<int*>(base + kFixed)) {}
-  static size_t bytes(int max_rows) {
-    return kFixed + (sizeof(int) * ((size_t)max_rows + 1) + 15) / 16 * 16;
-  }
+        mask(reinterpret_cast<unsigned*>(base + kV + kW + 32)),
+        wcnt(reinterpret_cast<int*>(base + kV + kW + 32 + kM)) {}
+  static constexpr size_t bytes() { return kV + kW + 32 + kM + 16; }
 };
+
+// The walk of one thread over `items` merge items from local coordinate
+// (r, k); bit i of m marks item i as a row end.  A nonzero item adds its
+// product to the open row; a row-end item writes the open row's sum to
+// s_y[r] (for the thread's first row end this is only the part from this
+// thread: the scan adds the earlier threads' part) and opens the next row.
+// What remains in `sum` is the partial of the row open after the last item.
+template <int IPT, bool FULL, class P>
+__device__ __forceinline__ void walk_items(unsigned m, const typename P::lds_t* s_p,
+                                           typename P::lds_t* s_y, int r, int k, int items,
+                                           typename P::acc_t& sum) {
+  using acc_t = typename P::acc_t;
+#pragma unroll
+  for (int i = 0; i < IPT; ++i) {
+    const bool end = (m >> i) & 1u;
+    // k <= nnz_t: a row-end item may read the first row-sum slot, whose
+    // value is not used
+    acc_t ns = sum;
+    P::add(ns, s_p[k]);
+    if (end) s_y[r] = P::to_lds(sum);
+    const bool nz = !end && (FULL || i < items);
+    sum = end ? P::zero() : (nz ? ns : sum);
+    k += nz ? 1 : 0;
+    r += end ? 1 : 0;
+  }
+}
 
 // One merge-path tile of y = alpha*A*x + beta*y (the work of one workgroup).
 // Rows ending in the tile are written to y; carry[2*tile] receives the
@@ -365,7 +434,6 @@ __device__ __forceinline__ void spmv_tile(
   constexpr int NW = WG / 64;
   using lds_t = typename P::lds_t;
   using acc_t = typename P::acc_t;
-  int* s_re = sm.re;
   lds_t* s_v = sm.v;
   static_assert(WG % 64 == 0 && WG / 64 <= 4, "scan scratch holds up to 4 waves");
   const int tid = threadIdx.x;
@@ -375,85 +443,79 @@ __device__ __forceinline__ void spmv_tile(
   const int rows_t = c1.x - r0;
   const int nnz_t = c1.y - k0;
 
-  // 1. Stage: row ends of the rows ending here, then products.  The open
-  //    row at the tile end never ends inside the tile: its end is +inf.
-  //    All global loads of the tile are issued before the first wait.
-  if (VEC)
-    stage_vec<WG, IPT, P>(s_re, s_v, row_ptr, col, val, x, r0, k0, rows_t, nnz_t);
-  else
-    stage_scalar<WG, IPT, P>(s_re, s_v, row_ptr, col, val, x, r0, k0, rows_t, nnz_t);
-  if (tid == 0) s_re[rows_t] = INT_MAX;
+  // 1. Stage: products to LDS and the row-end marks.  All global loads of
+  //    the tile are issued before the first wait.  s_p: the products
+  //    (offset by k0 & 3 on the vector path).
+  sm.mask[tid] = 0u;
+  if (tid < NW) sm.wcnt[tid] = 0;
+  lds_t* s_p = s_v;
+  if (VEC) {
+    stage_vec<WG, IPT, P>(sm.mask, sm.wcnt, s_v, row_ptr, col, val, x, r0, k0, rows_t, nnz_t);
+    s_p += k0 & 3;
+  } else {
+    stage_scalar<WG, IPT, P>(sm.mask, sm.wcnt, s_v, row_ptr, col, val, x, r0, k0, rows_t,
+                             nnz_t);
+  }
   __syncthreads();
 
-  // 2. Per-thread walk over merge items [tid*IPT, tid*IPT+IPT) of the tile,
-  //    starting from a merge-path search in the LDS row ends.
+  // 2. Per-thread walk over merge items [tid*IPT, tid*IPT+IPT) of the tile.
+  //    Its start row is the number of row ends before its first item: the
+  //    counts of the earlier waves plus a wave scan of the marks.  Every
+  //    tile but a matrix's last has WG*IPT items, so the walk without item
+  //    checks is the common (wave-uniform) case.
+  const int lane = tid & 63, wave = tid >> 6;
+  const unsigned m = sm.mask[tid];
+  const int nm = __popc(m);
+  int incl = nm;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int o = __shfl_up(incl, d);
+    if (lane >= d) incl += o;
+  }
+  int r = incl - nm;
+#pragma unroll
+  for (int w = 0; w < NW - 1; ++w)
+    if (w < wave) r += sm.wcnt[w];
   const int d0 = tid * IPT;
   const int items = min(max(nnz_t + rows_t - d0, 0), IPT);
-  lds_t* s_y = s_v + nnz_t;
-  acc_t sum = P::zero(), head = P::zero();
-  int head_row = 0;
-  bool flagged = false;
-  if (items > 0) {
-    int r = lds_merge_search(s_re, d0, nnz_t, rows_t);
-    int k = d0 - r;
-    int re = s_re[r];
-#pragma unroll
-    for (int i = 0; i < IPT; ++i) {
-      if (i < items) {
-        if (k < re) {
-          P::add(sum, s_v[k]);
-          ++k;
-        } else {
-          if (flagged) {
-            s_y[r] = P::to_lds(sum);  // row began and ended in this thread
-          } else {
-            head = sum;
-            head_row = r;
-            flagged = true;
-          }
-          sum = P::zero();
-          ++r;
-          re = s_re[r];
-        }
-      }
-    }
-  }
+  lds_t* s_y = s_p + nnz_t;
+  acc_t sum = P::zero();
+  if (items == IPT)
+    walk_items<IPT, true, P>(m, s_p, s_y, r, d0 - r, IPT, sum);
+  else if (items > 0)
+    walk_items<IPT, false, P>(m, s_p, s_y, r, d0 - r, items, sum);
+  const bool flagged = m != 0u;  // the thread closes row r (its head)
 
-  // 3. Segmented inclusive scan of the threads' open-row partials.  A
-  //    thread with a row end starts a new segment; seg is the nearest such
-  //    lane at or below this one.
-  const int lane = tid & 63, wave = tid >> 6;
+  // 3. Segmented inclusive scan (FP64) of the threads' open-row partials.
+  //    A thread with a row end starts a new segment; seg is the nearest
+  //    such lane at or below this one.
   const unsigned long long fmask = __ballot(flagged);
   const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
   const unsigned long long fb = fmask & upto;
   const int seg = fb ? 63 - __clzll(fb) : -1;
-  acc_t s = sum;
+  double s = P::to_f64(P::to_lds(sum));
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
-    const acc_t o = P::shfl_up(s, d);
-    if (lane >= d && lane - d >= seg) s = P::combine(o, s);
+    const double o = __shfl_up(s, d);
+    if (lane >= d && lane - d >= seg) s = o + s;
   }
-  const acc_t cin = P::shfl_up(s, 1);
+  const double cin = __shfl_up(s, 1);
   if (lane == 63) {
     sm.wtail[wave] = s;
     sm.wflag[wave] = fmask != 0ull;
   }
   __syncthreads();
-  acc_t win = P::zero();
+  double win = 0.0;
 #pragma unroll
   for (int w = 0; w < NW - 1; ++w)
-    if (w < wave) win = sm.wflag[w] ? sm.wtail[w] : P::combine(win, sm.wtail[w]);
+    if (w < wave) win = sm.wflag[w] ? sm.wtail[w] : win + sm.wtail[w];
   const bool ext = (fmask & ((1ull << lane) - 1ull)) == 0ull;
   if (flagged) {
-    acc_t c = (lane == 0) ? P::zero() : cin;
-    if (ext) c = P::combine(win, c);
-    s_y[head_row] = P::to_lds(P::combine(c, head));
+    double c = (lane == 0) ? 0.0 : cin;
+    if (ext) c = win + c;
+    s_y[r] = P::finish(c, s_y[r]);
   }
-  if (tid == WG - 1) {
-    acc_t t = s;
-    if (fmask == 0ull) t = P::combine(win, t);
-    *sm.tail = P::to_f64(t);
-  }
+  if (tid == WG - 1) *sm.tail = (fmask == 0ull) ? win + s : s;
   __syncthreads();
 
   // 4. Coalesced write-out of the rows that end in this tile, and the carry
