@@ -32,7 +32,7 @@ struct BatchMat {
   const void* val;
   const void* x;
   void* y;
-  int32_t nrow, nnz;
+  int32_t nrow, ncol, nnz;
   int32_t tile_base;   // first global tile of this matrix
   int32_t coord_base;  // first coordinate (tile_base + matrix index)
 };
@@ -79,7 +79,7 @@ __device__ __forceinline__ int batch_matrix_of_(const int* __restrict__ tile_bas
   return batch_matrix_of(tile_base, count, g);
 }
 
-template <int WG, int IPT, class P>
+template <int WG, int IPT, bool VEC, class P>
 __global__ __launch_bounds__(WG) void batch_spmv_kernel(
     const BatchMat* __restrict__ mats, const int* __restrict__ tile_base, int count,
     int total_tiles, const int4* __restrict__ coord, double* __restrict__ carry,
@@ -101,8 +101,8 @@ __global__ __launch_bounds__(WG) void batch_spmv_kernel(
     const int g = first + i;
     const int m = batch_matrix_of(tile_base, count, g);
     const BatchMat M = mats[m];
-    mbx_detail::spmv_tile<WG, IPT, false, P>(
-        sm, g - M.tile_base, M.row_ptr, M.col, static_cast<const typename P::val_t*>(M.val),
+    mbx_detail::spmv_tile<WG, IPT, VEC, P>(
+        sm, g - M.tile_base, M.ncol, M.row_ptr, M.col, static_cast<const typename P::val_t*>(M.val),
         static_cast<const typename P::x_t*>(M.x), static_cast<typename P::y_t*>(M.y),
         coord + M.coord_base, carry + 2ll * M.coord_base, alpha, beta, beta_zero);
     __syncthreads();  // the next tile reuses the LDS
@@ -131,6 +131,7 @@ __global__ __launch_bounds__(256) void batch_fixup_kernel(
 
 struct mbx_batch {
   int32_t count, kind, ipt, tile_items, total_tiles, ncoord, grid;
+  bool vec;            // every matrix suits the vector staging path
   int32_t max_rows;    // most rows ending in one tile over the batch
   int32_t split_rows;  // tiles whose first row began earlier (fixup needed)
   BatchMat* d_mats;
@@ -145,8 +146,8 @@ template <class P, int IPT>
 int batch_grid(int device, int max_rows) {
   int per_cu = 0, cus = 0;
   const size_t lds = TileSmem<256, IPT, P>::bytes();
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_spmv_kernel<256, IPT, P>, 256,
-                                                   lds) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_spmv_kernel<256, IPT, true, P>,
+                                                   256, lds) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
     return 0;
   return std::max(1, per_cu) * std::max(1, cus);
@@ -158,9 +159,14 @@ void batch_launch(const mbx_batch* b, hipStream_t s, typename P::scal_t alpha,
   const int bz = beta == typename P::scal_t(0);
   const int grid = std::min(b->grid, b->total_tiles);
   const size_t lds = TileSmem<256, IPT, P>::bytes();
-  batch_spmv_kernel<256, IPT, P><<<grid, 256, lds, s>>>(b->d_mats, b->d_tile_base, b->count,
-                                                      b->total_tiles, b->d_coord, b->d_carry,
-                                                      alpha, beta, bz);
+  if (b->vec)
+    batch_spmv_kernel<256, IPT, true, P><<<grid, 256, lds, s>>>(
+        b->d_mats, b->d_tile_base, b->count, b->total_tiles, b->d_coord, b->d_carry, alpha, beta,
+        bz);
+  else
+    batch_spmv_kernel<256, IPT, false, P><<<grid, 256, lds, s>>>(
+        b->d_mats, b->d_tile_base, b->count, b->total_tiles, b->d_coord, b->d_carry, alpha, beta,
+        bz);
   if (b->split_rows > 0)
     batch_fixup_kernel<P><<<(b->total_tiles + 255) / 256, 256, 0, s>>>(
       b->d_mats, b->d_tile_base, b->count, b->total_tiles, b->tile_items, b->d_coord,
@@ -190,6 +196,16 @@ int batch_run(const mbx_batch* b, hipStream_t s, double alpha, double beta, bool
     case MBX_VALUE_F32: return batch_dispatch<F32C>(b, s, alpha, beta, grid_only, device);
     case MBX_VALUE_BF16: return batch_dispatch<BF16C>(b, s, alpha, beta, grid_only, device);
     default: return -1;
+  }
+}
+
+bool vec_item_ok(const mbx_batch_item& it, int kind) {
+  const int32_t* rp = it.row_ptr;
+  const int32_t* col = it.col;
+  switch (kind) {
+    case MBX_VALUE_F64: return mbx_detail::vec_path_ok(rp, col, (const double*)it.val, it.ncol);
+    case MBX_VALUE_F32: return mbx_detail::vec_path_ok(rp, col, (const float*)it.val, it.ncol);
+    default: return mbx_detail::vec_path_ok(rp, col, (const uint16_t*)it.val, it.ncol);
   }
 }
 
@@ -239,6 +255,7 @@ int mbx_batch_create(const mbx_batch_item* items, int32_t count, int32_t value_k
   std::vector<BatchMat> h((size_t)count);
   std::vector<int> tb((size_t)count);
   long long tiles = 0;
+  bool vec = true;
   for (int m = 0; m < count; ++m) {
     const mbx_batch_item& it = items[m];
     if (it.nrow < 0 || it.ncol < 0 || it.nnz < 0) return MBX_EINVAL;
@@ -247,8 +264,9 @@ int mbx_batch_create(const mbx_batch_item* items, int32_t count, int32_t value_k
     const long long path = (long long)it.nrow + it.nnz;
     if (path + T >= (long long)INT32_MAX) return MBX_EOVERFLOW;
     const long long nt = (path + T - 1) / T;
-    h[(size_t)m] = BatchMat{it.row_ptr, it.col, it.val, it.x, it.y, it.nrow, it.nnz,
+    h[(size_t)m] = BatchMat{it.row_ptr, it.col, it.val, it.x, it.y, it.nrow, it.ncol, it.nnz,
                             (int32_t)tiles, (int32_t)(tiles + m)};
+    vec = vec && vec_item_ok(it, value_kind);
     tb[(size_t)m] = (int)tiles;
     tiles += nt;
     if (tiles + count >= (long long)INT32_MAX / 2) return MBX_EOVERFLOW;
@@ -257,6 +275,7 @@ int mbx_batch_create(const mbx_batch_item* items, int32_t count, int32_t value_k
   if (!b) return MBX_ENOMEM;
   b->count = count;
   b->kind = value_kind;
+  b->vec = vec;
   b->ipt = ipt;
   b->tile_items = (int)T;
   b->total_tiles = (int)tiles;

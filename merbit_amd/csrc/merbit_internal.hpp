@@ -49,12 +49,14 @@ __device__ __forceinline__ int lds_merge_search(const int* s_re, int d, int nnz_
 }
 
 // The vector staging path reads row_ptr and col in 16-byte groups, FP64 and
-// FP32 values in 16-byte and bfloat16 values in 8-byte groups.
+// FP32 values in 16-byte and bfloat16 values in 8-byte groups, and gathers x
+// by 32-bit byte offsets (x of at most 4 GiB: FP64 x is FP64, else FP32).
 template <class V>
-inline bool vec_aligned(const int32_t* rp, const int32_t* col, const V* val) {
+inline bool vec_path_ok(const int32_t* rp, const int32_t* col, const V* val, int32_t ncol) {
   const uintptr_t a = reinterpret_cast<uintptr_t>(rp) | reinterpret_cast<uintptr_t>(col);
   const uintptr_t v = reinterpret_cast<uintptr_t>(val);
-  return (a & 15u) == 0 && (v & (sizeof(V) == 2 ? 7u : 15u)) == 0;
+  const uint64_t x_bytes = (uint64_t)ncol * (sizeof(V) == 8 ? 8u : 4u);
+  return (a & 15u) == 0 && (v & (sizeof(V) == 2 ? 7u : 15u)) == 0 && x_bytes <= 0xFFFFFFFFull;
 }
 
 // Launch-side validation shared by every entry point taking a plan.

@@ -66,12 +66,12 @@ __global__ __launch_bounds__(WG) void tile_spmv_kernel(
     const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
     const typename P::val_t* __restrict__ val, const typename P::x_t* __restrict__ x,
     typename P::y_t* __restrict__ y, const int4* __restrict__ coord,
-    double* __restrict__ carry, int ntiles, typename P::scal_t alpha,
+    double* __restrict__ carry, int ntiles, int ncol, typename P::scal_t alpha,
     typename P::scal_t beta, int beta_zero) {
   extern __shared__ __align__(16) char smem[];
   TileSmem<WG, IPT, P> sm(smem);
-  mbx_detail::spmv_tile<WG, IPT, VEC, P>(sm, xcd_tile(blockIdx.x, ntiles), row_ptr, col, val, x, y,
-                                    coord, carry, alpha, beta, beta_zero);
+  mbx_detail::spmv_tile<WG, IPT, VEC, P>(sm, xcd_tile(blockIdx.x, ntiles), ncol, row_ptr, col,
+                                         val, x, y, coord, carry, alpha, beta, beta_zero);
 }
 
 // Carry fixup, one thread per tile (see fixup_tile).
@@ -107,15 +107,15 @@ hipError_t launch_tile(const mbx_plan* p, hipStream_t s, const int32_t* rp,
                        const typename P::x_t* x, typename P::y_t* y,
                        typename P::scal_t alpha, typename P::scal_t beta) {
   const int bz = beta == typename P::scal_t(0);
-  const bool vec = mbx_detail::vec_aligned(rp, col, val);
+  const bool vec = mbx_detail::vec_path_ok(rp, col, val, p->ncol);
   if (p->ev_begin) (void)hipEventRecord(p->ev_begin, s);
   const size_t lds = TileSmem<256, IPT, P>::bytes();
   if (vec)
     tile_spmv_kernel<256, IPT, true, P><<<p->ntiles, 256, lds, s>>>(
-        rp, col, val, x, y, p->d_coord, p->d_carry, p->ntiles, alpha, beta, bz);
+        rp, col, val, x, y, p->d_coord, p->d_carry, p->ntiles, p->ncol, alpha, beta, bz);
   else
     tile_spmv_kernel<256, IPT, false, P><<<p->ntiles, 256, lds, s>>>(
-        rp, col, val, x, y, p->d_coord, p->d_carry, p->ntiles, alpha, beta, bz);
+        rp, col, val, x, y, p->d_coord, p->d_carry, p->ntiles, p->ncol, alpha, beta, bz);
   if (p->ev_end) (void)hipEventRecord(p->ev_end, s);
   p->ev_begin = p->ev_end = nullptr;
   if (p->split_rows > 0) {  // some row crosses a tile boundary

@@ -215,6 +215,71 @@ __device__ __forceinline__ void load_group_clamped(const T* p, int g, ValGroup<T
   load_group(p, g, o, lo, hi);
 }
 
+// Cross-lane moves through DPP (GFX9 controls): row_shr:n shifts within
+// rows of 16 lanes, row_bcast:15 / row_bcast:31 pass lane 15 of a row to the
+// next row / lane 31 to rows 2 and 3, wave_shr:1 shifts the whole wave by
+// one.  Lanes without a source (or in rows left out by ROW_MASK) read 0.
+constexpr int kDppRowShr = 0x110;
+constexpr int kDppRowBcast15 = 0x142;
+constexpr int kDppRowBcast31 = 0x143;
+constexpr int kDppWaveShr1 = 0x138;
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ int dpp_move(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, ROW_MASK, 0xf, true);
+}
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ double dpp_move(double v) {
+  const int lo = dpp_move<CTRL, ROW_MASK>(__double2loint(v));
+  const int hi = dpp_move<CTRL, ROW_MASK>(__double2hiint(v));
+  return __hiloint2double(hi, lo);
+}
+
+// Inclusive prefix sum over the 64 lanes of a wave.
+__device__ __forceinline__ int wave_prefix_sum(int v) {
+  v += dpp_move<kDppRowShr | 1>(v);
+  v += dpp_move<kDppRowShr | 2>(v);
+  v += dpp_move<kDppRowShr | 4>(v);
+  v += dpp_move<kDppRowShr | 8>(v);
+  v += dpp_move<kDppRowBcast15, 0xa>(v);
+  v += dpp_move<kDppRowBcast31, 0xc>(v);
+  return v;
+}
+
+// Segmented inclusive sum over the 64 lanes of a wave: lane l adds the
+// values of lanes seg..l-1, where seg (< l, or -1) is the last lane at or
+// below l that starts a segment.  Earlier lanes' sums are added on the left.
+__device__ __forceinline__ double wave_segmented_sum(double s, int lane, int seg) {
+  const int reach = min(lane & 15, lane - seg);  // lanes back within row and segment
+  double o = dpp_move<kDppRowShr | 1>(s);
+  if (reach >= 1) s = o + s;
+  o = dpp_move<kDppRowShr | 2>(s);
+  if (reach >= 2) s = o + s;
+  o = dpp_move<kDppRowShr | 4>(s);
+  if (reach >= 4) s = o + s;
+  o = dpp_move<kDppRowShr | 8>(s);
+  if (reach >= 8) s = o + s;
+  o = dpp_move<kDppRowBcast15, 0xa>(s);
+  if ((lane & 16) && seg < (lane & ~15)) s = o + s;
+  o = dpp_move<kDppRowBcast31, 0xc>(s);
+  if (lane >= 32 && seg < 32) s = o + s;
+  return s;
+}
+
+// Raw buffer resource over `bytes` bytes at p: loads through it take a
+// 32-bit byte offset, and offsets past the end read 0 instead of faulting.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes,
+                                           0x00020000 /* 32-bit data format */);
+}
+__device__ __forceinline__ void buffer_gather(__amdgpu_buffer_rsrc_t r, int c, float& out) {
+  out = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, c << 2, 0, 0));
+}
+__device__ __forceinline__ void buffer_gather(__amdgpu_buffer_rsrc_t r, int c, double& out) {
+  typedef unsigned v2u __attribute__((ext_vector_type(2)));
+  const v2u w = __builtin_amdgcn_raw_buffer_load_b64(r, c << 3, 0, 0);
+  out = __hiloint2double((int)w.y, (int)w.x);
+}
+
 // Row-end marks.  The end item of local row i sits at merge position
 // p = (row_ptr[r0+1+i] - k0) + i of the tile; bit p % IPT of s_mask[p / IPT]
 // marks it for the thread owning that item and s_wcnt[wave of p] counts it.
@@ -271,7 +336,8 @@ __device__ __forceinline__ void stage_scalar(
 
 // Vector staging: 16-byte groups of 4 elements, group g of the tile by thread
 // g % WG (row_ptr, col and FP32 values 16-byte aligned, FP64 values 16-byte,
-// bfloat16 values 8-byte aligned).  Group indices are clamped into the tile's
+// bfloat16 values 8-byte aligned; x of at most 4 GiB, gathered by 32-bit
+// byte offsets).  Group indices are clamped into the tile's
 // range, so every load is in bounds and issued unconditionally (no branch
 // makes the compiler wait on a load before issuing the next); elements
 // outside the tile are masked where they are used.  The product of tile
@@ -292,7 +358,7 @@ template <int WG, int IPT, class P>
 __device__ __forceinline__ void stage_vec(
     unsigned* s_mask, int* s_wcnt, typename P::lds_t* s_v, const int32_t* __restrict__ row_ptr,
     const int32_t* __restrict__ col, const typename P::val_t* __restrict__ val,
-    const typename P::x_t* __restrict__ x, int r0, int k0, int rows_t, int nnz_t) {
+    const typename P::x_t* __restrict__ x, int ncol, int r0, int k0, int rows_t, int nnz_t) {
   constexpr int NG = (WG * IPT / 4 + 2 + WG - 1) / WG;  // groups per thread
   using VT = typename P::val_t;
   using lds_t = typename P::lds_t;
@@ -322,16 +388,19 @@ __device__ __forceinline__ void stage_vec(
       }
     }
   }
-  // x gathers: elements outside the tile read x[0] (present: the tile has
-  // nonzeros, so ncol > 0) and are dropped below.
+  // x gathers through a buffer resource over x: elements of a group outside
+  // the tile (nonzeros of the neighbouring tiles, or past the end of the
+  // column array) read some in-bounds value or 0, and their products are
+  // dropped below; no load can fault.
   typename P::x_t xv[NG][4];
   if (nnz_t > 0) {
+    const __amdgpu_buffer_rsrc_t rx =
+        buffer_rsrc(x, (uint32_t)ncol * (uint32_t)sizeof(typename P::x_t));
 #pragma unroll
     for (int j = 0; j < NG; ++j) {
       if (j == 0 || j * WG < nkg) {
-        const int lo = k0 - 4 * (kb + tid + j * WG), hi = lo + nnz_t;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) xv[j][v] = x[(v >= lo && v < hi) ? cg[j][v] : 0];
+        for (int v = 0; v < 4; ++v) buffer_gather(rx, cg[j][v], xv[j][v]);
       }
     }
   }
@@ -405,17 +474,21 @@ __device__ __forceinline__ void walk_items(unsigned m, const typename P::lds_t* 
                                            typename P::lds_t* s_y, int r, int k, int items,
                                            typename P::acc_t& sum) {
   using acc_t = typename P::acc_t;
+  using lds_t = typename P::lds_t;
+  // Item i is the (i - ends before it)-th nonzero of the thread: all product
+  // reads are issued before the first use (a row-end item reads a slot it
+  // does not use, at most the first row-sum slot).
+  lds_t pv[IPT];
+#pragma unroll
+  for (int i = 0; i < IPT; ++i) pv[i] = s_p[k + i - __popc(m & ((1u << i) - 1u))];
 #pragma unroll
   for (int i = 0; i < IPT; ++i) {
     const bool end = (m >> i) & 1u;
-    // k <= nnz_t: a row-end item may read the first row-sum slot, whose
-    // value is not used
-    acc_t ns = sum;
-    P::add(ns, s_p[k]);
     if (end) s_y[r] = P::to_lds(sum);
+    acc_t ns = sum;
+    P::add(ns, pv[i]);
     const bool nz = !end && (FULL || i < items);
     sum = end ? P::zero() : (nz ? ns : sum);
-    k += nz ? 1 : 0;
     r += end ? 1 : 0;
   }
 }
@@ -426,7 +499,7 @@ __device__ __forceinline__ void walk_items(unsigned m, const typename P::lds_t* 
 // a first row that began in an earlier tile (completed by fixup_tile).
 template <int WG, int IPT, bool VEC, class P>
 __device__ __forceinline__ void spmv_tile(
-    TileSmem<WG, IPT, P>& sm, int tile, const int32_t* __restrict__ row_ptr,
+    TileSmem<WG, IPT, P>& sm, int tile, int ncol, const int32_t* __restrict__ row_ptr,
     const int32_t* __restrict__ col, const typename P::val_t* __restrict__ val,
     const typename P::x_t* __restrict__ x, typename P::y_t* __restrict__ y,
     const int4* __restrict__ coord, double* __restrict__ carry, typename P::scal_t alpha,
@@ -450,7 +523,8 @@ __device__ __forceinline__ void spmv_tile(
   if (tid < NW) sm.wcnt[tid] = 0;
   lds_t* s_p = s_v;
   if (VEC) {
-    stage_vec<WG, IPT, P>(sm.mask, sm.wcnt, s_v, row_ptr, col, val, x, r0, k0, rows_t, nnz_t);
+    stage_vec<WG, IPT, P>(sm.mask, sm.wcnt, s_v, row_ptr, col, val, x, ncol, r0, k0, rows_t,
+                          nnz_t);
     s_p += k0 & 3;
   } else {
     stage_scalar<WG, IPT, P>(sm.mask, sm.wcnt, s_v, row_ptr, col, val, x, r0, k0, rows_t,
@@ -463,16 +537,10 @@ __device__ __forceinline__ void spmv_tile(
   //    counts of the earlier waves plus a wave scan of the marks.  Every
   //    tile but a matrix's last has WG*IPT items, so the walk without item
   //    checks is the common (wave-uniform) case.
-  const int lane = tid & 63, wave = tid >> 6;
+  const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const unsigned m = sm.mask[tid];
   const int nm = __popc(m);
-  int incl = nm;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int o = __shfl_up(incl, d);
-    if (lane >= d) incl += o;
-  }
-  int r = incl - nm;
+  int r = wave_prefix_sum(nm) - nm;
 #pragma unroll
   for (int w = 0; w < NW - 1; ++w)
     if (w < wave) r += sm.wcnt[w];
@@ -493,13 +561,8 @@ __device__ __forceinline__ void spmv_tile(
   const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
   const unsigned long long fb = fmask & upto;
   const int seg = fb ? 63 - __clzll(fb) : -1;
-  double s = P::to_f64(P::to_lds(sum));
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const double o = __shfl_up(s, d);
-    if (lane >= d && lane - d >= seg) s = o + s;
-  }
-  const double cin = __shfl_up(s, 1);
+  const double s = wave_segmented_sum(P::to_f64(P::to_lds(sum)), lane, seg);
+  const double cin = dpp_move<kDppWaveShr1>(s);
   if (lane == 63) {
     sm.wtail[wave] = s;
     sm.wflag[wave] = fmask != 0ull;
