@@ -3,7 +3,8 @@
 Headline workload: FP64 merge SpMV y = A*x on the R-MAT power-law matrix
 (2^20 rows, 2^24 entries with duplicates kept, a=.57 b=c=.19, values N(0,1),
 x ~ U[-1,1)).  A step is one SpMV (tile kernel + carry fixup) with the matrix
-and x already resident in HBM.  Three copies of the matrix are rotated so
+and x already resident in HBM; the step time is taken over the whole loop of
+steps, with no per-step events.  Three copies of the matrix are rotated so
 consecutive steps cannot be served from the 256 MB MALL; the warm (single
 copy) rate is reported beside it.
 
@@ -17,9 +18,10 @@ timed on the host cores.
 Beside the headline, `extras` reports the other measured forms: mixed
 precision (FP32 / BF16 values, compensated FP32 sums) on the R-MAT and on a
 banded 4M-row matrix, multi-RHS SpMM (k = 8, 32), 64 uneven matrices in one
-persistent graph launch (autotuned), the adjoint, and a plain device copy as
-the measured bandwidth ceiling.  Every rate uses the same algorithmic-bytes
-definition.
+persistent graph launch (autotuned), the adjoint on a 2.4M-node graph,
+rocSPARSE's CSR SpMV on the headline matrix (a measured comparator), and a
+plain device copy as the measured bandwidth ceiling.  Every rate uses the same
+algorithmic-bytes definition.
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
@@ -47,31 +49,34 @@ def log(msg):
 
 def time_steps(calls, steps, warmup, plan_events=None):
     """Run warmup + steps calls round-robin; return (ms per step, mean
-    tile-kernel ms or None)."""
+    tile-kernel ms or None).  The step time comes from events around the
+    whole loop only; with plan_events the loop is run once more with events
+    recorded around every tile kernel (on the stream it is launched on), for
+    the kernel's own duration."""
     import torch
     for i in range(warmup):
         calls[i % len(calls)]()
     torch.cuda.synchronize()
-    evs = None
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record()
+    for i in range(steps):
+        calls[i % len(calls)]()
+    stop.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / steps
+    kern = None
     if plan_events is not None:
         evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                for _ in range(steps)]
         for b, e in evs:  # materialise the underlying hipEvents
             b.record()
             e.record()
-    start = torch.cuda.Event(enable_timing=True)
-    stop = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    start.record()
-    for i in range(steps):
-        if evs is not None:
+        torch.cuda.synchronize()
+        for i in range(steps):
             plan_events(*evs[i])
-        calls[i % len(calls)]()
-    stop.record()
-    torch.cuda.synchronize()
-    ms = start.elapsed_time(stop) / steps
-    kern = None
-    if evs is not None:
+            calls[i % len(calls)]()
+        torch.cuda.synchronize()
         kern = float(np.mean([b.elapsed_time(e) for b, e in evs]))
     return ms, kern
 
@@ -243,6 +248,80 @@ def bench_batch(dev, args):
     return out
 
 
+def bench_rocsparse(A, x, dev, args, f64_ms):
+    """rocSPARSE CSR SpMV (FP64) on the same R-MAT copies: a measured
+    comparator, not part of the product path."""
+    import torch
+    from merbit_amd import workloads
+    from merbit_amd.compare import RocsparseSpmv
+    nrow, ncol, nnz = A.nrow, A.ncol, A.nnz
+    b_alg = workloads.algorithmic_bytes(nrow, ncol, nnz, 8, 8, 8)
+    copies = [dict(rp=torch.from_numpy(A.row_ptr).to(dev), col=torch.from_numpy(A.col).to(dev),
+                   val=torch.from_numpy(A.val).to(dev), x=torch.from_numpy(x).to(dev),
+                   y=torch.empty(nrow, dtype=torch.float64, device=dev))
+              for _ in range(max(1, args.copies))]
+    out = {}
+    for alg in ("adaptive", "lrb"):
+        try:
+            t0 = time.time()
+            ops = [RocsparseSpmv(c["rp"], c["col"], c["val"], c["x"], c["y"], alg)
+                   for c in copies]
+            prep_s = (time.time() - t0) / len(ops)
+        except Exception as e:  # an algorithm this rocSPARSE build refuses
+            out[alg] = {"error": str(e)}
+            continue
+        ms, _ = time_steps(ops, args.steps, args.warmup)
+        out[alg] = {"gbs": gbs(b_alg, ms), "us_per_step": round(ms * 1e3, 2),
+                    "merge_path_speedup": round(ms / f64_ms, 3),
+                    "preprocess_ms": round(prep_s * 1e3, 1),
+                    "buffer_bytes": ops[0].buffer_bytes}
+        log(f"rocsparse {alg}: {ms * 1e3:.1f} us/step ({out[alg]['gbs']} GB/s), "
+            f"merge path x{ms / f64_ms:.2f}")
+        for o in ops:
+            o.close()
+    del copies
+    return out
+
+
+def bench_adjoint(dev, args):
+    """The differentiable product on a random graph (2.4M nodes, Poisson(25)
+    out-degree, FP32 values): forward y = A x, dx = A^T g through the
+    transposed merge path, dval = g[row] x[col] (SDDMM)."""
+    import torch
+    from merbit_amd import workloads
+    from merbit_amd.autograd import MerbitOperator
+    t0 = time.time()
+    A, x, g = workloads.graph()
+    log(f"graph: nrow={A.nrow} nnz={A.nnz} ({time.time() - t0:.1f}s)")
+    rp = torch.from_numpy(A.row_ptr).to(dev)
+    col = torch.from_numpy(A.col).to(dev)
+    val = torch.from_numpy(A.val).to(dev)
+    xx = torch.from_numpy(x).to(dev)
+    gg = torch.from_numpy(g).to(dev)
+    op = MerbitOperator(rp, col, A.ncol, items_per_thread=args.ipt)
+    t0 = time.time()
+    op.adjoint_handle  # builds the transposed pattern once
+    torch.cuda.synchronize()
+    build_ms = (time.time() - t0) * 1e3
+    n, nnz = A.nrow, A.nnz
+    steps = max(20, args.steps // 4)
+    out = {"nrow": n, "nnz": nnz, "transpose_build_ms": round(build_ms, 1)}
+    fwd_ms, _ = time_steps([lambda: op.forward(val, xx)], steps, args.warmup)
+    dx_ms, _ = time_steps([lambda: op.adjoint_x(val, gg)], steps, args.warmup)
+    dv_ms, _ = time_steps([lambda: op.adjoint_val(gg, xx)], steps, args.warmup)
+    b_fwd = workloads.algorithmic_bytes(n, A.ncol, nnz, 4, 4, 4)
+    b_dx = workloads.algorithmic_bytes(A.ncol, n, nnz, 4, 4, 4)
+    b_dv = 4 * (n + 1) + 4 * nnz + 4 * nnz + 4 * n + 4 * A.ncol
+    for name, ms, b in (("forward", fwd_ms, b_fwd), ("dx", dx_ms, b_dx), ("dval", dv_ms, b_dv)):
+        out[name] = {"us": round(ms * 1e3, 2), "gbs": gbs(b, ms),
+                     "gflops": round(2 * nnz / (ms * 1e-3) / 1e9, 1)}
+    out["backward_us"] = round((dx_ms + dv_ms) * 1e3, 2)
+    log(f"adjoint: fwd {fwd_ms * 1e3:.1f} us, dx {dx_ms * 1e3:.1f} us, "
+        f"dval {dv_ms * 1e3:.1f} us")
+    op.close()
+    return out
+
+
 def bench_copy(dev):
     import torch
     n = 256 << 20  # 256 Mi doubles = 2 GiB
@@ -279,6 +358,8 @@ def main():
                     help="headline: only the FP64 R-MAT line (for profiling)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run the plain device copy (counter calibration)")
+    ap.add_argument("--compare", action="store_true",
+                    help="also time rocSPARSE on the headline matrix (implied by --only all)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -318,6 +399,9 @@ def main():
         extras["mixed_banded"] = bench_mixed("banded", B, xb, dev, args)
         del B, xb
         extras["batch_uneven"] = bench_batch(dev, args)
+        extras["adjoint_graph"] = bench_adjoint(dev, args)
+    if args.only == "all" or args.compare:
+        extras["rocsparse_rmat_f64"] = bench_rocsparse(A, x, dev, args, ms)
     if args.only == "all" or args.copy_probe:
         extras["copy_ceiling"] = bench_copy(dev)
 

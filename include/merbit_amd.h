@@ -186,6 +186,18 @@ int mbx_sddmm(const mbx_plan* plan, int32_t value_kind, const int32_t* d_row_ptr
               const int32_t* d_col, const void* d_g, const void* d_x, void* d_dval,
               void* stream);
 
+/* Reference-semantics input preparation on the device: the CSR of A^T of a
+ * square COO graph (n x n, int32 indices, duplicates kept) with values
+ * 1/out-degree(row) -- Fun_Values (src/class.cu:9-34), GRAPH::sort(false)
+ * (src/class.cu:143-177) and cusparseXcoo2csr (include/tile.cuh:571-574).
+ * Device arrays: d_row_ind, d_col_ind [nnz] in; d_row_ptr_t [n+1],
+ * d_col_t [nnz] (ascending within rows), d_val_t [nnz] (FP64 or FP32 by
+ * value_kind) out.  An index outside [0, n) is MBX_EINVAL.  Allocates
+ * temporaries and synchronises `stream`. */
+int mbx_ingest_ref(int32_t n, int32_t nnz, const int32_t* d_row_ind, const int32_t* d_col_ind,
+                   int32_t value_kind, int32_t* d_row_ptr_t, int32_t* d_col_t, void* d_val_t,
+                   void* stream);
+
 /* Status text, like the CSR5 error codes (include/csr5/detail/common.h:13-22). */
 const char* mbx_status_str(int status);
 

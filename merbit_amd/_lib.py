@@ -21,7 +21,7 @@ ABI_SYMBOLS = (
     "mbx_batch_graph_create", "mbx_spmv_graph_create", "mbx_graph_launch", "mbx_graph_destroy",
     "mbx_autotune_spmv", "mbx_autotune_spmm", "mbx_autotune_batch",
     "mbx_adjoint_create", "mbx_adjoint_destroy", "mbx_adjoint_transpose", "mbx_adjoint_dx",
-    "mbx_sddmm",
+    "mbx_sddmm", "mbx_ingest_ref",
     "mbx_status_str",
 )
 GEN_SYMBOLS = (
@@ -104,6 +104,7 @@ def lib() -> ctypes.CDLL:
     L.mbx_adjoint_transpose.argtypes = [vp, vp, vp, vp]
     L.mbx_adjoint_dx.argtypes = [vp, i32, vp, f64, vp, f64, vp, vp]
     L.mbx_sddmm.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp]
+    L.mbx_ingest_ref.argtypes = [i32, i32, vp, vp, i32, vp, vp, vp, vp]
     L.mbx_status_str.argtypes = [ctypes.c_int]
     L.mbx_status_str.restype = ctypes.c_char_p
     L.mbg_fill_uniform_f64.argtypes = [vp, i64, f64, f64, u64]

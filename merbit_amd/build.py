@@ -6,7 +6,9 @@ tree and every time it is called:
 * ``merbit_amd/libmerbit_amd.so`` -- HIP kernels + C ABI (include/merbit_amd.h)
   and the host-side generators (include/merbit_gen.h);
 * ``merbit_amd/bin/merbit_main`` -- the C++ config-ID driver (``./main <ID>``
-  of the reference, src/main.cu:3-84) over that library.
+  of the reference, src/main.cu:3-84) over that library;
+* ``merbit_amd/libmerbit_cmp.so`` -- rocSPARSE CSR SpMV behind a small C ABI,
+  a measured comparator for the benchmark (not used by the product path).
 """
 from __future__ import annotations
 
@@ -20,6 +22,7 @@ CSRC = os.path.join(PKG, "csrc")
 HOST = os.path.join(PKG, "host")
 LIB = os.path.join(PKG, "libmerbit_amd.so")
 BIN = os.path.join(PKG, "bin", "merbit_main")
+CMP_LIB = os.path.join(PKG, "libmerbit_cmp.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -39,6 +42,7 @@ LIB_SOURCES = [
     os.path.join(CSRC, "merbit_batch.hip"),
     os.path.join(CSRC, "merbit_tune.hip"),
     os.path.join(CSRC, "merbit_adjoint.hip"),
+    os.path.join(CSRC, "merbit_ingest.hip"),
     os.path.join(CSRC, "merbit_gen.cpp"),
     os.path.join(CSRC, "merbit_host.cpp"),
 ]
@@ -112,9 +116,24 @@ def build_driver(verbose: bool = False) -> str | None:
     return BIN
 
 
+def build_comparator(verbose: bool = False) -> str | None:
+    src = os.path.join(PKG, "compare", "rocsparse_cmp.cpp")
+    rocm = os.path.dirname(os.path.dirname(HIPCC))
+    if not os.path.exists(os.path.join(rocm, "include", "rocsparse", "rocsparse.h")):
+        return None
+    inc = os.path.join(ROOT, "include")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-fPIC", "-shared", "-D__HIP_PLATFORM_AMD__",
+           f"-I{inc}", f"-I{rocm}/include", src, f"-L{rocm}/lib", "-lrocsparse",
+           "-lamdhip64", f"-Wl,-rpath,{rocm}/lib", "-o", CMP_LIB + ".tmp"]
+    _run(cmd, verbose)
+    os.replace(CMP_LIB + ".tmp", CMP_LIB)
+    return CMP_LIB
+
+
 def build(verbose: bool = False) -> None:
     build_library(verbose)
     build_driver(verbose)
+    build_comparator(verbose)
 
 
 if __name__ == "__main__":

@@ -48,9 +48,35 @@ def ref_ingest(nrow: int, row_ind, col_ind) -> Csr:
     return Csr(rp, col[:m], val[:m], nrow)
 
 
+def device_ingest(nrow: int, row_ind, col_ind, dtype=None):
+    """The same matrix built on the GPU (mbx_ingest_ref): torch tensors
+    (row_ptr, col, val) of the CSR of A^T with values 1/out-degree(row).
+    row_ind / col_ind: int32 device tensors; dtype float64 (default) or
+    float32."""
+    import torch
+    from ._lib import MBX_VALUE_F32, MBX_VALUE_F64
+    from .spmv import _stream_handle
+    dtype = dtype or torch.float64
+    kind = {torch.float64: MBX_VALUE_F64, torch.float32: MBX_VALUE_F32}[dtype]
+    if row_ind.dtype != torch.int32 or col_ind.dtype != torch.int32 or not row_ind.is_cuda \
+            or row_ind.shape != col_ind.shape:
+        raise ValueError("row_ind and col_ind must be int32 device tensors of one length")
+    ri, ci = row_ind.contiguous(), col_ind.contiguous()
+    m = ri.numel()
+    dev = ri.device
+    rp = torch.empty(nrow + 1, dtype=torch.int32, device=dev)
+    col = torch.empty(max(m, 1), dtype=torch.int32, device=dev)
+    val = torch.empty(max(m, 1), dtype=dtype, device=dev)
+    check(lib().mbx_ingest_ref(nrow, m, ri.data_ptr(), ci.data_ptr(), kind, rp.data_ptr(),
+                               col.data_ptr(), val.data_ptr(), _stream_handle(None)),
+          "device ingest")
+    return rp, col[:m], val[:m]
+
+
 def load_graph(config_file: str, gid: int) -> Csr:
     n, ri, ci, _ = read_mtx(config_lookup(config_file, gid))
     return ref_ingest(n, ri, ci)
 
 
-__all__ = ["config_lookup", "read_mtx", "ref_ingest", "load_graph", "MerbitError"]
+__all__ = ["config_lookup", "read_mtx", "ref_ingest", "device_ingest", "load_graph",
+           "MerbitError"]
