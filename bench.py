@@ -116,9 +116,11 @@ def bench_rmat_f64(A, x, dev, args):
     calls = [(lambda c=c: plan.spmv(c["rp"], c["col"], c["val"], c["x"], c["y"]))
              for c in copies]
     ms, kern = time_steps(calls, args.steps, args.warmup, plan.set_events)
-    warm_ms, warm_kern = time_steps(calls[:1], args.steps, args.warmup, plan.set_events)
+    warm_ms = warm_kern = None
+    if not args.no_warm:
+        warm_ms, warm_kern = time_steps(calls[:1], args.steps, args.warmup, plan.set_events)
     log(f"rmat fp64: {ms * 1e3:.1f} us/step rotated ({gbs(b_alg, ms)} GB/s), tile kernel "
-        f"{kern * 1e3:.1f} us; warm {warm_ms * 1e3:.1f} us")
+        f"{kern * 1e3:.1f} us" + (f"; warm {warm_ms * 1e3:.1f} us" if warm_ms else ""))
     res = dict(b_alg=b_alg, ms=ms, kern=kern, warm_ms=warm_ms, warm_kern=warm_kern,
                ntiles=plan.ntiles, tile_items=plan.tile_items)
     plan.close()
@@ -354,12 +356,16 @@ def main():
     ap.add_argument("--copies", type=int, default=3)
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--batch-count", type=int, default=64)
-    ap.add_argument("--only", choices=["headline", "all"], default="all",
-                    help="headline: only the FP64 R-MAT line (for profiling)")
+    ap.add_argument("--only", choices=["headline", "banded", "all"], default="all",
+                    help="headline: only the FP64 R-MAT line; banded: the headline plus the "
+                         "FP64/FP32/BF16 banded products (for profiling)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run the plain device copy (counter calibration)")
     ap.add_argument("--compare", action="store_true",
                     help="also time rocSPARSE on the headline matrix (implied by --only all)")
+    ap.add_argument("--no-warm", action="store_true",
+                    help="skip the warm single-copy line (so a kernel trace of the run holds "
+                         "only rotated-copy products)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -385,19 +391,21 @@ def main():
         "pct_hbm_peak": round(100.0 * value / HBM_PEAK_GBS, 2),
         "algorithmic_bytes": b_alg,
         "tile_kernel_us": round(kern * 1e3, 2),
-        "warm_single_copy": {"gbs": gbs(b_alg, h["warm_ms"]),
-                             "us_per_step": round(h["warm_ms"] * 1e3, 2),
-                             "tile_kernel_us": round(h["warm_kern"] * 1e3, 2)},
+        "warm_single_copy": None if h["warm_ms"] is None else {
+            "gbs": gbs(b_alg, h["warm_ms"]), "us_per_step": round(h["warm_ms"] * 1e3, 2),
+            "tile_kernel_us": round(h["warm_kern"] * 1e3, 2)},
         "ntiles": h["ntiles"], "tile_items": h["tile_items"],
     }
     if args.only == "all":
         extras["mixed_rmat"] = bench_mixed("rmat", A, x, dev, args, f64_ms=ms, copies=3)
         extras["spmm_rmat"] = bench_spmm(A, dev, args)
+    if args.only in ("all", "banded"):
         t0 = time.time()
         B, xb = workloads.banded()
         log(f"banded: nrow={B.nrow} nnz={B.nnz} ({time.time() - t0:.1f}s)")
         extras["mixed_banded"] = bench_mixed("banded", B, xb, dev, args)
         del B, xb
+    if args.only == "all":
         extras["batch_uneven"] = bench_batch(dev, args)
         extras["adjoint_graph"] = bench_adjoint(dev, args)
     if args.only == "all" or args.compare:
