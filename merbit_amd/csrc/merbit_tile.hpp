@@ -337,14 +337,10 @@ __device__ __forceinline__ void stage_scalar(
 // Vector staging: 16-byte groups of 4 elements, group g of the tile by thread
 // g % WG (row_ptr, col and FP32 values 16-byte aligned, FP64 values 16-byte,
 // bfloat16 values 8-byte aligned; x of at most 4 GiB, gathered by 32-bit
-// byte offsets).  Group indices are clamped into the tile's
-// range, so every load is in bounds and issued unconditionally (no branch
-// makes the compiler wait on a load before issuing the next); elements
-// outside the tile are masked where they are used.  The product of tile
-// element i goes to s_v[i + (k0 & 3)], so every group starts on a 4-slot
-// boundary and groups wholly inside the tile are written with vector LDS
-// stores.  The marks must be zero on entry; the function contains a
-// workgroup barrier between issuing the loads and setting the marks.
+// byte offsets).  Group indices are clamped into the tile's range, so every
+// load is in bounds and issued unconditionally (no branch makes the compiler
+// wait on a load before issuing the next); elements outside the tile are
+// masked where they are used.
 __device__ __forceinline__ void lds_store4(double* dst, const double (&v)[4]) {
   reinterpret_cast<double2*>(dst)[0] = make_double2(v[0], v[1]);
   reinterpret_cast<double2*>(dst)[1] = make_double2(v[2], v[3]);
@@ -354,65 +350,109 @@ __device__ __forceinline__ void lds_store4(float2* dst, const float2 (&v)[4]) {
   reinterpret_cast<float4*>(dst)[1] = make_float4(v[2].x, v[2].y, v[3].x, v[3].y);
 }
 
+// Registers of one tile on the vector staging path: its coordinates, the
+// 16-byte row-end, column and value groups this thread loads, and the x
+// values gathered for its column groups.  Filled in three steps so that a
+// persistent workgroup can keep the next tile's loads in flight while it
+// works on the current one: tile_coords, tile_issue_loads (needs the
+// coordinates), tile_issue_gathers (needs the columns).
 template <int WG, int IPT, class P>
-__device__ __forceinline__ void stage_vec(
-    unsigned* s_mask, int* s_wcnt, typename P::lds_t* s_v, const int32_t* __restrict__ row_ptr,
-    const int32_t* __restrict__ col, const typename P::val_t* __restrict__ val,
-    const typename P::x_t* __restrict__ x, int ncol, int r0, int k0, int rows_t, int nnz_t) {
-  constexpr int NG = (WG * IPT / 4 + 2 + WG - 1) / WG;  // groups per thread
-  using VT = typename P::val_t;
-  using lds_t = typename P::lds_t;
+struct TileRegs {
+  static constexpr int NG = (WG * IPT / 4 + 2 + WG - 1) / WG;  // groups per thread
+  int4 c0, c1;
+  int rg[NG][4];
+  int cg[NG][4];
+  ValGroup<typename P::val_t> vg[NG];
+  typename P::x_t xv[NG][4];
+  __device__ int r0() const { return c0.x; }
+  __device__ int k0() const { return c0.y; }
+  __device__ int rows() const { return c1.x - c0.x; }
+  __device__ int nnz() const { return c1.y - c0.y; }
+};
+
+template <int WG, int IPT, class P>
+__device__ __forceinline__ void tile_coords(TileRegs<WG, IPT, P>& R,
+                                            const int4* __restrict__ coord, int tile) {
+  R.c0 = coord[tile];
+  R.c1 = coord[tile + 1];
+}
+
+// Row groups first (clamped into valid entries of row_ptr), then the nonzero
+// groups: no load waits for another.  Group rounds that no thread needs are
+// skipped (uniform branches).
+template <int WG, int IPT, class P>
+__device__ __forceinline__ void tile_issue_loads(TileRegs<WG, IPT, P>& R,
+                                                 const int32_t* __restrict__ row_ptr,
+                                                 const int32_t* __restrict__ col,
+                                                 const typename P::val_t* __restrict__ val) {
+  constexpr int NG = TileRegs<WG, IPT, P>::NG;
   const int tid = threadIdx.x;
-  const int e0 = r0 + 1;                   // row_ptr entries [e0, e0+rows_t)
+  const int e0 = R.r0() + 1, rows_t = R.rows(), k0 = R.k0(), nnz_t = R.nnz();
   const int rb = e0 >> 2, rlast = (e0 + rows_t - 1) >> 2;
   const int k1 = k0 + nnz_t;
   const int kb = k0 >> 2, klast = (k1 - 1) >> 2;
-  // Row groups first (clamped into valid entries of row_ptr), then the
-  // nonzero groups, then the LDS writes: no load waits for another.  Group
-  // rounds that no thread needs are skipped (uniform branches).
   const int nrg = rlast - rb + 1, nkg = klast - kb + 1;
   const int rl = max(rlast, rb - 1 < 0 ? 0 : rb - 1);
-  int rg[NG][4];
 #pragma unroll
   for (int j = 0; j < NG; ++j)
-    if (j == 0 || j * WG < nrg) load_group(row_ptr, min(rb + tid + j * WG, rl), rg[j]);
-  int cg[NG][4];
-  ValGroup<VT> vg[NG];
+    if (j == 0 || j * WG < nrg) load_group(row_ptr, min(rb + tid + j * WG, rl), R.rg[j]);
   if (nnz_t > 0) {  // uniform per tile
 #pragma unroll
     for (int j = 0; j < NG; ++j) {
       if (j == 0 || j * WG < nkg) {
         const int g = min(kb + tid + j * WG, klast);
-        load_group(col, g, cg[j]);
-        load_group_clamped(val, g, vg[j], k0, k1);
+        load_group(col, g, R.cg[j]);
+        load_group_clamped(val, g, R.vg[j], k0, k1);
       }
     }
   }
-  // x gathers through a buffer resource over x: elements of a group outside
-  // the tile (nonzeros of the neighbouring tiles, or past the end of the
-  // column array) read some in-bounds value or 0, and their products are
-  // dropped below; no load can fault.
-  typename P::x_t xv[NG][4];
+}
+
+// x gathers through a buffer resource over x: elements of a group outside
+// the tile (nonzeros of the neighbouring tiles, or past the end of the column
+// array) read some in-bounds value or 0, and their products are dropped by
+// tile_write_stage; no load can fault.
+template <int WG, int IPT, class P>
+__device__ __forceinline__ void tile_issue_gathers(TileRegs<WG, IPT, P>& R,
+                                                   const typename P::x_t* __restrict__ x,
+                                                   int ncol) {
+  constexpr int NG = TileRegs<WG, IPT, P>::NG;
+  const int nnz_t = R.nnz();
   if (nnz_t > 0) {
+    const int kb = R.k0() >> 2, nkg = ((R.k0() + nnz_t - 1) >> 2) - kb + 1;
     const __amdgpu_buffer_rsrc_t rx =
         buffer_rsrc(x, (uint32_t)ncol * (uint32_t)sizeof(typename P::x_t));
 #pragma unroll
     for (int j = 0; j < NG; ++j) {
       if (j == 0 || j * WG < nkg) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) buffer_gather(rx, cg[j][v], xv[j][v]);
+        for (int v = 0; v < 4; ++v) buffer_gather(rx, R.cg[j][v], R.xv[j][v]);
       }
     }
   }
-  __syncthreads();  // marks zeroed by every thread
-  // row-end marks; element v of row group rb + q is local row v - lo
+}
+
+// Row-end marks (the marks must be zero and visible to the workgroup) and
+// products to LDS: the product of tile element i goes to s_v[i + (k0 & 3)],
+// so every group starts on a 4-slot boundary and groups wholly inside the
+// tile are written with vector LDS stores.
+template <int WG, int IPT, class P>
+__device__ __forceinline__ void tile_write_stage(const TileRegs<WG, IPT, P>& R, unsigned* s_mask,
+                                                 int* s_wcnt, typename P::lds_t* s_v) {
+  constexpr int NG = TileRegs<WG, IPT, P>::NG;
+  using lds_t = typename P::lds_t;
+  const int tid = threadIdx.x;
+  const int e0 = R.r0() + 1, rows_t = R.rows(), k0 = R.k0(), nnz_t = R.nnz();
+  const int rb = e0 >> 2, nrg = ((e0 + rows_t - 1) >> 2) - rb + 1;
+  const int kb = k0 >> 2, nkg = ((k0 + nnz_t - 1) >> 2) - kb + 1;
+  // element v of row group rb + q is local row v - lo
 #pragma unroll
   for (int j = 0; j < NG; ++j) {
     if (j == 0 || j * WG < nrg) {
       const int lo = e0 - 4 * (rb + tid + j * WG), hi = lo + rows_t;
 #pragma unroll
       for (int v = 0; v < 4; ++v)
-        if (v >= lo && v < hi) mark_row_end<IPT>(s_mask, s_wcnt, v - lo, rg[j][v] - k0);
+        if (v >= lo && v < hi) mark_row_end<IPT>(s_mask, s_wcnt, v - lo, R.rg[j][v] - k0);
     }
   }
   if (nnz_t > 0) {
@@ -423,7 +463,7 @@ __device__ __forceinline__ void stage_vec(
         const int lo = k0 - 4 * (kb + q), hi = lo + nnz_t;
         lds_t pr[4];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) pr[v] = P::product(vg[j][v], xv[j][v]);
+        for (int v = 0; v < 4; ++v) pr[v] = P::product(R.vg[j][v], R.xv[j][v]);
         if (lo <= 0 && hi >= 4) {
           lds_store4(s_v + 4 * q, pr);
         } else {
@@ -493,50 +533,30 @@ __device__ __forceinline__ void walk_items(unsigned m, const typename P::lds_t* 
   }
 }
 
-// One merge-path tile of y = alpha*A*x + beta*y (the work of one workgroup).
-// Rows ending in the tile are written to y; carry[2*tile] receives the
-// partial of the row open at the tile end and carry[2*tile+1] the partial of
-// a first row that began in an earlier tile (completed by fixup_tile).
-template <int WG, int IPT, bool VEC, class P>
-__device__ __forceinline__ void spmv_tile(
-    TileSmem<WG, IPT, P>& sm, int tile, int ncol, const int32_t* __restrict__ row_ptr,
-    const int32_t* __restrict__ col, const typename P::val_t* __restrict__ val,
-    const typename P::x_t* __restrict__ x, typename P::y_t* __restrict__ y,
-    const int4* __restrict__ coord, double* __restrict__ carry, typename P::scal_t alpha,
+// The rest of one tile once its products and row-end marks are in LDS (and
+// a barrier has made them visible): per-thread walk, wave and cross-wave
+// carry scans, write-out.  Rows ending in the tile are written to y;
+// carry[2*tile] receives the partial of the row open at the tile end and
+// carry[2*tile+1] the partial of a first row that began in an earlier tile
+// (completed by fixup_tile).  s_p: the tile's products.
+template <int WG, int IPT, class P>
+__device__ __forceinline__ void finish_tile(
+    TileSmem<WG, IPT, P>& sm, typename P::lds_t* s_p, int tile, int4 c0, int4 c1,
+    typename P::y_t* __restrict__ y, double* __restrict__ carry, typename P::scal_t alpha,
     typename P::scal_t beta, int beta_zero) {
   constexpr int NW = WG / 64;
   using lds_t = typename P::lds_t;
   using acc_t = typename P::acc_t;
-  lds_t* s_v = sm.v;
-  static_assert(WG % 64 == 0 && WG / 64 <= 4, "scan scratch holds up to 4 waves");
   const int tid = threadIdx.x;
-  const int4 c0 = coord[tile];
-  const int4 c1 = coord[tile + 1];
   const int r0 = c0.x, k0 = c0.y;
   const int rows_t = c1.x - r0;
   const int nnz_t = c1.y - k0;
 
-  // 1. Stage: products to LDS and the row-end marks.  All global loads of
-  //    the tile are issued before the first wait.  s_p: the products
-  //    (offset by k0 & 3 on the vector path).
-  sm.mask[tid] = 0u;
-  if (tid < NW) sm.wcnt[tid] = 0;
-  lds_t* s_p = s_v;
-  if (VEC) {
-    stage_vec<WG, IPT, P>(sm.mask, sm.wcnt, s_v, row_ptr, col, val, x, ncol, r0, k0, rows_t,
-                          nnz_t);
-    s_p += k0 & 3;
-  } else {
-    stage_scalar<WG, IPT, P>(sm.mask, sm.wcnt, s_v, row_ptr, col, val, x, r0, k0, rows_t,
-                             nnz_t);
-  }
-  __syncthreads();
-
-  // 2. Per-thread walk over merge items [tid*IPT, tid*IPT+IPT) of the tile.
-  //    Its start row is the number of row ends before its first item: the
-  //    counts of the earlier waves plus a wave scan of the marks.  Every
-  //    tile but a matrix's last has WG*IPT items, so the walk without item
-  //    checks is the common (wave-uniform) case.
+  // Per-thread walk over merge items [tid*IPT, tid*IPT+IPT) of the tile.
+  // Its start row is the number of row ends before its first item: the
+  // counts of the earlier waves plus a wave scan of the marks.  Every tile
+  // but a matrix's last has WG*IPT items, so the walk without item checks
+  // is the common (wave-uniform) case.
   const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const unsigned m = sm.mask[tid];
   const int nm = __popc(m);
@@ -554,9 +574,9 @@ __device__ __forceinline__ void spmv_tile(
     walk_items<IPT, false, P>(m, s_p, s_y, r, d0 - r, items, sum);
   const bool flagged = m != 0u;  // the thread closes row r (its head)
 
-  // 3. Segmented inclusive scan (FP64) of the threads' open-row partials.
-  //    A thread with a row end starts a new segment; seg is the nearest
-  //    such lane at or below this one.
+  // Segmented inclusive scan (FP64) of the threads' open-row partials.  A
+  // thread with a row end starts a new segment; seg is the nearest such lane
+  // at or below this one.
   const unsigned long long fmask = __ballot(flagged);
   const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
   const unsigned long long fb = fmask & upto;
@@ -581,9 +601,9 @@ __device__ __forceinline__ void spmv_tile(
   if (tid == WG - 1) *sm.tail = (fmask == 0ull) ? win + s : s;
   __syncthreads();
 
-  // 4. Coalesced write-out of the rows that end in this tile, and the carry
-  //    slots: tail = partial of the row open at the tile end, head = partial
-  //    of a row that began in an earlier tile (completed by the fixup).
+  // Coalesced write-out of the rows that end in this tile, and the carry
+  // slots: tail = partial of the row open at the tile end, head = partial of
+  // a row that began in an earlier tile (completed by the fixup).
   const bool head_multi = rows_t > 0 && c0.z < k0;  // row r0 began earlier
   for (int i = tid; i < rows_t; i += WG) {
     const lds_t v = s_y[i];
@@ -593,6 +613,49 @@ __device__ __forceinline__ void spmv_tile(
       P::store(y + r0 + i, alpha, v, beta, beta_zero != 0);
   }
   if (tid == 0) carry[2 * tile] = *sm.tail;
+}
+
+// Zero the row-end marks: after the previous tile's mark reads (which precede
+// its carry-scan barrier) and before the barrier that precedes new marks.
+template <int WG, int IPT, class P>
+__device__ __forceinline__ void clear_marks(TileSmem<WG, IPT, P>& sm) {
+  sm.mask[threadIdx.x] = 0u;
+  if (threadIdx.x < WG / 64) sm.wcnt[threadIdx.x] = 0;
+}
+
+// One merge-path tile of y = alpha*A*x + beta*y (the work of one workgroup).
+template <int WG, int IPT, bool VEC, class P>
+__device__ __forceinline__ void spmv_tile(
+    TileSmem<WG, IPT, P>& sm, int tile, int ncol, const int32_t* __restrict__ row_ptr,
+    const int32_t* __restrict__ col, const typename P::val_t* __restrict__ val,
+    const typename P::x_t* __restrict__ x, typename P::y_t* __restrict__ y,
+    const int4* __restrict__ coord, double* __restrict__ carry, typename P::scal_t alpha,
+    typename P::scal_t beta, int beta_zero) {
+  static_assert(WG % 64 == 0 && WG / 64 <= 4, "scan scratch holds up to 4 waves");
+  typename P::lds_t* s_p = sm.v;
+  int4 c0, c1;
+  // Stage: products to LDS and the row-end marks, every global load of the
+  // tile issued before the first wait.
+  if (VEC) {
+    TileRegs<WG, IPT, P> R;
+    tile_coords(R, coord, tile);
+    tile_issue_loads(R, row_ptr, col, val);
+    tile_issue_gathers(R, x, ncol);
+    clear_marks(sm);
+    __syncthreads();
+    tile_write_stage(R, sm.mask, sm.wcnt, sm.v);
+    s_p += R.k0() & 3;
+    c0 = R.c0;
+    c1 = R.c1;
+  } else {
+    c0 = coord[tile];
+    c1 = coord[tile + 1];
+    clear_marks(sm);
+    stage_scalar<WG, IPT, P>(sm.mask, sm.wcnt, sm.v, row_ptr, col, val, x, c0.x, c0.y,
+                             c1.x - c0.x, c1.y - c0.y);
+  }
+  __syncthreads();
+  finish_tile(sm, s_p, tile, c0, c1, y, carry, alpha, beta, beta_zero);
 }
 
 // Carry fixup for tile t >= 1 whose first row began in an earlier tile ts and
