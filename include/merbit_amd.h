@@ -147,6 +147,24 @@ int mbx_spmv_graph_create(const mbx_plan* plan, int32_t value_kind, const int32_
 int mbx_graph_launch(void* graph_exec, void* stream);
 void mbx_graph_destroy(void* graph_exec);
 
+/* Pattern statistics of a planned matrix (features for choosing the tile
+ * shape): row-length moments about the mean and the empty-row count as the
+ * reference's GRAPH::desc / GRAPH::dangling report them (src/class.cu:181-320),
+ * and tile statistics from the plan (TILE::desc, include/tile.cuh:177-233:
+ * single-row tiles are its fast tiles).  Copies row_ptr and the plan's
+ * coordinates to the host (synchronous). */
+typedef struct mbx_pattern_stats {
+  int32_t nrow, ncol, nnz;
+  double mean, var, skew, kurt; /* row lengths */
+  int32_t empty_rows, max_row;
+  int32_t ntiles, tile_items;
+  int32_t single_row_tiles;     /* tiles that end no row */
+  int32_t split_tiles;          /* tiles whose first row began in an earlier tile */
+  int32_t max_rows_per_tile;
+  int32_t long_rows;            /* split rows spanning more than 64 tiles */
+} mbx_pattern_stats;
+int mbx_plan_stats(const mbx_plan* plan, const int32_t* d_row_ptr, mbx_pattern_stats* out);
+
 /* Timing autotuner (no reference counterpart: SIGMA and BLOCK are
  * compile-time in the reference, CMakeLists.txt:4-6): times every
  * instantiated items_per_thread on the given operands (which it overwrites in

@@ -21,7 +21,7 @@ ABI_SYMBOLS = (
     "mbx_batch_graph_create", "mbx_spmv_graph_create", "mbx_graph_launch", "mbx_graph_destroy",
     "mbx_autotune_spmv", "mbx_autotune_spmm", "mbx_autotune_batch",
     "mbx_adjoint_create", "mbx_adjoint_destroy", "mbx_adjoint_transpose", "mbx_adjoint_dx",
-    "mbx_sddmm", "mbx_ingest_ref",
+    "mbx_sddmm", "mbx_ingest_ref", "mbx_plan_stats",
     "mbx_status_str",
 )
 GEN_SYMBOLS = (
@@ -44,6 +44,17 @@ class BatchItem(ctypes.Structure):
     _fields_ = [("row_ptr", ctypes.c_void_p), ("col", ctypes.c_void_p),
                 ("val", ctypes.c_void_p), ("x", ctypes.c_void_p), ("y", ctypes.c_void_p),
                 ("nrow", ctypes.c_int32), ("ncol", ctypes.c_int32), ("nnz", ctypes.c_int32)]
+
+class PatternStats(ctypes.Structure):
+    """mbx_pattern_stats of include/merbit_amd.h."""
+    _fields_ = [("nrow", ctypes.c_int32), ("ncol", ctypes.c_int32), ("nnz", ctypes.c_int32),
+                ("mean", ctypes.c_double), ("var", ctypes.c_double), ("skew", ctypes.c_double),
+                ("kurt", ctypes.c_double), ("empty_rows", ctypes.c_int32),
+                ("max_row", ctypes.c_int32), ("ntiles", ctypes.c_int32),
+                ("tile_items", ctypes.c_int32), ("single_row_tiles", ctypes.c_int32),
+                ("split_tiles", ctypes.c_int32), ("max_rows_per_tile", ctypes.c_int32),
+                ("long_rows", ctypes.c_int32)]
+
 
 _lib = None
 
@@ -105,6 +116,7 @@ def lib() -> ctypes.CDLL:
     L.mbx_adjoint_dx.argtypes = [vp, i32, vp, f64, vp, f64, vp, vp]
     L.mbx_sddmm.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp]
     L.mbx_ingest_ref.argtypes = [i32, i32, vp, vp, i32, vp, vp, vp, vp]
+    L.mbx_plan_stats.argtypes = [vp, vp, ctypes.POINTER(PatternStats)]
     L.mbx_status_str.argtypes = [ctypes.c_int]
     L.mbx_status_str.restype = ctypes.c_char_p
     L.mbg_fill_uniform_f64.argtypes = [vp, i64, f64, f64, u64]

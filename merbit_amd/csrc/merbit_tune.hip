@@ -1,4 +1,5 @@
-// merbit_tune.hip -- timing autotuner for the tile shape.
+// merbit_tune.hip -- timing autotuner for the tile shape, and the pattern
+// statistics a caller (or a tuning model) can use as features.
 //
 // For a matrix (or batch) already in HBM, builds a plan for every
 // instantiated items-per-thread value, times `reps` products of each with
@@ -8,7 +9,12 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
 #include "merbit_amd.h"
+#include "merbit_internal.hpp"
 
 namespace {
 
@@ -56,6 +62,56 @@ int spmv_call(const mbx_plan* p, int kind, const int32_t* rp, const int32_t* col
 }  // namespace
 
 extern "C" {
+
+int mbx_plan_stats(const mbx_plan* p, const int32_t* d_row_ptr, mbx_pattern_stats* out) {
+  if (!p || !out || (p->nrow > 0 && !d_row_ptr)) return MBX_EINVAL;
+  std::vector<int32_t> rp((size_t)p->nrow + 1);
+  std::vector<int4> coord((size_t)p->ntiles + 1);
+  if (hipMemcpy(rp.data(), d_row_ptr, sizeof(int32_t) * rp.size(), hipMemcpyDeviceToHost) !=
+          hipSuccess ||
+      (p->ntiles > 0 && hipMemcpy(coord.data(), p->d_coord, sizeof(int4) * coord.size(),
+                                  hipMemcpyDeviceToHost) != hipSuccess))
+    return MBX_EHIP;
+  mbx_pattern_stats st{};
+  st.nrow = p->nrow;
+  st.ncol = p->ncol;
+  st.nnz = p->nnz;
+  st.ntiles = p->ntiles;
+  st.tile_items = p->tile_items;
+  // row-length moments about the mean (population), GRAPH::desc
+  // (src/class.cu:255-320); empty rows as GRAPH::dangling (class.cu:181-216)
+  const double n = (double)std::max(p->nrow, 1);
+  const double mean = (double)p->nnz / n;
+  double m2 = 0.0, m3 = 0.0, m4 = 0.0;
+  for (int32_t r = 0; r < p->nrow; ++r) {
+    const int32_t len = rp[(size_t)r + 1] - rp[(size_t)r];
+    if (len == 0) ++st.empty_rows;
+    st.max_row = std::max(st.max_row, len);
+    const double d = (double)len - mean;
+    m2 += d * d;
+    m3 += d * d * d;
+    m4 += d * d * d * d;
+  }
+  st.mean = mean;
+  st.var = m2 / n;
+  st.skew = st.var > 0.0 ? m3 / n / std::pow(st.var, 1.5) : 0.0;
+  st.kurt = st.var > 0.0 ? m4 / n / (st.var * st.var) : 0.0;
+  // tile statistics from the plan's coordinates (TILE::desc,
+  // include/tile.cuh:177-233: tiles inside one row are its fast tiles)
+  for (int32_t t = 0; t < p->ntiles; ++t) {
+    const int4 c0 = coord[(size_t)t], c1 = coord[(size_t)t + 1];
+    const int32_t rows = c1.x - c0.x;
+    st.max_rows_per_tile = std::max(st.max_rows_per_tile, rows);
+    if (rows == 0) ++st.single_row_tiles;
+    if (t > 0 && rows > 0 && c0.z < c0.y) {
+      ++st.split_tiles;
+      const long long ts = ((long long)c0.x + c0.z) / p->tile_items;
+      if (t - ts > 64) ++st.long_rows;
+    }
+  }
+  *out = st;
+  return MBX_OK;
+}
 
 int mbx_autotune_spmv(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol, int32_t nnz,
                       const int32_t* d_col, const void* d_val, int32_t value_kind,

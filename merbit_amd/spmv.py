@@ -70,6 +70,15 @@ class MerbitPlan:
               "mbx_plan_partition")
         return rows, nnz
 
+    def stats(self, row_ptr: torch.Tensor) -> dict:
+        """Pattern statistics (mbx_plan_stats): row-length moments, empty rows,
+        tile statistics of this plan's shape."""
+        from ._lib import PatternStats
+        st = PatternStats()
+        check(lib().mbx_plan_stats(self.handle, _dptr(row_ptr, torch.int32, "row_ptr"),
+                                   ctypes.byref(st)), "mbx_plan_stats")
+        return {name: getattr(st, name) for name, _ in PatternStats._fields_}
+
     def set_events(self, begin, end) -> None:
         """Record torch.cuda.Event pair around the next call's tile kernel."""
         check(lib().mbx_plan_set_events(self.handle, begin.cuda_event, end.cuda_event),

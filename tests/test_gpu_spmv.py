@@ -207,3 +207,39 @@ def test_unaligned_operands(orc, device, kind):
         plan.close()
         ya, absr = orc.spmv_accurate(rp, col, vals, xs)
         assert cond_err(y, ya, absr) <= tol, name
+
+
+def test_pattern_stats(orc, device):
+    """mbx_plan_stats against numpy moments of the row lengths and tile
+    statistics recomputed from the partition coordinates."""
+    from merbit_amd.spmv import MerbitPlan
+    rng = np.random.default_rng(44)
+    cases = [("pareto", (rng.pareto(1.0, 30000) * 4).astype(int), 30000)]
+    cases += [c for c in edge_case_patterns(256 * 4) if c[0] in ("giant_row", "all_empty",
+                                                                 "run_long", "many_short")]
+    for name, lens, ncol in cases:
+        say(f"stats {name}")
+        rp, _ = random_csr(rng, len(lens), ncol, lens=lens)
+        plan = MerbitPlan(dev(rp, device), ncol, int(rp[-1]), items_per_thread=4)
+        st = plan.stats(dev(rp, device))
+        ln = np.diff(rp).astype(np.float64)
+        n = len(ln)
+        assert (st["nrow"], st["nnz"], st["ntiles"]) == (n, int(rp[-1]), plan.ntiles)
+        assert st["empty_rows"] == int((ln == 0).sum()) and st["max_row"] == int(ln.max())
+        d = ln - ln.mean()
+        var = (d ** 2).mean()
+        assert abs(st["mean"] - ln.mean()) <= 1e-12 * max(1.0, ln.mean())
+        assert abs(st["var"] - var) <= 1e-9 * max(1.0, var)
+        if var > 0:
+            assert abs(st["skew"] - (d ** 3).mean() / var ** 1.5) <= 1e-9 * max(1.0, abs(st["skew"]))
+            assert abs(st["kurt"] - (d ** 4).mean() / var ** 2) <= 1e-9 * max(1.0, st["kurt"])
+        rows, nnz = plan.partition()
+        T = plan.tile_items
+        nt = plan.ntiles
+        per = rows[1:nt + 1] - rows[:nt]
+        assert st["single_row_tiles"] == int((per == 0).sum())
+        assert st["max_rows_per_tile"] == int(per.max())
+        split = [t for t in range(1, nt) if per[t] > 0 and rp[rows[t]] < nnz[t]]
+        assert st["split_tiles"] == len(split)
+        assert st["long_rows"] == sum(1 for t in split if t - (rows[t] + rp[rows[t]]) // T > 64)
+        plan.close()
