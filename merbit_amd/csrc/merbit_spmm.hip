@@ -56,6 +56,49 @@ __global__ __launch_bounds__(256) void repack_rows_kernel(const float* __restric
   }
 }
 
+// Column-major X with 16-byte aligned columns (ldx % 4 == 0) -> row-major Xr:
+// 16-byte loads down each column, 16-byte stores along each row, transposed
+// through LDS (128 rows x 64 right-hand sides per workgroup).
+__global__ __launch_bounds__(256) void repack_cols_vec_kernel(const float* __restrict__ X,
+                                                              long long ldx, int ncol, int k,
+                                                              float* __restrict__ Xr,
+                                                              int ldr) {
+  __shared__ float t[64][kTransRows + 1];
+  const int c0 = blockIdx.x * kTransRows;
+  const int j0 = blockIdx.y * 64;
+  const int kk = min(64, k - j0);
+  constexpr int C4 = kTransRows / 4;  // float4 per column segment
+  for (int e = threadIdx.x; e < kk * C4; e += 256) {
+    const int j = e / C4, cc = 4 * (e - j * C4);
+    const int c = c0 + cc;
+    const float* src = X + (long long)(j0 + j) * ldx + c;
+    float4 v;
+    if (c + 3 < ncol) {
+      v = *reinterpret_cast<const float4*>(src);
+    } else {
+      v.x = c < ncol ? src[0] : 0.f;
+      v.y = c + 1 < ncol ? src[1] : 0.f;
+      v.z = c + 2 < ncol ? src[2] : 0.f;
+      v.w = 0.f;
+    }
+    t[j][cc] = v.x;
+    t[j][cc + 1] = v.y;
+    t[j][cc + 2] = v.z;
+    t[j][cc + 3] = v.w;
+  }
+  __syncthreads();
+  const int k4 = (kk + 3) / 4;  // float4 groups per row segment (the last one padded with 0)
+  for (int e = threadIdx.x; e < kTransRows * k4; e += 256) {
+    const int cc = e / k4, g = e - cc * k4;
+    const int c = c0 + cc;
+    if (c >= ncol) continue;
+    const int j = 4 * g;
+    const float4 v = make_float4(t[j][cc], j + 1 < kk ? t[j + 1][cc] : 0.f,
+                                 j + 2 < kk ? t[j + 2][cc] : 0.f, j + 3 < kk ? t[j + 3][cc] : 0.f);
+    *reinterpret_cast<float4*>(Xr + (long long)c * ldr + j0 + j) = v;
+  }
+}
+
 template <int KC>
 struct Vec {
   float v[KC];
@@ -273,16 +316,17 @@ __global__ __launch_bounds__(WG) void tile_spmm_kernel(
   }
 }
 
-// One wave per tile t >= 1 whose first row began in an earlier tile ts and
-// ends in t; lane q completes right-hand side q (q += 64) summing
-// tail[ts..t-1] then head[t] in order.
+// One thread per (tile t >= 1, right-hand side q): a row that began in an
+// earlier tile ts and ends in t is completed by summing tail[ts..t-1] then
+// head[t] in order (FP64).
 __global__ __launch_bounds__(256) void spmm_fixup_kernel(
-    const int32_t* __restrict__ row_ptr, const int4* __restrict__ coord,
-    const double* __restrict__ carry, int k, float* __restrict__ Y, long long ldy,
-    int y_row_major, int ntiles, int tile_items, float alpha, float beta, int beta_zero) {
-  const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (t < 1 || t >= ntiles) return;
+    const int4* __restrict__ coord, const double* __restrict__ carry, int k,
+    float* __restrict__ Y, long long ldy, int y_row_major, int ntiles, int tile_items,
+    float alpha, float beta, int beta_zero) {
+  const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (g >= (long long)ntiles * k) return;
+  const int t = (int)(g / k), q = (int)(g - (long long)t * k);
+  if (t < 1) return;
   const int4 c0 = coord[t];
   const int4 c1 = coord[t + 1];
   const int r0 = c0.x, k0 = c0.y;
@@ -290,14 +334,22 @@ __global__ __launch_bounds__(256) void spmm_fixup_kernel(
   const int rs = c0.z;
   if (rs >= k0) return;
   const int ts = (int)(((long long)r0 + rs) / tile_items);
-  for (int q = lane; q < k; q += 64) {
-    double acc = 0.0;
-    for (int s = ts; s < t; ++s) acc += carry[(2ll * s) * k + q];
-    acc += carry[(2ll * t + 1) * k + q];
-    float* yp = y_row_major ? Y + (long long)r0 * ldy + q : Y + r0 + (long long)q * ldy;
-    *yp = beta_zero ? (float)((double)alpha * acc)
-                    : (float)((double)alpha * acc + (double)beta * (double)*yp);
+  // in order; loads issued 8 at a time so a long run is not one round trip
+  // per tile
+  double acc = 0.0;
+  int s = ts;
+  for (; s + 8 <= t; s += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = carry[(2ll * (s + u)) * k + q];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc += v[u];
   }
+  for (; s < t; ++s) acc += carry[(2ll * s) * k + q];
+  acc += carry[(2ll * t + 1) * k + q];
+  float* yp = y_row_major ? Y + (long long)r0 * ldy + q : Y + r0 + (long long)q * ldy;
+  *yp = beta_zero ? (float)((double)alpha * acc)
+                  : (float)((double)alpha * acc + (double)beta * (double)*yp);
 }
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
@@ -327,8 +379,8 @@ void launch_spmm(const mbx_plan* p, hipStream_t s, const int32_t* rp, const int3
   if (p->ev_end) (void)hipEventRecord(p->ev_end, s);
   p->ev_begin = p->ev_end = nullptr;
   if (p->ntiles > 1)
-    spmm_fixup_kernel<<<(p->ntiles + 3) / 4, 256, 0, s>>>(
-        rp, p->d_coord, carry, k, Y, ldy, yrm, p->ntiles, p->tile_items, alpha, beta, bz);
+    spmm_fixup_kernel<<<(int)(((long long)p->ntiles * k + 255) / 256), 256, 0, s>>>(
+        p->d_coord, carry, k, Y, ldy, yrm, p->ntiles, p->tile_items, alpha, beta, bz);
 }
 
 }  // namespace
@@ -368,8 +420,12 @@ int mbx_spmm_f32(const mbx_plan* p, const int32_t* rp, const int32_t* col, const
         ws + align_up(sizeof(double) * 2 * ((size_t)p->ntiles + 1) * (size_t)k, 256));
     if (p->ncol > 0) {
       dim3 grid((p->ncol + kTransRows - 1) / kTransRows, (k + 63) / 64);
-      repack_rows_kernel<<<grid, 256, 0, s>>>(X, ldx, x_layout == MBX_ROW_MAJOR, p->ncol, k, xr,
-                                              ldr);
+      if (x_layout == MBX_COL_MAJOR && ldx % 4 == 0 &&
+          (reinterpret_cast<uintptr_t>(X) & 15u) == 0)
+        repack_cols_vec_kernel<<<grid, 256, 0, s>>>(X, ldx, p->ncol, k, xr, ldr);
+      else
+        repack_rows_kernel<<<grid, 256, 0, s>>>(X, ldx, x_layout == MBX_ROW_MAJOR, p->ncol, k,
+                                                xr, ldr);
     }
     Xr = xr;
   }

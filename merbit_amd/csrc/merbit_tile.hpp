@@ -682,8 +682,17 @@ __device__ __forceinline__ void fixup_tile(bool valid, int t, const int4* __rest
     ts = (int)(((long long)r0 + c0.z) / tile_items);
   }
   if (valid && t - ts <= kShort) {
+    // in order; loads issued 8 at a time so a run is not one round trip per tile
     double acc = 0.0;
-    for (int s = ts; s < t; ++s) acc += carry[2 * s];
+    int s = ts;
+    for (; s + 8 <= t; s += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = carry[2 * (s + u)];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc += v[u];
+    }
+    for (; s < t; ++s) acc += carry[2 * s];
     P::store_f64(y + r0, alpha, acc + carry[2 * t + 1], beta, beta_zero != 0);
   }
   unsigned long long longs = __ballot(valid && t - ts > kShort);

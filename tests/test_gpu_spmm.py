@@ -95,3 +95,23 @@ def test_spmm_rmat(orc, device, k):
     Y = plan.spmm(d_rp, d_col, d_val, dev(X.T.copy(), device).t()).cpu().numpy()
     check_spmm(orc, A.row_ptr, A.col, val, X, Y)
     plan.close()
+
+
+@pytest.mark.parametrize("k", [5, 64, 70])
+def test_spmm_padded_columns(orc, device, k):
+    """Column-major X whose leading dimension is a multiple of 4 but whose
+    column count is not (the 16-byte repack path with a ragged tail), and
+    more right-hand sides than one repack block holds."""
+    from merbit_amd.spmv import MerbitPlan
+    rng = np.random.default_rng(70 + k)
+    ncol, ldx = 2501, 2504
+    rp, col = random_csr(rng, 1700, ncol, lens=(rng.pareto(1.1, 1700) * 4).astype(int))
+    val = rng.uniform(-1, 1, int(rp[-1])).astype(np.float32)
+    X = rng.uniform(-1, 1, (ncol, k)).astype(np.float32)
+    say(f"spmm padded columns k={k}")
+    Xbig = torch.full((k, ldx), float("nan"), dtype=torch.float32, device=device)
+    Xbig[:, :ncol] = dev(X.T.copy(), device)
+    plan = MerbitPlan(dev(rp, device), ncol, int(rp[-1]), items_per_thread=4)
+    Y = plan.spmm(dev(rp, device), dev(col, device), dev(val, device), Xbig[:, :ncol].t())
+    check_spmm(orc, rp, col, val, X, Y.cpu().numpy())
+    plan.close()
