@@ -271,12 +271,14 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* p, uin
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes,
                                            0x00020000 /* 32-bit data format */);
 }
+// The byte offset is formed in unsigned arithmetic (a column near 2^29 gives
+// an offset past INT_MAX; the hardware reads it as unsigned).
 __device__ __forceinline__ void buffer_gather(__amdgpu_buffer_rsrc_t r, int c, float& out) {
-  out = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, c << 2, 0, 0));
+  out = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)((unsigned)c << 2), 0, 0));
 }
 __device__ __forceinline__ void buffer_gather(__amdgpu_buffer_rsrc_t r, int c, double& out) {
   typedef unsigned v2u __attribute__((ext_vector_type(2)));
-  const v2u w = __builtin_amdgcn_raw_buffer_load_b64(r, c << 3, 0, 0);
+  const v2u w = __builtin_amdgcn_raw_buffer_load_b64(r, (int)((unsigned)c << 3), 0, 0);
   out = __hiloint2double((int)w.y, (int)w.x);
 }
 

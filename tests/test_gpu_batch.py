@@ -176,3 +176,32 @@ def test_autotuners(orc, device):
     mats = make_mats(device, 6, "f32", 3)
     ipt, us = autotune_batch(mats, reps=2)
     assert ipt in (2, 4, 6, 8, 12, 16) and us > 0
+
+
+def test_batch_unaligned_member(device):
+    """A batch with one matrix whose arrays are not 16-byte aligned runs the
+    scalar staging path for every matrix; results still equal single calls
+    bit for bit."""
+    from merbit_amd.batch import MerbitBatch
+    from merbit_amd.spmv import MerbitPlan
+    say("batch with an unaligned member")
+    mats = make_mats(device, 6, "f32", 3)
+    rp, col, val, x, y, ncol = mats[2]
+    # views starting one element in: 4-byte aligned, not 16-byte aligned
+    rp_u = torch.empty(rp.numel() + 1, dtype=rp.dtype, device=device)[1:]
+    rp_u.copy_(rp)
+    col_u = torch.empty(col.numel() + 1, dtype=col.dtype, device=device)[1:]
+    col_u.copy_(col)
+    val_u = torch.empty(val.numel() + 1, dtype=val.dtype, device=device)[1:]
+    val_u.copy_(val)
+    mats[2] = (rp_u, col_u, val_u, x, y, ncol)
+    b = MerbitBatch(mats, items_per_thread=8)
+    b.spmv()
+    torch.cuda.synchronize()
+    got = [m[4].clone() for m in mats]
+    for (rp, col, val, x, y, ncol), g in zip(mats, got):
+        plan = MerbitPlan(rp, ncol, col.numel(), items_per_thread=8)
+        ref = plan.spmv(rp, col, val, x)
+        plan.close()
+        assert torch.equal(ref.view(torch.uint8), g.view(torch.uint8))
+    b.close()
