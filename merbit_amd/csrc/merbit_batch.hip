@@ -142,12 +142,17 @@ struct mbx_batch {
 
 namespace {
 
+// Resident workgroups of the kernel variant the batch will launch.
 template <class P, int IPT>
-int batch_grid(int device, int max_rows) {
+int batch_grid(int device, bool vec) {
   int per_cu = 0, cus = 0;
   const size_t lds = TileSmem<256, IPT, P>::bytes();
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_spmv_kernel<256, IPT, true, P>,
-                                                   256, lds) != hipSuccess ||
+  const hipError_t e =
+      vec ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, batch_spmv_kernel<256, IPT, true, P>, 256, lds)
+          : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, batch_spmv_kernel<256, IPT, false, P>, 256, lds);
+  if (e != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
     return 0;
   return std::max(1, per_cu) * std::max(1, cus);
@@ -180,7 +185,7 @@ int batch_dispatch(const mbx_batch* b, hipStream_t s, double alpha, double beta,
   switch (b->ipt) {
 #define MBX_CASE(I)                                                  \
   case I:                                                           \
-    if (grid_only) return batch_grid<P, I>(device, b->max_rows);    \
+    if (grid_only) return batch_grid<P, I>(device, b->vec);         \
     batch_launch<P, I>(b, s, (S)alpha, (S)beta);                    \
     return 0;
     MBX_CASE(2) MBX_CASE(4) MBX_CASE(6) MBX_CASE(8) MBX_CASE(12) MBX_CASE(16)

@@ -7,13 +7,13 @@
 
 #include "merbit_amd.h"
 
-// The merge-path plan: one (rows consumed, nnz consumed) coordinate per tile
-// boundary of wg*ipt merge items, plus FP64 carry slots (tail, head) per tile
-// used by the SpMV carry fixup.
+// The merge-path plan: one (rows consumed, nnz consumed, row_ptr[rows])
+// coordinate per tile boundary of wg*ipt merge items, plus FP64 carry slots
+// (tail, head) per tile used by the SpMV carry fixup.
 struct mbx_plan {
   int32_t nrow, ncol, nnz;
   int32_t wg, ipt, tile_items, ntiles;
-  int32_t max_rows;    // most rows ending in one tile (sizes the LDS row ends)
+  int32_t max_rows;    // most rows ending in one tile (pattern statistic)
   int32_t split_rows;  // tiles whose first row began earlier (fixup needed)
   int4* d_coord;  // per tile boundary: rows, nnz consumed, row_ptr[rows]
   double* d_carry;
@@ -33,9 +33,9 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
   return j * q + min(j, r) + i;
 }
 
-// Merge-path search in LDS: first row r in [max(0,d-nnz_t), min(d,rows_t)]
-// whose local end s_re[r] exceeds d-r-1 (an nnz index is consumed before an
-// equal row end), include/utils.cuh:5-29.
+// Merge-path search in LDS (the SpMM and SDDMM tiles): first row r in
+// [max(0,d-nnz_t), min(d,rows_t)] whose local end s_re[r] exceeds d-r-1 (an
+// nnz index is consumed before an equal row end), include/utils.cuh:5-29.
 __device__ __forceinline__ int lds_merge_search(const int* s_re, int d, int nnz_t, int rows_t) {
   int lo = max(0, d - nnz_t), hi = min(d, rows_t);
   while (lo < hi) {

@@ -4,19 +4,20 @@
 //
 //   plan      partition_kernel: merge-path search of every tile boundary over
 //             (row ends row_ptr[1..nrow], nnz index 0..nnz-1); stores the
-//             (rows consumed, nnz consumed) coordinate of each tile start.
-//             Reference: merge_search include/utils.cuh:5-29 and the TILE plan
-//             include/tile.cuh:28-122 (which stores a per-32-lane descriptor;
-//             here only per-tile coordinates are kept and each thread finds
-//             its own start by a search in LDS).
+//             (rows consumed, nnz consumed, row_ptr[rows]) coordinate of each
+//             tile start.  Reference: merge_search include/utils.cuh:5-29 and
+//             the TILE plan include/tile.cuh:28-122 (which stores a per-32-lane
+//             descriptor; here only per-tile coordinates are kept and every
+//             thread finds its start from row-end marks inside the tile).
 //   tile      tile_spmv_kernel: one workgroup per tile of WG*IPT merge items.
-//             Stages the tile's row ends and its products val*x[col] in LDS,
-//             walks IPT merge items per thread, then a 64-lane segmented scan
-//             (shuffles + ballot) and a cross-wave scan finish the rows that
-//             end in the tile.  Rows wholly inside the tile are written to y
-//             with coalesced stores; the partial of the row open at the tile
-//             end (tail) and of a row that began in an earlier tile (head) go
-//             to per-tile carry slots.
+//             Loads the tile's row ends, columns and values in 16-byte
+//             groups, gathers x, stores the products val*x[col] in LDS and
+//             marks the merge position of every row end; each thread walks
+//             its IPT items, then DPP wave scans and a cross-wave step finish
+//             the rows that end in the tile.  Rows wholly inside the tile are
+//             written to y with coalesced stores; the partial of the row open
+//             at the tile end (tail) and of a row that began in an earlier
+//             tile (head) go to per-tile carry slots.
 //             Reference: merbit_kernel include/tile.cuh:476-561 (fast_track,
 //             load_smem, normal_track, warp_segmented_sum, load_mem).
 //   fixup     carry_fixup_kernel: for every row that spans tiles, sums the
@@ -28,7 +29,7 @@
 //
 // Value policies: F64 (plain FP64 sums, the parity path) and F32C / BF16C
 // (FP32 or bfloat16 values, FP32 x/y, TwoProduct + TwoSum compensated FP32
-// sums, FP64 carries).
+// sums per thread, FP64 partials across threads and tiles).
 
 #include <hip/hip_runtime.h>
 #include <limits.h>
