@@ -102,6 +102,16 @@ def load_traffic(name):
 # ---------------------------------------------------------------------------
 
 
+def pick_ipt(args, rp, col, val, x, y):
+    """Items per thread: --ipt when given, else the library's timing
+    autotuner (mbx_autotune_spmv) on these operands."""
+    if args.ipt:
+        return args.ipt
+    from merbit_amd.batch import autotune_spmv
+    ipt, _ = autotune_spmv(rp, col, val, x, y, reps=20)
+    return ipt
+
+
 def bench_rmat_f64(A, x, dev, args):
     import torch
     from merbit_amd import workloads
@@ -112,17 +122,19 @@ def bench_rmat_f64(A, x, dev, args):
                    val=torch.from_numpy(A.val).to(dev), x=torch.from_numpy(x).to(dev),
                    y=torch.empty(nrow, dtype=torch.float64, device=dev))
               for _ in range(max(1, args.copies))]
-    plan = MerbitPlan(copies[0]["rp"], ncol, nnz, items_per_thread=args.ipt)
+    c0 = copies[0]
+    ipt = pick_ipt(args, c0["rp"], c0["col"], c0["val"], c0["x"], c0["y"])
+    plan = MerbitPlan(c0["rp"], ncol, nnz, items_per_thread=ipt)
     calls = [(lambda c=c: plan.spmv(c["rp"], c["col"], c["val"], c["x"], c["y"]))
              for c in copies]
     ms, kern = time_steps(calls, args.steps, args.warmup, plan.set_events)
     warm_ms = warm_kern = None
     if not args.no_warm:
         warm_ms, warm_kern = time_steps(calls[:1], args.steps, args.warmup, plan.set_events)
-    log(f"rmat fp64: {ms * 1e3:.1f} us/step rotated ({gbs(b_alg, ms)} GB/s), tile kernel "
+    log(f"rmat fp64 (ipt {ipt}): {ms * 1e3:.1f} us/step rotated ({gbs(b_alg, ms)} GB/s), tile kernel "
         f"{kern * 1e3:.1f} us" + (f"; warm {warm_ms * 1e3:.1f} us" if warm_ms else ""))
     res = dict(b_alg=b_alg, ms=ms, kern=kern, warm_ms=warm_ms, warm_kern=warm_kern,
-               ntiles=plan.ntiles, tile_items=plan.tile_items)
+               ntiles=plan.ntiles, tile_items=plan.tile_items, ipt=ipt)
     plan.close()
     del copies
     return res
@@ -137,7 +149,6 @@ def bench_mixed(name, A, x, dev, args, f64_ms=None, copies=1):
     nrow, ncol, nnz = A.nrow, A.ncol, A.nnz
     rp = torch.from_numpy(A.row_ptr).to(dev)
     cols = [torch.from_numpy(A.col).to(dev) for _ in range(copies)]
-    plan = MerbitPlan(rp, ncol, nnz, items_per_thread=args.ipt)
     v32 = np.ascontiguousarray(A.val, dtype=np.float32)
     x32 = np.ascontiguousarray(x, dtype=np.float32)
     out = {}
@@ -145,12 +156,15 @@ def bench_mixed(name, A, x, dev, args, f64_ms=None, copies=1):
         vals = [torch.from_numpy(v32.astype(np.float64)).to(dev) for _ in range(copies)]
         xx = torch.from_numpy(x32.astype(np.float64)).to(dev)
         y = torch.empty(nrow, dtype=torch.float64, device=dev)
+        ipt = pick_ipt(args, rp, cols[0], vals[0], xx, y)
+        plan = MerbitPlan(rp, ncol, nnz, items_per_thread=ipt)
         calls = [(lambda i=i: plan.spmv(rp, cols[i], vals[i], xx, y)) for i in range(copies)]
         f64_ms, kern = time_steps(calls, args.steps, args.warmup, plan.set_events)
         b64 = workloads.algorithmic_bytes(nrow, ncol, nnz, 8, 8, 8)
         out["f64"] = {"gbs": gbs(b64, f64_ms), "us_per_step": round(f64_ms * 1e3, 2),
-                      "tile_kernel_us": round(kern * 1e3, 2),
+                      "tile_kernel_us": round(kern * 1e3, 2), "items_per_thread": ipt,
                       "pct_hbm_peak": round(100 * b64 / (f64_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 1)}
+        plan.close()
         del vals, y
     for kind in ("f32", "bf16"):
         stored = v32 if kind == "f32" else gen.to_bf16(v32)
@@ -159,6 +173,8 @@ def bench_mixed(name, A, x, dev, args, f64_ms=None, copies=1):
         vals = [torch.from_numpy(stored).to(dev) for _ in range(copies)]
         xx = torch.from_numpy(x32).to(dev)
         y = torch.empty(nrow, dtype=torch.float32, device=dev)
+        ipt = pick_ipt(args, rp, cols[0], vals[0], xx, y)
+        plan = MerbitPlan(rp, ncol, nnz, items_per_thread=ipt)
         calls = [(lambda i=i: plan.spmv(rp, cols[i], vals[i], xx, y)) for i in range(copies)]
         ms, kern = time_steps(calls, args.steps, args.warmup, plan.set_events)
         ref = plan.spmv(rp, cols[0], torch.from_numpy(wide).to(dev),
@@ -167,12 +183,12 @@ def bench_mixed(name, A, x, dev, args, f64_ms=None, copies=1):
         b = workloads.algorithmic_bytes(nrow, ncol, nnz, s_val, 4, 4)
         out[kind] = {"gbs": gbs(b, ms), "gflops": round(2 * nnz / (ms * 1e-3) / 1e9, 1),
                      "us_per_step": round(ms * 1e3, 2), "tile_kernel_us": round(kern * 1e3, 2),
-                     "speedup_vs_f64": round(f64_ms / ms, 3),
+                     "speedup_vs_f64": round(f64_ms / ms, 3), "items_per_thread": ipt,
                      "rel_err_vs_f64_same_values": rel_err(y, ref)}
-        log(f"{name} {kind}: {ms * 1e3:.1f} us/step, x{f64_ms / ms:.2f} vs fp64, "
+        log(f"{name} {kind}: {ms * 1e3:.1f} us/step (ipt {ipt}), x{f64_ms / ms:.2f} vs fp64, "
             f"err {out[kind]['rel_err_vs_f64_same_values']:.2e}")
+        plan.close()
         del vals, y, ref
-    plan.close()
     return out
 
 
@@ -300,7 +316,7 @@ def bench_adjoint(dev, args):
     val = torch.from_numpy(A.val).to(dev)
     xx = torch.from_numpy(x).to(dev)
     gg = torch.from_numpy(g).to(dev)
-    op = MerbitOperator(rp, col, A.ncol, items_per_thread=args.ipt)
+    op = MerbitOperator(rp, col, A.ncol, items_per_thread=args.ipt or 8)
     t0 = time.time()
     op.adjoint_handle  # builds the transposed pattern once
     torch.cuda.synchronize()
@@ -352,7 +368,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--ipt", type=int, default=8, help="items per thread (SIGMA)")
+    ap.add_argument("--ipt", type=int, default=0,
+                    help="items per thread (SIGMA); 0 = picked by the timing autotuner")
     ap.add_argument("--copies", type=int, default=3)
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--batch-count", type=int, default=64)
@@ -428,7 +445,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "fp64 merge SpMV, R-MAT power law 2^20 rows 2^24 nnz "
                                "(a=.57,b=c=.19), 3 rotated copies",
-                   "nrow": A.nrow, "nnz": A.nnz, "items_per_thread": args.ipt,
+                   "nrow": A.nrow, "nnz": A.nnz, "items_per_thread": h["ipt"],
                    "wg_size": 256, "copies_rotated": max(1, args.copies)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
