@@ -102,14 +102,24 @@ def load_traffic(name):
 # ---------------------------------------------------------------------------
 
 
-def pick_ipt(args, rp, col, val, x, y):
-    """Items per thread: --ipt when given, else the library's timing
-    autotuner (mbx_autotune_spmv) on these operands."""
+def pick_ipt(args, rp, ncol, nnz, make_calls):
+    """Items per thread: --ipt when given, else the candidate whose plan runs
+    the timed workload (make_calls(plan): the rotated products) fastest over
+    a short timing of each.  Tuning on the rotated copies matters: on one
+    copy resident in the 256 MB cache the candidates 8, 12 and 16 time alike
+    on the R-MAT matrix, while rotated copies separate them by 5 %."""
     if args.ipt:
         return args.ipt
-    from merbit_amd.batch import autotune_spmv
-    ipt, _ = autotune_spmv(rp, col, val, x, y, reps=20)
-    return ipt
+    from merbit_amd.spmv import ITEMS_PER_THREAD, MerbitPlan
+    best = None
+    for ipt in ITEMS_PER_THREAD:
+        plan = MerbitPlan(rp, ncol, nnz, items_per_thread=ipt)
+        calls = make_calls(plan)
+        ms = min(time_steps(calls, 10 * len(calls), 2 * len(calls))[0] for _ in range(3))
+        plan.close()
+        if best is None or ms < best[1]:
+            best = (ipt, ms)
+    return best[0]
 
 
 def bench_rmat_f64(A, x, dev, args):
@@ -123,10 +133,14 @@ def bench_rmat_f64(A, x, dev, args):
                    y=torch.empty(nrow, dtype=torch.float64, device=dev))
               for _ in range(max(1, args.copies))]
     c0 = copies[0]
-    ipt = pick_ipt(args, c0["rp"], c0["col"], c0["val"], c0["x"], c0["y"])
+
+    def make_calls(plan):
+        return [(lambda c=c: plan.spmv(c["rp"], c["col"], c["val"], c["x"], c["y"]))
+                for c in copies]
+
+    ipt = pick_ipt(args, c0["rp"], ncol, nnz, make_calls)
     plan = MerbitPlan(c0["rp"], ncol, nnz, items_per_thread=ipt)
-    calls = [(lambda c=c: plan.spmv(c["rp"], c["col"], c["val"], c["x"], c["y"]))
-             for c in copies]
+    calls = make_calls(plan)
     ms, kern = time_steps(calls, args.steps, args.warmup, plan.set_events)
     warm_ms = warm_kern = None
     if not args.no_warm:
@@ -156,9 +170,12 @@ def bench_mixed(name, A, x, dev, args, f64_ms=None, copies=1):
         vals = [torch.from_numpy(v32.astype(np.float64)).to(dev) for _ in range(copies)]
         xx = torch.from_numpy(x32.astype(np.float64)).to(dev)
         y = torch.empty(nrow, dtype=torch.float64, device=dev)
-        ipt = pick_ipt(args, rp, cols[0], vals[0], xx, y)
+        def make_calls(plan, vals=vals, xx=xx, y=y):
+            return [(lambda i=i: plan.spmv(rp, cols[i], vals[i], xx, y)) for i in range(copies)]
+
+        ipt = pick_ipt(args, rp, ncol, nnz, make_calls)
         plan = MerbitPlan(rp, ncol, nnz, items_per_thread=ipt)
-        calls = [(lambda i=i: plan.spmv(rp, cols[i], vals[i], xx, y)) for i in range(copies)]
+        calls = make_calls(plan)
         f64_ms, kern = time_steps(calls, args.steps, args.warmup, plan.set_events)
         b64 = workloads.algorithmic_bytes(nrow, ncol, nnz, 8, 8, 8)
         out["f64"] = {"gbs": gbs(b64, f64_ms), "us_per_step": round(f64_ms * 1e3, 2),
@@ -173,9 +190,12 @@ def bench_mixed(name, A, x, dev, args, f64_ms=None, copies=1):
         vals = [torch.from_numpy(stored).to(dev) for _ in range(copies)]
         xx = torch.from_numpy(x32).to(dev)
         y = torch.empty(nrow, dtype=torch.float32, device=dev)
-        ipt = pick_ipt(args, rp, cols[0], vals[0], xx, y)
+        def make_calls(plan, vals=vals, xx=xx, y=y):
+            return [(lambda i=i: plan.spmv(rp, cols[i], vals[i], xx, y)) for i in range(copies)]
+
+        ipt = pick_ipt(args, rp, ncol, nnz, make_calls)
         plan = MerbitPlan(rp, ncol, nnz, items_per_thread=ipt)
-        calls = [(lambda i=i: plan.spmv(rp, cols[i], vals[i], xx, y)) for i in range(copies)]
+        calls = make_calls(plan)
         ms, kern = time_steps(calls, args.steps, args.warmup, plan.set_events)
         ref = plan.spmv(rp, cols[0], torch.from_numpy(wide).to(dev),
                         torch.from_numpy(x32.astype(np.float64)).to(dev))
