@@ -139,6 +139,12 @@ def bench_rmat_f64(A, x, dev, args):
                 for c in copies]
 
     ipt = pick_ipt(args, c0["rp"], ncol, nnz, make_calls)
+    plan_ms = []
+    for _ in range(5):  # plan construction (the reference's tim1), host clock
+        t0 = time.perf_counter()
+        plan = MerbitPlan(c0["rp"], ncol, nnz, items_per_thread=ipt)
+        plan_ms.append((time.perf_counter() - t0) * 1e3)
+        plan.close()
     plan = MerbitPlan(c0["rp"], ncol, nnz, items_per_thread=ipt)
     calls = make_calls(plan)
     ms, kern = time_steps(calls, args.steps, args.warmup, plan.set_events)
@@ -148,7 +154,8 @@ def bench_rmat_f64(A, x, dev, args):
     log(f"rmat fp64 (ipt {ipt}): {ms * 1e3:.1f} us/step rotated ({gbs(b_alg, ms)} GB/s), tile kernel "
         f"{kern * 1e3:.1f} us" + (f"; warm {warm_ms * 1e3:.1f} us" if warm_ms else ""))
     res = dict(b_alg=b_alg, ms=ms, kern=kern, warm_ms=warm_ms, warm_kern=warm_kern,
-               ntiles=plan.ntiles, tile_items=plan.tile_items, ipt=ipt)
+               ntiles=plan.ntiles, tile_items=plan.tile_items, ipt=ipt,
+               plan_ms=float(np.median(plan_ms)))
     plan.close()
     del copies
     return res
@@ -287,8 +294,9 @@ def bench_batch(dev, args):
 
 
 def bench_rocsparse(A, x, dev, args, f64_ms):
-    """rocSPARSE CSR SpMV (FP64) on the same R-MAT copies: a measured
-    comparator, not part of the product path."""
+    """rocSPARSE SpMV (FP64; CSR adaptive and LRB, COO segmented and atomic)
+    on the same R-MAT copies: a measured comparator, not part of the product
+    path."""
     import torch
     from merbit_amd import workloads
     from merbit_amd.compare import RocsparseSpmv
@@ -299,7 +307,7 @@ def bench_rocsparse(A, x, dev, args, f64_ms):
                    y=torch.empty(nrow, dtype=torch.float64, device=dev))
               for _ in range(max(1, args.copies))]
     out = {}
-    for alg in ("adaptive", "lrb"):
+    for alg in ("adaptive", "lrb", "coo", "coo_atomic"):
         try:
             t0 = time.time()
             ops = [RocsparseSpmv(c["rp"], c["col"], c["val"], c["x"], c["y"], alg)
@@ -375,12 +383,15 @@ def cpu_baseline(A, x, b_alg):
     threads = max(1, min(threads, os.cpu_count() or 1))
     t0 = time.time()
     t, used, _ = oracle.spmv_csr_omp_time(A.row_ptr, A.col, A.val, x, threads,
-                                          warmup=2, reps=10)
-    log(f"cpu baseline {t * 1e3:.2f} ms/SpMV on {used} threads ({time.time() - t0:.1f}s)")
+                                          warmup=3, reps=20)
+    t1, _, _ = oracle.spmv_csr_omp_time(A.row_ptr, A.col, A.val, x, 1, warmup=3, reps=20)
+    log(f"cpu baseline {t * 1e3:.2f} ms/SpMV on {used} threads, {t1 * 1e3:.2f} ms on 1 "
+        f"({time.time() - t0:.1f}s)")
     return {"value": round(b_alg / t / 1e9, 3), "unit": "GB/s", "cores": used,
             "kind": "port",
             "sample": "full R-MAT FP64 product (2^20 rows, 2^24 nnz), OpenMP row-parallel "
-                      "CSR, median of 10 after 2 warm-ups", "ms_per_spmv": round(t * 1e3, 3)}
+                      "CSR, median of 20 after 3 warm-ups", "ms_per_spmv": round(t * 1e3, 3),
+            "one_thread": {"value": round(b_alg / t1 / 1e9, 3), "ms_per_spmv": round(t1 * 1e3, 3)}}
 
 
 def main():
@@ -432,6 +443,7 @@ def main():
             "gbs": gbs(b_alg, h["warm_ms"]), "us_per_step": round(h["warm_ms"] * 1e3, 2),
             "tile_kernel_us": round(h["warm_kern"] * 1e3, 2)},
         "ntiles": h["ntiles"], "tile_items": h["tile_items"],
+        "plan_create_ms": round(h["plan_ms"], 3),
     }
     if args.only == "all":
         extras["mixed_rmat"] = bench_mixed("rmat", A, x, dev, args, f64_ms=ms, copies=3)

@@ -147,6 +147,26 @@ int mbx_spmv_graph_create(const mbx_plan* plan, int32_t value_kind, const int32_
 int mbx_graph_launch(void* graph_exec, void* stream);
 void mbx_graph_destroy(void* graph_exec);
 
+/* PageRank power iterations on the reference's operated matrix (CSR of P^T,
+ * values 1/out-degree: mbx_ingest_ref's output), the use of PARAMETER::damp
+ * (include/class.cuh:119-146) that the reference's loop (include/tile.cuh:
+ * 596-603, y = A*x repeated with x fixed) leaves out.  Each iteration is
+ *   y = damp*A*x;  x' = y + (sum(x) - sum(y))/n
+ * (teleport plus the damped mass of dangling nodes, spread evenly; the mass
+ * sum(x) is kept).  The square plan's product runs `iters` times from d_x0
+ * (not modified) and leaves the last iterate in d_y.  d_work: 256-byte
+ * aligned device memory of mbx_pagerank_workspace_bytes(nrow) bytes.  FP64;
+ * the sums are deterministic.  The graph variant captures all iterations as
+ * one HIP graph for mbx_graph_launch. */
+size_t mbx_pagerank_workspace_bytes(int32_t nrow);
+int mbx_pagerank_f64(const mbx_plan* plan, const int32_t* d_row_ptr, const int32_t* d_col,
+                     const double* d_val, double damp, int32_t iters, const double* d_x0,
+                     double* d_y, void* d_work, size_t work_bytes, void* stream);
+int mbx_pagerank_graph_create(const mbx_plan* plan, const int32_t* d_row_ptr,
+                              const int32_t* d_col, const double* d_val, double damp,
+                              int32_t iters, const double* d_x0, double* d_y, void* d_work,
+                              size_t work_bytes, void** graph_exec);
+
 /* Pattern statistics of a planned matrix (features for choosing the tile
  * shape): row-length moments about the mean and the empty-row count as the
  * reference's GRAPH::desc / GRAPH::dangling report them (src/class.cu:181-320),

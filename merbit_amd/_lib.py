@@ -22,6 +22,7 @@ ABI_SYMBOLS = (
     "mbx_autotune_spmv", "mbx_autotune_spmm", "mbx_autotune_batch",
     "mbx_adjoint_create", "mbx_adjoint_destroy", "mbx_adjoint_transpose", "mbx_adjoint_dx",
     "mbx_sddmm", "mbx_ingest_ref", "mbx_plan_stats",
+    "mbx_pagerank_workspace_bytes", "mbx_pagerank_f64", "mbx_pagerank_graph_create",
     "mbx_status_str",
 )
 GEN_SYMBOLS = (
@@ -117,6 +118,11 @@ def lib() -> ctypes.CDLL:
     L.mbx_sddmm.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp]
     L.mbx_ingest_ref.argtypes = [i32, i32, vp, vp, i32, vp, vp, vp, vp]
     L.mbx_plan_stats.argtypes = [vp, vp, ctypes.POINTER(PatternStats)]
+    L.mbx_pagerank_workspace_bytes.argtypes = [i32]
+    L.mbx_pagerank_workspace_bytes.restype = ctypes.c_size_t
+    L.mbx_pagerank_f64.argtypes = [vp, vp, vp, vp, f64, i32, vp, vp, vp, ctypes.c_size_t, vp]
+    L.mbx_pagerank_graph_create.argtypes = [vp, vp, vp, vp, f64, i32, vp, vp, vp,
+                                            ctypes.c_size_t, ctypes.POINTER(vp)]
     L.mbx_status_str.argtypes = [ctypes.c_int]
     L.mbx_status_str.restype = ctypes.c_char_p
     L.mbg_fill_uniform_f64.argtypes = [vp, i64, f64, f64, u64]

@@ -1,6 +1,6 @@
 """Measured comparators for the benchmark (not used by the product path).
 
-``RocsparseSpmv`` wraps rocSPARSE's CSR SpMV (libmerbit_cmp.so, built from
+``RocsparseSpmv`` wraps rocSPARSE's CSR and COO SpMV (libmerbit_cmp.so, built from
 rocsparse_cmp.cpp) on device tensors; the benchmark times it beside the
 merge-path kernel on the same arrays, as the reference times its vendor
 library rows (src/spmv.cu:6-64, 124-157)."""
@@ -15,7 +15,7 @@ from .._lib import MBX_VALUE_F32, MBX_VALUE_F64, MerbitError
 
 _PATH = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                      "libmerbit_cmp.so")
-_ALGS = {"adaptive": 0, "lrb": 1, "rowsplit": 2, "nnzsplit": 3}
+_ALGS = {"adaptive": 0, "lrb": 1, "rowsplit": 2, "nnzsplit": 3, "coo": 4, "coo_atomic": 5}
 _L = None
 
 
@@ -40,7 +40,9 @@ def _lib():
 
 
 class RocsparseSpmv:
-    """y = A x by rocSPARSE CSR SpMV (FP64 or FP32), preprocessed once."""
+    """y = A x by rocSPARSE SpMV (FP64 or FP32), preprocessed once.  ``alg``:
+    CSR adaptive / lrb / rowsplit / nnzsplit, or COO coo / coo_atomic (COO
+    row indices converted from row_ptr at construction)."""
 
     def __init__(self, row_ptr: torch.Tensor, col: torch.Tensor, val: torch.Tensor,
                  x: torch.Tensor, y: torch.Tensor, alg: str = "adaptive"):

@@ -3,7 +3,9 @@
 // beside the merge-path kernel on the same device arrays, in place of the
 // reference's vendor-library rows (src/spmv.cu:6-64 Fun_CSRMV, 124-157
 // Fun_MERGEMV).  The analysis / preprocessing stage runs once at create, as
-// the reference does before its timed loop.
+// the reference does before its timed loop.  The COO algorithms (the
+// reference's Fun_COOMV row, spmv.cu:69-118, which runs on its sorted COO
+// arrays) take their row indices from rocsparse_csr2coo, run once at create.
 #include <hip/hip_runtime.h>
 #include <rocsparse/rocsparse.h>
 
@@ -15,7 +17,14 @@
 extern "C" {
 
 // Algorithm ids accepted by mbc_rocsparse_create.
-enum { MBC_ADAPTIVE = 0, MBC_LRB = 1, MBC_ROWSPLIT = 2, MBC_NNZSPLIT = 3 };
+enum {
+  MBC_ADAPTIVE = 0,
+  MBC_LRB = 1,
+  MBC_ROWSPLIT = 2,
+  MBC_NNZSPLIT = 3,
+  MBC_COO = 4,         // segmented COO (deterministic)
+  MBC_COO_ATOMIC = 5,  // atomic COO (summation order not fixed)
+};
 
 struct mbc_rocsparse {
   rocsparse_handle handle = nullptr;
@@ -25,6 +34,7 @@ struct mbc_rocsparse {
   rocsparse_spmv_alg alg = rocsparse_spmv_alg_csr_adaptive;
   void* buffer = nullptr;
   size_t buffer_bytes = 0;
+  int32_t* coo_row = nullptr;  // COO algorithms: row index of every entry
 };
 
 void mbc_rocsparse_destroy(mbc_rocsparse* h);
@@ -71,15 +81,34 @@ int mbc_rocsparse_create(int32_t value_kind, int32_t alg, int32_t nrow, int32_t 
     case MBC_LRB: h->alg = rocsparse_spmv_alg_csr_lrb; break;
     case MBC_ROWSPLIT: h->alg = rocsparse_spmv_alg_csr_rowsplit; break;
     case MBC_NNZSPLIT: h->alg = rocsparse_spmv_alg_csr_nnzsplit; break;
+    case MBC_COO: h->alg = rocsparse_spmv_alg_coo; break;
+    case MBC_COO_ATOMIC: h->alg = rocsparse_spmv_alg_coo_atomic; break;
     default: return fail(h, MBX_EUNSUPPORTED);
   }
+  const bool coo = alg == MBC_COO || alg == MBC_COO_ATOMIC;
   if (rocsparse_create_handle(&h->handle) != rocsparse_status_success) return fail(h, MBX_EHIP);
   if (rocsparse_set_stream(h->handle, (hipStream_t)stream) != rocsparse_status_success)
     return fail(h, MBX_EHIP);
-  if (rocsparse_create_csr_descr(&h->mat, nrow, ncol, nnz, const_cast<int32_t*>(row_ptr),
-                                 const_cast<int32_t*>(col), const_cast<void*>(val),
-                                 rocsparse_indextype_i32, rocsparse_indextype_i32,
-                                 rocsparse_index_base_zero, h->type) != rocsparse_status_success ||
+  if (coo) {
+    if (hipMalloc((void**)&h->coo_row, sizeof(int32_t) * (nnz > 0 ? (size_t)nnz : 1)) !=
+        hipSuccess) {
+      h->coo_row = nullptr;
+      return fail(h, MBX_ENOMEM);
+    }
+    if (nnz > 0 && rocsparse_csr2coo(h->handle, row_ptr, nnz, nrow, h->coo_row,
+                                     rocsparse_index_base_zero) != rocsparse_status_success)
+      return fail(h, MBX_EHIP);
+  }
+  const rocsparse_status mst =
+      coo ? rocsparse_create_coo_descr(&h->mat, nrow, ncol, nnz, h->coo_row,
+                                       const_cast<int32_t*>(col), const_cast<void*>(val),
+                                       rocsparse_indextype_i32, rocsparse_index_base_zero,
+                                       h->type)
+          : rocsparse_create_csr_descr(&h->mat, nrow, ncol, nnz, const_cast<int32_t*>(row_ptr),
+                                       const_cast<int32_t*>(col), const_cast<void*>(val),
+                                       rocsparse_indextype_i32, rocsparse_indextype_i32,
+                                       rocsparse_index_base_zero, h->type);
+  if (mst != rocsparse_status_success ||
       rocsparse_create_dnvec_descr(&h->x, ncol, const_cast<void*>(x), h->type) !=
           rocsparse_status_success ||
       rocsparse_create_dnvec_descr(&h->y, nrow, y, h->type) != rocsparse_status_success)
@@ -115,6 +144,7 @@ void mbc_rocsparse_destroy(mbc_rocsparse* h) {
   if (h->mat) rocsparse_destroy_spmat_descr(h->mat);
   if (h->handle) rocsparse_destroy_handle(h->handle);
   if (h->buffer) (void)hipFree(h->buffer);
+  if (h->coo_row) (void)hipFree(h->coo_row);
   delete h;
 }
 

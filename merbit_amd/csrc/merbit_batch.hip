@@ -218,31 +218,6 @@ bool ipt_ok(int ipt) {
   return ipt == 2 || ipt == 4 || ipt == 6 || ipt == 8 || ipt == 12 || ipt == 16;
 }
 
-// Capture `repeats` calls of fn(stream) into an executable graph.
-template <class F>
-int capture_graph(int repeats, F fn, void** exec_out) {
-  if (!exec_out || repeats < 1) return MBX_EINVAL;
-  *exec_out = nullptr;
-  hipStream_t cs = nullptr;
-  if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return MBX_EHIP;
-  int st = MBX_OK;
-  hipGraph_t g = nullptr;
-  if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-    st = MBX_EHIP;
-  } else {
-    for (int i = 0; i < repeats && st == MBX_OK; ++i) st = fn(cs);
-    if (hipStreamEndCapture(cs, &g) != hipSuccess && st == MBX_OK) st = MBX_EHIP;
-  }
-  hipGraphExec_t exec = nullptr;
-  if (st == MBX_OK && hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) != hipSuccess)
-    st = MBX_EHIP;
-  if (g) (void)hipGraphDestroy(g);
-  (void)hipStreamDestroy(cs);
-  if (st == MBX_OK) *exec_out = exec;
-  (void)hipGetLastError();
-  return st;
-}
-
 }  // namespace
 
 extern "C" {
@@ -366,8 +341,8 @@ int mbx_batch_spmv(const mbx_batch* b, double alpha, double beta, void* stream) 
 int mbx_batch_graph_create(const mbx_batch* b, double alpha, double beta, int32_t repeats,
                            void** graph_exec) {
   if (!b) return MBX_EINVAL;
-  return capture_graph(repeats, [&](hipStream_t s) { return mbx_batch_spmv(b, alpha, beta, s); },
-                       graph_exec);
+  return mbx_detail::capture_graph(
+      repeats, [&](hipStream_t s) { return mbx_batch_spmv(b, alpha, beta, s); }, graph_exec);
 }
 
 int mbx_spmv_graph_create(const mbx_plan* plan, int32_t value_kind, const int32_t* d_row_ptr,
@@ -390,7 +365,7 @@ int mbx_spmv_graph_create(const mbx_plan* plan, int32_t value_kind, const int32_
         return MBX_EINVAL;
     }
   };
-  return capture_graph(repeats, fn, graph_exec);
+  return mbx_detail::capture_graph(repeats, fn, graph_exec);
 }
 
 int mbx_graph_launch(void* graph_exec, void* stream) {

@@ -68,4 +68,29 @@ inline bool plan_csr_args_ok(const mbx_plan* p, const void* rp, const void* col,
   return true;
 }
 
+// Capture `repeats` calls of fn(stream) into an executable graph.
+template <class F>
+inline int capture_graph(int repeats, F fn, void** exec_out) {
+  if (!exec_out || repeats < 1) return MBX_EINVAL;
+  *exec_out = nullptr;
+  hipStream_t cs = nullptr;
+  if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return MBX_EHIP;
+  int st = MBX_OK;
+  hipGraph_t g = nullptr;
+  if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    st = MBX_EHIP;
+  } else {
+    for (int i = 0; i < repeats && st == MBX_OK; ++i) st = fn(cs);
+    if (hipStreamEndCapture(cs, &g) != hipSuccess && st == MBX_OK) st = MBX_EHIP;
+  }
+  hipGraphExec_t exec = nullptr;
+  if (st == MBX_OK && hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) != hipSuccess)
+    st = MBX_EHIP;
+  if (g) (void)hipGraphDestroy(g);
+  (void)hipStreamDestroy(cs);
+  if (st == MBX_OK) *exec_out = exec;
+  (void)hipGetLastError();
+  return st;
+}
+
 }  // namespace mbx_detail

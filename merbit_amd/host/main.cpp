@@ -2,13 +2,16 @@
 //
 //   merbit_main <ID> [--config FILE] [--log FILE] [--device N] [--type f64|f32]
 //               [--sigma 4|6|8|12|16] [--perrnd N] [--maxiter N] [--check]
+//               [--pagerank [--damp D]]
 //   merbit_main --mtx FILE [...]
 //
 // Loads graph <ID> from the registry (lines "ID path [nrow ncol nnz]"), builds
 // the operated matrix with the reference's semantics, runs Fun_Merbit
 // (maxiter*perrnd SpMVs, default 20*20 = 400 as src/main.cu:32) and appends
 // " gid&MERBIT&sigma&tim1&tim2" to the log.  --check verifies y against a
-// sequential FP64 CSR product on the host (Fun_Check).
+// sequential FP64 CSR product on the host (Fun_Check).  --pagerank runs
+// Fun_PageRank instead: maxiter damped power iterations (FP64), checked
+// against the same iterations on the host.
 
 #include <cstdlib>
 #include <cstring>
@@ -26,19 +29,69 @@ struct Args {
   std::string mtx;
   std::string type = "f64";
   int device = 0, sigma = 8, perrnd = 20, maxiter = 20;
-  bool check = false;
+  double damp = 0.85;
+  bool check = false, pagerank = false;
 };
 
 int usage() {
   std::cerr << "usage: merbit_main <ID> [--config FILE] [--log FILE] [--device N] "
                "[--type f64|f32] [--sigma 4|6|8|12|16] [--perrnd N] [--maxiter N] [--check]\n"
+               "                   [--pagerank [--damp D]]\n"
                "       merbit_main --mtx FILE [same options]\n";
   return 2;
 }
 
+// PageRank iterations on the host, in the order the device computes them
+// mathematically (y = damp*A*x, then the uniform correction); the check's
+// tolerance absorbs the different summation order.
+std::vector<double> host_pagerank(const GRAPH<double>& g, const std::vector<double>& x0,
+                                  double damp, int iters) {
+  std::vector<double> x = x0, y((size_t)g.nrow);
+  for (int t = 0; t < iters; ++t) {
+    double sx = 0.0, sy = 0.0;
+    for (int r = 0; r < g.nrow; ++r) {
+      double s = 0.0;
+      for (int k = g.h_offset[r]; k < g.h_offset[r + 1]; ++k)
+        s += g.h_values[k] * x[(size_t)g.h_col[k]];
+      y[(size_t)r] = damp * s;
+      sx += x[(size_t)r];
+      sy += y[(size_t)r];
+    }
+    const double c = (sx - sy) / (double)g.nrow;
+    for (int r = 0; r < g.nrow; ++r) x[(size_t)r] = y[(size_t)r] + c;
+  }
+  return x;
+}
+
+template <unsigned S>
+int run_pagerank(const ENV& env, GRAPH<double>& graph, const Args& a) {
+  PARAMETER<double> para(&graph, a.damp, a.perrnd, a.maxiter);
+  const MerbitTimes t = Fun_PageRank<256, S>(env, para);
+  std::cout << "graph = " << graph.graph_id << ", PageRank damp = " << a.damp
+            << ", iterations = " << a.maxiter << ", nrow = " << graph.nrow
+            << ", nnz = " << graph.nnz << "\n"
+            << "tim1 = " << t.plan_ms << " ms, " << t.spmv_ms << " ms/iteration\n";
+  if (!a.check) return 0;
+  std::vector<double> y((size_t)graph.nrow);
+  merbit::hip_check(hipMemcpy(y.data(), para.pagerank_real, sizeof(double) * y.size(),
+                              hipMemcpyDeviceToHost),
+                    "hipMemcpy");
+  const std::vector<double> ref =
+      host_pagerank(graph, std::vector<double>((size_t)graph.nrow, 1.0), a.damp, a.maxiter);
+  const int bad = Fun_Check<double>(ref.data(), y.data(), graph.nrow, 1e-12);
+  std::cout << (bad ? "CHECK FAILED" : "check ok") << " (" << bad << " rows)\n";
+  return bad ? 1 : 0;
+}
+
 template <typename V, unsigned S>
 int run_sigma(const ENV& env, GRAPH<V>& graph, const Args& a) {
-  PARAMETER<V> para(&graph, 0.85, a.perrnd, a.maxiter);
+  if constexpr (std::is_same<V, double>::value) {
+    if (a.pagerank) return run_pagerank<S>(env, graph, a);
+  } else if (a.pagerank) {
+    std::cerr << "--pagerank is FP64 only\n";
+    return 2;
+  }
+  PARAMETER<V> para(&graph, a.damp, a.perrnd, a.maxiter);
   const MerbitTimes t = Fun_Merbit<V, 256, S>(env, para);
   const double bytes = 4.0 * (graph.nrow + 1) + graph.nnz * (4.0 + sizeof(V)) +
                        2.0 * graph.nrow * sizeof(V);
@@ -93,6 +146,8 @@ int main(int argc, char** argv) {
     else if (s == "--perrnd" && (v = next())) a.perrnd = std::atoi(v);
     else if (s == "--maxiter" && (v = next())) a.maxiter = std::atoi(v);
     else if (s == "--check") a.check = true;
+    else if (s == "--pagerank") a.pagerank = true;
+    else if (s == "--damp" && (v = next())) a.damp = std::atof(v);
     else if (!s.empty() && s[0] != '-' && a.gid < 0) a.gid = std::atoi(s.c_str());
     else return usage();
   }

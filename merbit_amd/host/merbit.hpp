@@ -298,6 +298,44 @@ MerbitTimes Fun_Merbit(const ENV& env, const PARAMETER<V>& para) {
   return t;
 }
 
+// PageRank with the damping factor the reference carries in PARAMETER::damp
+// but does not apply (include/class.cuh:119-146): para.maxiter power
+// iterations y = damp*A*x, x' = y + (sum(x) - sum(y))/n from x0 = para.person,
+// captured as one HIP graph; the last iterate lands in para.pagerank_real.
+// FP64 only.  Returns the plan time and the time per iteration.
+template <unsigned BLOCK, unsigned SIGMA>
+MerbitTimes Fun_PageRank(const ENV& env, const PARAMETER<double>& para) {
+  Timer timer;
+  MerbitTimes t;
+  const GRAPH<double>& g = *para.graph;
+  timer.start(env.stream);
+  TILE<BLOCK, SIGMA> tile(g.offset, g.nrow, g.nnz, env.stream);
+  t.plan_ms = timer.stop(env.stream);
+  if (para.maxiter < 1 || g.nrow == 0) return t;
+  const size_t wbytes = mbx_pagerank_workspace_bytes(g.nrow);
+  void* work = nullptr;
+  merbit::hip_check(hipMalloc(&work, wbytes), "hipMalloc");
+  void* graph = nullptr;
+  const int st = mbx_pagerank_graph_create(tile.plan, g.offset, g.col, g.values, para.damp,
+                                           para.maxiter, para.person, para.pagerank_real, work,
+                                           wbytes, &graph);
+  if (st != MBX_OK) {
+    merbit::device_free(work);
+    merbit::check(st, "PageRank graph capture");
+  }
+  timer.start(env.stream);
+  const int lst = mbx_graph_launch(graph, env.stream);
+  const float ms = timer.stop(env.stream);
+  mbx_graph_destroy(graph);
+  merbit::device_free(work);
+  merbit::check(lst, "graph launch");
+  t.spmv_ms = ms / (float)para.maxiter;
+  if (FSTR.is_open())
+    FSTR << " " << g.graph_id << "&PAGERANK&" << SIGMA << "&" << t.plan_ms << "&" << t.spmv_ms
+         << "\n";
+  return t;
+}
+
 // include/utils.cuh:116-132: flags rows with |X-Y| > rtol*|X|, prints the
 // first few and the total flagged error; returns the number of flagged rows.
 template <typename V>

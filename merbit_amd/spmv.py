@@ -121,6 +121,34 @@ class MerbitPlan:
         check(st, f"mbx_spmv_{kind}")
         return y
 
+    def pagerank(self, row_ptr: torch.Tensor, col: torch.Tensor, val: torch.Tensor,
+                 damp: float = 0.85, iters: int = 20, x0: torch.Tensor | None = None,
+                 y: torch.Tensor | None = None, stream=None) -> torch.Tensor:
+        """PageRank power iterations (mbx_pagerank_f64) on this square plan's
+        FP64 matrix, the CSR of P^T with values 1/out-degree
+        (PARAMETER::damp of include/class.cuh:119-146 put to use):
+        y = damp*A*x, x' = y + (sum(x) - sum(y))/n, `iters` times from x0
+        (default uniform 1/n).  Returns the last iterate."""
+        if self.nrow != self.ncol:
+            raise ValueError("PageRank needs a square matrix")
+        dev = row_ptr.device
+        if x0 is None:
+            x0 = torch.full((self.nrow,), 1.0 / max(self.nrow, 1), dtype=torch.float64,
+                            device=dev)
+        if x0.numel() != self.nrow or col.numel() != self.nnz or val.numel() != self.nnz:
+            raise ValueError("arrays do not match the plan")
+        if y is None:
+            y = torch.empty(self.nrow, dtype=torch.float64, device=dev)
+        nbytes = int(lib().mbx_pagerank_workspace_bytes(self.nrow))
+        work = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        check(lib().mbx_pagerank_f64(self.handle, _dptr(row_ptr, torch.int32, "row_ptr"),
+                                     _dptr(col, torch.int32, "col"),
+                                     _dptr(val, torch.float64, "val"), float(damp), int(iters),
+                                     _dptr(x0, torch.float64, "x0"),
+                                     _dptr(y, torch.float64, "y"), work.data_ptr(), nbytes,
+                                     _stream_handle(stream)), "mbx_pagerank_f64")
+        return y
+
     @staticmethod
     def _layout(t: torch.Tensor, name: str):
         """(layout, leading dimension) of a 2-D tensor viewed as n x k."""

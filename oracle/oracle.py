@@ -159,6 +159,21 @@ def csr_transpose(row_ptr, col, ncol: int):
     return cp, ri[:nnz], perm[:nnz]
 
 
+def pagerank(row_ptr, col, val, damp: float, iters: int, x0):
+    """Damped power iterations on the CSR of P^T (values 1/out-degree), the
+    use of PARAMETER::damp (include/class.cuh:119-146) that the reference's
+    loop leaves out: y = damp*A*x (Dot2-accurate rows), x' = y + (sum(x) -
+    sum(y))/n with correctly rounded sums (math.fsum).  Pinned in the tests
+    by the fixed point from a dense linear solve and by mass conservation."""
+    import math
+    x = np.ascontiguousarray(x0, dtype=np.float64).copy()
+    n = len(x)
+    for _ in range(int(iters)):
+        y = damp * spmv_accurate(row_ptr, col, val, x)[0]
+        x = y + (math.fsum(x) - math.fsum(y)) / n
+    return x
+
+
 def spmv_csr_omp_time(row_ptr, col, val, x, nthreads: int, warmup=3, reps=20):
     """Median seconds of the OpenMP FP64 CSR baseline and threads used."""
     rp = _c(row_ptr, np.int32)

@@ -11,7 +11,7 @@ from helpers import cond_err, random_csr
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("alg", ["adaptive", "lrb"])
+@pytest.mark.parametrize("alg", ["adaptive", "lrb", "coo", "coo_atomic"])
 def test_rocsparse_matches_oracle(orc, device, alg):
     from merbit_amd.compare import RocsparseSpmv
     print(f"[case] rocsparse {alg}", flush=True)

@@ -291,3 +291,46 @@ def test_omp_baseline_matches(orc):
     t, used, y = orc.spmv_csr_omp_time(rp, col, val, x, nthreads=2, warmup=1, reps=3)
     assert t > 0 and used >= 1
     assert np.array_equal(y, orc.spmv_csr(rp, col, val, x))
+
+
+def _dangling_graph(seed=5, n=300, m=900):
+    rng = np.random.default_rng(seed)
+    row = rng.integers(0, n, m).astype(np.int32)
+    col = rng.integers(0, n, m).astype(np.int32)
+    return n, row, col
+
+
+def test_pagerank_fixed_point_dense_solve(orc):
+    """The oracle's iterations converge to the fixed point of
+    x = d P^T x + (d/n) 1 a^T x + (1-d) s/n 1 (a: dangling indicator, s the
+    mass), solved densely from the loaded edges without the oracle."""
+    n, row, col = _dangling_graph()
+    deg = np.bincount(row, minlength=n).astype(float)
+    dangling = deg == 0
+    assert dangling.sum() > 0
+    P = np.zeros((n, n))
+    np.add.at(P, (row, col), 1.0 / deg[row])
+    d = 0.85
+    x0 = np.random.default_rng(6).uniform(0.5, 1.5, n)
+    s = x0.sum()
+    M = np.eye(n) - d * P.T - (d / n) * np.outer(np.ones(n), dangling.astype(float))
+    x_star = np.linalg.solve(M, np.full(n, (1 - d) * s / n))
+    rp, c, v = orc.ref_ingest(n, row, col)
+    x = orc.pagerank(rp, c, v, d, 300, x0)
+    assert np.max(np.abs(x - x_star) / np.abs(x_star)) < 1e-12
+    assert abs(x_star.sum() - s) < 1e-12 * s
+
+
+def test_pagerank_mass_and_no_damping(orc):
+    """Mass is kept every iteration; with d = 0 one iteration gives the uniform
+    vector s/n; with no dangling nodes and d = 1 the iteration is P^T x."""
+    n, row, col = _dangling_graph(seed=9)
+    rp, c, v = orc.ref_ingest(n, row, col)
+    x0 = np.random.default_rng(2).uniform(0, 1, n)
+    for it in (1, 2, 7):
+        assert abs(orc.pagerank(rp, c, v, 0.85, it, x0).sum() - x0.sum()) < 1e-12 * x0.sum()
+    assert np.allclose(orc.pagerank(rp, c, v, 0.0, 1, x0), x0.sum() / n, rtol=1e-15, atol=0)
+    ring_r = np.arange(n, dtype=np.int32)
+    ring_c = ((ring_r + 1) % n).astype(np.int32)
+    rp2, c2, v2 = orc.ref_ingest(n, ring_r, ring_c)
+    assert np.array_equal(orc.pagerank(rp2, c2, v2, 1.0, 1, x0), np.roll(x0, 1))
