@@ -54,6 +54,34 @@ def test_batch_matches_single_calls(device, kind, ipt):
     b.close()
 
 
+@pytest.mark.slow
+def test_batch_full(orc, device):
+    """The 64-matrix uneven batch at full size (6.3M rows, 214M entries,
+    FP32): one batch product equals the 64 single-matrix products bit for
+    bit, and every eighth matrix meets the mixed bar against the oracle."""
+    from merbit_amd.batch import MerbitBatch
+    from merbit_amd.spmv import MerbitPlan
+    from merbit_amd import workloads
+    say("batch 64 matrices full size, f32")
+    mats_h, xs_h = workloads.uneven_batch()
+    mats = [(torch.from_numpy(A.row_ptr).to(device), torch.from_numpy(A.col).to(device),
+             torch.from_numpy(A.val).to(device), torch.from_numpy(x).to(device),
+             torch.full((A.nrow,), float("nan"), dtype=torch.float32, device=device), A.ncol)
+            for A, x in zip(mats_h, xs_h)]
+    b = MerbitBatch(mats, items_per_thread=16)
+    b.spmv()
+    torch.cuda.synchronize()
+    for i, ((rp, col, val, x, y, ncol), A, xh) in enumerate(zip(mats, mats_h, xs_h)):
+        plan = MerbitPlan(rp, ncol, col.numel(), items_per_thread=16)
+        ref = plan.spmv(rp, col, val, x)
+        plan.close()
+        assert torch.equal(ref.view(torch.uint8), y.view(torch.uint8)), i
+        if i % 8 == 0:
+            ya, absr = orc.spmv_accurate(A.row_ptr, A.col, A.val, xh)
+            assert cond_err(y.cpu().numpy().astype(np.float64), ya, absr) <= 1e-6, i
+    b.close()
+
+
 @pytest.mark.parametrize("kind", ["f64", "f32"])
 def test_batch_long_rows(orc, device, kind):
     """Rows spanning more than 64 tiles are completed by a whole wave of the

@@ -4,6 +4,8 @@ the Dot2-accurate oracle and against the reference-order emulator; mixed
 precision bar: <= 1e-6 against the FP64 oracle on the same stored values."""
 from __future__ import annotations
 
+import math
+
 import numpy as np
 import pytest
 import torch
@@ -149,6 +151,37 @@ def test_f64_rmat_full(orc, device):
     ye = orc.merbit_emulate(A.row_ptr, A.col, A.val, x, sigma=14)
     assert cond_err(y1, ye, absr) <= F64_TOL
     M.plan.close()
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("kind", ["f64", "f32", "bf16"])
+def test_banded_full(orc, device, kind):
+    """The banded workload at full size (4M rows, 27 diagonals, 108M entries)
+    against the oracle's accurate product on the same stored values, and the
+    FP64 result with x = 1 against the row sums (a checksum of checksums:
+    sum(y) = sum of all values, accumulated accurately by the oracle)."""
+    from merbit_amd import gen, workloads
+    from merbit_amd.spmv import MerbitPlan
+    say(f"banded 4M rows, 108M entries, {kind}")
+    A, x32 = workloads.banded()
+    if kind == "f64":
+        vals, xs, tol = A.val.astype(np.float64), x32.astype(np.float64), F64_TOL
+    else:
+        vals = A.val if kind == "f32" else gen.to_bf16(A.val)
+        xs, tol = x32, MIXED_TOL
+    d_rp = dev(A.row_ptr, device)
+    plan = MerbitPlan(d_rp, A.ncol, A.nnz, items_per_thread=12)
+    d_col, d_val = dev(A.col, device), dev(vals, device)
+    y = plan.spmv(d_rp, d_col, d_val, dev(xs, device)).cpu().numpy().astype(np.float64)
+    ya, absr = orc.spmv_accurate(A.row_ptr, A.col, vals, xs)
+    assert cond_err(y, ya, absr) <= tol
+    if kind == "f64":
+        ones = np.ones(A.ncol)
+        y1 = plan.spmv(d_rp, d_col, d_val, dev(ones, device)).cpu().numpy()
+        rs, rabs = orc.spmv_accurate(A.row_ptr, A.col, vals, ones)
+        assert cond_err(y1, rs, rabs) <= F64_TOL
+        assert abs(math.fsum(y1) - math.fsum(rs)) <= 1e-12 * math.fsum(rabs)
+    plan.close()
 
 
 @pytest.mark.parametrize("kind", ["f32", "bf16"])
