@@ -417,8 +417,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not "
-              "supported (the merge-path SpMV here runs on a single MI355X)",
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the merge-path SpMV library runs on a single MI355X)",
               file=sys.stderr)
         sys.exit(2)
 

@@ -5,7 +5,22 @@
 //                        (CSC of A, built once per pattern on the device with a
 //                        stable radix sort, so every column keeps its rows in
 //                        ascending order and the sums are deterministic); the
-//                        values are gathered into CSC order per call.
+//                        values are moved into CSC order per call by a
+//                        two-pass bucketed permutation (below).
+//
+// Bucketed permutation.  Moving nnz values from CSR to CSC order is a random
+// permutation; done as one gather it reads a whole memory line per value from
+// an array larger than the caches.  Instead, CSC positions are cut into
+// buckets of kBucket consecutive positions, and CSR positions into chunks of
+// kChunk.  Pass 1 (one workgroup per chunk) reads the chunk's values
+// coalesced, regroups them in LDS by bucket and writes each bucket's run
+// contiguously into a staging array laid out bucket after bucket; pass 2
+// (one workgroup per bucket) loads the bucket's staging slice into LDS and
+// writes the bucket's CSC positions in order, each value read from LDS.
+// Every global access is coalesced or a short contiguous run.  The schedule
+// (LDS slot per CSR entry, staging index per pass-1 slot, staging index per
+// CSC position) is computed once at create with two stable radix sorts; the
+// values moved are bit-for-bit those of the plain gather val[perm[p]].
 //   dval = g[row]*x[col] (SDDMM over the pattern) walking the forward plan's
 //                        merge-path tiles, which give every nonzero its row.
 
@@ -55,11 +70,94 @@ __global__ void transpose_rows_kernel(const int32_t* __restrict__ perm,
   if (p < nnz) row_idx[p] = row[perm[p]];
 }
 
+// Both sizes are limited by LDS: a chunk's or a bucket's values must fit a
+// workgroup's LDS at 8 bytes each (128 KB of the 160 KB per CU).
+constexpr int kChunk = 16384;   // CSR entries per pass-1 workgroup (LDS slots, < 65536)
+constexpr int kBucket = 16384;  // CSC positions per bucket
+constexpr int kPassWg = 256;
+
+// Pass 1: the chunk's values into LDS at their bucket-grouped slots, then
+// out to the staging array, slot order (consecutive lanes write consecutive
+// staging entries inside each bucket run).  Consecutive chunks run on one
+// XCD, so the runs they add to the same staging lines meet in one L2.
 template <class T>
-__global__ void gather_kernel(const T* __restrict__ val, const int32_t* __restrict__ perm,
-                              int nnz, T* __restrict__ out) {
+__global__ __launch_bounds__(kPassWg) void perm_pass1_kernel(const T* __restrict__ val,
+                                                             const uint16_t* __restrict__ lslot,
+                                                             const int32_t* __restrict__ spos,
+                                                             int nnz, int nchunks,
+                                                             T* __restrict__ stage) {
+  extern __shared__ __align__(16) char smem[];
+  T* s_v = reinterpret_cast<T*>(smem);  // kChunk values
+  const long long base = (long long)xcd_tile(blockIdx.x, nchunks) * kChunk;
+  const int n = (int)min((long long)kChunk, (long long)nnz - base);
+#pragma unroll 8
+  for (int j = threadIdx.x; j < n; j += kPassWg) s_v[lslot[base + j]] = val[base + j];
+  __syncthreads();
+#pragma unroll 8
+  for (int j = threadIdx.x; j < n; j += kPassWg) stage[spos[base + j]] = s_v[j];
+}
+
+// Pass 2: one workgroup per bucket loads the bucket's staging slice into LDS
+// (coalesced), then writes the bucket's CSC positions in order, each value
+// read from LDS at its staging index.
+template <class T>
+__global__ __launch_bounds__(kPassWg) void perm_pass2_kernel(const T* __restrict__ stage,
+                                                                 const int32_t* __restrict__ lidx,
+                                                                 int nnz, T* __restrict__ out) {
+  extern __shared__ __align__(16) char smem[];
+  T* s_v = reinterpret_cast<T*>(smem);
+  const long long base = (long long)blockIdx.x * kBucket;
+  const int n = (int)min((long long)kBucket, (long long)nnz - base);
+#pragma unroll 8
+  for (int j = threadIdx.x; j < n; j += kPassWg) s_v[j] = stage[base + j];
+  __syncthreads();
+#pragma unroll 8
+  for (int j = threadIdx.x; j < n; j += kPassWg) out[base + j] = s_v[lidx[base + j] - base];
+}
+
+// inv[perm[p]] = p: the CSC position of every CSR entry.
+__global__ void perm_inv_kernel(const int32_t* __restrict__ perm, int nnz,
+                                int32_t* __restrict__ inv) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < nnz) out[p] = val[perm[p]];
+  if (p < nnz) inv[perm[p]] = p;
+}
+
+// Sort keys of CSR entry k: pass-1 order is chunk(k) above bucket(inv[k]);
+// staging order is bucket(inv[k]) (both sorts are stable, so k breaks ties).
+__global__ void perm_key1_kernel(const int32_t* __restrict__ inv, int nnz, int bucket_bits,
+                                 uint32_t* __restrict__ key1, uint32_t* __restrict__ key2,
+                                 int32_t* __restrict__ iota) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  const uint32_t b = (uint32_t)(inv[k] / kBucket);
+  key1[k] = ((uint32_t)(k / kChunk) << bucket_bits) | b;
+  key2[k] = b;
+  iota[k] = k;
+}
+
+// lslot[k] = slot of CSR entry k in its chunk, from the pass-1 order.
+__global__ void perm_lslot_kernel(const int32_t* __restrict__ order1, int nnz,
+                                  uint16_t* __restrict__ lslot) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nnz) lslot[order1[i]] = (uint16_t)(i % kChunk);
+}
+
+// sidx[k] = staging index of CSR entry k (its rank in (bucket, k) order).
+__global__ void perm_sidx_kernel(const int32_t* __restrict__ order2, int nnz,
+                                 int32_t* __restrict__ sidx) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < nnz) sidx[order2[r]] = r;
+}
+
+// spos[i] = sidx[order1[i]] and lidx[p] = sidx[perm[p]].
+__global__ void perm_final_kernel(const int32_t* __restrict__ order1,
+                                  const int32_t* __restrict__ perm,
+                                  const int32_t* __restrict__ sidx, int nnz,
+                                  int32_t* __restrict__ spos, int32_t* __restrict__ lidx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnz) return;
+  spos[i] = sidx[order1[i]];
+  lidx[i] = sidx[perm[i]];
 }
 
 // dval[k] = g[row(k)] * x[col(k)] for the nonzeros of one forward tile.
@@ -148,7 +246,91 @@ struct mbx_adjoint {
   int32_t* perm;      // nnz: CSR position of every CSC entry
   void* vt;           // nnz values in CSC order (8 bytes each)
   mbx_plan* plan_t;   // merge-path plan of A^T
+  // bucketed permutation schedule (see the file comment)
+  uint16_t* lslot;    // nnz: LDS slot of CSR entry k within its chunk
+  int32_t* spos;      // nnz: staging index of pass-1 slot i (chunk-major)
+  int32_t* lidx;      // nnz: staging index of CSC position p
+  void* stage;        // nnz values, bucket after bucket (8 bytes each)
 };
+
+namespace {
+
+// The bucketed-permutation schedule of an adjoint whose perm is built
+// (stream-ordered; temporaries freed after a synchronisation).
+int build_perm_schedule(mbx_adjoint* a, hipStream_t s) {
+  const int nnz = a->nnz;
+  if (nnz == 0) return MBX_OK;
+  const size_t m = (size_t)nnz;
+  int bucket_bits = 1, chunk_bits = 1;
+  while (bucket_bits < 31 && (1ll << bucket_bits) * kBucket < (long long)nnz) ++bucket_bits;
+  while (chunk_bits < 31 && (1ll << chunk_bits) * kChunk < (long long)nnz) ++chunk_bits;
+  if (bucket_bits + chunk_bits > 32) return MBX_EOVERFLOW;
+  int32_t *inv = nullptr, *iota = nullptr, *order1 = nullptr, *order2 = nullptr;
+  uint32_t *key1 = nullptr, *key2 = nullptr, *skey = nullptr;
+  void* tmp = nullptr;
+  int st = MBX_OK;
+  if (hipMalloc((void**)&inv, 4 * m) != hipSuccess || hipMalloc((void**)&iota, 4 * m) != hipSuccess ||
+      hipMalloc((void**)&order1, 4 * m) != hipSuccess ||
+      hipMalloc((void**)&order2, 4 * m) != hipSuccess ||
+      hipMalloc((void**)&key1, 4 * m) != hipSuccess || hipMalloc((void**)&key2, 4 * m) != hipSuccess ||
+      hipMalloc((void**)&skey, 4 * m) != hipSuccess)
+    st = MBX_ENOMEM;
+  if (st == MBX_OK) {
+    perm_inv_kernel<<<blocks(nnz), kBlock, 0, s>>>(a->perm, nnz, inv);
+    perm_key1_kernel<<<blocks(nnz), kBlock, 0, s>>>(inv, nnz, bucket_bits, key1, key2, iota);
+    size_t b1 = 0, b2 = 0;
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, b1, key1, skey, iota, order1, nnz, 0,
+                                           bucket_bits + chunk_bits, s) != hipSuccess ||
+        hipcub::DeviceRadixSort::SortPairs(nullptr, b2, key2, skey, iota, order2, nnz, 0,
+                                           bucket_bits, s) != hipSuccess)
+      st = MBX_EHIP;
+    const size_t tb = b1 > b2 ? b1 : b2;
+    if (st == MBX_OK && hipMalloc(&tmp, tb ? tb : 1) != hipSuccess) st = MBX_ENOMEM;
+    if (st == MBX_OK &&
+        (hipcub::DeviceRadixSort::SortPairs(tmp, b1, key1, skey, iota, order1, nnz, 0,
+                                            bucket_bits + chunk_bits, s) != hipSuccess ||
+         hipcub::DeviceRadixSort::SortPairs(tmp, b2, key2, skey, iota, order2, nnz, 0,
+                                            bucket_bits, s) != hipSuccess))
+      st = MBX_EHIP;
+    if (st == MBX_OK) {
+      int32_t* sidx = inv;  // inv is dead once the keys are built
+      perm_lslot_kernel<<<blocks(nnz), kBlock, 0, s>>>(order1, nnz, a->lslot);
+      perm_sidx_kernel<<<blocks(nnz), kBlock, 0, s>>>(order2, nnz, sidx);
+      perm_final_kernel<<<blocks(nnz), kBlock, 0, s>>>(order1, a->perm, sidx, nnz, a->spos,
+                                                       a->lidx);
+      if (hipGetLastError() != hipSuccess) st = MBX_EHIP;
+    }
+  }
+  if (hipStreamSynchronize(s) != hipSuccess && st == MBX_OK) st = MBX_EHIP;
+  if (inv) (void)hipFree(inv);
+  if (iota) (void)hipFree(iota);
+  if (order1) (void)hipFree(order1);
+  if (order2) (void)hipFree(order2);
+  if (key1) (void)hipFree(key1);
+  if (key2) (void)hipFree(key2);
+  if (skey) (void)hipFree(skey);
+  if (tmp) (void)hipFree(tmp);
+  return st;
+}
+
+// Values of `val` (CSR order) into CSC order in vt, by the two passes.
+template <class T>
+void permute_values(const mbx_adjoint* a, const T* val, hipStream_t s) {
+  const int nnz = a->nnz;
+  const int g1 = (int)(((long long)nnz + kChunk - 1) / kChunk);
+  const size_t lds1 = sizeof(T) * kChunk;
+  (void)hipFuncSetAttribute((const void*)perm_pass1_kernel<T>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1);
+  perm_pass1_kernel<T><<<g1, kPassWg, lds1, s>>>(val, a->lslot, a->spos, nnz, g1,
+                                                  (T*)a->stage);
+  const int g2 = (int)(((long long)nnz + kBucket - 1) / kBucket);
+  const size_t lds2 = sizeof(T) * kBucket;
+  (void)hipFuncSetAttribute((const void*)perm_pass2_kernel<T>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2);
+  perm_pass2_kernel<T><<<g2, kPassWg, lds2, s>>>((const T*)a->stage, a->lidx, nnz, (T*)a->vt);
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -174,7 +356,11 @@ int mbx_adjoint_create(const int32_t* d_row_ptr, const int32_t* d_col, int32_t n
   if (hipMalloc((void**)&a->col_ptr, sizeof(int32_t) * ((size_t)ncol + 1)) != hipSuccess ||
       hipMalloc((void**)&a->row_idx, sizeof(int32_t) * m) != hipSuccess ||
       hipMalloc((void**)&a->perm, sizeof(int32_t) * m) != hipSuccess ||
-      hipMalloc(&a->vt, 8 * m) != hipSuccess || hipMalloc((void**)&row, sizeof(int32_t) * m) != hipSuccess ||
+      hipMalloc(&a->vt, 8 * m) != hipSuccess || hipMalloc(&a->stage, 8 * m) != hipSuccess ||
+      hipMalloc((void**)&a->lslot, sizeof(uint16_t) * m) != hipSuccess ||
+      hipMalloc((void**)&a->spos, sizeof(int32_t) * m) != hipSuccess ||
+      hipMalloc((void**)&a->lidx, sizeof(int32_t) * m) != hipSuccess ||
+      hipMalloc((void**)&row, sizeof(int32_t) * m) != hipSuccess ||
       hipMalloc((void**)&iota, sizeof(int32_t) * m) != hipSuccess ||
       hipMalloc((void**)&keys, sizeof(uint32_t) * m) != hipSuccess) {
     st = MBX_ENOMEM;
@@ -201,6 +387,7 @@ int mbx_adjoint_create(const int32_t* d_row_ptr, const int32_t* d_col, int32_t n
       st = MBX_EHIP;
   }
   if (st == MBX_OK && hipGetLastError() != hipSuccess) st = MBX_EHIP;
+  if (st == MBX_OK) st = build_perm_schedule(a, s);
   if (st == MBX_OK)
     st = mbx_plan_create(a->col_ptr, ncol, nrow, nnz, items_per_thread, 256, stream, &a->plan_t);
   // the temporaries die here: wait for the kernels that use them
@@ -223,6 +410,10 @@ void mbx_adjoint_destroy(mbx_adjoint* a) {
   if (a->row_idx) (void)hipFree(a->row_idx);
   if (a->perm) (void)hipFree(a->perm);
   if (a->vt) (void)hipFree(a->vt);
+  if (a->stage) (void)hipFree(a->stage);
+  if (a->lslot) (void)hipFree(a->lslot);
+  if (a->spos) (void)hipFree(a->spos);
+  if (a->lidx) (void)hipFree(a->lidx);
   mbx_plan_destroy(a->plan_t);
   delete a;
 }
@@ -248,21 +439,15 @@ int mbx_adjoint_dx(const mbx_adjoint* a, int32_t value_kind, const void* d_val, 
   const int nnz = a->nnz;
   switch (value_kind) {
     case MBX_VALUE_F64:
-      if (nnz > 0)
-        gather_kernel<double><<<blocks(nnz), kBlock, 0, s>>>((const double*)d_val, a->perm, nnz,
-                                                             (double*)a->vt);
+      if (nnz > 0) permute_values<double>(a, (const double*)d_val, s);
       return mbx_spmv_f64(a->plan_t, a->col_ptr, a->row_idx, (const double*)a->vt, alpha,
                           (const double*)d_g, beta, (double*)d_dx, stream);
     case MBX_VALUE_F32:
-      if (nnz > 0)
-        gather_kernel<float><<<blocks(nnz), kBlock, 0, s>>>((const float*)d_val, a->perm, nnz,
-                                                           (float*)a->vt);
+      if (nnz > 0) permute_values<float>(a, (const float*)d_val, s);
       return mbx_spmv_f32(a->plan_t, a->col_ptr, a->row_idx, (const float*)a->vt, (float)alpha,
                           (const float*)d_g, (float)beta, (float*)d_dx, stream);
     case MBX_VALUE_BF16:
-      if (nnz > 0)
-        gather_kernel<uint16_t><<<blocks(nnz), kBlock, 0, s>>>((const uint16_t*)d_val, a->perm,
-                                                              nnz, (uint16_t*)a->vt);
+      if (nnz > 0) permute_values<uint16_t>(a, (const uint16_t*)d_val, s);
       return mbx_spmv_bf16(a->plan_t, a->col_ptr, a->row_idx, (const uint16_t*)a->vt,
                            (float)alpha, (const float*)d_g, (float)beta, (float*)d_dx, stream);
     default:

@@ -142,3 +142,45 @@ def test_adjoint_graph_c5(orc, device):
     rows = np.repeat(np.arange(A.nrow), np.diff(A.row_ptr))
     assert np.array_equal(dval, g[rows] * x[A.col])
     op.close()
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32", "bf16"])
+def test_adjoint_permutation_boundaries(orc, device, dtype):
+    """The CSR-to-CSC value permutation runs in chunks and buckets of 16384
+    entries: at sizes around those boundaries dx must equal, bit for bit, the
+    same tile SpMV on the transposed pattern fed with val[perm] gathered by
+    torch."""
+    from merbit_amd import gen
+    from merbit_amd._lib import MBX_VALUE_BF16, MBX_VALUE_F32, MBX_VALUE_F64, check, lib
+    from merbit_amd.spmv import MerbitPlan
+    kinds = {"f64": MBX_VALUE_F64, "f32": MBX_VALUE_F32, "bf16": MBX_VALUE_BF16}
+    rng = np.random.default_rng(11)
+    for nnz in (1, 16383, 16384, 16385, 3 * 16384 + 7, 200_003):
+        say(f"permutation {dtype} nnz={nnz}")
+        nrow, ncol = max(1, nnz // 9), max(1, nnz // 7)
+        lens = rng.multinomial(nnz, np.full(nrow, 1.0 / nrow))
+        rp, col = random_csr(rng, nrow, ncol, lens=lens)
+        assert int(rp[-1]) == nnz
+        v32 = rng.uniform(-1, 1, nnz).astype(np.float32)
+        if dtype == "f64":
+            val, g = v32.astype(np.float64), rng.standard_normal(nrow)
+        else:
+            val = v32 if dtype == "f32" else gen.to_bf16(v32)
+            g = rng.standard_normal(nrow).astype(np.float32)
+        op = make_op(rp, col, ncol, device)
+        d_val, d_g = dev(val, device), dev(g, device)
+        dx = torch.empty(ncol, dtype=d_g.dtype, device=device)
+        check(lib().mbx_adjoint_dx(op.adjoint_handle, kinds[dtype], d_val.data_ptr(), 1.0,
+                                   d_g.data_ptr(), 0.0, dx.data_ptr(),
+                                   torch.cuda.current_stream().cuda_stream), "mbx_adjoint_dx")
+        cp, ri, pm = orc.csr_transpose(rp, col, ncol)
+        d_cp = dev(cp, device)
+        plan_t = MerbitPlan(d_cp, nrow, nnz, items_per_thread=8)
+        idx = dev(pm.astype(np.int64), device)
+        # gather the raw bits (torch indexes int16, not uint16)
+        vt = (d_val.view(torch.int16)[idx].view(torch.uint16) if dtype == "bf16"
+              else d_val[idx].contiguous())
+        want = plan_t.spmv(d_cp, dev(ri, device), vt, d_g)
+        assert torch.equal(dx.view(torch.uint8), want.view(torch.uint8)), nnz
+        plan_t.close()
+        op.close()
