@@ -76,11 +76,18 @@ constexpr int kChunk = 16384;   // CSR entries per pass-1 workgroup (LDS slots, 
 constexpr int kBucket = 16384;  // CSC positions per bucket
 constexpr int kPassWg = 256;
 
+// Four consecutive elements as one vector access (16 bytes for FP32, 8 for
+// bfloat16, two 16-byte halves for FP64); the pointer must be aligned to the
+// group size.
+template <class T>
+using vec4_t = T __attribute__((ext_vector_type(4)));
+
 // Pass 1: the chunk's values into LDS at their bucket-grouped slots, then
 // out to the staging array, slot order (consecutive lanes write consecutive
 // staging entries inside each bucket run).  Consecutive chunks run on one
-// XCD, so the runs they add to the same staging lines meet in one L2.
-template <class T>
+// XCD, so the runs they add to the same staging lines meet in one L2.  VEC:
+// values and slots read four at a time (val aligned to 4 elements).
+template <class T, bool VEC>
 __global__ __launch_bounds__(kPassWg) void perm_pass1_kernel(const T* __restrict__ val,
                                                              const uint16_t* __restrict__ lslot,
                                                              const int32_t* __restrict__ spos,
@@ -90,29 +97,58 @@ __global__ __launch_bounds__(kPassWg) void perm_pass1_kernel(const T* __restrict
   T* s_v = reinterpret_cast<T*>(smem);  // kChunk values
   const long long base = (long long)xcd_tile(blockIdx.x, nchunks) * kChunk;
   const int n = (int)min((long long)kChunk, (long long)nnz - base);
-#pragma unroll 8
-  for (int j = threadIdx.x; j < n; j += kPassWg) s_v[lslot[base + j]] = val[base + j];
+  const int n4 = VEC ? (n & ~3) : 0;
+  if (VEC) {
+#pragma unroll 4
+    for (int j = 4 * threadIdx.x; j < n4; j += 4 * kPassWg) {
+      const vec4_t<T> v = *reinterpret_cast<const vec4_t<T>*>(val + base + j);
+      const vec4_t<uint16_t> l = *reinterpret_cast<const vec4_t<uint16_t>*>(lslot + base + j);
+      s_v[l.x] = v.x;
+      s_v[l.y] = v.y;
+      s_v[l.z] = v.z;
+      s_v[l.w] = v.w;
+    }
+  }
+  for (int j = n4 + threadIdx.x; j < n; j += kPassWg) s_v[lslot[base + j]] = val[base + j];
   __syncthreads();
+  // one slot per lane: a store instruction covers 64 consecutive slots, so
+  // it touches as few bucket runs (staging lines) as possible
 #pragma unroll 8
   for (int j = threadIdx.x; j < n; j += kPassWg) stage[spos[base + j]] = s_v[j];
 }
 
 // Pass 2: one workgroup per bucket loads the bucket's staging slice into LDS
-// (coalesced), then writes the bucket's CSC positions in order, each value
-// read from LDS at its staging index.
+// (coalesced, four values at a time), then writes the bucket's CSC positions
+// in order, each value read from LDS at its staging index.  Buckets start at
+// multiples of kBucket in library-allocated arrays, so every group of four
+// is aligned.
 template <class T>
 __global__ __launch_bounds__(kPassWg) void perm_pass2_kernel(const T* __restrict__ stage,
-                                                                 const int32_t* __restrict__ lidx,
-                                                                 int nnz, T* __restrict__ out) {
+                                                             const int32_t* __restrict__ lidx,
+                                                             int nnz, T* __restrict__ out) {
   extern __shared__ __align__(16) char smem[];
   T* s_v = reinterpret_cast<T*>(smem);
   const long long base = (long long)blockIdx.x * kBucket;
   const int n = (int)min((long long)kBucket, (long long)nnz - base);
-#pragma unroll 8
-  for (int j = threadIdx.x; j < n; j += kPassWg) s_v[j] = stage[base + j];
+  const int n4 = n & ~3;
+#pragma unroll 4
+  for (int j = 4 * threadIdx.x; j < n4; j += 4 * kPassWg)
+    *reinterpret_cast<vec4_t<T>*>(s_v + j) =
+        *reinterpret_cast<const vec4_t<T>*>(stage + base + j);
+  for (int j = n4 + threadIdx.x; j < n; j += kPassWg) s_v[j] = stage[base + j];
   __syncthreads();
-#pragma unroll 8
-  for (int j = threadIdx.x; j < n; j += kPassWg) out[base + j] = s_v[lidx[base + j] - base];
+  const int b = (int)base;  // staging indices of this bucket are base + [0, n)
+#pragma unroll 4
+  for (int j = 4 * threadIdx.x; j < n4; j += 4 * kPassWg) {
+    const vec4_t<int32_t> i = *reinterpret_cast<const vec4_t<int32_t>*>(lidx + base + j);
+    vec4_t<T> v;
+    v.x = s_v[i.x - b];
+    v.y = s_v[i.y - b];
+    v.z = s_v[i.z - b];
+    v.w = s_v[i.w - b];
+    *reinterpret_cast<vec4_t<T>*>(out + base + j) = v;
+  }
+  for (int j = n4 + threadIdx.x; j < n; j += kPassWg) out[base + j] = s_v[lidx[base + j] - b];
 }
 
 // inv[perm[p]] = p: the CSC position of every CSR entry.
@@ -319,10 +355,17 @@ void permute_values(const mbx_adjoint* a, const T* val, hipStream_t s) {
   const int nnz = a->nnz;
   const int g1 = (int)(((long long)nnz + kChunk - 1) / kChunk);
   const size_t lds1 = sizeof(T) * kChunk;
-  (void)hipFuncSetAttribute((const void*)perm_pass1_kernel<T>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1);
-  perm_pass1_kernel<T><<<g1, kPassWg, lds1, s>>>(val, a->lslot, a->spos, nnz, g1,
-                                                  (T*)a->stage);
+  if ((reinterpret_cast<uintptr_t>(val) & (4 * sizeof(T) - 1)) == 0) {
+    (void)hipFuncSetAttribute((const void*)perm_pass1_kernel<T, true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1);
+    perm_pass1_kernel<T, true><<<g1, kPassWg, lds1, s>>>(val, a->lslot, a->spos, nnz, g1,
+                                                          (T*)a->stage);
+  } else {
+    (void)hipFuncSetAttribute((const void*)perm_pass1_kernel<T, false>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1);
+    perm_pass1_kernel<T, false><<<g1, kPassWg, lds1, s>>>(val, a->lslot, a->spos, nnz, g1,
+                                                           (T*)a->stage);
+  }
   const int g2 = (int)(((long long)nnz + kBucket - 1) / kBucket);
   const size_t lds2 = sizeof(T) * kBucket;
   (void)hipFuncSetAttribute((const void*)perm_pass2_kernel<T>,

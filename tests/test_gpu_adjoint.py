@@ -182,5 +182,13 @@ def test_adjoint_permutation_boundaries(orc, device, dtype):
               else d_val[idx].contiguous())
         want = plan_t.spmv(d_cp, dev(ri, device), vt, d_g)
         assert torch.equal(dx.view(torch.uint8), want.view(torch.uint8)), nnz
+        # values one element into a larger buffer: not 4-element aligned
+        holder = torch.zeros(nnz + 1, dtype=d_val.dtype, device=device)
+        holder[1:] = d_val
+        dx_u = torch.empty_like(dx)
+        check(lib().mbx_adjoint_dx(op.adjoint_handle, kinds[dtype], holder[1:].data_ptr(), 1.0,
+                                   d_g.data_ptr(), 0.0, dx_u.data_ptr(),
+                                   torch.cuda.current_stream().cuda_stream), "mbx_adjoint_dx")
+        assert torch.equal(dx_u.view(torch.uint8), want.view(torch.uint8)), nnz
         plan_t.close()
         op.close()
