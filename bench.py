@@ -404,9 +404,10 @@ def main():
     ap.add_argument("--copies", type=int, default=3)
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--batch-count", type=int, default=64)
-    ap.add_argument("--only", choices=["headline", "banded", "all"], default="all",
-                    help="headline: only the FP64 R-MAT line; banded: the headline plus the "
-                         "FP64/FP32/BF16 banded products (for profiling)")
+    ap.add_argument("--only", choices=["headline", "banded", "spmm", "batch", "adjoint", "all"],
+                    default="all",
+                    help="headline: only the FP64 R-MAT line; banded / spmm / batch / adjoint: "
+                         "the headline plus that part (for profiling)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run the plain device copy (counter calibration)")
     ap.add_argument("--compare", action="store_true",
@@ -447,6 +448,7 @@ def main():
     }
     if args.only == "all":
         extras["mixed_rmat"] = bench_mixed("rmat", A, x, dev, args, f64_ms=ms, copies=3)
+    if args.only in ("all", "spmm"):
         extras["spmm_rmat"] = bench_spmm(A, dev, args)
     if args.only in ("all", "banded"):
         t0 = time.time()
@@ -454,8 +456,9 @@ def main():
         log(f"banded: nrow={B.nrow} nnz={B.nnz} ({time.time() - t0:.1f}s)")
         extras["mixed_banded"] = bench_mixed("banded", B, xb, dev, args)
         del B, xb
-    if args.only == "all":
+    if args.only in ("all", "batch"):
         extras["batch_uneven"] = bench_batch(dev, args)
+    if args.only in ("all", "adjoint"):
         extras["adjoint_graph"] = bench_adjoint(dev, args)
     if args.only == "all" or args.compare:
         extras["rocsparse_rmat_f64"] = bench_rocsparse(A, x, dev, args, ms)
