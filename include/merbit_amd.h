@@ -44,7 +44,8 @@ typedef struct mbx_plan mbx_plan;
  * Runs the merge-path partition search over (row_ptr, nnz index) on `stream`
  * and keeps one (row, nnz) coordinate per tile of wg_size*items_per_thread
  * merge items plus the per-tile carry slots.  items_per_thread plays the
- * reference's SIGMA and wg_size its BLOCK; 0 selects the defaults (8, 256).
+ * reference's SIGMA (instantiated: 2, 4, 6, 8, 10, 12, 14, 16) and wg_size
+ * its BLOCK (256); 0 selects the defaults (8, 256).
  * Synchronises `stream` once: the largest number of rows ending in one tile
  * is read back to size the tile kernel's LDS. */
 int mbx_plan_create(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol,

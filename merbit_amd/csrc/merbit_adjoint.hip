@@ -520,7 +520,8 @@ int mbx_sddmm(const mbx_plan* p, int32_t value_kind, const int32_t* d_row_ptr,
           d_row_ptr, d_col, (const float*)d_g, (const float*)d_x, (float*)d_dval,           \
           p->d_coord, p->ntiles);                                                           \
     break;
-    MBX_CASE(2) MBX_CASE(4) MBX_CASE(6) MBX_CASE(8) MBX_CASE(12) MBX_CASE(16)
+    MBX_CASE(2) MBX_CASE(4) MBX_CASE(6) MBX_CASE(8) MBX_CASE(10) MBX_CASE(12) MBX_CASE(14)
+    MBX_CASE(16)
 #undef MBX_CASE
     default:
       return MBX_EUNSUPPORTED;

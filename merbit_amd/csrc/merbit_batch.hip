@@ -188,7 +188,8 @@ int batch_dispatch(const mbx_batch* b, hipStream_t s, double alpha, double beta,
     if (grid_only) return batch_grid<P, I>(device, b->vec);         \
     batch_launch<P, I>(b, s, (S)alpha, (S)beta);                    \
     return 0;
-    MBX_CASE(2) MBX_CASE(4) MBX_CASE(6) MBX_CASE(8) MBX_CASE(12) MBX_CASE(16)
+    MBX_CASE(2) MBX_CASE(4) MBX_CASE(6) MBX_CASE(8) MBX_CASE(10) MBX_CASE(12) MBX_CASE(14)
+    MBX_CASE(16)
 #undef MBX_CASE
     default: return -1;
   }
@@ -215,7 +216,8 @@ bool vec_item_ok(const mbx_batch_item& it, int kind) {
 }
 
 bool ipt_ok(int ipt) {
-  return ipt == 2 || ipt == 4 || ipt == 6 || ipt == 8 || ipt == 12 || ipt == 16;
+  return ipt == 2 || ipt == 4 || ipt == 6 || ipt == 8 || ipt == 10 || ipt == 12 || ipt == 14 ||
+         ipt == 16;
 }
 
 }  // namespace

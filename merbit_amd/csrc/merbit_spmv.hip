@@ -97,7 +97,7 @@ namespace {
 bool supported_shape(int wg, int ipt) {
   if (wg != 256) return false;
   switch (ipt) {
-    case 2: case 4: case 6: case 8: case 12: case 16: return true;
+    case 2: case 4: case 6: case 8: case 10: case 12: case 14: case 16: return true;
     default: return false;
   }
 }
@@ -143,7 +143,9 @@ int run_spmv(const mbx_plan* p, const int32_t* rp, const int32_t* col,
     case 4: e = launch_tile<P, 4>(p, s, rp, col, val, x, y, alpha, beta); break;
     case 6: e = launch_tile<P, 6>(p, s, rp, col, val, x, y, alpha, beta); break;
     case 8: e = launch_tile<P, 8>(p, s, rp, col, val, x, y, alpha, beta); break;
+    case 10: e = launch_tile<P, 10>(p, s, rp, col, val, x, y, alpha, beta); break;
     case 12: e = launch_tile<P, 12>(p, s, rp, col, val, x, y, alpha, beta); break;
+    case 14: e = launch_tile<P, 14>(p, s, rp, col, val, x, y, alpha, beta); break;
     case 16: e = launch_tile<P, 16>(p, s, rp, col, val, x, y, alpha, beta); break;
     default: return MBX_EUNSUPPORTED;
   }

@@ -18,7 +18,7 @@
 
 namespace {
 
-const int kSpmvCandidates[] = {2, 4, 6, 8, 12, 16};
+const int kSpmvCandidates[] = {2, 4, 6, 8, 10, 12, 14, 16};
 const int kSpmmCandidates[] = {2, 4};
 
 template <class Call>

@@ -1,7 +1,7 @@
 // main.cpp -- config-ID driver: ./merbit_main <ID> (src/main.cu:3-84).
 //
 //   merbit_main <ID> [--config FILE] [--log FILE] [--device N] [--type f64|f32]
-//               [--sigma 4|6|8|12|16] [--perrnd N] [--maxiter N] [--check]
+//               [--sigma 2|4|6|8|10|12|14|16] [--perrnd N] [--maxiter N] [--check]
 //               [--pagerank [--damp D]]
 //   merbit_main --mtx FILE [...]
 //
@@ -35,7 +35,7 @@ struct Args {
 
 int usage() {
   std::cerr << "usage: merbit_main <ID> [--config FILE] [--log FILE] [--device N] "
-               "[--type f64|f32] [--sigma 4|6|8|12|16] [--perrnd N] [--maxiter N] [--check]\n"
+               "[--type f64|f32] [--sigma 2|4|...|16] [--perrnd N] [--maxiter N] [--check]\n"
                "                   [--pagerank [--damp D]]\n"
                "       merbit_main --mtx FILE [same options]\n";
   return 2;
@@ -120,10 +120,13 @@ int run(const Args& a) {
   ENV env(a.device, a.config, a.log);
   GRAPH<V> graph = a.mtx.empty() ? GRAPH<V>(a.config, a.gid) : GRAPH<V>::from_mtx(a.mtx, a.gid);
   switch (a.sigma) {
+    case 2: return run_sigma<V, 2>(env, graph, a);
     case 4: return run_sigma<V, 4>(env, graph, a);
     case 6: return run_sigma<V, 6>(env, graph, a);
     case 8: return run_sigma<V, 8>(env, graph, a);
+    case 10: return run_sigma<V, 10>(env, graph, a);
     case 12: return run_sigma<V, 12>(env, graph, a);
+    case 14: return run_sigma<V, 14>(env, graph, a);
     case 16: return run_sigma<V, 16>(env, graph, a);
     default: std::cerr << "unsupported sigma " << a.sigma << "\n"; return 2;
   }

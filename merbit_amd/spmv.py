@@ -16,7 +16,7 @@ import torch
 from ._lib import check, lib
 
 # Items per thread with an instantiated tile kernel (the reference's SIGMA).
-ITEMS_PER_THREAD = (2, 4, 6, 8, 12, 16)
+ITEMS_PER_THREAD = (2, 4, 6, 8, 10, 12, 14, 16)
 
 _VALUE_KINDS = {torch.float64: "f64", torch.float32: "f32", torch.uint16: "bf16",
                 torch.bfloat16: "bf16"}

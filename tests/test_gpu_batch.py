@@ -189,7 +189,7 @@ def test_autotuners(orc, device):
     xx = torch.from_numpy(x).to(device)
     y = torch.empty(A.nrow, dtype=torch.float64, device=device)
     ipt, us = autotune_spmv(rp, col, val, xx, y, reps=3)
-    assert ipt in (2, 4, 6, 8, 12, 16) and us > 0
+    assert ipt in (2, 4, 6, 8, 10, 12, 14, 16) and us > 0
     plan = MerbitPlan(rp, A.ncol, A.nnz, items_per_thread=ipt)
     yy = plan.spmv(rp, col, val, xx).cpu().numpy()
     ya, absr = orc.spmv_accurate(A.row_ptr, A.col, A.val, x)
@@ -203,7 +203,7 @@ def test_autotuners(orc, device):
     say("autotune batch")
     mats = make_mats(device, 6, "f32", 3)
     ipt, us = autotune_batch(mats, reps=2)
-    assert ipt in (2, 4, 6, 8, 12, 16) and us > 0
+    assert ipt in (2, 4, 6, 8, 10, 12, 14, 16) and us > 0
 
 
 def test_batch_unaligned_member(device):

@@ -27,7 +27,7 @@ def write_rmat_mtx(path, scale, seed):
         f.writelines(rows)
 
 
-@pytest.mark.parametrize("typ,sigma", [("f64", 8), ("f64", 4), ("f32", 16)])
+@pytest.mark.parametrize("typ,sigma", [("f64", 8), ("f64", 4), ("f64", 14), ("f32", 16), ("f32", 10)])
 def test_driver_config_ids(device, tmp_path, typ, sigma):
     write_rmat_mtx(tmp_path / "rmat.mtx", 14, 3)
     cfg = tmp_path / "config.txt"

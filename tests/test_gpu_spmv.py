@@ -56,7 +56,7 @@ def test_partition_bit_exact(orc, device, ipt):
         plan.close()
 
 
-@pytest.mark.parametrize("ipt", [4, 6, 8, 12, 16])
+@pytest.mark.parametrize("ipt", [4, 6, 8, 10, 12, 14, 16])
 def test_f64_edge_cases(orc, device, ipt):
     rng = np.random.default_rng(1000 + ipt)
     for name, lens, ncol in edge_case_patterns(256 * ipt):
