@@ -368,6 +368,52 @@ def bench_adjoint(dev, args):
     return out
 
 
+def bench_pagerank(A, dev, args):
+    """The reference's pipeline on the R-MAT edge list: device ingest (CSR of
+    P^T, values 1/out-degree), then 20 damped PageRank iterations captured
+    as one HIP graph, against 20 plain products on the same matrix."""
+    import ctypes
+
+    import torch
+    from merbit_amd._lib import check, lib
+    from merbit_amd.ingest import device_ingest
+    from merbit_amd.spmv import MerbitPlan
+    n = A.nrow
+    row = torch.repeat_interleave(torch.arange(n, dtype=torch.int32, device=dev),
+                                  torch.from_numpy(np.diff(A.row_ptr)).to(dev))
+    col = torch.from_numpy(A.col).to(dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rp, ct, vt = device_ingest(n, row, col)
+    torch.cuda.synchronize()
+    ingest_ms = (time.perf_counter() - t0) * 1e3
+    del row, col
+    iters = 20
+    plan = MerbitPlan(rp, n, vt.numel(), items_per_thread=args.ipt or 12)
+    x0 = torch.full((n,), 1.0 / n, dtype=torch.float64, device=dev)
+    y = torch.empty(n, dtype=torch.float64, device=dev)
+    nbytes = int(lib().mbx_pagerank_workspace_bytes(n))
+    work = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    g = ctypes.c_void_p()
+    check(lib().mbx_pagerank_graph_create(plan.handle, rp.data_ptr(), ct.data_ptr(),
+                                          vt.data_ptr(), 0.85, iters, x0.data_ptr(),
+                                          y.data_ptr(), work.data_ptr(), nbytes,
+                                          ctypes.byref(g)), "mbx_pagerank_graph_create")
+    s = torch.cuda.current_stream().cuda_stream
+    pr_ms, _ = time_steps([lambda: check(lib().mbx_graph_launch(g, s), "mbx_graph_launch")],
+                          max(5, args.steps // 10), 2)
+    lib().mbx_graph_destroy(g)
+    spmv_ms, _ = time_steps([lambda: plan.spmv(rp, ct, vt, x0, y)], args.steps, args.warmup)
+    mass = float(plan.pagerank(rp, ct, vt, damp=0.85, iters=iters, x0=x0).sum())
+    plan.close()
+    out = {"nrow": n, "nnz": int(vt.numel()), "iterations": iters, "damp": 0.85,
+           "ingest_ms": round(ingest_ms, 2), "us_per_iteration": round(pr_ms * 1e3 / iters, 2),
+           "us_per_plain_spmv": round(spmv_ms * 1e3, 2), "mass_after": mass}
+    log(f"pagerank: ingest {ingest_ms:.1f} ms, {out['us_per_iteration']} us/iteration "
+        f"(plain SpMV {out['us_per_plain_spmv']} us), mass {mass:.15f}")
+    return out
+
+
 def bench_copy(dev):
     import torch
     n = 256 << 20  # 256 Mi doubles = 2 GiB
@@ -404,10 +450,11 @@ def main():
     ap.add_argument("--copies", type=int, default=3)
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--batch-count", type=int, default=64)
-    ap.add_argument("--only", choices=["headline", "banded", "spmm", "batch", "adjoint", "all"],
+    ap.add_argument("--only", choices=["headline", "banded", "spmm", "batch", "adjoint", "pagerank",
+                                       "all"],
                     default="all",
-                    help="headline: only the FP64 R-MAT line; banded / spmm / batch / adjoint: "
-                         "the headline plus that part (for profiling)")
+                    help="headline: only the FP64 R-MAT line; banded / spmm / batch / adjoint "
+                         "/ pagerank: the headline plus that part (for profiling)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run the plain device copy (counter calibration)")
     ap.add_argument("--compare", action="store_true",
@@ -460,6 +507,8 @@ def main():
         extras["batch_uneven"] = bench_batch(dev, args)
     if args.only in ("all", "adjoint"):
         extras["adjoint_graph"] = bench_adjoint(dev, args)
+    if args.only in ("all", "pagerank"):
+        extras["pagerank_rmat"] = bench_pagerank(A, dev, args)
     if args.only == "all" or args.compare:
         extras["rocsparse_rmat_f64"] = bench_rocsparse(A, x, dev, args, ms)
     if args.only == "all" or args.copy_probe:
