@@ -51,6 +51,10 @@ def test_host_side_argument_checks():
     assert L.mbx_spmv_f32(None, None, None, None, 1.0, None, 0.0, None, None) == -1
     assert L.mbx_spmv_bf16(None, None, None, None, 1.0, None, 0.0, None, None) == -1
     assert L.mbx_plan_info(None, None, None, None) == -1
+    assert L.mbx_pagerank_f64(None, None, None, None, 0.85, 20, None, None, None, 0, None) == -1
+    assert L.mbx_pagerank_workspace_bytes(-1) == 0
+    assert L.mbx_pagerank_workspace_bytes(1000) >= 8 * 1000
+    assert L.mbx_ingest_ref(-1, 0, None, None, 0, None, None, None, None) == -1
     assert _lib.status_str(0) == "ok"
     assert "overflow" not in _lib.status_str(-1)
     assert "int32" in _lib.status_str(-2)
