@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 import torch
 
-from helpers import cond_err
+from helpers import cond_err, random_csr
 
 pytestmark = pytest.mark.gpu
 
@@ -232,4 +232,44 @@ def test_batch_unaligned_member(device):
         ref = plan.spmv(rp, col, val, x)
         plan.close()
         assert torch.equal(ref.view(torch.uint8), g.view(torch.uint8))
+    b.close()
+
+
+@pytest.mark.parametrize("kind", ["f64", "bf16"])
+def test_batch_degenerate_members(device, kind):
+    """Members with no rows, with rows but no entries, a single entry, and a
+    normal matrix between them: every member equals its single-matrix
+    product bit for bit, and a member with no entries gets y = 0."""
+    from merbit_amd import gen
+    from merbit_amd.batch import MerbitBatch
+    from merbit_amd.spmv import MerbitPlan
+    say(f"batch degenerate members {kind}")
+    rng = np.random.default_rng(31)
+    shapes = [([], 5), ([0] * 1000, 40), ([1], 1), (list(rng.poisson(9, 3000)), 2500), ([], 0),
+              ([0, 0, 3, 0], 7)]
+    mats = []
+    for lens, ncol in shapes:
+        rp, col = random_csr(rng, len(lens), max(ncol, 1), lens=lens) if lens else \
+            (np.zeros(1, np.int32), np.zeros(0, np.int32))
+        v32 = rng.uniform(-1, 1, int(rp[-1])).astype(np.float32)
+        if kind == "f64":
+            v, x = v32.astype(np.float64), rng.uniform(-1, 1, ncol)
+        else:
+            v, x = gen.to_bf16(v32), rng.uniform(-1, 1, ncol).astype(np.float32)
+        ydt = torch.float64 if kind == "f64" else torch.float32
+        mats.append((torch.from_numpy(rp).to(device), torch.from_numpy(col).to(device),
+                     torch.from_numpy(v).to(device), torch.from_numpy(x).to(device),
+                     torch.full((len(lens),), float("nan"), dtype=ydt, device=device), ncol))
+    b = MerbitBatch(mats, items_per_thread=8)
+    b.spmv()
+    torch.cuda.synchronize()
+    for i, (rp, col, val, x, y, ncol) in enumerate(mats):
+        if y.numel() == 0:
+            continue
+        plan = MerbitPlan(rp, ncol, col.numel(), items_per_thread=8)
+        ref = plan.spmv(rp, col, val, x)
+        plan.close()
+        assert torch.equal(ref.view(torch.uint8), y.view(torch.uint8)), i
+        if col.numel() == 0:
+            assert torch.all(y == 0), i
     b.close()
