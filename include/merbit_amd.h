@@ -55,6 +55,16 @@ int mbx_plan_create(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol,
 /* Replaces ~TILE, include/tile.cuh:19-24.  Waits for the device. */
 void mbx_plan_destroy(mbx_plan* plan);
 
+/* A second plan object on the same partition with carry slots of its own
+ * (no reference counterpart: the reference's plan serves one stream).
+ * Products on a plan use its carry slots, so two products on one plan must
+ * not overlap in time; a plan and its clones share the read-only tile
+ * coordinates and may run products concurrently on different streams when
+ * their outputs differ.  Allocates 16*(ntiles+1) bytes; no kernel runs.
+ * Every plan object is destroyed on its own; the coordinates are freed with
+ * the last one. */
+int mbx_plan_clone(const mbx_plan* plan, mbx_plan** out);
+
 /* Plan geometry: number of tiles, merge items per tile, workgroup size. */
 int mbx_plan_info(const mbx_plan* plan, int32_t* ntiles, int32_t* tile_items,
                   int32_t* wg_size);

@@ -5,11 +5,14 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
+
 #include "merbit_amd.h"
 
 // The merge-path plan: one (rows consumed, nnz consumed, row_ptr[rows])
 // coordinate per tile boundary of wg*ipt merge items, plus FP64 carry slots
-// (tail, head) per tile used by the SpMV carry fixup.
+// (tail, head) per tile used by the SpMV carry fixup.  Clones share the
+// coordinates (counted by coord_refs) and own their carry slots.
 struct mbx_plan {
   int32_t nrow, ncol, nnz;
   int32_t wg, ipt, tile_items, ntiles;
@@ -17,6 +20,7 @@ struct mbx_plan {
   int32_t split_rows;  // tiles whose first row began earlier (fixup needed)
   int4* d_coord;  // per tile boundary: rows, nnz consumed, row_ptr[rows]
   double* d_carry;
+  std::atomic<int>* coord_refs;  // plan objects sharing d_coord
   mutable hipEvent_t ev_begin, ev_end;
 };
 

@@ -181,7 +181,12 @@ int mbx_plan_create(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol,
   p->ntiles = (int)((path + T - 1) / T);
   p->d_coord = nullptr;
   p->d_carry = nullptr;
+  p->coord_refs = new (std::nothrow) std::atomic<int>(1);
   p->ev_begin = p->ev_end = nullptr;
+  if (!p->coord_refs) {
+    mbx_plan_destroy(p);
+    return MBX_ENOMEM;
+  }
   const size_t nt1 = (size_t)p->ntiles + 1;
   if (hipMalloc((void**)&p->d_coord, sizeof(int4) * nt1) != hipSuccess ||
       hipMalloc((void**)&p->d_carry, sizeof(double) * 2 * nt1) != hipSuccess) {
@@ -218,9 +223,31 @@ int mbx_plan_create(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol,
 
 void mbx_plan_destroy(mbx_plan* p) {
   if (!p) return;
-  if (p->d_coord) (void)hipFree(p->d_coord);
+  // the coordinates go with the last plan object that shares them
+  if (!p->coord_refs || p->coord_refs->fetch_sub(1) == 1) {
+    if (p->d_coord) (void)hipFree(p->d_coord);
+    delete p->coord_refs;
+  }
   if (p->d_carry) (void)hipFree(p->d_carry);
   delete p;
+}
+
+int mbx_plan_clone(const mbx_plan* p, mbx_plan** out) {
+  if (!p || !out || !p->coord_refs) return MBX_EINVAL;
+  *out = nullptr;
+  mbx_plan* c = new (std::nothrow) mbx_plan(*p);
+  if (!c) return MBX_ENOMEM;
+  c->ev_begin = c->ev_end = nullptr;
+  c->d_carry = nullptr;
+  if (hipMalloc((void**)&c->d_carry, sizeof(double) * 2 * ((size_t)p->ntiles + 1)) !=
+      hipSuccess) {
+    (void)hipGetLastError();
+    delete c;
+    return MBX_ENOMEM;
+  }
+  p->coord_refs->fetch_add(1);
+  *out = c;
+  return MBX_OK;
 }
 
 int mbx_plan_info(const mbx_plan* p, int32_t* ntiles, int32_t* tile_items, int32_t* wg_size) {

@@ -197,6 +197,19 @@ class MerbitPlan:
         return int(lib().mbx_spmm_workspace_bytes(self.handle, int(X.shape[1]), X.data_ptr(),
                                                   ldx, xl))
 
+    def clone(self) -> "MerbitPlan":
+        """A plan object on the same partition with carry slots of its own
+        (mbx_plan_clone): products on a plan and on its clones may run
+        concurrently on different streams when their y differ."""
+        h = ctypes.c_void_p()
+        check(lib().mbx_plan_clone(self.handle, ctypes.byref(h)), "mbx_plan_clone")
+        c = MerbitPlan.__new__(MerbitPlan)
+        c.nrow, c.ncol, c.nnz = self.nrow, self.ncol, self.nnz
+        c.ntiles, c.tile_items, c.wg_size = self.ntiles, self.tile_items, self.wg_size
+        c.items_per_thread = self.items_per_thread
+        c._h = h
+        return c
+
     def close(self) -> None:
         if getattr(self, "_h", None) is not None:
             lib().mbx_plan_destroy(self._h)

@@ -276,3 +276,38 @@ def test_pattern_stats(orc, device):
         assert st["split_tiles"] == len(split)
         assert st["long_rows"] == sum(1 for t in split if t - (rows[t] + rp[rows[t]]) // T > 64)
         plan.close()
+
+
+def test_plan_clones_on_concurrent_streams(orc, device):
+    """A plan and its clone share the partition and own their carry slots:
+    products on two streams at once equal the products run one by one, and
+    destroying the original leaves the clone working."""
+    from merbit_amd.spmv import MerbitPlan
+    say("plan clone, two streams")
+    rng = np.random.default_rng(41)
+    lens = (rng.pareto(1.0, 40000) * 6).astype(int)
+    lens[7] = 20 * 2048  # a row across many tiles: the carry slots are in use
+    rp, col = random_csr(rng, len(lens), 30000, lens=lens)
+    val = rng.standard_normal(int(rp[-1]))
+    xs = [rng.uniform(-1, 1, 30000) for _ in range(2)]
+    d_rp, d_col, d_val = dev(rp, device), dev(col, device), dev(val, device)
+    d_x = [dev(x, device) for x in xs]
+    plan = MerbitPlan(d_rp, 30000, int(rp[-1]))
+    twin = plan.clone()
+    want = [plan.spmv(d_rp, d_col, d_val, d_x[i]).clone() for i in range(2)]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    ys = [torch.empty(len(lens), dtype=torch.float64, device=device) for _ in range(2)]
+    torch.cuda.synchronize()
+    for _ in range(20):
+        for i, (p, s) in enumerate(((plan, streams[0]), (twin, streams[1]))):
+            with torch.cuda.stream(s):
+                p.spmv(d_rp, d_col, d_val, d_x[i], ys[i], stream=s)
+    torch.cuda.synchronize()
+    for i in range(2):
+        assert torch.equal(ys[i], want[i]), i
+    plan.close()
+    y = twin.spmv(d_rp, d_col, d_val, d_x[1])
+    assert torch.equal(y, want[1])
+    ya, absr = orc.spmv_accurate(rp, col, val, xs[1])
+    assert cond_err(y.cpu().numpy(), ya, absr) <= F64_TOL
+    twin.close()
