@@ -103,26 +103,34 @@ def load_traffic(name):
 
 
 def pick_ipt(args, rp, ncol, nnz, make_calls):
-    """Items per thread: --ipt when given, else the candidate whose plan runs
-    the timed workload (make_calls(plan): the rotated products) fastest over
-    a short timing of each (the smallest of those within 1 % of the
+    """Tile shape and stream cache policy: --ipt / --stream when given, else
+    the (items per thread, policy) pair whose plan runs the timed workload
+    (make_calls(plan): the rotated products) fastest over a short timing of
+    each (the smallest shape, cached first, of those within 1 % of the
     fastest).  Tuning on the rotated copies matters: on one copy resident in
     the 256 MB cache the candidates 8, 12 and 16 time alike on the R-MAT
-    matrix, while rotated copies separate them by 5 %."""
-    if args.ipt:
-        return args.ipt
+    matrix, while rotated copies separate them by 5 %; and a matrix read
+    once per product gains from non-temporal stream loads that cost a
+    cache-resident one 6 %."""
     from merbit_amd.spmv import ITEMS_PER_THREAD, MerbitPlan
+    ipts = [args.ipt] if args.ipt else list(ITEMS_PER_THREAD)
+    modes = [args.stream] if args.stream else ["cached", "nontemporal"]
+    if len(ipts) == 1 and len(modes) == 1:
+        return ipts[0], modes[0]
     times = {}
-    for ipt in ITEMS_PER_THREAD:
-        plan = MerbitPlan(rp, ncol, nnz, items_per_thread=ipt)
-        calls = make_calls(plan)
-        times[ipt] = min(time_steps(calls, 10 * len(calls), 2 * len(calls))[0]
-                         for _ in range(3))
-        plan.close()
-    # candidates within 1 % of the fastest time alike: take the smallest of
+    for ipt in ipts:
+        for mode in modes:
+            plan = MerbitPlan(rp, ncol, nnz, items_per_thread=ipt)
+            plan.set_stream(mode)
+            calls = make_calls(plan)
+            times[(ipt, mode)] = min(time_steps(calls, 10 * len(calls), 2 * len(calls))[0]
+                                     for _ in range(3))
+            plan.close()
+    # candidates within 1 % of the fastest time alike: take the first of
     # them, so that run-to-run noise does not flip the choice
     fastest = min(times.values())
-    return min(ipt for ipt, ms in times.items() if ms <= 1.01 * fastest)
+    return min((k for k, ms in times.items() if ms <= 1.01 * fastest),
+               key=lambda k: (k[0], modes.index(k[1])))
 
 
 def bench_rmat_f64(A, x, dev, args):
@@ -141,7 +149,7 @@ def bench_rmat_f64(A, x, dev, args):
         return [(lambda c=c: plan.spmv(c["rp"], c["col"], c["val"], c["x"], c["y"]))
                 for c in copies]
 
-    ipt = pick_ipt(args, c0["rp"], ncol, nnz, make_calls)
+    ipt, stream = pick_ipt(args, c0["rp"], ncol, nnz, make_calls)
     plan_ms = []
     for _ in range(5):  # plan construction (the reference's tim1), host clock
         t0 = time.perf_counter()
@@ -149,15 +157,17 @@ def bench_rmat_f64(A, x, dev, args):
         plan_ms.append((time.perf_counter() - t0) * 1e3)
         plan.close()
     plan = MerbitPlan(c0["rp"], ncol, nnz, items_per_thread=ipt)
+    plan.set_stream(stream)
     calls = make_calls(plan)
     ms, kern = time_steps(calls, args.steps, args.warmup, plan.set_events)
     warm_ms = warm_kern = None
-    if not args.no_warm:
+    if not args.no_warm:  # one copy over and over: cache-resident, plain loads
+        plan.set_stream("cached")
         warm_ms, warm_kern = time_steps(calls[:1], args.steps, args.warmup, plan.set_events)
-    log(f"rmat fp64 (ipt {ipt}): {ms * 1e3:.1f} us/step rotated ({gbs(b_alg, ms)} GB/s), tile kernel "
+    log(f"rmat fp64 (ipt {ipt}, {stream} stream): {ms * 1e3:.1f} us/step rotated ({gbs(b_alg, ms)} GB/s), tile kernel "
         f"{kern * 1e3:.1f} us" + (f"; warm {warm_ms * 1e3:.1f} us" if warm_ms else ""))
     res = dict(b_alg=b_alg, ms=ms, kern=kern, warm_ms=warm_ms, warm_kern=warm_kern,
-               ntiles=plan.ntiles, tile_items=plan.tile_items, ipt=ipt,
+               ntiles=plan.ntiles, tile_items=plan.tile_items, ipt=ipt, stream=stream,
                plan_ms=float(np.median(plan_ms)))
     plan.close()
     del copies
@@ -183,14 +193,15 @@ def bench_mixed(name, A, x, dev, args, f64_ms=None, copies=1):
         def make_calls(plan, vals=vals, xx=xx, y=y):
             return [(lambda i=i: plan.spmv(rp, cols[i], vals[i], xx, y)) for i in range(copies)]
 
-        ipt = pick_ipt(args, rp, ncol, nnz, make_calls)
+        ipt, stream = pick_ipt(args, rp, ncol, nnz, make_calls)
         plan = MerbitPlan(rp, ncol, nnz, items_per_thread=ipt)
+        plan.set_stream(stream)
         calls = make_calls(plan)
         f64_ms, kern = time_steps(calls, args.steps, args.warmup, plan.set_events)
         b64 = workloads.algorithmic_bytes(nrow, ncol, nnz, 8, 8, 8)
         out["f64"] = {"gbs": gbs(b64, f64_ms), "us_per_step": round(f64_ms * 1e3, 2),
                       "tile_kernel_us": round(kern * 1e3, 2), "items_per_thread": ipt,
-                      "pct_hbm_peak": round(100 * b64 / (f64_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 1)}
+                      "stream": stream, "pct_hbm_peak": round(100 * b64 / (f64_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 1)}
         plan.close()
         del vals, y
     for kind in ("f32", "bf16"):
@@ -203,8 +214,9 @@ def bench_mixed(name, A, x, dev, args, f64_ms=None, copies=1):
         def make_calls(plan, vals=vals, xx=xx, y=y):
             return [(lambda i=i: plan.spmv(rp, cols[i], vals[i], xx, y)) for i in range(copies)]
 
-        ipt = pick_ipt(args, rp, ncol, nnz, make_calls)
+        ipt, stream = pick_ipt(args, rp, ncol, nnz, make_calls)
         plan = MerbitPlan(rp, ncol, nnz, items_per_thread=ipt)
+        plan.set_stream(stream)
         calls = make_calls(plan)
         ms, kern = time_steps(calls, args.steps, args.warmup, plan.set_events)
         ref = plan.spmv(rp, cols[0], torch.from_numpy(wide).to(dev),
@@ -214,8 +226,10 @@ def bench_mixed(name, A, x, dev, args, f64_ms=None, copies=1):
         out[kind] = {"gbs": gbs(b, ms), "gflops": round(2 * nnz / (ms * 1e-3) / 1e9, 1),
                      "us_per_step": round(ms * 1e3, 2), "tile_kernel_us": round(kern * 1e3, 2),
                      "speedup_vs_f64": round(f64_ms / ms, 3), "items_per_thread": ipt,
+                     "stream": stream,
                      "rel_err_vs_f64_same_values": rel_err(y, ref)}
-        log(f"{name} {kind}: {ms * 1e3:.1f} us/step (ipt {ipt}), x{f64_ms / ms:.2f} vs fp64, "
+        log(f"{name} {kind}: {ms * 1e3:.1f} us/step (ipt {ipt}, {stream}), x{f64_ms / ms:.2f} "
+            f"vs fp64, "
             f"err {out[kind]['rel_err_vs_f64_same_values']:.2e}")
         plan.close()
         del vals, y, ref
@@ -450,6 +464,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--ipt", type=int, default=0,
                     help="items per thread (SIGMA); 0 = picked by the timing autotuner")
+    ap.add_argument("--stream", choices=["cached", "nontemporal"], default=None,
+                    help="stream cache policy; default: picked with the tile shape")
     ap.add_argument("--copies", type=int, default=3)
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--batch-count", type=int, default=64)
@@ -533,6 +549,7 @@ def main():
         "config": {"workload": "fp64 merge SpMV, R-MAT power law 2^20 rows 2^24 nnz "
                                "(a=.57,b=c=.19), 3 rotated copies",
                    "nrow": A.nrow, "nnz": A.nnz, "items_per_thread": h["ipt"],
+                   "stream_loads": h["stream"],
                    "wg_size": 256, "copies_rotated": max(1, args.copies)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

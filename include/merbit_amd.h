@@ -75,6 +75,17 @@ int mbx_plan_info(const mbx_plan* plan, int32_t* ntiles, int32_t* tile_items,
 int mbx_plan_partition(const mbx_plan* plan, int32_t* h_rows, int32_t* h_nnz,
                        int32_t count);
 
+/* Cache policy of a product's row_ptr/col/value stream (no reference
+ * counterpart).  MBX_STREAM_CACHED: plain loads, so a matrix that fits the
+ * 256 MB MALL stays there for the next product (repeated products of one
+ * matrix).  MBX_STREAM_NONTEMPORAL: non-temporal loads for the parts the
+ * value kind gains from (row_ptr and col; FP32/BF16 values too), so data
+ * read once does not displace the x lines the gathers reuse (matrices that
+ * do not stay resident).  MBX_STREAM_AUTO (the default): non-temporal when
+ * the product's bytes exceed 240 MiB.  Results are identical in every mode. */
+enum { MBX_STREAM_AUTO = 0, MBX_STREAM_CACHED = 1, MBX_STREAM_NONTEMPORAL = 2 };
+int mbx_plan_set_stream(mbx_plan* plan, int32_t mode);
+
 /* Optional timing hook: when ev_begin/ev_end (hipEvent_t) are non-NULL, the
  * next spmv call on this plan records them immediately before and after its
  * tile kernel on the call's stream, then clears them. */
@@ -144,6 +155,9 @@ void mbx_batch_destroy(mbx_batch* batch);
 int mbx_batch_info(const mbx_batch* batch, int32_t* total_tiles, int32_t* tile_items,
                    int32_t* grid);
 int mbx_batch_spmv(const mbx_batch* batch, double alpha, double beta, void* stream);
+/* The stream cache policy (MBX_STREAM_*) of the batch's products; AUTO
+ * decides on the bytes of the whole batch. */
+int mbx_batch_set_stream(mbx_batch* batch, int32_t mode);
 
 /* HIP graphs of repeated products (the reference's timed loop of
  * maxiter*perrnd launches, include/tile.cuh:596-603): `repeats` back-to-back

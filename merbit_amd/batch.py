@@ -83,6 +83,12 @@ class MerbitBatch:
         self.total_tiles, self.tile_items, self.grid = tt.value, ti.value, gr.value
         self.items_per_thread = items_per_thread
 
+    def set_stream(self, mode: str) -> None:
+        """Cache policy of the batch's stream loads (mbx_batch_set_stream):
+        "auto", "cached" or "nontemporal" (MerbitPlan.set_stream)."""
+        from .spmv import STREAM_MODES
+        check(lib().mbx_batch_set_stream(self._h, STREAM_MODES[mode]), "mbx_batch_set_stream")
+
     def spmv(self, alpha: float = 1.0, beta: float = 0.0, stream=None) -> None:
         check(lib().mbx_batch_spmv(self._h, alpha, beta, _stream_handle(stream)),
               "mbx_batch_spmv")

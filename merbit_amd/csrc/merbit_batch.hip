@@ -138,6 +138,8 @@ struct mbx_batch {
   int* d_tile_base;
   int4* d_coord;  // per tile boundary: rows, nnz consumed, row_ptr[rows]
   double* d_carry;
+  long long footprint;  // bytes one batch product reads and writes
+  int32_t stream_mode;  // MBX_STREAM_*
 };
 
 namespace {
@@ -164,7 +166,12 @@ void batch_launch(const mbx_batch* b, hipStream_t s, typename P::scal_t alpha,
   const int bz = beta == typename P::scal_t(0);
   const int grid = std::min(b->grid, b->total_tiles);
   const size_t lds = TileSmem<256, IPT, P>::bytes();
-  if (b->vec)
+  const bool nt = mbx_detail::stream_nontemporal(b->stream_mode, b->footprint);
+  if (b->vec && nt)
+    batch_spmv_kernel<256, IPT, true, mbx_detail::Streamed<P>><<<grid, 256, lds, s>>>(
+        b->d_mats, b->d_tile_base, b->count, b->total_tiles, b->d_coord, b->d_carry, alpha, beta,
+        bz);
+  else if (b->vec)
     batch_spmv_kernel<256, IPT, true, P><<<grid, 256, lds, s>>>(
         b->d_mats, b->d_tile_base, b->count, b->total_tiles, b->d_coord, b->d_carry, alpha, beta,
         bz);
@@ -236,8 +243,10 @@ int mbx_batch_create(const mbx_batch_item* items, int32_t count, int32_t value_k
   const long long T = 256ll * ipt;
   std::vector<BatchMat> h((size_t)count);
   std::vector<int> tb((size_t)count);
-  long long tiles = 0;
+  long long tiles = 0, footprint = 0;
   bool vec = true;
+  const long long s_val = value_kind == MBX_VALUE_F64 ? 8 : value_kind == MBX_VALUE_F32 ? 4 : 2;
+  const long long s_vec = value_kind == MBX_VALUE_F64 ? 8 : 4;
   for (int m = 0; m < count; ++m) {
     const mbx_batch_item& it = items[m];
     if (it.nrow < 0 || it.ncol < 0 || it.nnz < 0) return MBX_EINVAL;
@@ -249,6 +258,8 @@ int mbx_batch_create(const mbx_batch_item* items, int32_t count, int32_t value_k
     h[(size_t)m] = BatchMat{it.row_ptr, it.col, it.val, it.x, it.y, it.nrow, it.ncol, it.nnz,
                             (int32_t)tiles, (int32_t)(tiles + m)};
     vec = vec && vec_item_ok(it, value_kind);
+    footprint += 4ll * (it.nrow + 1) + (long long)it.nnz * (4 + s_val) +
+                 s_vec * ((long long)it.ncol + it.nrow);
     tb[(size_t)m] = (int)tiles;
     tiles += nt;
     if (tiles + count >= (long long)INT32_MAX / 2) return MBX_EOVERFLOW;
@@ -258,6 +269,8 @@ int mbx_batch_create(const mbx_batch_item* items, int32_t count, int32_t value_k
   b->count = count;
   b->kind = value_kind;
   b->vec = vec;
+  b->footprint = footprint;
+  b->stream_mode = MBX_STREAM_AUTO;
   b->ipt = ipt;
   b->tile_items = (int)T;
   b->total_tiles = (int)tiles;
@@ -330,6 +343,12 @@ int mbx_batch_info(const mbx_batch* b, int32_t* total_tiles, int32_t* tile_items
   if (total_tiles) *total_tiles = b->total_tiles;
   if (tile_items) *tile_items = b->tile_items;
   if (grid) *grid = std::min(b->grid, b->total_tiles);
+  return MBX_OK;
+}
+
+int mbx_batch_set_stream(mbx_batch* b, int32_t mode) {
+  if (!b || mode < MBX_STREAM_AUTO || mode > MBX_STREAM_NONTEMPORAL) return MBX_EINVAL;
+  b->stream_mode = mode;
   return MBX_OK;
 }
 

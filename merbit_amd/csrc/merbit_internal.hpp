@@ -21,6 +21,7 @@ struct mbx_plan {
   int4* d_coord;  // per tile boundary: rows, nnz consumed, row_ptr[rows]
   double* d_carry;
   std::atomic<int>* coord_refs;  // plan objects sharing d_coord
+  int32_t stream_mode;           // MBX_STREAM_*
   mutable hipEvent_t ev_begin, ev_end;
 };
 
@@ -61,6 +62,14 @@ inline bool vec_path_ok(const int32_t* rp, const int32_t* col, const V* val, int
   const uintptr_t v = reinterpret_cast<uintptr_t>(val);
   const uint64_t x_bytes = (uint64_t)ncol * (sizeof(V) == 8 ? 8u : 4u);
   return (a & 15u) == 0 && (v & (sizeof(V) == 2 ? 7u : 15u)) == 0 && x_bytes <= 0xFFFFFFFFull;
+}
+
+// Whether a product reads its row_ptr/col/value stream non-temporally:
+// asked for, or (MBX_STREAM_AUTO) when the product's bytes exceed what stays
+// resident in the 256 MB MALL between products.
+constexpr long long kResidentBytes = 240ll << 20;
+inline bool stream_nontemporal(int32_t mode, long long footprint) {
+  return mode == MBX_STREAM_NONTEMPORAL || (mode == MBX_STREAM_AUTO && footprint > kResidentBytes);
 }
 
 // Launch-side validation shared by every entry point taking a plan.

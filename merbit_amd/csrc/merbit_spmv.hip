@@ -109,9 +109,17 @@ hipError_t launch_tile(const mbx_plan* p, hipStream_t s, const int32_t* rp,
                        typename P::scal_t alpha, typename P::scal_t beta) {
   const int bz = beta == typename P::scal_t(0);
   const bool vec = mbx_detail::vec_path_ok(rp, col, val, p->ncol);
+  const long long footprint = 4ll * (p->nrow + 1) +
+                              (long long)p->nnz * (4 + sizeof(typename P::val_t)) +
+                              (long long)p->ncol * sizeof(typename P::x_t) +
+                              (long long)p->nrow * sizeof(typename P::y_t);
+  const bool nt = mbx_detail::stream_nontemporal(p->stream_mode, footprint);
   if (p->ev_begin) (void)hipEventRecord(p->ev_begin, s);
   const size_t lds = TileSmem<256, IPT, P>::bytes();
-  if (vec)
+  if (vec && nt)
+    tile_spmv_kernel<256, IPT, true, mbx_detail::Streamed<P>><<<p->ntiles, 256, lds, s>>>(
+        rp, col, val, x, y, p->d_coord, p->d_carry, p->ntiles, p->ncol, alpha, beta, bz);
+  else if (vec)
     tile_spmv_kernel<256, IPT, true, P><<<p->ntiles, 256, lds, s>>>(
         rp, col, val, x, y, p->d_coord, p->d_carry, p->ntiles, p->ncol, alpha, beta, bz);
   else
@@ -182,6 +190,7 @@ int mbx_plan_create(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol,
   p->d_coord = nullptr;
   p->d_carry = nullptr;
   p->coord_refs = new (std::nothrow) std::atomic<int>(1);
+  p->stream_mode = MBX_STREAM_AUTO;
   p->ev_begin = p->ev_end = nullptr;
   if (!p->coord_refs) {
     mbx_plan_destroy(p);
@@ -273,6 +282,12 @@ int mbx_plan_partition(const mbx_plan* p, int32_t* h_rows, int32_t* h_nnz, int32
     h_nnz[i] = tmp[i].y;
   }
   delete[] tmp;
+  return MBX_OK;
+}
+
+int mbx_plan_set_stream(mbx_plan* p, int32_t mode) {
+  if (!p || mode < MBX_STREAM_AUTO || mode > MBX_STREAM_NONTEMPORAL) return MBX_EINVAL;
+  p->stream_mode = mode;
   return MBX_OK;
 }
 

@@ -22,9 +22,14 @@ __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
 }
 
 struct F64 {
-  // plain loads for the row_ptr/col/val stream (non-temporal ones measured
-  // 1-3 % slower for FP64)
-  static constexpr bool kStreamNT = false;
+  // Stream loads (row_ptr, col, values) are plain.  Streamed<F64> reads
+  // row_ptr and col non-temporally (2 % faster on a cold R-MAT, 1 % on the
+  // banded matrix) and keeps the values plain (non-temporal FP64 values
+  // cost the banded product 9 %).
+  static constexpr bool kIndexNT = false;
+  static constexpr bool kValueNT = false;
+  static constexpr bool kStreamedIndexNT = true;
+  static constexpr bool kStreamedValueNT = false;
   using val_t = double;
   using x_t = double;
   using y_t = double;
@@ -57,9 +62,13 @@ struct F64 {
 // wavefront scan, the cross-wave carries and the tile carries) are FP64.
 template <class V>
 struct CompF32 {
-  // non-temporal loads for the row_ptr/col/val stream: 4-8 % faster on the
-  // banded and R-MAT matrices
-  static constexpr bool kStreamNT = true;
+  // Stream loads are plain; Streamed<CompF32> reads row_ptr, col and the
+  // values non-temporally (4-8 % faster on the cold banded and R-MAT
+  // matrices).
+  static constexpr bool kIndexNT = false;
+  static constexpr bool kValueNT = false;
+  static constexpr bool kStreamedIndexNT = true;
+  static constexpr bool kStreamedValueNT = true;
   using val_t = V;
   using x_t = float;
   using y_t = float;
@@ -111,6 +120,17 @@ struct CompF32 {
 
 using F32C = CompF32<float>;
 using BF16C = CompF32<uint16_t>;
+
+// P in its streamed mode: the stream loads P marks are non-temporal, so that
+// data read once per product does not displace the x lines the gathers
+// reuse.  For products whose data does not stay cache-resident from one
+// product to the next; when it does, the plain loads of P keep it there
+// (non-temporal loads cost a warm R-MAT product 6 %).
+template <class P>
+struct Streamed : P {
+  static constexpr bool kIndexNT = P::kStreamedIndexNT;
+  static constexpr bool kValueNT = P::kStreamedValueNT;
+};
 
 // out[0] = most rows ending in one tile, out[1] = tiles whose first row
 // began in an earlier tile (they need the carry fixup kernel).
@@ -428,14 +448,14 @@ __device__ __forceinline__ void tile_issue_loads(TileRegs<WG, IPT, P>& R,
 #pragma unroll
   for (int j = 0; j < NG; ++j)
     if (j == 0 || j * WG < nrg)
-      load_group<P::kStreamNT>(row_ptr, min(rb + tid + j * WG, rl), R.rg[j]);
+      load_group<P::kIndexNT>(row_ptr, min(rb + tid + j * WG, rl), R.rg[j]);
   if (nnz_t > 0) {  // uniform per tile
 #pragma unroll
     for (int j = 0; j < NG; ++j) {
       if (j == 0 || j * WG < nkg) {
         const int g = min(kb + tid + j * WG, klast);
-        load_group<P::kStreamNT>(col, g, R.cg[j]);
-        load_group_clamped<P::kStreamNT>(val, g, R.vg[j], k0, k1);
+        load_group<P::kIndexNT>(col, g, R.cg[j]);
+        load_group_clamped<P::kValueNT>(val, g, R.vg[j], k0, k1);
       }
     }
   }

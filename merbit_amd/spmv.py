@@ -15,6 +15,9 @@ import torch
 
 from ._lib import check, lib
 
+# Stream cache policies of mbx_plan_set_stream / mbx_batch_set_stream.
+STREAM_MODES = {"auto": 0, "cached": 1, "nontemporal": 2}
+
 # Items per thread with an instantiated tile kernel (the reference's SIGMA).
 ITEMS_PER_THREAD = (2, 4, 6, 8, 10, 12, 14, 16)
 
@@ -81,6 +84,13 @@ class MerbitPlan:
         check(lib().mbx_plan_stats(self.handle, _dptr(row_ptr, torch.int32, "row_ptr"),
                                    ctypes.byref(st)), "mbx_plan_stats")
         return {name: getattr(st, name) for name, _ in PatternStats._fields_}
+
+    def set_stream(self, mode: str) -> None:
+        """Cache policy of the row_ptr/col/value stream (mbx_plan_set_stream):
+        "auto" (non-temporal past 240 MiB per product), "cached" (plain
+        loads: the matrix can stay in the MALL between products) or
+        "nontemporal"."""
+        check(lib().mbx_plan_set_stream(self.handle, STREAM_MODES[mode]), "mbx_plan_set_stream")
 
     def set_events(self, begin, end) -> None:
         """Record torch.cuda.Event pair around the next call's tile kernel."""

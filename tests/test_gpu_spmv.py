@@ -311,3 +311,40 @@ def test_plan_clones_on_concurrent_streams(orc, device):
     ya, absr = orc.spmv_accurate(rp, col, val, xs[1])
     assert cond_err(y.cpu().numpy(), ya, absr) <= F64_TOL
     twin.close()
+
+
+@pytest.mark.parametrize("kind", ["f64", "f32", "bf16"])
+def test_stream_policies_identical(device, kind):
+    """Cached, non-temporal and automatic stream loads change only cache
+    hints: the products are bit-identical, for a plan and for a batch."""
+    from merbit_amd import gen
+    from merbit_amd.batch import MerbitBatch
+    from merbit_amd.spmv import MerbitPlan
+    say(f"stream policies {kind}")
+    rng = np.random.default_rng(43)
+    lens = (rng.pareto(1.0, 30000) * 6).astype(int)
+    rp, col = random_csr(rng, len(lens), 20000, lens=lens)
+    v32 = rng.uniform(-1, 1, int(rp[-1])).astype(np.float32)
+    if kind == "f64":
+        val, x = v32.astype(np.float64), rng.uniform(-1, 1, 20000)
+    else:
+        val = v32 if kind == "f32" else gen.to_bf16(v32)
+        x = rng.uniform(-1, 1, 20000).astype(np.float32)
+    d_rp, d_col, d_val, d_x = (dev(a, device) for a in (rp, col, val, x))
+    plan = MerbitPlan(d_rp, 20000, int(rp[-1]), items_per_thread=12)
+    outs = {}
+    for mode in ("auto", "cached", "nontemporal"):
+        plan.set_stream(mode)
+        outs[mode] = plan.spmv(d_rp, d_col, d_val, d_x).clone()
+    plan.close()
+    ydt = outs["auto"].dtype
+    mats = [(d_rp, d_col, d_val, d_x, torch.empty(len(lens), dtype=ydt, device=device), 20000)]
+    b = MerbitBatch(mats, items_per_thread=12)
+    bouts = {}
+    for mode in ("cached", "nontemporal"):
+        b.set_stream(mode)
+        b.spmv()
+        bouts[mode] = mats[0][4].clone()
+    b.close()
+    for o in list(outs.values()) + list(bouts.values()):
+        assert torch.equal(o.view(torch.uint8), outs["cached"].view(torch.uint8))
