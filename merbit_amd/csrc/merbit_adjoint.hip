@@ -101,8 +101,10 @@ __global__ __launch_bounds__(kPassWg) void perm_pass1_kernel(const T* __restrict
   if (VEC) {
 #pragma unroll 4
     for (int j = 4 * threadIdx.x; j < n4; j += 4 * kPassWg) {
-      const vec4_t<T> v = *reinterpret_cast<const vec4_t<T>*>(val + base + j);
-      const vec4_t<uint16_t> l = *reinterpret_cast<const vec4_t<uint16_t>*>(lslot + base + j);
+      const vec4_t<T> v =
+          __builtin_nontemporal_load(reinterpret_cast<const vec4_t<T>*>(val + base + j));
+      const vec4_t<uint16_t> l =
+          __builtin_nontemporal_load(reinterpret_cast<const vec4_t<uint16_t>*>(lslot + base + j));
       s_v[l.x] = v.x;
       s_v[l.y] = v.y;
       s_v[l.z] = v.z;
@@ -114,7 +116,8 @@ __global__ __launch_bounds__(kPassWg) void perm_pass1_kernel(const T* __restrict
   // one slot per lane: a store instruction covers 64 consecutive slots, so
   // it touches as few bucket runs (staging lines) as possible
 #pragma unroll 8
-  for (int j = threadIdx.x; j < n; j += kPassWg) stage[spos[base + j]] = s_v[j];
+  for (int j = threadIdx.x; j < n; j += kPassWg)
+    stage[__builtin_nontemporal_load(spos + base + j)] = s_v[j];
 }
 
 // Pass 2: one workgroup per bucket loads the bucket's staging slice into LDS
@@ -140,7 +143,8 @@ __global__ __launch_bounds__(kPassWg) void perm_pass2_kernel(const T* __restrict
   const int b = (int)base;  // staging indices of this bucket are base + [0, n)
 #pragma unroll 4
   for (int j = 4 * threadIdx.x; j < n4; j += 4 * kPassWg) {
-    const vec4_t<int32_t> i = *reinterpret_cast<const vec4_t<int32_t>*>(lidx + base + j);
+    const vec4_t<int32_t> i =
+        __builtin_nontemporal_load(reinterpret_cast<const vec4_t<int32_t>*>(lidx + base + j));
     vec4_t<T> v;
     v.x = s_v[i.x - b];
     v.y = s_v[i.y - b];
@@ -220,8 +224,10 @@ __global__ __launch_bounds__(WG) void sddmm_tile_kernel(
       const int i = tid + j * WG;
       re[j] = INT_MAX;
       ci[j] = 0;
-      if (i < rows_t) re[j] = row_ptr[r0 + 1 + i];
-      if (i < nnz_t) ci[j] = col[k0 + i];
+      // the pattern is read once per call and dval written once: non-temporal
+      // accesses keep them from displacing the x and g lines the gathers reuse
+      if (i < rows_t) re[j] = __builtin_nontemporal_load(row_ptr + r0 + 1 + i);
+      if (i < nnz_t) ci[j] = __builtin_nontemporal_load(col + k0 + i);
     }
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
@@ -265,7 +271,7 @@ __global__ __launch_bounds__(WG) void sddmm_tile_kernel(
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
     const int i = tid + j * WG;
-    if (i < nnz_t) dval[k0 + i] = gv[j] * xv[j];
+    if (i < nnz_t) __builtin_nontemporal_store(gv[j] * xv[j], dval + k0 + i);
   }
 }
 
