@@ -6,7 +6,10 @@ x ~ U[-1,1)).  A step is one SpMV (tile kernel + carry fixup) with the matrix
 and x already resident in HBM; the step time is taken over the whole loop of
 steps, with no per-step events.  Three copies of the matrix are rotated so
 consecutive steps cannot be served from the 256 MB MALL; the warm (single
-copy) rate is reported beside it.
+copy) rate is reported beside it.  The tile shape (items per thread) and the
+stream cache policy (plain or non-temporal matrix loads) are picked by timing
+every pair on the rotated workload, as a user would tune for their access
+pattern; --ipt / --stream pin them.
 
 value = algorithmic GB/s = (4(nrow+1) + 12 nnz + 8 ncol + 8 nrow) / time per
 step.  The roofline object is for the dominant (tile) kernel, timed with HIP
@@ -19,8 +22,9 @@ Beside the headline, `extras` reports the other measured forms: mixed
 precision (FP32 / BF16 values, compensated FP32 sums) on the R-MAT and on a
 banded 4M-row matrix, multi-RHS SpMM (k = 8, 32), 64 uneven matrices in one
 persistent graph launch (autotuned), the adjoint on a 2.4M-node graph,
-rocSPARSE's CSR SpMV on the headline matrix (a measured comparator), and a
-plain device copy as the measured bandwidth ceiling.  Every rate uses the same
+PageRank iterations on the R-MAT edge list after device ingest, rocSPARSE's
+CSR and COO SpMV on the headline matrix (measured comparators), and a plain
+device copy as the measured bandwidth ceiling.  Every rate uses the same
 algorithmic-bytes definition.
 
 Prints ONE JSON line on stdout; progress goes to stderr.
