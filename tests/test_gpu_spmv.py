@@ -348,3 +348,42 @@ def test_stream_policies_identical(device, kind):
     b.close()
     for o in list(outs.values()) + list(bouts.values()):
         assert torch.equal(o.view(torch.uint8), outs["cached"].view(torch.uint8))
+
+
+@pytest.mark.slow
+def test_f64_near_int32_limit(device):
+    """nrow + nnz just under 2^31 (2^21 ragged rows, 2.14e9 entries, 26 GB of
+    CSR built on the device): with x[c] = c every row sum is an integer below
+    2^53, so the FP64 result is exact in any order and must equal the closed
+    form."""
+    from merbit_amd.spmv import MerbitPlan
+    n = 1 << 21
+    base = 1022
+    say(f"int32 limit: {n} rows, ~{n * base} entries")
+    r = torch.arange(n, dtype=torch.int64, device=device)
+    lens = base + (r % 5) - 2                      # ragged, same total
+    rp = torch.zeros(n + 1, dtype=torch.int64, device=device)
+    rp[1:] = torch.cumsum(lens, 0)
+    nnz = int(rp[-1])
+    assert n + nnz + 16 * 256 < 2**31 - 1
+    start = (7 * r) % n                             # first column of each row
+    # col[k] = (start[row] + k - rp[row]) mod n, built in chunks of rows
+    col = torch.empty(nnz, dtype=torch.int32, device=device)
+    step = 1 << 17
+    for r0 in range(0, n, step):
+        r1 = min(n, r0 + step)
+        k0, k1 = int(rp[r0]), int(rp[r1])
+        rows = torch.repeat_interleave(torch.arange(r0, r1, device=device), lens[r0:r1])
+        k = torch.arange(k0, k1, dtype=torch.int64, device=device)
+        col[k0:k1] = ((start[rows] + k - rp[rows]) % n).to(torch.int32)
+        del rows, k
+    val = torch.ones(nnz, dtype=torch.float64, device=device)
+    x = torch.arange(n, dtype=torch.float64, device=device)
+    rp32 = rp.to(torch.int32)
+    plan = MerbitPlan(rp32, n, nnz, items_per_thread=12)
+    y = plan.spmv(rp32, col, val, x)
+    plan.close()
+    # sum_j ((s + j) mod n), j < L, with at most one wrap (L < n)
+    L = lens
+    want = L * start + L * (L - 1) // 2 - n * torch.clamp(start + L - n, min=0)
+    assert torch.equal(y, want.to(torch.float64))
