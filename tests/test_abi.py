@@ -117,3 +117,40 @@ def test_bf16_round_to_nearest_even():
     assert np.all((h[ties] & 1) == 0)
     # above the largest bf16 plus half an ulp the value rounds to infinity
     assert gen.to_bf16(np.array([3.4e38, -3.4e38], np.float32)).tolist() == [0x7F80, 0xFF80]
+
+
+def test_c_program_links_the_abi(tmp_path):
+    """A C99 program includes the public headers, links libmerbit_amd.so and
+    calls host-checked entry points (no device work): the boundary is usable
+    from plain C."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "abi_check.c"
+    src.write_text(r"""
+#include <stdio.h>
+#include <string.h>
+#include "merbit_amd.h"
+#include "merbit_gen.h"
+#include "merbit_host.h"
+int main(void) {
+  mbx_plan* p = 0;
+  if (mbx_plan_create(0, -1, 0, 0, 8, 256, 0, &p) != MBX_EINVAL) return 1;
+  if (mbx_plan_create(0, 0, 0, 0, 7, 256, 0, &p) != MBX_EUNSUPPORTED) return 2;
+  if (mbx_spmv_f64(0, 0, 0, 0, 1.0, 0, 0.0, 0, 0) != MBX_EINVAL) return 3;
+  if (mbx_plan_set_stream(0, MBX_STREAM_CACHED) != MBX_EINVAL) return 4;
+  if (mbx_pagerank_workspace_bytes(-1) != 0) return 5;
+  if (strcmp(mbx_status_str(MBX_OK), "ok") != 0) return 6;
+  printf("abi ok\n");
+  return 0;
+}
+""")
+    lib_dir = os.path.join(ROOT, "merbit_amd")
+    exe = tmp_path / "abi_check"
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", f"-I{ROOT}/include", str(src),
+                        f"-L{lib_dir}", "-lmerbit_amd", f"-Wl,-rpath,{lib_dir}", "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "abi ok" in r.stdout, (r.returncode, r.stdout, r.stderr)
