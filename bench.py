@@ -4,9 +4,10 @@ Headline workload: FP64 merge SpMV y = A*x on the R-MAT power-law matrix
 (2^20 rows, 2^24 entries with duplicates kept, a=.57 b=c=.19, values N(0,1),
 x ~ U[-1,1)).  A step is one SpMV (tile kernel + carry fixup) with the matrix
 and x already resident in HBM; the step time is taken over the whole loop of
-steps, with no per-step events.  Three copies of the matrix are rotated so
-consecutive steps cannot be served from the 256 MB MALL; the warm (single
-copy) rate is reported beside it.  The tile shape (items per thread) and the
+steps, with no per-step events, the steps captured as one HIP graph and
+replayed once (--no-graph: eager launches).  Three copies of the matrix are
+rotated so consecutive steps cannot be served from the 256 MB MALL; the warm
+(single copy) rate is reported beside it.  The tile shape (items per thread) and the
 stream cache policy (plain or non-temporal matrix loads) are picked by timing
 every pair on the rotated workload, as a user would tune for their access
 pattern; --ipt / --stream pin them.
@@ -51,24 +52,37 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def time_steps(calls, steps, warmup, plan_events=None):
+def time_steps(calls, steps, warmup, plan_events=None, graph=False):
     """Run warmup + steps calls round-robin; return (ms per step, mean
     tile-kernel ms or None).  The step time comes from events around the
-    whole loop only; with plan_events the loop is run once more with events
-    recorded around every tile kernel (on the stream it is launched on), for
-    the kernel's own duration."""
+    whole loop only.  graph: the `steps` calls are captured once as one HIP
+    graph (as the reference's timed loop of repeated launches) and the timed
+    region is its single replay.  With plan_events the loop is run once more,
+    eagerly, with events recorded around every tile kernel (on the stream it
+    is launched on), for the kernel's own duration."""
     import torch
     for i in range(warmup):
         calls[i % len(calls)]()
     torch.cuda.synchronize()
+    g = None
+    if graph:
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            for i in range(steps):
+                calls[i % len(calls)]()
+        torch.cuda.synchronize()
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
     start.record()
-    for i in range(steps):
-        calls[i % len(calls)]()
+    if g is not None:
+        g.replay()
+    else:
+        for i in range(steps):
+            calls[i % len(calls)]()
     stop.record()
     torch.cuda.synchronize()
     ms = start.elapsed_time(stop) / steps
+    del g
     kern = None
     if plan_events is not None:
         evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
@@ -163,7 +177,8 @@ def bench_rmat_f64(A, x, dev, args):
     plan = MerbitPlan(c0["rp"], ncol, nnz, items_per_thread=ipt)
     plan.set_stream(stream)
     calls = make_calls(plan)
-    ms, kern = time_steps(calls, args.steps, args.warmup, plan.set_events)
+    ms, kern = time_steps(calls, args.steps, args.warmup, plan.set_events,
+                          graph=not args.no_graph)
     warm_ms = warm_kern = None
     if not args.no_warm:  # one copy over and over: cache-resident, plain loads
         plan.set_stream("cached")
@@ -486,6 +501,8 @@ def main():
                     help="skip the warm single-copy line (so a kernel trace of the run holds "
                          "only rotated-copy products)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="time the headline steps as eager launches instead of one graph replay")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -554,7 +571,8 @@ def main():
                                "(a=.57,b=c=.19), 3 rotated copies",
                    "nrow": A.nrow, "nnz": A.nnz, "items_per_thread": h["ipt"],
                    "stream_loads": h["stream"],
-                   "wg_size": 256, "copies_rotated": max(1, args.copies)},
+                   "wg_size": 256, "copies_rotated": max(1, args.copies),
+                   "timed_as": "eager launches" if args.no_graph else "one HIP graph of the steps"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": load_traffic("rmat_f64")},
