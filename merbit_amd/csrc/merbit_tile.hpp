@@ -344,12 +344,14 @@ __device__ __forceinline__ void mark_row_end(unsigned* s_mask, int* s_wcnt, int 
 
 // Scalar staging: element i of the tile is handled by thread i % WG.  The
 // marks must be zero on entry; the function contains a workgroup barrier
-// between issuing the loads and setting the marks.
+// between issuing the loads and setting the marks.  x is gathered through a
+// buffer resource when it spans at most 4 GiB (a column index outside
+// [0, ncol) then reads 0, as on the vector path), directly otherwise.
 template <int WG, int IPT, class P>
 __device__ __forceinline__ void stage_scalar(
     unsigned* s_mask, int* s_wcnt, typename P::lds_t* s_v, const int32_t* __restrict__ row_ptr,
     const int32_t* __restrict__ col, const typename P::val_t* __restrict__ val,
-    const typename P::x_t* __restrict__ x, int r0, int k0, int rows_t, int nnz_t) {
+    const typename P::x_t* __restrict__ x, int ncol, int r0, int k0, int rows_t, int nnz_t) {
   const int tid = threadIdx.x;
   int re[IPT];
   int ci[IPT];
@@ -367,11 +369,22 @@ __device__ __forceinline__ void stage_scalar(
       vv[j] = val[k0 + i];
     }
   }
+  const uint64_t x_bytes = (uint64_t)ncol * sizeof(typename P::x_t);
+  if (x_bytes <= 0xFFFFFFFFull) {  // uniform
+    const __amdgpu_buffer_rsrc_t rx = buffer_rsrc(x, (uint32_t)x_bytes);
 #pragma unroll
-  for (int j = 0; j < IPT; ++j) {
-    const int i = tid + j * WG;
-    xv[j] = typename P::x_t(0);
-    if (i < nnz_t) xv[j] = x[ci[j]];
+    for (int j = 0; j < IPT; ++j) {
+      const int i = tid + j * WG;
+      xv[j] = typename P::x_t(0);
+      if (i < nnz_t) buffer_gather(rx, ci[j], xv[j]);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const int i = tid + j * WG;
+      xv[j] = typename P::x_t(0);
+      if (i < nnz_t) xv[j] = x[ci[j]];
+    }
   }
   __syncthreads();  // marks zeroed by every thread
 #pragma unroll
@@ -704,7 +717,7 @@ __device__ __forceinline__ void spmv_tile(
     c0 = coord[tile];
     c1 = coord[tile + 1];
     clear_marks(sm);
-    stage_scalar<WG, IPT, P>(sm.mask, sm.wcnt, sm.v, row_ptr, col, val, x, c0.x, c0.y,
+    stage_scalar<WG, IPT, P>(sm.mask, sm.wcnt, sm.v, row_ptr, col, val, x, ncol, c0.x, c0.y,
                              c1.x - c0.x, c1.y - c0.y);
   }
   __syncthreads();
