@@ -216,8 +216,7 @@ def bench_mixed(name, A, x, dev, args, f64_ms=None, copies=1):
         plan = MerbitPlan(rp, ncol, nnz, items_per_thread=ipt)
         plan.set_stream(stream)
         calls = make_calls(plan)
-        f64_ms, kern = time_steps(calls, args.steps, args.warmup, plan.set_events,
-                                  graph=not args.no_graph)
+        f64_ms, kern = time_steps(calls, args.steps, args.warmup, plan.set_events)
         b64 = workloads.algorithmic_bytes(nrow, ncol, nnz, 8, 8, 8)
         out["f64"] = {"gbs": gbs(b64, f64_ms), "us_per_step": round(f64_ms * 1e3, 2),
                       "tile_kernel_us": round(kern * 1e3, 2), "items_per_thread": ipt,
@@ -238,8 +237,7 @@ def bench_mixed(name, A, x, dev, args, f64_ms=None, copies=1):
         plan = MerbitPlan(rp, ncol, nnz, items_per_thread=ipt)
         plan.set_stream(stream)
         calls = make_calls(plan)
-        ms, kern = time_steps(calls, args.steps, args.warmup, plan.set_events,
-                              graph=not args.no_graph)
+        ms, kern = time_steps(calls, args.steps, args.warmup, plan.set_events)
         ref = plan.spmv(rp, cols[0], torch.from_numpy(wide).to(dev),
                         torch.from_numpy(x32.astype(np.float64)).to(dev))
         plan.spmv(rp, cols[0], vals[0], xx, y)
