@@ -121,34 +121,38 @@ def load_traffic(name):
 
 
 def pick_ipt(args, rp, ncol, nnz, make_calls):
-    """Tile shape and stream cache policy: --ipt / --stream when given, else
-    the (items per thread, policy) pair whose plan runs the timed workload
-    (make_calls(plan): the rotated products) fastest over a short timing of
-    each (the smallest shape, cached first, of those within 1 % of the
-    fastest).  Tuning on the rotated copies matters: on one copy resident in
-    the 256 MB cache the candidates 8, 12 and 16 time alike on the R-MAT
-    matrix, while rotated copies separate them by 5 %; and a matrix read
-    once per product gains from non-temporal stream loads that cost a
-    cache-resident one 6 %."""
+    """Launch shape: --ipt / --stream / --residency when given, else the
+    (items per thread, stream policy, tiles per CU) triple whose plan runs
+    the timed workload (make_calls(plan): the rotated products) fastest over
+    a short timing of each (the first, in shape-policy-residency order, of
+    those within 1 % of the fastest).  Tuning on the rotated copies matters:
+    on one copy resident in the 256 MB cache the candidates 8, 12 and 16 time
+    alike on the R-MAT matrix, while rotated copies separate them by 5 %; a
+    matrix read once per product gains from non-temporal stream loads that
+    cost a cache-resident one 6 %; and fewer tiles per CU cut the L1 misses
+    of gather-bound products while HBM-bound ones want every tile resident."""
     from merbit_amd.spmv import ITEMS_PER_THREAD, MerbitPlan
     ipts = [args.ipt] if args.ipt else list(ITEMS_PER_THREAD)
     modes = [args.stream] if args.stream else ["cached", "nontemporal"]
-    if len(ipts) == 1 and len(modes) == 1:
-        return ipts[0], modes[0]
+    wgs = [args.residency] if args.residency is not None else [0, 3, 4, 5]
+    if len(ipts) == 1 and len(modes) == 1 and len(wgs) == 1:
+        return ipts[0], modes[0], wgs[0]
     times = {}
     for ipt in ipts:
+        plan = MerbitPlan(rp, ncol, nnz, items_per_thread=ipt)
+        calls = make_calls(plan)
         for mode in modes:
-            plan = MerbitPlan(rp, ncol, nnz, items_per_thread=ipt)
             plan.set_stream(mode)
-            calls = make_calls(plan)
-            times[(ipt, mode)] = min(time_steps(calls, 10 * len(calls), 2 * len(calls))[0]
-                                     for _ in range(3))
-            plan.close()
+            for wg in wgs:
+                plan.set_residency(wg)
+                times[(ipt, mode, wg)] = min(
+                    time_steps(calls, 10 * len(calls), 2 * len(calls))[0] for _ in range(3))
+        plan.close()
     # candidates within 1 % of the fastest time alike: take the first of
     # them, so that run-to-run noise does not flip the choice
     fastest = min(times.values())
     return min((k for k, ms in times.items() if ms <= 1.01 * fastest),
-               key=lambda k: (k[0], modes.index(k[1])))
+               key=lambda k: (k[0], modes.index(k[1]), wgs.index(k[2])))
 
 
 def bench_rmat_f64(A, x, dev, args):
@@ -167,7 +171,7 @@ def bench_rmat_f64(A, x, dev, args):
         return [(lambda c=c: plan.spmv(c["rp"], c["col"], c["val"], c["x"], c["y"]))
                 for c in copies]
 
-    ipt, stream = pick_ipt(args, c0["rp"], ncol, nnz, make_calls)
+    ipt, stream, resid = pick_ipt(args, c0["rp"], ncol, nnz, make_calls)
     plan_ms = []
     for _ in range(5):  # plan construction (the reference's tim1), host clock
         t0 = time.perf_counter()
@@ -176,17 +180,21 @@ def bench_rmat_f64(A, x, dev, args):
         plan.close()
     plan = MerbitPlan(c0["rp"], ncol, nnz, items_per_thread=ipt)
     plan.set_stream(stream)
+    plan.set_residency(resid)
     calls = make_calls(plan)
     ms, kern = time_steps(calls, args.steps, args.warmup, plan.set_events,
                           graph=not args.no_graph)
     warm_ms = warm_kern = None
     if not args.no_warm:  # one copy over and over: cache-resident, plain loads
         plan.set_stream("cached")
+        plan.set_residency(0)
         warm_ms, warm_kern = time_steps(calls[:1], args.steps, args.warmup, plan.set_events)
-    log(f"rmat fp64 (ipt {ipt}, {stream} stream): {ms * 1e3:.1f} us/step rotated ({gbs(b_alg, ms)} GB/s), tile kernel "
+    log(f"rmat fp64 (ipt {ipt}, {stream} stream, {resid or 'all'} tiles/CU): "
+        f"{ms * 1e3:.1f} us/step rotated ({gbs(b_alg, ms)} GB/s), tile kernel "
         f"{kern * 1e3:.1f} us" + (f"; warm {warm_ms * 1e3:.1f} us" if warm_ms else ""))
     res = dict(b_alg=b_alg, ms=ms, kern=kern, warm_ms=warm_ms, warm_kern=warm_kern,
                ntiles=plan.ntiles, tile_items=plan.tile_items, ipt=ipt, stream=stream,
+               resid=resid,
                plan_ms=float(np.median(plan_ms)))
     plan.close()
     del copies
@@ -212,15 +220,17 @@ def bench_mixed(name, A, x, dev, args, f64_ms=None, copies=1):
         def make_calls(plan, vals=vals, xx=xx, y=y):
             return [(lambda i=i: plan.spmv(rp, cols[i], vals[i], xx, y)) for i in range(copies)]
 
-        ipt, stream = pick_ipt(args, rp, ncol, nnz, make_calls)
+        ipt, stream, resid = pick_ipt(args, rp, ncol, nnz, make_calls)
         plan = MerbitPlan(rp, ncol, nnz, items_per_thread=ipt)
         plan.set_stream(stream)
+        plan.set_residency(resid)
         calls = make_calls(plan)
         f64_ms, kern = time_steps(calls, args.steps, args.warmup, plan.set_events)
         b64 = workloads.algorithmic_bytes(nrow, ncol, nnz, 8, 8, 8)
         out["f64"] = {"gbs": gbs(b64, f64_ms), "us_per_step": round(f64_ms * 1e3, 2),
                       "tile_kernel_us": round(kern * 1e3, 2), "items_per_thread": ipt,
-                      "stream": stream, "pct_hbm_peak": round(100 * b64 / (f64_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 1)}
+                      "stream": stream, "tiles_per_cu": resid,
+                      "pct_hbm_peak": round(100 * b64 / (f64_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 1)}
         plan.close()
         del vals, y
     for kind in ("f32", "bf16"):
@@ -233,9 +243,10 @@ def bench_mixed(name, A, x, dev, args, f64_ms=None, copies=1):
         def make_calls(plan, vals=vals, xx=xx, y=y):
             return [(lambda i=i: plan.spmv(rp, cols[i], vals[i], xx, y)) for i in range(copies)]
 
-        ipt, stream = pick_ipt(args, rp, ncol, nnz, make_calls)
+        ipt, stream, resid = pick_ipt(args, rp, ncol, nnz, make_calls)
         plan = MerbitPlan(rp, ncol, nnz, items_per_thread=ipt)
         plan.set_stream(stream)
+        plan.set_residency(resid)
         calls = make_calls(plan)
         ms, kern = time_steps(calls, args.steps, args.warmup, plan.set_events)
         ref = plan.spmv(rp, cols[0], torch.from_numpy(wide).to(dev),
@@ -245,9 +256,10 @@ def bench_mixed(name, A, x, dev, args, f64_ms=None, copies=1):
         out[kind] = {"gbs": gbs(b, ms), "gflops": round(2 * nnz / (ms * 1e-3) / 1e9, 1),
                      "us_per_step": round(ms * 1e3, 2), "tile_kernel_us": round(kern * 1e3, 2),
                      "speedup_vs_f64": round(f64_ms / ms, 3), "items_per_thread": ipt,
-                     "stream": stream,
+                     "stream": stream, "tiles_per_cu": resid,
                      "rel_err_vs_f64_same_values": rel_err(y, ref)}
-        log(f"{name} {kind}: {ms * 1e3:.1f} us/step (ipt {ipt}, {stream}), x{f64_ms / ms:.2f} "
+        log(f"{name} {kind}: {ms * 1e3:.1f} us/step (ipt {ipt}, {stream}, {resid or 'all'}/CU), "
+            f"x{f64_ms / ms:.2f} "
             f"vs fp64, "
             f"err {out[kind]['rel_err_vs_f64_same_values']:.2e}")
         plan.close()
@@ -302,6 +314,14 @@ def bench_batch(dev, args):
                       torch.empty(A.nrow, dtype=torch.float32, device=dev), A.ncol))
     ipt, tuned_us = autotune_batch(items, reps=3)
     b = MerbitBatch(items, items_per_thread=ipt)
+    # tiles per CU, timed like the single-matrix shapes (first within 1 %)
+    wgs = [args.residency] if args.residency is not None else [0, 3, 4, 5]
+    t_wg = {}
+    for wg in wgs:
+        b.set_residency(wg)
+        t_wg[wg] = min(time_steps([b.spmv], 5, 2)[0] for _ in range(3))
+    resid = next(wg for wg in wgs if t_wg[wg] <= 1.01 * min(t_wg.values()))
+    b.set_residency(resid)
     reps = 10
     g = b.graph(repeats=reps)
     ms, _ = time_steps([g.launch], max(5, args.steps // 20), 2)
@@ -311,10 +331,13 @@ def bench_batch(dev, args):
     # per-matrix launches for comparison (same arithmetic, one launch pair each)
     from merbit_amd.spmv import MerbitPlan
     plans = [MerbitPlan(it[0], it[5], it[1].numel(), items_per_thread=ipt) for it in items]
+    for p in plans:
+        p.set_residency(resid)
     calls = [lambda: [p.spmv(it[0], it[1], it[2], it[3], it[4]) for p, it in zip(plans, items)]]
     ms_loop, _ = time_steps(calls, max(5, args.steps // 20), 2)
     out = {"matrices": len(mats), "rows": int(sum(A.nrow for A in mats)), "nnz": int(nnz),
-           "autotuned_items_per_thread": ipt, "tile_items": b.tile_items,
+           "autotuned_items_per_thread": ipt, "tiles_per_cu": resid or "all",
+           "tile_items": b.tile_items,
            "persistent_grid": b.grid, "gbs": gbs(nbytes, ms),
            "gflops": round(2 * nnz / (ms * 1e-3) / 1e9, 1), "us_per_batch": round(ms * 1e3, 1),
            "us_per_batch_separate_launches": round(ms_loop * 1e3, 1),
@@ -485,6 +508,8 @@ def main():
                     help="items per thread (SIGMA); 0 = picked by the timing autotuner")
     ap.add_argument("--stream", choices=["cached", "nontemporal"], default=None,
                     help="stream cache policy; default: picked with the tile shape")
+    ap.add_argument("--residency", type=int, default=None,
+                    help="tiles per CU (0 = as many as fit); default: picked with the shape")
     ap.add_argument("--copies", type=int, default=3)
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--batch-count", type=int, default=64)
@@ -570,7 +595,7 @@ def main():
         "config": {"workload": "fp64 merge SpMV, R-MAT power law 2^20 rows 2^24 nnz "
                                "(a=.57,b=c=.19), 3 rotated copies",
                    "nrow": A.nrow, "nnz": A.nnz, "items_per_thread": h["ipt"],
-                   "stream_loads": h["stream"],
+                   "stream_loads": h["stream"], "tiles_per_cu": h["resid"] or "all",
                    "wg_size": 256, "copies_rotated": max(1, args.copies),
                    "timed_as": "eager launches" if args.no_graph else "one HIP graph of the steps"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

@@ -86,6 +86,13 @@ int mbx_plan_partition(const mbx_plan* plan, int32_t* h_rows, int32_t* h_nnz,
 enum { MBX_STREAM_AUTO = 0, MBX_STREAM_CACHED = 1, MBX_STREAM_NONTEMPORAL = 2 };
 int mbx_plan_set_stream(mbx_plan* plan, int32_t mode);
 
+/* Tiles resident per CU for the plan's SpMV launches (no reference
+ * counterpart): 0 (the default) lets as many run as registers and LDS
+ * allow; 1..16 reserves extra LDS per tile so that at most that many share
+ * a CU.  Fewer resident tiles can mean fewer L1 misses for their x gathers
+ * (a timing knob for the autotuner; results are identical). */
+int mbx_plan_set_residency(mbx_plan* plan, int32_t wg_per_cu);
+
 /* Optional timing hook: when ev_begin/ev_end (hipEvent_t) are non-NULL, the
  * next spmv call on this plan records them immediately before and after its
  * tile kernel on the call's stream, then clears them. */
@@ -158,6 +165,9 @@ int mbx_batch_spmv(const mbx_batch* batch, double alpha, double beta, void* stre
 /* The stream cache policy (MBX_STREAM_*) of the batch's products; AUTO
  * decides on the bytes of the whole batch. */
 int mbx_batch_set_stream(mbx_batch* batch, int32_t mode);
+/* Tiles resident per CU for the batch's launches (mbx_plan_set_residency);
+ * the persistent grid shrinks to wg_per_cu per CU. */
+int mbx_batch_set_residency(mbx_batch* batch, int32_t wg_per_cu);
 
 /* HIP graphs of repeated products (the reference's timed loop of
  * maxiter*perrnd launches, include/tile.cuh:596-603): `repeats` back-to-back

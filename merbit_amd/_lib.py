@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(_PKG, "libmerbit_amd.so")
 ABI_SYMBOLS = (
     "mbx_plan_create", "mbx_plan_destroy", "mbx_plan_clone", "mbx_plan_info",
     "mbx_plan_partition", "mbx_plan_set_stream", "mbx_batch_set_stream",
+    "mbx_plan_set_residency", "mbx_batch_set_residency",
     "mbx_plan_set_events", "mbx_spmv_f64", "mbx_spmv_f32", "mbx_spmv_bf16",
     "mbx_spmm_workspace_bytes", "mbx_spmm_f32",
     "mbx_batch_create", "mbx_batch_destroy", "mbx_batch_info", "mbx_batch_spmv",
@@ -87,6 +88,8 @@ def lib() -> ctypes.CDLL:
     L.mbx_plan_destroy.argtypes = [vp]
     L.mbx_plan_clone.argtypes = [vp, ctypes.POINTER(vp)]
     L.mbx_plan_set_stream.argtypes = [vp, i32]
+    L.mbx_plan_set_residency.argtypes = [vp, i32]
+    L.mbx_batch_set_residency.argtypes = [vp, i32]
     L.mbx_batch_set_stream.argtypes = [vp, i32]
     L.mbx_plan_destroy.restype = None
     L.mbx_plan_info.argtypes = [vp, vp, vp, vp]

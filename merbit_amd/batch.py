@@ -89,6 +89,11 @@ class MerbitBatch:
         from .spmv import STREAM_MODES
         check(lib().mbx_batch_set_stream(self._h, STREAM_MODES[mode]), "mbx_batch_set_stream")
 
+    def set_residency(self, wg_per_cu: int) -> None:
+        """Tiles resident per CU for the batch's launches
+        (mbx_batch_set_residency); 0 = as many as fit."""
+        check(lib().mbx_batch_set_residency(self._h, int(wg_per_cu)), "mbx_batch_set_residency")
+
     def spmv(self, alpha: float = 1.0, beta: float = 0.0, stream=None) -> None:
         check(lib().mbx_batch_spmv(self._h, alpha, beta, _stream_handle(stream)),
               "mbx_batch_spmv")

@@ -140,6 +140,8 @@ struct mbx_batch {
   double* d_carry;
   long long footprint;  // bytes one batch product reads and writes
   int32_t stream_mode;  // MBX_STREAM_*
+  int32_t wg_per_cu;    // resident tiles per CU; 0 = as many as fit
+  int32_t cus;          // compute units of the device
 };
 
 namespace {
@@ -164,21 +166,26 @@ template <class P, int IPT>
 void batch_launch(const mbx_batch* b, hipStream_t s, typename P::scal_t alpha,
                   typename P::scal_t beta) {
   const int bz = beta == typename P::scal_t(0);
-  const int grid = std::min(b->grid, b->total_tiles);
-  const size_t lds = TileSmem<256, IPT, P>::bytes();
+  int grid = std::min(b->grid, b->total_tiles);
+  if (b->wg_per_cu > 0) grid = std::min(grid, b->wg_per_cu * b->cus);  // persistent
+  const size_t lds = mbx_detail::residency_lds(TileSmem<256, IPT, P>::bytes(), b->wg_per_cu);
   const bool nt = mbx_detail::stream_nontemporal(b->stream_mode, b->footprint);
-  if (b->vec && nt)
+  if (b->vec && nt) {
+    mbx_detail::allow_lds(batch_spmv_kernel<256, IPT, true, mbx_detail::Streamed<P>>, lds);
     batch_spmv_kernel<256, IPT, true, mbx_detail::Streamed<P>><<<grid, 256, lds, s>>>(
         b->d_mats, b->d_tile_base, b->count, b->total_tiles, b->d_coord, b->d_carry, alpha, beta,
         bz);
-  else if (b->vec)
+  } else if (b->vec) {
+    mbx_detail::allow_lds(batch_spmv_kernel<256, IPT, true, P>, lds);
     batch_spmv_kernel<256, IPT, true, P><<<grid, 256, lds, s>>>(
         b->d_mats, b->d_tile_base, b->count, b->total_tiles, b->d_coord, b->d_carry, alpha, beta,
         bz);
-  else
+  } else {
+    mbx_detail::allow_lds(batch_spmv_kernel<256, IPT, false, P>, lds);
     batch_spmv_kernel<256, IPT, false, P><<<grid, 256, lds, s>>>(
         b->d_mats, b->d_tile_base, b->count, b->total_tiles, b->d_coord, b->d_carry, alpha, beta,
         bz);
+  }
   if (b->split_rows > 0)
     batch_fixup_kernel<P><<<(b->total_tiles + 255) / 256, 256, 0, s>>>(
       b->d_mats, b->d_tile_base, b->count, b->total_tiles, b->tile_items, b->d_coord,
@@ -271,6 +278,8 @@ int mbx_batch_create(const mbx_batch_item* items, int32_t count, int32_t value_k
   b->vec = vec;
   b->footprint = footprint;
   b->stream_mode = MBX_STREAM_AUTO;
+  b->wg_per_cu = 0;
+  b->cus = 1;
   b->ipt = ipt;
   b->tile_items = (int)T;
   b->total_tiles = (int)tiles;
@@ -319,6 +328,10 @@ int mbx_batch_create(const mbx_batch_item* items, int32_t count, int32_t value_k
   b->max_rows = h_stats[0];
   b->split_rows = h_stats[1];
   b->grid = batch_run(b, s, 0.0, 0.0, true, device);
+  if (hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, device) !=
+          hipSuccess ||
+      b->cus < 1)
+    b->cus = 1;
   // the host vectors die here: wait for the copies that read them
   if (hipGetLastError() != hipSuccess || b->grid <= 0 || hipStreamSynchronize(s) != hipSuccess) {
     mbx_batch_destroy(b);
@@ -343,6 +356,12 @@ int mbx_batch_info(const mbx_batch* b, int32_t* total_tiles, int32_t* tile_items
   if (total_tiles) *total_tiles = b->total_tiles;
   if (tile_items) *tile_items = b->tile_items;
   if (grid) *grid = std::min(b->grid, b->total_tiles);
+  return MBX_OK;
+}
+
+int mbx_batch_set_residency(mbx_batch* b, int32_t wg_per_cu) {
+  if (!b || wg_per_cu < 0 || wg_per_cu > 16) return MBX_EINVAL;
+  b->wg_per_cu = wg_per_cu;
   return MBX_OK;
 }
 

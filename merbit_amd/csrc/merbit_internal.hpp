@@ -22,6 +22,7 @@ struct mbx_plan {
   double* d_carry;
   std::atomic<int>* coord_refs;  // plan objects sharing d_coord
   int32_t stream_mode;           // MBX_STREAM_*
+  int32_t wg_per_cu;             // resident tiles per CU; 0 = as many as fit
   mutable hipEvent_t ev_begin, ev_end;
 };
 
@@ -70,6 +71,25 @@ inline bool vec_path_ok(const int32_t* rp, const int32_t* col, const V* val, int
 constexpr long long kResidentBytes = 240ll << 20;
 inline bool stream_nontemporal(int32_t mode, long long footprint) {
   return mode == MBX_STREAM_NONTEMPORAL || (mode == MBX_STREAM_AUTO && footprint > kResidentBytes);
+}
+
+// LDS a tile launch reserves: what the tile needs, or more when the caller
+// limits the tiles resident on a CU to wg_per_cu (the LDS then admits no
+// more; fewer tiles per CU can mean fewer L1 misses for their gathers).
+constexpr size_t kLdsPerCu = 160 * 1024;
+inline size_t residency_lds(size_t need, int wg_per_cu) {
+  if (wg_per_cu <= 0) return need;
+  const size_t cap = kLdsPerCu / (size_t)wg_per_cu - 1024;
+  return need > cap ? need : cap;
+}
+
+// Launch a tile kernel with `lds` bytes of dynamic LDS (the attribute allows
+// more than 64 KB when asked).
+template <class K>
+inline void allow_lds(K* kernel, size_t lds) {
+  if (lds > 65536)
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
 // Launch-side validation shared by every entry point taking a plan.

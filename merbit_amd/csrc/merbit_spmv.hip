@@ -115,16 +115,20 @@ hipError_t launch_tile(const mbx_plan* p, hipStream_t s, const int32_t* rp,
                               (long long)p->nrow * sizeof(typename P::y_t);
   const bool nt = mbx_detail::stream_nontemporal(p->stream_mode, footprint);
   if (p->ev_begin) (void)hipEventRecord(p->ev_begin, s);
-  const size_t lds = TileSmem<256, IPT, P>::bytes();
-  if (vec && nt)
+  const size_t lds = mbx_detail::residency_lds(TileSmem<256, IPT, P>::bytes(), p->wg_per_cu);
+  if (vec && nt) {
+    mbx_detail::allow_lds(tile_spmv_kernel<256, IPT, true, mbx_detail::Streamed<P>>, lds);
     tile_spmv_kernel<256, IPT, true, mbx_detail::Streamed<P>><<<p->ntiles, 256, lds, s>>>(
         rp, col, val, x, y, p->d_coord, p->d_carry, p->ntiles, p->ncol, alpha, beta, bz);
-  else if (vec)
+  } else if (vec) {
+    mbx_detail::allow_lds(tile_spmv_kernel<256, IPT, true, P>, lds);
     tile_spmv_kernel<256, IPT, true, P><<<p->ntiles, 256, lds, s>>>(
         rp, col, val, x, y, p->d_coord, p->d_carry, p->ntiles, p->ncol, alpha, beta, bz);
-  else
+  } else {
+    mbx_detail::allow_lds(tile_spmv_kernel<256, IPT, false, P>, lds);
     tile_spmv_kernel<256, IPT, false, P><<<p->ntiles, 256, lds, s>>>(
         rp, col, val, x, y, p->d_coord, p->d_carry, p->ntiles, p->ncol, alpha, beta, bz);
+  }
   if (p->ev_end) (void)hipEventRecord(p->ev_end, s);
   p->ev_begin = p->ev_end = nullptr;
   if (p->split_rows > 0) {  // some row crosses a tile boundary
@@ -191,6 +195,7 @@ int mbx_plan_create(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol,
   p->d_carry = nullptr;
   p->coord_refs = new (std::nothrow) std::atomic<int>(1);
   p->stream_mode = MBX_STREAM_AUTO;
+  p->wg_per_cu = 0;
   p->ev_begin = p->ev_end = nullptr;
   if (!p->coord_refs) {
     mbx_plan_destroy(p);
@@ -288,6 +293,12 @@ int mbx_plan_partition(const mbx_plan* p, int32_t* h_rows, int32_t* h_nnz, int32
 int mbx_plan_set_stream(mbx_plan* p, int32_t mode) {
   if (!p || mode < MBX_STREAM_AUTO || mode > MBX_STREAM_NONTEMPORAL) return MBX_EINVAL;
   p->stream_mode = mode;
+  return MBX_OK;
+}
+
+int mbx_plan_set_residency(mbx_plan* p, int32_t wg_per_cu) {
+  if (!p || wg_per_cu < 0 || wg_per_cu > 16) return MBX_EINVAL;
+  p->wg_per_cu = wg_per_cu;
   return MBX_OK;
 }
 

@@ -92,6 +92,12 @@ class MerbitPlan:
         "nontemporal"."""
         check(lib().mbx_plan_set_stream(self.handle, STREAM_MODES[mode]), "mbx_plan_set_stream")
 
+    def set_residency(self, wg_per_cu: int) -> None:
+        """Tiles resident per CU for this plan's launches
+        (mbx_plan_set_residency); 0 = as many as fit."""
+        check(lib().mbx_plan_set_residency(self.handle, int(wg_per_cu)),
+              "mbx_plan_set_residency")
+
     def set_events(self, begin, end) -> None:
         """Record torch.cuda.Event pair around the next call's tile kernel."""
         check(lib().mbx_plan_set_events(self.handle, begin.cuda_event, end.cuda_event),

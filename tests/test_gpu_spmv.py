@@ -316,7 +316,8 @@ def test_plan_clones_on_concurrent_streams(orc, device):
 @pytest.mark.parametrize("kind", ["f64", "f32", "bf16"])
 def test_stream_policies_identical(device, kind):
     """Cached, non-temporal and automatic stream loads change only cache
-    hints: the products are bit-identical, for a plan and for a batch."""
+    hints, and a cap on tiles per CU only the LDS reserved per tile: the
+    products are bit-identical, for a plan and for a batch."""
     from merbit_amd import gen
     from merbit_amd.batch import MerbitBatch
     from merbit_amd.spmv import MerbitPlan
@@ -336,6 +337,10 @@ def test_stream_policies_identical(device, kind):
     for mode in ("auto", "cached", "nontemporal"):
         plan.set_stream(mode)
         outs[mode] = plan.spmv(d_rp, d_col, d_val, d_x).clone()
+    for wg in (1, 3, 16):  # tiles per CU (2 and 1 need the over-64 KB LDS attribute)
+        plan.set_residency(wg)
+        outs[f"resid{wg}"] = plan.spmv(d_rp, d_col, d_val, d_x).clone()
+    plan.set_residency(0)
     plan.close()
     ydt = outs["auto"].dtype
     mats = [(d_rp, d_col, d_val, d_x, torch.empty(len(lens), dtype=ydt, device=device), 20000)]
@@ -345,6 +350,10 @@ def test_stream_policies_identical(device, kind):
         b.set_stream(mode)
         b.spmv()
         bouts[mode] = mats[0][4].clone()
+    for wg in (1, 3):
+        b.set_residency(wg)
+        b.spmv()
+        bouts[f"resid{wg}"] = mats[0][4].clone()
     b.close()
     for o in list(outs.values()) + list(bouts.values()):
         assert torch.equal(o.view(torch.uint8), outs["cached"].view(torch.uint8))
