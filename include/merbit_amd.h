@@ -224,6 +224,16 @@ int mbx_plan_stats(const mbx_plan* plan, const int32_t* d_row_ptr, mbx_pattern_s
  * compile-time in the reference, CMakeLists.txt:4-6): times every
  * instantiated items_per_thread on the given operands (which it overwrites in
  * y) and returns the fastest with its time per product in microseconds. */
+/* As mbx_autotune_spmv, over the whole launch shape: every items_per_thread
+ * with both stream policies (MBX_STREAM_CACHED, MBX_STREAM_NONTEMPORAL) and
+ * tile residencies 0, 3, 4, 5 (mbx_plan_set_stream / _set_residency).
+ * Returns the fastest triple; apply it to a plan of that items_per_thread
+ * with the two setters. */
+int mbx_autotune_spmv_launch(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol, int32_t nnz,
+                             const int32_t* d_col, const void* d_val, int32_t value_kind,
+                             const void* d_x, void* d_y, int32_t reps, void* stream,
+                             int32_t* best_ipt, int32_t* best_stream, int32_t* best_wg_per_cu,
+                             float* best_us);
 int mbx_autotune_spmv(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol, int32_t nnz,
                       const int32_t* d_col, const void* d_val, int32_t value_kind,
                       const void* d_x, void* d_y, int32_t reps, void* stream, int32_t* best_ipt,

@@ -140,6 +140,47 @@ int mbx_autotune_spmv(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol, int3
   return *best_ipt ? MBX_OK : MBX_EOVERFLOW;
 }
 
+int mbx_autotune_spmv_launch(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol, int32_t nnz,
+                             const int32_t* d_col, const void* d_val, int32_t value_kind,
+                             const void* d_x, void* d_y, int32_t reps, void* stream,
+                             int32_t* best_ipt, int32_t* best_stream, int32_t* best_wg_per_cu,
+                             float* best_us) {
+  if (!best_ipt || !best_stream || !best_wg_per_cu || !best_us || reps < 1) return MBX_EINVAL;
+  hipStream_t s = (hipStream_t)stream;
+  const int kModes[] = {MBX_STREAM_CACHED, MBX_STREAM_NONTEMPORAL};
+  const int kResidency[] = {0, 3, 4, 5};
+  *best_ipt = 0;
+  *best_us = 0.f;
+  for (int ipt : kSpmvCandidates) {
+    mbx_plan* p = nullptr;
+    int st = mbx_plan_create(d_row_ptr, nrow, ncol, nnz, ipt, 256, stream, &p);
+    if (st == MBX_EOVERFLOW) continue;
+    if (st != MBX_OK) return st;
+    for (int mode : kModes) {
+      for (int wg : kResidency) {
+        p->stream_mode = mode;
+        p->wg_per_cu = wg;
+        float us = 0.f;
+        st = time_calls(
+            s, reps,
+            [&] { return spmv_call(p, value_kind, d_row_ptr, d_col, d_val, d_x, d_y, s); }, &us);
+        if (st != MBX_OK) {
+          mbx_plan_destroy(p);
+          return st;
+        }
+        if (*best_ipt == 0 || us < *best_us) {
+          *best_ipt = ipt;
+          *best_stream = mode;
+          *best_wg_per_cu = wg;
+          *best_us = us;
+        }
+      }
+    }
+    mbx_plan_destroy(p);
+  }
+  return *best_ipt ? MBX_OK : MBX_EOVERFLOW;
+}
+
 int mbx_autotune_spmm(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol, int32_t nnz,
                       const int32_t* d_col, const float* d_val, int32_t k, const float* d_x,
                       int64_t ldx, int32_t x_layout, float* d_y, int64_t ldy, int32_t y_layout,

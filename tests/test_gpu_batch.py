@@ -195,6 +195,17 @@ def test_autotuners(orc, device):
     ya, absr = orc.spmv_accurate(A.row_ptr, A.col, A.val, x)
     assert cond_err(yy, ya, absr) <= 1e-12
     plan.close()
+    say("autotune spmv launch shape")
+    from merbit_amd.batch import autotune_spmv_launch
+    ipt, mode, wg, us = autotune_spmv_launch(rp, col, val, xx, y, reps=2)
+    assert ipt in (2, 4, 6, 8, 10, 12, 14, 16) and mode in ("cached", "nontemporal")
+    assert wg in (0, 3, 4, 5) and us > 0
+    plan = MerbitPlan(rp, A.ncol, A.nnz, items_per_thread=ipt)
+    plan.set_stream(mode)
+    plan.set_residency(wg)
+    yy = plan.spmv(rp, col, val, xx).cpu().numpy()
+    assert cond_err(yy, ya, absr) <= 1e-12
+    plan.close()
     say("autotune spmm")
     X = torch.rand((A.ncol, 8), device=device)
     Y = torch.empty((A.nrow, 8), device=device)
