@@ -280,7 +280,16 @@ def bench_spmm(A, dev, args):
         X = torch.from_numpy(gen.uniform(ncol * k, -1, 1, workloads.SEED["spmm_x"],
                                          np.float32)).to(dev).view(k, ncol).t()  # col-major
         Y = torch.empty((k, nrow), dtype=torch.float32, device=dev).t()
-        plan = MerbitPlan(rp, ncol, nnz, items_per_thread=4)
+        # items per thread: the faster of the two SpMM tile shapes on this k
+        tuned = {}
+        for ipt in (2, 4):
+            plan = MerbitPlan(rp, ncol, nnz, items_per_thread=ipt)
+            ws = torch.empty(plan.spmm_workspace_bytes(X), dtype=torch.uint8, device=dev)
+            tuned[ipt] = min(time_steps([lambda: plan.spmm(rp, col, val, X, Y, workspace=ws)],
+                                        10, 2)[0] for _ in range(2))
+            plan.close()
+        ipt = min(tuned, key=tuned.get)
+        plan = MerbitPlan(rp, ncol, nnz, items_per_thread=ipt)
         ws = torch.empty(plan.spmm_workspace_bytes(X), dtype=torch.uint8, device=dev)
         ms, kern = time_steps([lambda: plan.spmm(rp, col, val, X, Y, workspace=ws)],
                               max(20, args.steps // 4), args.warmup, plan.set_events)
@@ -291,9 +300,10 @@ def bench_spmm(A, dev, args):
                         "us_per_step": round(ms * 1e3, 2),
                         "tile_kernel_us": round(kern * 1e3, 2),
                         "us_per_rhs": round(ms * 1e3 / k, 2),
+                        "items_per_thread": ipt,
                         "rel_err_col0_vs_f64": rel_err(Y[:, 0], y0),
                         "layout": "column-major X and Y, repacked per call"}
-        log(f"spmm k={k}: {ms * 1e3:.1f} us/step ({out[f'k{k}']['gbs']} GB/s)")
+        log(f"spmm k={k} (ipt {ipt}): {ms * 1e3:.1f} us/step ({out[f'k{k}']['gbs']} GB/s)")
         plan.close()
         del X, Y, ws
     return out

@@ -128,7 +128,9 @@ enum { MBX_COL_MAJOR = 0, MBX_ROW_MAJOR = 1 };
  * plan made with items_per_thread 2 or 4 (MBX_EUNSUPPORTED otherwise).  The
  * caller provides a 256-byte aligned device workspace of at least
  * mbx_spmm_workspace_bytes(plan, k, X, ldx, x_layout) bytes; a column-major
- * (or unaligned row-major) X is repacked into it as rows once per call. */
+ * (or unaligned row-major) X is repacked into it as rows once per call.
+ * Items per thread 2 suits wide X (k > 8: 16 right-hand sides per pass),
+ * 4 suits k <= 8; mbx_autotune_spmm times both. */
 size_t mbx_spmm_workspace_bytes(const mbx_plan* plan, int32_t k, const float* d_x,
                                 int64_t ldx, int32_t x_layout);
 int mbx_spmm_f32(const mbx_plan* plan, const int32_t* d_row_ptr, const int32_t* d_col,

@@ -10,6 +10,11 @@
 // repacked once per call into the caller's workspace.  Carries of rows that
 // cross tiles (FP64, per right-hand side) are completed by a fixup kernel in a
 // fixed order, as for SpMV.
+//
+// Two tile kernels: for k <= 4 one lane per path thread (tile_spmm_kernel,
+// KC = 4); for wider X, G lanes per path thread (tile_spmm_rows_kernel), so
+// that G lanes read one nonzero's X segment together and each lane carries a
+// single float4 through the walk and the scan.
 
 #include <hip/hip_runtime.h>
 #include <limits.h>
@@ -316,6 +321,199 @@ __global__ __launch_bounds__(WG) void tile_spmm_kernel(
   }
 }
 
+// The same tile SpMM with G lanes per merge-path thread: lane g of a group
+// owns right-hand sides [4g, 4g+4) of the chunk (KC = 4G), so a chunk covers
+// a whole X row segment of 16G bytes.  The product stage maps G consecutive
+// lanes onto one nonzero's segment: one wave instruction reads 64/G whole
+// segments (one 128-byte line each at G = 8) instead of 64 scattered 16-byte
+// pieces, and each nonzero's row is fetched once per chunk of 4G columns.
+// The walk and the segmented scan then move one float4 per lane.  The tile is
+// the plan's (TI = 256 * items per thread path steps); each of the WG / G
+// path threads walks TI * G / WG of them.  Products live in dynamic LDS
+// (TI * G float4).
+template <int WG, int TI, int G>
+__global__ __launch_bounds__(WG) void tile_spmm_rows_kernel(
+    const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
+    const float* __restrict__ val, const float* __restrict__ Xr, int ldr, int k,
+    float* __restrict__ Y, long long ldy, int y_row_major, const int4* __restrict__ coord,
+    double* __restrict__ carry, int ntiles, float alpha, float beta, int beta_zero) {
+  constexpr int KC = 4 * G;
+  constexpr int NW = WG / 64;
+  constexpr int PW = 64 / G;         // path threads per wave
+  constexpr int IPP = TI * G / WG;   // path steps per path thread
+  constexpr int U = 8;               // product-stage loads in flight per lane
+  static_assert((TI * G) % WG == 0 && 64 % G == 0, "tile shape");
+  __shared__ int s_re[TI + 1];
+  __shared__ float s_wtail[NW][KC];
+  __shared__ int s_wflag[NW];
+  __shared__ float s_tail[KC];
+  extern __shared__ float4 s_v[];  // [TI * G]: products [0, nnz_t), row sums after
+
+  const int tile = xcd_tile(blockIdx.x, ntiles);
+  const int tid = threadIdx.x;
+  const int4 c0 = coord[tile];
+  const int4 c1 = coord[tile + 1];
+  const int r0 = c0.x, k0 = c0.y;
+  const int rows_t = c1.x - r0;
+  const int nnz_t = c1.y - k0;
+
+  for (int i = tid; i < rows_t; i += WG) s_re[i] = row_ptr[r0 + 1 + i] - k0;
+  if (tid == 0) s_re[rows_t] = INT_MAX;
+  __syncthreads();
+
+  const int lane = tid & 63, wave = tid >> 6;
+  const int g = lane % G, pl = lane / G;
+  const int d0 = (tid / G) * IPP;
+  const int items = min(max(nnz_t + rows_t - d0, 0), IPP);
+  const int r_start = items > 0 ? lds_merge_search(s_re, d0, nnz_t, rows_t) : 0;
+  const int k_start = d0 - r_start;
+  const bool head_multi = rows_t > 0 && c0.z < k0;
+  float4* s_y = s_v + nnz_t * G;
+  const int n_prod = nnz_t * G;
+
+  for (int cb = 0; cb < k; cb += KC) {
+    const int kc = min(KC, k - cb);
+    // 1. products: G consecutive lanes per nonzero, one float4 each
+    for (int base = 0; base < n_prod; base += U * WG) {
+      int ci[U];
+      float vv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = base + u * WG + tid;
+        ci[u] = 0;
+        vv[u] = 0.f;
+        if (e < n_prod) {
+          ci[u] = col[k0 + e / G];
+          vv[u] = val[k0 + e / G];
+        }
+      }
+      float4 xv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = base + u * WG + tid;
+        xv[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (e < n_prod && 4 * g < kc)
+          xv[u] = *reinterpret_cast<const float4*>(Xr + (long long)ci[u] * ldr + cb + 4 * g);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = base + u * WG + tid;
+        if (e < n_prod)
+          s_v[e] = make_float4(vv[u] * xv[u].x, vv[u] * xv[u].y, vv[u] * xv[u].z,
+                               vv[u] * xv[u].w);
+      }
+    }
+    __syncthreads();
+
+    // 2. walk: every lane of a path thread follows the same path, on its float4
+    float4 sum = make_float4(0.f, 0.f, 0.f, 0.f), head = sum;
+    int head_row = 0;
+    bool flagged = false;
+    if (items > 0) {
+      int r = r_start, kk = k_start;
+      int re = s_re[r];
+#pragma unroll
+      for (int i = 0; i < IPP; ++i) {
+        if (i < items) {
+          if (kk < re) {
+            const float4 p = s_v[kk * G + g];
+            sum.x += p.x;
+            sum.y += p.y;
+            sum.z += p.z;
+            sum.w += p.w;
+            ++kk;
+          } else {
+            if (flagged) {
+              s_y[r * G + g] = sum;
+            } else {
+              head = sum;
+              head_row = r;
+              flagged = true;
+            }
+            sum = make_float4(0.f, 0.f, 0.f, 0.f);
+            ++r;
+            re = s_re[r];
+          }
+        }
+      }
+    }
+
+    // 3. segmented scan over the path threads, per component
+    const unsigned long long fmask = __ballot(flagged);
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    const unsigned long long fb = fmask & upto;
+    const int seg = fb ? (63 - __clzll(fb)) / G : -1;
+    float4 s = sum;
+#pragma unroll
+    for (int d = 1; d < PW; d <<= 1) {
+      const bool take = pl >= d && pl - d >= seg;
+      const float ox = __shfl_up(s.x, d * G), oy = __shfl_up(s.y, d * G);
+      const float oz = __shfl_up(s.z, d * G), ow = __shfl_up(s.w, d * G);
+      if (take) s = make_float4(ox + s.x, oy + s.y, oz + s.z, ow + s.w);
+    }
+    const float4 cin = make_float4(__shfl_up(s.x, G), __shfl_up(s.y, G), __shfl_up(s.z, G),
+                                   __shfl_up(s.w, G));
+    if (pl == PW - 1) {
+      s_wtail[wave][4 * g] = s.x;
+      s_wtail[wave][4 * g + 1] = s.y;
+      s_wtail[wave][4 * g + 2] = s.z;
+      s_wtail[wave][4 * g + 3] = s.w;
+      if (g == 0) s_wflag[wave] = fmask != 0ull;
+    }
+    __syncthreads();
+    float4 win = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int w = 0; w < NW - 1; ++w)
+      if (w < wave) {
+        const bool f = s_wflag[w];
+        const float* t = s_wtail[w] + 4 * g;
+        win = f ? make_float4(t[0], t[1], t[2], t[3])
+                : make_float4(win.x + t[0], win.y + t[1], win.z + t[2], win.w + t[3]);
+      }
+    const bool ext = (fmask & ((1ull << (pl * G)) - 1ull)) == 0ull;
+    if (flagged) {
+      float4 c = (pl == 0) ? make_float4(0.f, 0.f, 0.f, 0.f) : cin;
+      if (ext) c = make_float4(win.x + c.x, win.y + c.y, win.z + c.z, win.w + c.w);
+      s_y[head_row * G + g] = make_float4(c.x + head.x, c.y + head.y, c.z + head.z, c.w + head.w);
+    }
+    if (tid >= WG - G) {
+      const float4 t = (fmask == 0ull)
+                           ? make_float4(win.x + s.x, win.y + s.y, win.z + s.z, win.w + s.w)
+                           : s;
+      s_tail[4 * g] = t.x;
+      s_tail[4 * g + 1] = t.y;
+      s_tail[4 * g + 2] = t.z;
+      s_tail[4 * g + 3] = t.w;
+    }
+    __syncthreads();
+
+    // 4. write-out, as the per-lane kernel
+    const float* sy = reinterpret_cast<const float*>(s_y);
+    const int n_out = rows_t * kc;
+    for (int e = tid; e < n_out; e += WG) {
+      int i, q;
+      if (y_row_major) {
+        i = e / kc;
+        q = e - i * kc;
+      } else {
+        q = e / rows_t;
+        i = e - q * rows_t;
+      }
+      const float v = sy[i * KC + q];
+      const int jcol = cb + q;
+      if (i == 0 && head_multi) {
+        carry[(2ll * tile + 1) * k + jcol] = (double)v;
+      } else {
+        float* yp = y_row_major ? Y + (long long)(r0 + i) * ldy + jcol
+                                : Y + (r0 + i) + (long long)jcol * ldy;
+        *yp = beta_zero ? alpha * v : alpha * v + beta * *yp;
+      }
+    }
+    if (tid < kc) carry[(2ll * tile) * k + cb + tid] = (double)s_tail[tid];
+    __syncthreads();
+  }
+}
+
 // One thread per (tile t >= 1, right-hand side q): a row that began in an
 // earlier tile ts and ends in t is completed by summing tail[ts..t-1] then
 // head[t] in order (FP64).
@@ -383,6 +581,24 @@ void launch_spmm(const mbx_plan* p, hipStream_t s, const int32_t* rp, const int3
         p->d_coord, carry, k, Y, ldy, yrm, p->ntiles, p->tile_items, alpha, beta, bz);
 }
 
+template <int WG, int TI, int G>
+void launch_spmm_rows(const mbx_plan* p, hipStream_t s, const int32_t* rp, const int32_t* col,
+                      const float* val, const float* Xr, int ldr, int k, float* Y,
+                      long long ldy, int yrm, double* carry, float alpha, float beta) {
+  const int bz = beta == 0.f;
+  const size_t lds = sizeof(float4) * (size_t)TI * G;
+  auto kern = tile_spmm_rows_kernel<WG, TI, G>;
+  mbx_detail::allow_lds(kern, lds + sizeof(int) * (TI + 1) + 4096);
+  if (p->ev_begin) (void)hipEventRecord(p->ev_begin, s);
+  kern<<<p->ntiles, WG, lds, s>>>(rp, col, val, Xr, ldr, k, Y, ldy, yrm, p->d_coord, carry,
+                                  p->ntiles, alpha, beta, bz);
+  if (p->ev_end) (void)hipEventRecord(p->ev_end, s);
+  p->ev_begin = p->ev_end = nullptr;
+  if (p->ntiles > 1)
+    spmm_fixup_kernel<<<(int)(((long long)p->ntiles * k + 255) / 256), 256, 0, s>>>(
+        p->d_coord, carry, k, Y, ldy, yrm, p->ntiles, p->tile_items, alpha, beta, bz);
+}
+
 }  // namespace
 
 extern "C" {
@@ -430,13 +646,24 @@ int mbx_spmm_f32(const mbx_plan* p, const int32_t* rp, const int32_t* col, const
     Xr = xr;
   }
   const int yrm = y_layout == MBX_ROW_MAJOR;
-  const bool wide = k > 4;
-  if (p->ipt == 2) {
-    if (wide) launch_spmm<2, 8>(p, s, rp, col, val, Xr, ldr, k, Y, ldy, yrm, carry, alpha, beta);
-    else launch_spmm<2, 4>(p, s, rp, col, val, Xr, ldr, k, Y, ldy, yrm, carry, alpha, beta);
+  // k <= 4: one lane per path thread with KC = 4.  Wider: G lanes per path
+  // thread, G chosen so each path thread walks 8 steps (G * items per thread)
+  // and no group of a chunk is idle for k = 8.
+  if (k <= 4) {
+    if (p->ipt == 2)
+      launch_spmm<2, 4>(p, s, rp, col, val, Xr, ldr, k, Y, ldy, yrm, carry, alpha, beta);
+    else
+      launch_spmm<4, 4>(p, s, rp, col, val, Xr, ldr, k, Y, ldy, yrm, carry, alpha, beta);
+  } else if (p->ipt == 2) {
+    if (k <= 8)
+      launch_spmm_rows<256, 512, 2>(p, s, rp, col, val, Xr, ldr, k, Y, ldy, yrm, carry, alpha,
+                                    beta);
+    else
+      launch_spmm_rows<256, 512, 4>(p, s, rp, col, val, Xr, ldr, k, Y, ldy, yrm, carry, alpha,
+                                    beta);
   } else {
-    if (wide) launch_spmm<4, 8>(p, s, rp, col, val, Xr, ldr, k, Y, ldy, yrm, carry, alpha, beta);
-    else launch_spmm<4, 4>(p, s, rp, col, val, Xr, ldr, k, Y, ldy, yrm, carry, alpha, beta);
+    launch_spmm_rows<256, 1024, 2>(p, s, rp, col, val, Xr, ldr, k, Y, ldy, yrm, carry, alpha,
+                                   beta);
   }
   return hipGetLastError() == hipSuccess ? MBX_OK : MBX_EHIP;
 }

@@ -52,16 +52,17 @@ def test_spmm_edge_cases(orc, device, ipt, k):
         plan.close()
 
 
-def test_spmm_alpha_beta_and_ld(orc, device):
+@pytest.mark.parametrize("ipt", [2, 4])
+@pytest.mark.parametrize("k", [6, 21])
+def test_spmm_alpha_beta_and_ld(orc, device, ipt, k):
     from merbit_amd.spmv import MerbitPlan
-    rng = np.random.default_rng(4)
+    rng = np.random.default_rng(4 + k)
     rp, col = random_csr(rng, 3000, 2500, lens=(rng.pareto(1.2, 3000) * 5).astype(int))
     val = rng.uniform(-1, 1, int(rp[-1])).astype(np.float32)
-    k = 6
     X = rng.uniform(-1, 1, (2500, k)).astype(np.float32)
     Y0 = rng.uniform(-1, 1, (3000, k)).astype(np.float32)
     d_rp, d_col, d_val = dev(rp, device), dev(col, device), dev(val, device)
-    plan = MerbitPlan(d_rp, 2500, int(rp[-1]), items_per_thread=4)
+    plan = MerbitPlan(d_rp, 2500, int(rp[-1]), items_per_thread=ipt)
     # leading dimensions larger than the matrix: views into bigger buffers
     Xbig = torch.zeros((k + 3, 2600), dtype=torch.float32, device=device)
     Xbig[:k, :2500] = dev(X.T.copy(), device)
@@ -69,7 +70,7 @@ def test_spmm_alpha_beta_and_ld(orc, device):
     Ybig = torch.zeros((3000, k + 5), dtype=torch.float32, device=device)
     Ybig[:, :k] = dev(Y0, device)
     Yv = Ybig[:, :k]
-    say("spmm alpha beta ld")
+    say(f"spmm alpha beta ld ipt={ipt} k={k}")
     plan.spmm(d_rp, d_col, d_val, Xv, Yv, alpha=-2.0, beta=0.5)
     Y = Yv.cpu().numpy().astype(np.float64)
     for j in range(k):
@@ -82,23 +83,25 @@ def test_spmm_alpha_beta_and_ld(orc, device):
 
 
 @pytest.mark.slow
+@pytest.mark.parametrize("ipt", [2, 4])
 @pytest.mark.parametrize("k", [8, 32])
-def test_spmm_rmat(orc, device, k):
+def test_spmm_rmat(orc, device, k, ipt):
     from merbit_amd import gen, workloads
     from merbit_amd.spmv import MerbitPlan
-    say(f"spmm rmat scale 18 k={k}")
+    say(f"spmm rmat scale 18 k={k} ipt={ipt}")
     A, _ = workloads.rmat(18)
     val = A.val.astype(np.float32)
     X = gen.uniform(A.ncol * k, -1, 1, 77, np.float32).reshape(k, A.ncol).T
     d_rp, d_col, d_val = dev(A.row_ptr, device), dev(A.col, device), dev(val, device)
-    plan = MerbitPlan(d_rp, A.ncol, A.nnz, items_per_thread=4)
+    plan = MerbitPlan(d_rp, A.ncol, A.nnz, items_per_thread=ipt)
     Y = plan.spmm(d_rp, d_col, d_val, dev(X.T.copy(), device).t()).cpu().numpy()
     check_spmm(orc, A.row_ptr, A.col, val, X, Y)
     plan.close()
 
 
+@pytest.mark.parametrize("ipt", [2, 4])
 @pytest.mark.parametrize("k", [5, 64, 70])
-def test_spmm_padded_columns(orc, device, k):
+def test_spmm_padded_columns(orc, device, k, ipt):
     """Column-major X whose leading dimension is a multiple of 4 but whose
     column count is not (the 16-byte repack path with a ragged tail), and
     more right-hand sides than one repack block holds."""
@@ -108,10 +111,10 @@ def test_spmm_padded_columns(orc, device, k):
     rp, col = random_csr(rng, 1700, ncol, lens=(rng.pareto(1.1, 1700) * 4).astype(int))
     val = rng.uniform(-1, 1, int(rp[-1])).astype(np.float32)
     X = rng.uniform(-1, 1, (ncol, k)).astype(np.float32)
-    say(f"spmm padded columns k={k}")
+    say(f"spmm padded columns k={k} ipt={ipt}")
     Xbig = torch.full((k, ldx), float("nan"), dtype=torch.float32, device=device)
     Xbig[:, :ncol] = dev(X.T.copy(), device)
-    plan = MerbitPlan(dev(rp, device), ncol, int(rp[-1]), items_per_thread=4)
+    plan = MerbitPlan(dev(rp, device), ncol, int(rp[-1]), items_per_thread=ipt)
     Y = plan.spmm(dev(rp, device), dev(col, device), dev(val, device), Xbig[:, :ncol].t())
     check_spmm(orc, rp, col, val, X, Y.cpu().numpy())
     plan.close()
