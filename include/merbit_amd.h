@@ -248,6 +248,36 @@ int mbx_autotune_spmm(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol, int3
 int mbx_autotune_batch(const mbx_batch_item* items, int32_t count, int32_t value_kind,
                        int32_t reps, void* stream, int32_t* best_ipt, float* best_us);
 
+/* Column-blocked form of a pattern (no reference counterpart).  For products
+ * whose x gathers spread evenly over an x larger than one XCD's 4 MB L2 (a
+ * random graph), the columns are split into nblocks ranges and A into the
+ * CSR matrices A_j over all rows keeping the columns of range j; a product
+ * chains y = alpha*A_0*x_0 + beta*y, then y = alpha*A_j*x_j + y, each a
+ * merge-path SpMV whose x slice stays in L2.  Row sums are split over the
+ * blocks and added in block order: deterministic, rounded differently from
+ * mbx_spmv_* on the CSR.
+ *   mbx_colblock_create: builds the block-major pattern (block after block,
+ *       rows in order, a row's entries in CSR order; block starts padded to
+ *       4 slots) and one plan per block; nblocks 0..16 (0: slices of at most
+ *       2.5 MiB of 4-byte x), clamped to ncol.  Synchronises `stream`.
+ *   mbx_colblock_info: block count, value slots (the length of a blocked
+ *       value array: nnz plus padding) and the nblocks+1 column bounds.
+ *   mbx_colblock_values: moves values of the given kind from CSR order into
+ *       a blocked array of value_slots entries (padding slots set to 0).
+ *   mbx_colblock_spmv: the chained product on blocked values; x and y as
+ *       for mbx_spmv_f64 (MBX_VALUE_F64) or mbx_spmv_f32/bf16 (FP32 x, y). */
+typedef struct mbx_colblock mbx_colblock;
+int mbx_colblock_create(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow,
+                        int32_t ncol, int32_t nnz, int32_t nblocks, int32_t items_per_thread,
+                        void* stream, mbx_colblock** out);
+void mbx_colblock_destroy(mbx_colblock* cb);
+int mbx_colblock_info(const mbx_colblock* cb, int32_t* nblocks, int64_t* value_slots,
+                      int32_t* col_bounds);
+int mbx_colblock_values(const mbx_colblock* cb, int32_t value_kind, const void* d_val,
+                        void* d_val_blocked, void* stream);
+int mbx_colblock_spmv(const mbx_colblock* cb, int32_t value_kind, const void* d_val_blocked,
+                      double alpha, const void* d_x, double beta, void* d_y, void* stream);
+
 /* Adjoint of y = A*x (no reference counterpart).  An adjoint object holds the
  * transposed pattern (CSC of A built on the device by a stable radix sort:
  * rows stay ascending within every column, so dx sums deterministically) and

@@ -25,6 +25,8 @@ ABI_SYMBOLS = (
     "mbx_adjoint_create", "mbx_adjoint_destroy", "mbx_adjoint_transpose", "mbx_adjoint_dx",
     "mbx_sddmm", "mbx_ingest_ref", "mbx_plan_stats",
     "mbx_pagerank_workspace_bytes", "mbx_pagerank_f64", "mbx_pagerank_graph_create",
+    "mbx_colblock_create", "mbx_colblock_destroy", "mbx_colblock_info", "mbx_colblock_values",
+    "mbx_colblock_spmv",
     "mbx_status_str",
 )
 GEN_SYMBOLS = (
@@ -123,6 +125,12 @@ def lib() -> ctypes.CDLL:
                                     ctypes.POINTER(f32)]
     L.mbx_autotune_batch.argtypes = [vp, i32, i32, i32, vp, ctypes.POINTER(i32),
                                      ctypes.POINTER(f32)]
+    L.mbx_colblock_create.argtypes = [vp, vp, i32, i32, i32, i32, i32, vp, ctypes.POINTER(vp)]
+    L.mbx_colblock_destroy.argtypes = [vp]
+    L.mbx_colblock_destroy.restype = None
+    L.mbx_colblock_info.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(ctypes.c_int64), vp]
+    L.mbx_colblock_values.argtypes = [vp, i32, vp, vp, vp]
+    L.mbx_colblock_spmv.argtypes = [vp, i32, vp, f64, vp, f64, vp, vp]
     L.mbx_adjoint_create.argtypes = [vp, vp, i32, i32, i32, i32, vp, ctypes.POINTER(vp)]
     L.mbx_adjoint_destroy.argtypes = [vp]
     L.mbx_adjoint_destroy.restype = None

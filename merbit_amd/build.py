@@ -45,6 +45,7 @@ LIB_SOURCES = [
     os.path.join(CSRC, "merbit_adjoint.hip"),
     os.path.join(CSRC, "merbit_ingest.hip"),
     os.path.join(CSRC, "merbit_pagerank.hip"),
+    os.path.join(CSRC, "merbit_colblock.hip"),
     os.path.join(CSRC, "merbit_gen.cpp"),
     os.path.join(CSRC, "merbit_host.cpp"),
 ]

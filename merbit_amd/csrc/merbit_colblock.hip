@@ -1,0 +1,310 @@
+// merbit_colblock.hip -- column-blocked form of a CSR pattern for products
+// whose x gathers are spread evenly over a long x (random graphs).
+//
+// The merge-path product gathers x[col] once per nonzero.  When x is larger
+// than one XCD's 4 MB L2 and the columns have no hot set (a random graph:
+// 2.4M FP32 x values, 9.6 MB), most gathers miss L2 and the product is bound
+// by line fills from the MALL, not by HBM.  Splitting the columns into B
+// ranges and the matrix into B CSR matrices over all rows,
+//   A = A_0 + A_1 + ... + A_{B-1},  A_j keeping the columns [lo_j, lo_{j+1}),
+// lets each product A_j x_j gather from a slice of x that stays in L2.  The
+// products are chained on one stream (y = alpha*A_0 x_0 + beta*y, then
+// y = alpha*A_j x_j + y), each a plain merge-path SpMV over its own plan.
+//
+// The B matrices live in one block-major nonzero order: block after block,
+// rows in order within a block, entries of a row in CSR order.  Block starts
+// are padded to 4 entries so every block's col and values keep the 16-byte
+// alignment of the vector staging path.  The pattern is converted once
+// (mbx_colblock_create); values are moved into block-major order with
+// mbx_colblock_values whenever they change, so the form pays when one set of
+// values serves many products.  A row's sum is split over the blocks and
+// added in block order: results are deterministic, but rounded differently
+// from the single CSR product (FP64 agrees to a few units in the last place).
+
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+#include <stdint.h>
+
+#include <new>
+
+#include "merbit_amd.h"
+#include "merbit_internal.hpp"
+
+constexpr int kMaxBlocks = 16;
+
+struct mbx_colblock {
+  int32_t nrow = 0, ncol = 0, nnz = 0, nb = 0;
+  int32_t stride = 0;                 // row_ptr entries per block (nrow+1 rounded to 4)
+  int32_t lo[kMaxBlocks + 1] = {};    // column ranges
+  int64_t base[kMaxBlocks + 1] = {};  // padded first slot of each block
+  int32_t* row_ptr = nullptr;         // nb * stride: block j's row pointer at j * stride
+  int32_t* col = nullptr;             // slots: column minus lo_j, block-major
+  int32_t* perm = nullptr;            // slots: CSR position of each slot, -1 for padding
+  mbx_plan* plan[kMaxBlocks] = {};
+};
+
+namespace {
+
+struct Bounds {
+  int32_t v[kMaxBlocks + 1];
+};
+
+__device__ __forceinline__ int block_of(int c, int nb, int ncol) {
+  const long long j = ((long long)c * nb) / ncol;
+  return (int)(j < 0 ? 0 : (j >= nb ? nb - 1 : j));
+}
+
+// One thread per row: entries of the row in each column block.
+__global__ void cb_count_kernel(const int32_t* __restrict__ row_ptr,
+                                const int32_t* __restrict__ col, int nrow, int ncol, int nb,
+                                int stride, int32_t* __restrict__ cnt) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrow) return;
+  int c16[kMaxBlocks];
+#pragma unroll
+  for (int j = 0; j < kMaxBlocks; ++j) c16[j] = 0;
+  for (int k = row_ptr[r]; k < row_ptr[r + 1]; ++k) {
+    const int j = block_of(col[k], nb, ncol);
+#pragma unroll
+    for (int q = 0; q < kMaxBlocks; ++q) c16[q] += (q == j);
+  }
+  for (int j = 0; j < nb; ++j) cnt[(long long)j * stride + r] = c16[j];
+}
+
+// Exclusive-scanned counts -> each block's own row pointer (starting at 0).
+__global__ void cb_rebase_kernel(const int32_t* __restrict__ scanned, int nrow, int nb,
+                                 int stride, Bounds start, int32_t* __restrict__ row_ptr) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)nb * stride) return;
+  const int j = (int)(i / stride), r = (int)(i - (long long)j * stride);
+  row_ptr[i] = r <= nrow ? scanned[i] - start.v[j] : 0;
+}
+
+// One thread per row: place the row's entries in its blocks' rows.
+__global__ void cb_scatter_kernel(const int32_t* __restrict__ row_ptr,
+                                  const int32_t* __restrict__ col, int nrow, int ncol, int nb,
+                                  int stride, Bounds lo, Bounds base,
+                                  const int32_t* __restrict__ brp, int32_t* __restrict__ bcol,
+                                  int32_t* __restrict__ perm) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrow) return;
+  int seen[kMaxBlocks];
+#pragma unroll
+  for (int j = 0; j < kMaxBlocks; ++j) seen[j] = 0;
+  for (int k = row_ptr[r]; k < row_ptr[r + 1]; ++k) {
+    const int c = col[k];
+    const int j = block_of(c, nb, ncol);
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < kMaxBlocks; ++q)
+      if (q == j) s = seen[q]++;
+    const long long p = (long long)base.v[j] + brp[(long long)j * stride + r] + s;
+    bcol[p] = c - lo.v[j];
+    perm[p] = k;
+  }
+}
+
+template <class T>
+__global__ void cb_values_kernel(const int32_t* __restrict__ perm, long long slots,
+                                 const T* __restrict__ val, T* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= slots) return;
+  const int k = perm[i];
+  out[i] = k >= 0 ? val[k] : T(0);
+}
+
+void destroy(mbx_colblock* cb) {
+  if (!cb) return;
+  (void)hipDeviceSynchronize();
+  for (int j = 0; j < kMaxBlocks; ++j)
+    if (cb->plan[j]) mbx_plan_destroy(cb->plan[j]);
+  if (cb->row_ptr) (void)hipFree(cb->row_ptr);
+  if (cb->col) (void)hipFree(cb->col);
+  if (cb->perm) (void)hipFree(cb->perm);
+  delete cb;
+}
+
+int value_bytes(int32_t kind) {
+  switch (kind) {
+    case MBX_VALUE_F64: return 8;
+    case MBX_VALUE_F32: return 4;
+    case MBX_VALUE_BF16: return 2;
+    default: return 0;
+  }
+}
+
+int build(mbx_colblock* cb, const int32_t* rp, const int32_t* col, int ipt, hipStream_t s) {
+  const int nrow = cb->nrow, nb = cb->nb, stride = cb->stride;
+  const long long n = (long long)nb * stride;
+  int32_t* cnt = nullptr;
+  int32_t* scanned = nullptr;
+  void* tmp = nullptr;
+  size_t tmp_bytes = 0;
+  int st = MBX_OK;
+  if (hipMalloc(&cnt, sizeof(int32_t) * n) != hipSuccess ||
+      hipMalloc(&scanned, sizeof(int32_t) * n) != hipSuccess ||
+      hipMalloc(&cb->row_ptr, sizeof(int32_t) * n) != hipSuccess)
+    st = MBX_ENOMEM;
+  if (st == MBX_OK && hipMemsetAsync(cnt, 0, sizeof(int32_t) * n, s) != hipSuccess) st = MBX_EHIP;
+  if (st == MBX_OK && nrow > 0)
+    cb_count_kernel<<<(nrow + 255) / 256, 256, 0, s>>>(rp, col, nrow, cb->ncol, nb, stride, cnt);
+  if (st == MBX_OK &&
+      hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, cnt, scanned, (int)n, s) != hipSuccess)
+    st = MBX_EHIP;
+  if (st == MBX_OK && hipMalloc(&tmp, tmp_bytes > 0 ? tmp_bytes : 1) != hipSuccess)
+    st = MBX_ENOMEM;
+  if (st == MBX_OK &&
+      hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, cnt, scanned, (int)n, s) != hipSuccess)
+    st = MBX_EHIP;
+  // block starts in the unpadded order, read back to size and pad the blocks
+  int32_t h_start[kMaxBlocks] = {};
+  for (int j = 0; j < nb && st == MBX_OK; ++j)
+    if (hipMemcpyAsync(&h_start[j], scanned + (long long)j * stride, sizeof(int32_t),
+                       hipMemcpyDeviceToHost, s) != hipSuccess)
+      st = MBX_EHIP;
+  if (st == MBX_OK && hipStreamSynchronize(s) != hipSuccess) st = MBX_EHIP;
+  Bounds start{}, lo{}, base{};
+  int32_t nnz_b[kMaxBlocks] = {};
+  if (st == MBX_OK) {
+    for (int j = 0; j < nb; ++j) {
+      start.v[j] = h_start[j];
+      nnz_b[j] = (j + 1 < nb ? h_start[j + 1] : cb->nnz) - h_start[j];
+      lo.v[j] = cb->lo[j];
+    }
+    cb->base[0] = 0;
+    for (int j = 0; j < nb; ++j) cb->base[j + 1] = cb->base[j] + (nnz_b[j] + 3) / 4 * 4;
+    if (cb->base[nb] > INT32_MAX) st = MBX_EOVERFLOW;
+    for (int j = 0; j <= nb; ++j) base.v[j] = (int32_t)cb->base[j];
+  }
+  const long long slots = cb->base[nb];
+  if (st == MBX_OK &&
+      (hipMalloc(&cb->col, sizeof(int32_t) * (slots > 0 ? slots : 1)) != hipSuccess ||
+       hipMalloc(&cb->perm, sizeof(int32_t) * (slots > 0 ? slots : 1)) != hipSuccess))
+    st = MBX_ENOMEM;
+  if (st == MBX_OK && slots > 0 &&
+      (hipMemsetAsync(cb->col, 0, sizeof(int32_t) * slots, s) != hipSuccess ||
+       hipMemsetAsync(cb->perm, 0xFF, sizeof(int32_t) * slots, s) != hipSuccess))
+    st = MBX_EHIP;
+  if (st == MBX_OK) {
+    cb_rebase_kernel<<<(int)((n + 255) / 256), 256, 0, s>>>(scanned, nrow, nb, stride, start,
+                                                             cb->row_ptr);
+    if (nrow > 0)
+      cb_scatter_kernel<<<(nrow + 255) / 256, 256, 0, s>>>(rp, col, nrow, cb->ncol, nb, stride,
+                                                           lo, base, cb->row_ptr, cb->col,
+                                                           cb->perm);
+    if (hipGetLastError() != hipSuccess) st = MBX_EHIP;
+  }
+  for (int j = 0; j < nb && st == MBX_OK; ++j)
+    st = mbx_plan_create(cb->row_ptr + (long long)j * stride, nrow, cb->lo[j + 1] - cb->lo[j],
+                         nnz_b[j], ipt, 256, s, &cb->plan[j]);
+  if (hipStreamSynchronize(s) != hipSuccess && st == MBX_OK) st = MBX_EHIP;
+  if (tmp) (void)hipFree(tmp);
+  if (scanned) (void)hipFree(scanned);
+  if (cnt) (void)hipFree(cnt);
+  return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mbx_colblock_create(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow,
+                        int32_t ncol, int32_t nnz, int32_t nblocks, int32_t items_per_thread,
+                        void* stream, mbx_colblock** out) {
+  if (!out) return MBX_EINVAL;
+  *out = nullptr;
+  if (nrow < 0 || ncol < 0 || nnz < 0 || nblocks < 0 || nblocks > kMaxBlocks) return MBX_EINVAL;
+  if (!d_row_ptr || (nnz > 0 && !d_col)) return MBX_EINVAL;
+  if (nnz > 0 && ncol == 0) return MBX_EINVAL;
+  int nb = nblocks;
+  if (nb == 0) {  // slices of at most 2.5 MiB of 4-byte x
+    const long long slice = (5ll << 20) / 2 / 4;
+    nb = (int)((ncol + slice - 1) / slice);
+    nb = nb < 1 ? 1 : (nb > kMaxBlocks ? kMaxBlocks : nb);
+  }
+  if (ncol > 0 && nb > ncol) nb = ncol;
+  if (nb < 1) nb = 1;
+  const long long stride = ((long long)nrow + 1 + 3) / 4 * 4;
+  if (stride * nb > INT32_MAX) return MBX_EOVERFLOW;
+  mbx_colblock* cb = new (std::nothrow) mbx_colblock;
+  if (!cb) return MBX_ENOMEM;
+  cb->nrow = nrow;
+  cb->ncol = ncol;
+  cb->nnz = nnz;
+  cb->nb = nb;
+  cb->stride = (int32_t)stride;
+  for (int j = 0; j <= nb; ++j)  // lo_j = ceil(j * ncol / nb): c is in block floor(c * nb / ncol)
+    cb->lo[j] = (int32_t)(((long long)j * ncol + nb - 1) / nb);
+  const int st = build(cb, d_row_ptr, d_col, items_per_thread, (hipStream_t)stream);
+  if (st != MBX_OK) {
+    destroy(cb);
+    return st;
+  }
+  *out = cb;
+  return MBX_OK;
+}
+
+void mbx_colblock_destroy(mbx_colblock* cb) { destroy(cb); }
+
+int mbx_colblock_info(const mbx_colblock* cb, int32_t* nblocks, int64_t* value_slots,
+                      int32_t* col_bounds) {
+  if (!cb) return MBX_EINVAL;
+  if (nblocks) *nblocks = cb->nb;
+  if (value_slots) *value_slots = cb->base[cb->nb];
+  if (col_bounds)
+    for (int j = 0; j <= cb->nb; ++j) col_bounds[j] = cb->lo[j];
+  return MBX_OK;
+}
+
+int mbx_colblock_values(const mbx_colblock* cb, int32_t value_kind, const void* d_val,
+                        void* d_val_blocked, void* stream) {
+  if (!cb) return MBX_EINVAL;
+  const int vb = value_bytes(value_kind);
+  if (vb == 0) return MBX_EINVAL;
+  const long long slots = cb->base[cb->nb];
+  if (slots == 0) return MBX_OK;
+  if (!d_val_blocked || (cb->nnz > 0 && !d_val)) return MBX_EINVAL;
+  hipStream_t s = (hipStream_t)stream;
+  const int grid = (int)((slots + 255) / 256);
+  if (vb == 8)
+    cb_values_kernel<<<grid, 256, 0, s>>>(cb->perm, slots, (const uint64_t*)d_val,
+                                          (uint64_t*)d_val_blocked);
+  else if (vb == 4)
+    cb_values_kernel<<<grid, 256, 0, s>>>(cb->perm, slots, (const uint32_t*)d_val,
+                                          (uint32_t*)d_val_blocked);
+  else
+    cb_values_kernel<<<grid, 256, 0, s>>>(cb->perm, slots, (const uint16_t*)d_val,
+                                          (uint16_t*)d_val_blocked);
+  return hipGetLastError() == hipSuccess ? MBX_OK : MBX_EHIP;
+}
+
+int mbx_colblock_spmv(const mbx_colblock* cb, int32_t value_kind, const void* d_val_blocked,
+                      double alpha, const void* d_x, double beta, void* d_y, void* stream) {
+  if (!cb) return MBX_EINVAL;
+  const int vb = value_bytes(value_kind);
+  if (vb == 0) return MBX_EINVAL;
+  if (cb->nrow > 0 && !d_y) return MBX_EINVAL;
+  if (cb->nnz > 0 && (!d_val_blocked || !d_x)) return MBX_EINVAL;
+  const int xb = value_kind == MBX_VALUE_F64 ? 8 : 4;
+  for (int j = 0; j < cb->nb; ++j) {
+    const int32_t* rp = cb->row_ptr + (long long)j * cb->stride;
+    const int32_t* col = cb->col + cb->base[j];
+    const char* val = static_cast<const char*>(d_val_blocked) + cb->base[j] * vb;
+    const char* x = d_x ? static_cast<const char*>(d_x) + (long long)cb->lo[j] * xb : nullptr;
+    const double bj = j == 0 ? beta : 1.0;
+    int st;
+    if (value_kind == MBX_VALUE_F64)
+      st = mbx_spmv_f64(cb->plan[j], rp, col, (const double*)val, alpha, (const double*)x, bj,
+                        (double*)d_y, stream);
+    else if (value_kind == MBX_VALUE_F32)
+      st = mbx_spmv_f32(cb->plan[j], rp, col, (const float*)val, (float)alpha,
+                        (const float*)x, (float)bj, (float*)d_y, stream);
+    else
+      st = mbx_spmv_bf16(cb->plan[j], rp, col, (const uint16_t*)val, (float)alpha,
+                         (const float*)x, (float)bj, (float*)d_y, stream);
+    if (st != MBX_OK) return st;
+  }
+  return MBX_OK;
+}
+
+}  // extern "C"

@@ -254,3 +254,93 @@ class DeviceCsr:
 
     def spmv(self, x, y=None, alpha=1.0, beta=0.0, stream=None):
         return self.plan.spmv(self.row_ptr, self.col, self.val, x, y, alpha, beta, stream)
+
+
+_KIND_CODES = {"f64": 0, "f32": 1, "bf16": 2}
+
+
+class ColumnBlockedCsr:
+    """Column-blocked form of one CSR pattern (mbx_colblock_*): the columns
+    split into ``nblocks`` ranges, A = A_0 + ... + A_{B-1} with A_j keeping
+    the columns of range j, and a product chaining the block products so each
+    gathers from a slice of x that stays in one XCD's L2.  For patterns whose
+    x gathers spread evenly over a long x (random graphs); values are moved
+    into the block-major order once with ``values`` and reused."""
+
+    def __init__(self, row_ptr: torch.Tensor, col: torch.Tensor, ncol: int,
+                 nblocks: int = 0, items_per_thread: int = 8, stream=None):
+        self.nrow = int(row_ptr.numel()) - 1
+        self.ncol = int(ncol)
+        self.nnz = int(col.numel())
+        h = ctypes.c_void_p()
+        check(lib().mbx_colblock_create(_dptr(row_ptr, torch.int32, "row_ptr"),
+                                        _dptr(col, torch.int32, "col"), self.nrow, self.ncol,
+                                        self.nnz, nblocks, items_per_thread,
+                                        _stream_handle(stream), ctypes.byref(h)),
+              "mbx_colblock_create")
+        self._h = h
+        nb, slots = ctypes.c_int32(), ctypes.c_int64()
+        bounds = (ctypes.c_int32 * 17)()
+        check(lib().mbx_colblock_info(h, ctypes.byref(nb), ctypes.byref(slots), bounds),
+              "mbx_colblock_info")
+        self.nblocks = nb.value
+        self.value_slots = slots.value
+        self.col_bounds = [bounds[j] for j in range(self.nblocks + 1)]
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        if self._h is None:
+            raise RuntimeError("column-blocked pattern is closed")
+        return self._h
+
+    def values(self, val: torch.Tensor, out: torch.Tensor | None = None,
+               stream=None) -> torch.Tensor:
+        """``val`` (CSR order) moved into the block-major order."""
+        kind = _VALUE_KINDS.get(val.dtype)
+        if kind is None:
+            raise TypeError(f"unsupported value dtype {val.dtype}")
+        if val.numel() != self.nnz:
+            raise ValueError("val does not match the pattern")
+        if out is None:
+            out = torch.empty(max(self.value_slots, 1), dtype=val.dtype, device=val.device)
+        if out.numel() < self.value_slots or out.dtype != val.dtype:
+            raise ValueError("out is too short or of another dtype")
+        check(lib().mbx_colblock_values(self.handle, _KIND_CODES[kind],
+                                        _dptr(val, val.dtype, "val"),
+                                        _dptr(out, out.dtype, "out"), _stream_handle(stream)),
+              "mbx_colblock_values")
+        return out
+
+    def spmv(self, val_blocked: torch.Tensor, x: torch.Tensor, y: torch.Tensor | None = None,
+             alpha: float = 1.0, beta: float = 0.0, stream=None) -> torch.Tensor:
+        """y = alpha*A*x + beta*y on values in block-major order."""
+        kind = _VALUE_KINDS.get(val_blocked.dtype)
+        if kind is None:
+            raise TypeError(f"unsupported value dtype {val_blocked.dtype}")
+        if val_blocked.numel() < self.value_slots:
+            raise ValueError("val_blocked is shorter than the blocked pattern")
+        xdt = torch.float64 if kind == "f64" else torch.float32
+        if x.numel() != self.ncol:
+            raise ValueError("x has the wrong length")
+        if y is None:
+            y = torch.empty(self.nrow, dtype=xdt, device=x.device)
+            beta = 0.0
+        if y.numel() != self.nrow:
+            raise ValueError("y has the wrong length")
+        check(lib().mbx_colblock_spmv(self.handle, _KIND_CODES[kind],
+                                      _dptr(val_blocked, val_blocked.dtype, "val_blocked"),
+                                      alpha, _dptr(x, xdt, "x"), beta, _dptr(y, xdt, "y"),
+                                      _stream_handle(stream)),
+              "mbx_colblock_spmv")
+        return y
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None:
+            lib().mbx_colblock_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

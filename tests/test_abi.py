@@ -55,6 +55,14 @@ def test_host_side_argument_checks():
     assert L.mbx_pagerank_workspace_bytes(-1) == 0
     assert L.mbx_pagerank_workspace_bytes(1000) >= 8 * 1000
     assert L.mbx_ingest_ref(-1, 0, None, None, 0, None, None, None, None) == -1
+    h = ctypes.c_void_p()
+    assert L.mbx_colblock_create(None, None, 4, 4, 0, 0, 8, None, ctypes.byref(h)) == -1
+    assert L.mbx_colblock_create(1, 1, 4, 4, 8, 17, 8, None, ctypes.byref(h)) == -1
+    assert L.mbx_colblock_create(1, 1, 4, 0, 8, 0, 8, None, ctypes.byref(h)) == -1
+    assert L.mbx_colblock_create(1, 1, 4, 4, 8, 0, 8, None, None) == -1
+    assert L.mbx_colblock_info(None, None, None, None) == -1
+    assert L.mbx_colblock_values(None, 0, None, None, None) == -1
+    assert L.mbx_colblock_spmv(None, 0, None, 1.0, None, 0.0, None, None) == -1
     assert _lib.status_str(0) == "ok"
     assert "overflow" not in _lib.status_str(-1)
     assert "int32" in _lib.status_str(-2)

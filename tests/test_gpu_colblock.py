@@ -1,0 +1,132 @@
+"""GPU parity of the column-blocked product (mbx_colblock_*).
+
+* The block-major layout is checked bit for bit: values moved by
+  mbx_colblock_values must equal a numpy stable sort of the CSR entries by
+  (column block, row, position), with block starts padded to 4 slots and
+  zeros in the padding.
+* The chained product is checked against the oracle's accurate CSR product
+  with the bars of the plain product: condition-scaled error <= 1e-12 for
+  FP64 and <= 1e-6 for FP32/BF16 values (rows are split over the blocks, so
+  the rounding differs from mbx_spmv_* but stays within them)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import cond_err, edge_case_patterns, random_csr
+
+pytestmark = pytest.mark.gpu
+
+F64_TOL = 1e-12
+MIXED_TOL = 1e-6
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def dev(a, device):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device)
+
+
+def expected_blocked(rp, col, ncol, nb, val):
+    """Block-major values with padded block starts (numpy restatement)."""
+    nrow = len(rp) - 1
+    rows = np.repeat(np.arange(nrow), np.diff(rp))
+    blk = (col.astype(np.int64) * nb) // max(ncol, 1)
+    order = np.lexsort((np.arange(len(col)), rows, blk))
+    counts = np.bincount(blk, minlength=nb) if len(col) else np.zeros(nb, np.int64)
+    base = np.concatenate([[0], np.cumsum((counts + 3) // 4 * 4)])
+    out = np.zeros(int(base[-1]), val.dtype)
+    start = 0
+    for j in range(nb):
+        out[base[j]:base[j] + counts[j]] = val[order[start:start + counts[j]]]
+        start += counts[j]
+    return out
+
+
+def cases(rng):
+    out = [(n, lens, ncol) for n, lens, ncol in edge_case_patterns(2048)]
+    out.append(("random_graph", rng.poisson(25, 30000), 30000))
+    out.append(("narrow", rng.poisson(4, 3000), 3))
+    return out
+
+
+@pytest.mark.parametrize("nb", [0, 1, 3, 16])
+def test_colblock_layout_and_f64(orc, device, nb):
+    from merbit_amd.spmv import ColumnBlockedCsr
+    rng = np.random.default_rng(500 + nb)
+    for name, lens, ncol in cases(rng):
+        say(f"colblock f64 {name} nb={nb}")
+        rp, col = random_csr(rng, len(lens), ncol, lens=lens)
+        if name == "random_graph":  # columns need not be sorted within rows
+            for r in range(0, len(lens), 7):
+                rng.shuffle(col[rp[r]:rp[r + 1]])
+        val = rng.standard_normal(int(rp[-1]))
+        x = rng.uniform(-1, 1, ncol)
+        cb = ColumnBlockedCsr(dev(rp, device), dev(col, device), ncol, nblocks=nb)
+        assert 1 <= cb.nblocks <= 16 and cb.nblocks <= max(ncol, 1)
+        if nb:
+            assert cb.nblocks == min(nb, ncol)
+        vb = cb.values(dev(val, device))
+        want = expected_blocked(rp, col, ncol, cb.nblocks, val)
+        assert cb.value_slots == len(want)
+        assert np.array_equal(vb[:cb.value_slots].cpu().numpy().view(np.uint64),
+                              want.view(np.uint64)), name
+        y = cb.spmv(vb, dev(x, device)).cpu().numpy()
+        ya, absr = orc.spmv_accurate(rp, col, val, x)
+        assert cond_err(y, ya, absr) <= F64_TOL, name
+        # alpha, beta; y read once
+        y0 = rng.uniform(-1, 1, len(lens))
+        yb = cb.spmv(vb, dev(x, device), dev(y0, device), alpha=-2.0, beta=0.5).cpu().numpy()
+        den = 2.0 * absr + 0.5 * np.abs(y0) + 1e-300
+        assert np.max(np.abs(yb - (-2.0 * ya + 0.5 * y0)) / den, initial=0.0) <= F64_TOL, name
+        cb.close()
+
+
+@pytest.mark.parametrize("kind", ["f32", "bf16"])
+def test_colblock_mixed(orc, device, kind):
+    from merbit_amd import gen
+    from merbit_amd.spmv import ColumnBlockedCsr
+    rng = np.random.default_rng(77)
+    for name, lens, ncol in cases(rng):
+        say(f"colblock {kind} {name}")
+        rp, col = random_csr(rng, len(lens), ncol, lens=lens)
+        v32 = rng.standard_normal(int(rp[-1])).astype(np.float32)
+        vals = v32 if kind == "f32" else gen.to_bf16(v32)
+        x = rng.uniform(-1, 1, ncol).astype(np.float32)
+        cb = ColumnBlockedCsr(dev(rp, device), dev(col, device), ncol, nblocks=4)
+        vb = cb.values(dev(vals, device))
+        want = expected_blocked(rp, col, ncol, cb.nblocks, vals)
+        assert np.array_equal(vb[:cb.value_slots].cpu().numpy(), want), name
+        y = cb.spmv(vb, dev(x, device)).cpu().numpy()
+        ya, absr = orc.spmv_accurate(rp, col, vals, x)
+        assert cond_err(y, ya, absr) <= MIXED_TOL, name
+        cb.close()
+
+
+def test_colblock_matches_csr_product_on_a_graph(device):
+    """On a random graph the blocked product agrees with the plain CSR
+    product to FP64 rounding, and repeated calls are bit-identical."""
+    from merbit_amd import workloads
+    from merbit_amd.spmv import ColumnBlockedCsr, MerbitPlan
+    say("colblock graph 200k")
+    A, x, _ = workloads.graph(200_000)
+    d_rp, d_col = dev(A.row_ptr, device), dev(A.col, device)
+    val = dev(A.val.astype(np.float64), device)
+    xx = dev(x.astype(np.float64), device)
+    plan = MerbitPlan(d_rp, A.ncol, A.nnz)
+    y_csr = plan.spmv(d_rp, d_col, val, xx).cpu().numpy()
+    cb = ColumnBlockedCsr(d_rp, d_col, A.ncol)
+    vb = cb.values(val)
+    outs = [cb.spmv(vb, xx).cpu().numpy() for _ in range(3)]
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    absr = np.abs(A.val.astype(np.float64))
+    s = np.zeros(A.nrow)
+    np.add.at(s, np.repeat(np.arange(A.nrow), np.diff(A.row_ptr)),
+              absr * np.abs(x.astype(np.float64))[A.col])
+    assert cond_err(outs[0], y_csr, s) <= F64_TOL
+    cb.close()
+    plan.close()
