@@ -444,14 +444,18 @@ def bench_adjoint(dev, args):
     yb = torch.empty(n, dtype=torch.float32, device=dev)
     vals_ms, _ = time_steps([lambda: cb.values(val, out=vb)], steps, args.warmup)
     fwdb_ms, _ = time_steps([lambda: cb.spmv(vb, xx, yb)], steps, args.warmup)
+    db = torch.zeros(max(cb.value_slots, 1), dtype=torch.float32, device=dev)
+    dvb_ms, _ = time_steps([lambda: cb.sddmm(gg, xx, out=db)], steps, args.warmup)
     out["forward_column_blocked"] = {
         "us": round(fwdb_ms * 1e3, 2), "gbs": gbs(b_fwd, fwdb_ms),
         "gflops": round(2 * nnz / (fwdb_ms * 1e-3) / 1e9, 1), "blocks": cb.nblocks,
         "speedup_vs_csr": round(fwd_ms / fwdb_ms, 2),
+        "dval_blocked_us": round(dvb_ms * 1e3, 2), "dval_speedup": round(dv_ms / dvb_ms, 2),
         "values_to_blocked_us": round(vals_ms * 1e3, 2), "build_ms": round(cb_build_ms, 1),
         "rel_diff_vs_csr": rel_err(yb, op.forward(val, xx))}
     log(f"graph forward, {cb.nblocks} column blocks: {fwdb_ms * 1e3:.1f} us "
-        f"(x{fwd_ms / fwdb_ms:.2f} vs CSR; values to block order {vals_ms * 1e3:.1f} us, "
+        f"(x{fwd_ms / fwdb_ms:.2f} vs CSR), dval {dvb_ms * 1e3:.1f} us (x{dv_ms / dvb_ms:.2f}); "
+        f"values to block order {vals_ms * 1e3:.1f} us, "
         f"build {cb_build_ms:.1f} ms)")
     cb.close()
     op.close()

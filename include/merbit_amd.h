@@ -265,7 +265,10 @@ int mbx_autotune_batch(const mbx_batch_item* items, int32_t count, int32_t value
  *   mbx_colblock_values: moves values of the given kind from CSR order into
  *       a blocked array of value_slots entries (padding slots set to 0).
  *   mbx_colblock_spmv: the chained product on blocked values; x and y as
- *       for mbx_spmv_f64 (MBX_VALUE_F64) or mbx_spmv_f32/bf16 (FP32 x, y). */
+ *       for mbx_spmv_f64 (MBX_VALUE_F64) or mbx_spmv_f32/bf16 (FP32 x, y).
+ *   mbx_colblock_sddmm: dval = g[row]*x[col] in block-major order (FP64 or
+ *       FP32), as mbx_sddmm on each block with its x slice; padding slots
+ *       are not written. */
 typedef struct mbx_colblock mbx_colblock;
 int mbx_colblock_create(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow,
                         int32_t ncol, int32_t nnz, int32_t nblocks, int32_t items_per_thread,
@@ -277,6 +280,8 @@ int mbx_colblock_values(const mbx_colblock* cb, int32_t value_kind, const void* 
                         void* d_val_blocked, void* stream);
 int mbx_colblock_spmv(const mbx_colblock* cb, int32_t value_kind, const void* d_val_blocked,
                       double alpha, const void* d_x, double beta, void* d_y, void* stream);
+int mbx_colblock_sddmm(const mbx_colblock* cb, int32_t value_kind, const void* d_g,
+                       const void* d_x, void* d_dval_blocked, void* stream);
 
 /* Adjoint of y = A*x (no reference counterpart).  An adjoint object holds the
  * transposed pattern (CSC of A built on the device by a stable radix sort:

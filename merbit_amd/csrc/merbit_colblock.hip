@@ -307,4 +307,21 @@ int mbx_colblock_spmv(const mbx_colblock* cb, int32_t value_kind, const void* d_
   return MBX_OK;
 }
 
+int mbx_colblock_sddmm(const mbx_colblock* cb, int32_t value_kind, const void* d_g,
+                       const void* d_x, void* d_dval_blocked, void* stream) {
+  if (!cb) return MBX_EINVAL;
+  if (value_kind != MBX_VALUE_F64 && value_kind != MBX_VALUE_F32) return MBX_EINVAL;
+  if (cb->nnz > 0 && (!d_g || !d_x || !d_dval_blocked)) return MBX_EINVAL;
+  const int vb = value_kind == MBX_VALUE_F64 ? 8 : 4;
+  for (int j = 0; j < cb->nb; ++j) {
+    const int32_t* rp = cb->row_ptr + (long long)j * cb->stride;
+    const char* x = d_x ? static_cast<const char*>(d_x) + (long long)cb->lo[j] * vb : nullptr;
+    char* dval = d_dval_blocked ? static_cast<char*>(d_dval_blocked) + cb->base[j] * vb : nullptr;
+    const int st = mbx_sddmm(cb->plan[j], value_kind, rp, cb->col + cb->base[j], d_g, x, dval,
+                             stream);
+    if (st != MBX_OK) return st;
+  }
+  return MBX_OK;
+}
+
 }  // extern "C"

@@ -334,6 +334,23 @@ class ColumnBlockedCsr:
               "mbx_colblock_spmv")
         return y
 
+    def sddmm(self, g: torch.Tensor, x: torch.Tensor, out: torch.Tensor | None = None,
+              stream=None) -> torch.Tensor:
+        """dval = g[row]*x[col] in block-major order (FP64 or FP32)."""
+        if g.dtype not in (torch.float64, torch.float32) or x.dtype != g.dtype:
+            raise TypeError("sddmm takes FP64 or FP32 g and x of one dtype")
+        if g.numel() != self.nrow or x.numel() != self.ncol:
+            raise ValueError("g or x has the wrong length")
+        if out is None:
+            out = torch.zeros(max(self.value_slots, 1), dtype=g.dtype, device=g.device)
+        if out.numel() < self.value_slots or out.dtype != g.dtype:
+            raise ValueError("out is too short or of another dtype")
+        kind = 0 if g.dtype == torch.float64 else 1
+        check(lib().mbx_colblock_sddmm(self.handle, kind, _dptr(g, g.dtype, "g"),
+                                       _dptr(x, x.dtype, "x"), _dptr(out, out.dtype, "out"),
+                                       _stream_handle(stream)), "mbx_colblock_sddmm")
+        return out
+
     def close(self) -> None:
         if getattr(self, "_h", None) is not None:
             lib().mbx_colblock_destroy(self._h)
