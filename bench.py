@@ -442,6 +442,16 @@ def bench_adjoint(dev, args):
     cb_build_ms = (time.time() - t0) * 1e3
     vb = cb.values(val)
     yb = torch.empty(n, dtype=torch.float32, device=dev)
+    tuned = {}
+    for mode in ("cached", "nontemporal"):
+        cb.set_stream(mode)
+        for wg in (0, 3, 4):
+            cb.set_residency(wg)
+            tuned[(mode, wg)] = min(time_steps([lambda: cb.spmv(vb, xx, yb)], 10, 2)[0]
+                                    for _ in range(2))
+    cb_mode, cb_wg = min(tuned, key=tuned.get)
+    cb.set_stream(cb_mode)
+    cb.set_residency(cb_wg)
     vals_ms, _ = time_steps([lambda: cb.values(val, out=vb)], steps, args.warmup)
     fwdb_ms, _ = time_steps([lambda: cb.spmv(vb, xx, yb)], steps, args.warmup)
     db = torch.zeros(max(cb.value_slots, 1), dtype=torch.float32, device=dev)
@@ -449,6 +459,7 @@ def bench_adjoint(dev, args):
     out["forward_column_blocked"] = {
         "us": round(fwdb_ms * 1e3, 2), "gbs": gbs(b_fwd, fwdb_ms),
         "gflops": round(2 * nnz / (fwdb_ms * 1e-3) / 1e9, 1), "blocks": cb.nblocks,
+        "stream_loads": cb_mode, "tiles_per_cu": cb_wg,
         "speedup_vs_csr": round(fwd_ms / fwdb_ms, 2),
         "dval_blocked_us": round(dvb_ms * 1e3, 2), "dval_speedup": round(dv_ms / dvb_ms, 2),
         "values_to_blocked_us": round(vals_ms * 1e3, 2), "build_ms": round(cb_build_ms, 1),

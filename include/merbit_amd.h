@@ -259,7 +259,7 @@ int mbx_autotune_batch(const mbx_batch_item* items, int32_t count, int32_t value
  *   mbx_colblock_create: builds the block-major pattern (block after block,
  *       rows in order, a row's entries in CSR order; block starts padded to
  *       4 slots) and one plan per block; nblocks 0..16 (0: slices of at most
- *       2.5 MiB of 4-byte x), clamped to ncol.  Synchronises `stream`.
+ *       3.5 MiB of 4-byte x), clamped to ncol.  Synchronises `stream`.
  *   mbx_colblock_info: block count, value slots (the length of a blocked
  *       value array: nnz plus padding) and the nblocks+1 column bounds.
  *   mbx_colblock_values: moves values of the given kind from CSR order into
@@ -268,7 +268,12 @@ int mbx_autotune_batch(const mbx_batch_item* items, int32_t count, int32_t value
  *       for mbx_spmv_f64 (MBX_VALUE_F64) or mbx_spmv_f32/bf16 (FP32 x, y).
  *   mbx_colblock_sddmm: dval = g[row]*x[col] in block-major order (FP64 or
  *       FP32), as mbx_sddmm on each block with its x slice; padding slots
- *       are not written. */
+ *       are not written.
+ *   mbx_colblock_set_stream / _set_residency: the stream cache policy and
+ *       tiles per CU of every block's product, as mbx_plan_set_stream /
+ *       mbx_plan_set_residency; MBX_STREAM_AUTO (the default) judges the
+ *       bytes of the whole chained product.  Products on one column-blocked
+ *       form must not overlap (they share its plans' carry slots). */
 typedef struct mbx_colblock mbx_colblock;
 int mbx_colblock_create(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow,
                         int32_t ncol, int32_t nnz, int32_t nblocks, int32_t items_per_thread,
@@ -282,6 +287,8 @@ int mbx_colblock_spmv(const mbx_colblock* cb, int32_t value_kind, const void* d_
                       double alpha, const void* d_x, double beta, void* d_y, void* stream);
 int mbx_colblock_sddmm(const mbx_colblock* cb, int32_t value_kind, const void* d_g,
                        const void* d_x, void* d_dval_blocked, void* stream);
+int mbx_colblock_set_stream(mbx_colblock* cb, int32_t mode);
+int mbx_colblock_set_residency(mbx_colblock* cb, int32_t wg_per_cu);
 
 /* Adjoint of y = A*x (no reference counterpart).  An adjoint object holds the
  * transposed pattern (CSC of A built on the device by a stable radix sort:

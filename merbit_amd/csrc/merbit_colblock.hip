@@ -35,6 +35,7 @@ constexpr int kMaxBlocks = 16;
 struct mbx_colblock {
   int32_t nrow = 0, ncol = 0, nnz = 0, nb = 0;
   int32_t stride = 0;                 // row_ptr entries per block (nrow+1 rounded to 4)
+  int32_t stream_mode = MBX_STREAM_AUTO;
   int32_t lo[kMaxBlocks + 1] = {};    // column ranges
   int64_t base[kMaxBlocks + 1] = {};  // padded first slot of each block
   int32_t* row_ptr = nullptr;         // nb * stride: block j's row pointer at j * stride
@@ -217,8 +218,8 @@ int mbx_colblock_create(const int32_t* d_row_ptr, const int32_t* d_col, int32_t 
   if (!d_row_ptr || (nnz > 0 && !d_col)) return MBX_EINVAL;
   if (nnz > 0 && ncol == 0) return MBX_EINVAL;
   int nb = nblocks;
-  if (nb == 0) {  // slices of at most 2.5 MiB of 4-byte x
-    const long long slice = (5ll << 20) / 2 / 4;
+  if (nb == 0) {  // slices of at most 3.5 MiB of 4-byte x (one XCD's L2 is 4 MB)
+    const long long slice = (7ll << 20) / 2 / 4;
     nb = (int)((ncol + slice - 1) / slice);
     nb = nb < 1 ? 1 : (nb > kMaxBlocks ? kMaxBlocks : nb);
   }
@@ -256,6 +257,18 @@ int mbx_colblock_info(const mbx_colblock* cb, int32_t* nblocks, int64_t* value_s
   return MBX_OK;
 }
 
+int mbx_colblock_set_stream(mbx_colblock* cb, int32_t mode) {
+  if (!cb || mode < MBX_STREAM_AUTO || mode > MBX_STREAM_NONTEMPORAL) return MBX_EINVAL;
+  cb->stream_mode = mode;
+  return MBX_OK;
+}
+
+int mbx_colblock_set_residency(mbx_colblock* cb, int32_t wg_per_cu) {
+  if (!cb || wg_per_cu < 0 || wg_per_cu > 16) return MBX_EINVAL;
+  for (int j = 0; j < cb->nb; ++j) mbx_plan_set_residency(cb->plan[j], wg_per_cu);
+  return MBX_OK;
+}
+
 int mbx_colblock_values(const mbx_colblock* cb, int32_t value_kind, const void* d_val,
                         void* d_val_blocked, void* stream) {
   if (!cb) return MBX_EINVAL;
@@ -286,7 +299,16 @@ int mbx_colblock_spmv(const mbx_colblock* cb, int32_t value_kind, const void* d_
   if (cb->nrow > 0 && !d_y) return MBX_EINVAL;
   if (cb->nnz > 0 && (!d_val_blocked || !d_x)) return MBX_EINVAL;
   const int xb = value_kind == MBX_VALUE_F64 ? 8 : 4;
+  // AUTO judges the whole chained product: its stream is read once per
+  // product even when each block's part would stay in the MALL
+  int mode = cb->stream_mode;
+  if (mode == MBX_STREAM_AUTO) {
+    const long long bytes = (long long)cb->nb * 4 * (cb->nrow + 1) + (4ll + vb) * cb->nnz;
+    mode = mbx_detail::stream_nontemporal(MBX_STREAM_AUTO, bytes) ? MBX_STREAM_NONTEMPORAL
+                                                                   : MBX_STREAM_CACHED;
+  }
   for (int j = 0; j < cb->nb; ++j) {
+    cb->plan[j]->stream_mode = mode;
     const int32_t* rp = cb->row_ptr + (long long)j * cb->stride;
     const int32_t* col = cb->col + cb->base[j];
     const char* val = static_cast<const char*>(d_val_blocked) + cb->base[j] * vb;

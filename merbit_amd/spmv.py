@@ -334,6 +334,16 @@ class ColumnBlockedCsr:
               "mbx_colblock_spmv")
         return y
 
+    def set_stream(self, mode: str) -> None:
+        """Stream cache policy of every block's product (see MerbitPlan)."""
+        check(lib().mbx_colblock_set_stream(self.handle, STREAM_MODES[mode]),
+              "mbx_colblock_set_stream")
+
+    def set_residency(self, wg_per_cu: int) -> None:
+        """Tiles per CU of every block's product (0: no cap)."""
+        check(lib().mbx_colblock_set_residency(self.handle, wg_per_cu),
+              "mbx_colblock_set_residency")
+
     def sddmm(self, g: torch.Tensor, x: torch.Tensor, out: torch.Tensor | None = None,
               stream=None) -> torch.Tensor:
         """dval = g[row]*x[col] in block-major order (FP64 or FP32)."""

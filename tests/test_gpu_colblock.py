@@ -108,7 +108,8 @@ def test_colblock_mixed(orc, device, kind):
 
 def test_colblock_matches_csr_product_on_a_graph(device):
     """On a random graph the blocked product agrees with the plain CSR
-    product to FP64 rounding, and repeated calls are bit-identical."""
+    product to FP64 rounding, and repeated calls are bit-identical under
+    every stream policy and residency."""
     from merbit_amd import workloads
     from merbit_amd.spmv import ColumnBlockedCsr, MerbitPlan
     say("colblock graph 200k")
@@ -118,9 +119,14 @@ def test_colblock_matches_csr_product_on_a_graph(device):
     xx = dev(x.astype(np.float64), device)
     plan = MerbitPlan(d_rp, A.ncol, A.nnz)
     y_csr = plan.spmv(d_rp, d_col, val, xx).cpu().numpy()
-    cb = ColumnBlockedCsr(d_rp, d_col, A.ncol)
+    cb = ColumnBlockedCsr(d_rp, d_col, A.ncol, nblocks=3)
     vb = cb.values(val)
     outs = [cb.spmv(vb, xx).cpu().numpy() for _ in range(3)]
+    # the cache policy and tiles per CU change no arithmetic
+    for mode, wg in (("nontemporal", 4), ("cached", 3), ("auto", 0)):
+        cb.set_stream(mode)
+        cb.set_residency(wg)
+        outs.append(cb.spmv(vb, xx).cpu().numpy())
     for o in outs[1:]:
         assert np.array_equal(o, outs[0])
     absr = np.abs(A.val.astype(np.float64))
