@@ -456,16 +456,33 @@ def bench_adjoint(dev, args):
     fwdb_ms, _ = time_steps([lambda: cb.spmv(vb, xx, yb)], steps, args.warmup)
     db = torch.zeros(max(cb.value_slots, 1), dtype=torch.float32, device=dev)
     dvb_ms, _ = time_steps([lambda: cb.sddmm(gg, xx, out=db)], steps, args.warmup)
+    t0 = time.time()
+    # column-blocked A^T (blocks of A's rows); with 3 blocks of A, 16 of A^T
+    # keep the value gather into A^T's order the most local
+    nbt = cb.prepare_adjoint(rp, col, nblocks=16)
+    torch.cuda.synchronize()
+    adj_build_ms = (time.time() - t0) * 1e3
+    cb.set_stream(cb_mode)
+    cb.set_residency(cb_wg)
+    dxb = torch.empty(A.ncol, dtype=torch.float32, device=dev)
+    dxb_ms, _ = time_steps([lambda: cb.adjoint_x(vb, gg, dxb)], steps, args.warmup)
+    dx_ref = op.adjoint_x(val, gg)
     out["forward_column_blocked"] = {
         "us": round(fwdb_ms * 1e3, 2), "gbs": gbs(b_fwd, fwdb_ms),
         "gflops": round(2 * nnz / (fwdb_ms * 1e-3) / 1e9, 1), "blocks": cb.nblocks,
         "stream_loads": cb_mode, "tiles_per_cu": cb_wg,
         "speedup_vs_csr": round(fwd_ms / fwdb_ms, 2),
         "dval_blocked_us": round(dvb_ms * 1e3, 2), "dval_speedup": round(dv_ms / dvb_ms, 2),
+        "dx_blocked_us": round(dxb_ms * 1e3, 2), "dx_speedup": round(dx_ms / dxb_ms, 2),
+        "dx_blocks": nbt, "dx_build_ms": round(adj_build_ms, 1),
+        "dx_rel_diff_vs_csr": rel_err(dxb, dx_ref),
+        "step_us": round((fwdb_ms + dvb_ms + dxb_ms) * 1e3, 2),
+        "step_speedup": round((fwd_ms + dv_ms + dx_ms) / (fwdb_ms + dvb_ms + dxb_ms), 2),
         "values_to_blocked_us": round(vals_ms * 1e3, 2), "build_ms": round(cb_build_ms, 1),
         "rel_diff_vs_csr": rel_err(yb, op.forward(val, xx))}
     log(f"graph forward, {cb.nblocks} column blocks: {fwdb_ms * 1e3:.1f} us "
-        f"(x{fwd_ms / fwdb_ms:.2f} vs CSR), dval {dvb_ms * 1e3:.1f} us (x{dv_ms / dvb_ms:.2f}); "
+        f"(x{fwd_ms / fwdb_ms:.2f} vs CSR), dval {dvb_ms * 1e3:.1f} us (x{dv_ms / dvb_ms:.2f}), "
+        f"dx {dxb_ms * 1e3:.1f} us (x{dx_ms / dxb_ms:.2f}, {nbt} row blocks); "
         f"values to block order {vals_ms * 1e3:.1f} us, "
         f"build {cb_build_ms:.1f} ms)")
     cb.close()

@@ -269,11 +269,21 @@ int mbx_autotune_batch(const mbx_batch_item* items, int32_t count, int32_t value
  *   mbx_colblock_sddmm: dval = g[row]*x[col] in block-major order (FP64 or
  *       FP32), as mbx_sddmm on each block with its x slice; padding slots
  *       are not written.
+ *   mbx_colblock_adjoint_prepare: builds the column-blocked form of A^T
+ *       (blocks over A's rows, nblocks 0..16 as above, reported in
+ *       *nblocks_out when not NULL) from the same CSR pattern the form was
+ *       made from, and the map from its slots to this form's slots.
+ *       Synchronises `stream`.
+ *   mbx_colblock_dx: dx = alpha*A^T*g + beta*dx from values in this form's
+ *       block-major order: one gather into the A^T form's order (into a
+ *       buffer the form owns), then its chained product, whose g slices stay
+ *       in L2.  MBX_EINVAL before mbx_colblock_adjoint_prepare.
  *   mbx_colblock_set_stream / _set_residency: the stream cache policy and
- *       tiles per CU of every block's product, as mbx_plan_set_stream /
- *       mbx_plan_set_residency; MBX_STREAM_AUTO (the default) judges the
- *       bytes of the whole chained product.  Products on one column-blocked
- *       form must not overlap (they share its plans' carry slots). */
+ *       tiles per CU of every block's product (and of the A^T form's), as
+ *       mbx_plan_set_stream / mbx_plan_set_residency; MBX_STREAM_AUTO (the
+ *       default) judges the bytes of the whole chained product.  Calls on one
+ *       column-blocked form must not overlap (they share its plans' carry
+ *       slots and its dx value buffer). */
 typedef struct mbx_colblock mbx_colblock;
 int mbx_colblock_create(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow,
                         int32_t ncol, int32_t nnz, int32_t nblocks, int32_t items_per_thread,
@@ -289,6 +299,11 @@ int mbx_colblock_sddmm(const mbx_colblock* cb, int32_t value_kind, const void* d
                        const void* d_x, void* d_dval_blocked, void* stream);
 int mbx_colblock_set_stream(mbx_colblock* cb, int32_t mode);
 int mbx_colblock_set_residency(mbx_colblock* cb, int32_t wg_per_cu);
+int mbx_colblock_adjoint_prepare(mbx_colblock* cb, const int32_t* d_row_ptr,
+                                 const int32_t* d_col, int32_t nblocks,
+                                 int32_t items_per_thread, void* stream, int32_t* nblocks_out);
+int mbx_colblock_dx(const mbx_colblock* cb, int32_t value_kind, const void* d_val_blocked,
+                    double alpha, const void* d_g, double beta, void* d_dx, void* stream);
 
 /* Adjoint of y = A*x (no reference counterpart).  An adjoint object holds the
  * transposed pattern (CSC of A built on the device by a stable radix sort:

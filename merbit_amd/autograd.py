@@ -109,6 +109,39 @@ class _SpMV(torch.autograd.Function):
         return dval, dx, None
 
 
-def spmv(op: MerbitOperator, val: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
-    """Differentiable y = A x with A's pattern from `op` and values `val`."""
+class ColumnBlockedOperator:
+    """The operator of ``MerbitOperator`` on the column-blocked form of the
+    pattern (``ColumnBlockedCsr``), for random graphs whose x and g outgrow
+    L2.  Its values live in the form's block-major order: convert a CSR-order
+    parameter once with ``to_blocked`` and train that tensor; dval comes back
+    in the same order (zero in the padding slots).  Use with ``spmv``."""
+
+    def __init__(self, row_ptr: torch.Tensor, col: torch.Tensor, ncol: int,
+                 nblocks: int = 0, adjoint_blocks: int = 0, items_per_thread: int = 8):
+        from .spmv import ColumnBlockedCsr
+        self.nrow = row_ptr.numel() - 1
+        self.ncol = int(ncol)
+        self.form = ColumnBlockedCsr(row_ptr, col, ncol, nblocks, items_per_thread)
+        self.form.prepare_adjoint(row_ptr, col, adjoint_blocks, items_per_thread)
+        self.value_slots = self.form.value_slots
+
+    def to_blocked(self, val: torch.Tensor) -> torch.Tensor:
+        return self.form.values(val)
+
+    def forward(self, val: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
+        return self.form.spmv(val, x)
+
+    def adjoint_x(self, val: torch.Tensor, g: torch.Tensor) -> torch.Tensor:
+        return self.form.adjoint_x(val, g)
+
+    def adjoint_val(self, g: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
+        return self.form.sddmm(g, x)
+
+    def close(self) -> None:
+        self.form.close()
+
+
+def spmv(op, val: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
+    """Differentiable y = A x with A's pattern from `op` (a MerbitOperator,
+    values in CSR order, or a ColumnBlockedOperator, values in its order)."""
     return _SpMV.apply(val, x, op)

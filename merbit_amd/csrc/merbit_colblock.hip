@@ -20,6 +20,14 @@
 // values serves many products.  A row's sum is split over the blocks and
 // added in block order: results are deterministic, but rounded differently
 // from the single CSR product (FP64 agrees to a few units in the last place).
+//
+// The adjoint dx = A^T g gathers g by A's rows, so it is blocked the other
+// way: mbx_colblock_adjoint_prepare builds the CSC of A (the adjoint's
+// radix sort), a column-blocked form of A^T over it (blocks of A's rows) and
+// the composed map from that form's slots to this form's slots.  dx is then
+// one gather of the values into the A^T form's order and its chained
+// product.  The gather is local: a block of A^T reads the entries of one
+// range of A's rows, a contiguous part of this form's blocks.
 
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
@@ -42,6 +50,11 @@ struct mbx_colblock {
   int32_t* col = nullptr;             // slots: column minus lo_j, block-major
   int32_t* perm = nullptr;            // slots: CSR position of each slot, -1 for padding
   mbx_plan* plan[kMaxBlocks] = {};
+  // adjoint (mbx_colblock_adjoint_prepare): the column-blocked form of A^T,
+  // the slot of this form each of its slots reads, and a value buffer
+  mbx_colblock* t = nullptr;
+  int32_t* tperm = nullptr;
+  void* tval = nullptr;
 };
 
 namespace {
@@ -114,9 +127,41 @@ __global__ void cb_values_kernel(const int32_t* __restrict__ perm, long long slo
   out[i] = k >= 0 ? val[k] : T(0);
 }
 
+// Inverse of the slot -> CSR position map over the real slots.
+__global__ void cb_inverse_kernel(const int32_t* __restrict__ perm, long long slots,
+                                  int32_t* __restrict__ inv) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= slots) return;
+  const int k = perm[i];
+  if (k >= 0) inv[k] = (int32_t)i;
+}
+
+// Slot q of the blocked A^T reads CSC position tperm_csc[q], which holds CSR
+// entry csc_perm[.], which sits in slot inv[.] of the blocked A.
+__global__ void cb_compose_kernel(const int32_t* __restrict__ tperm_csc, long long slots,
+                                  const int32_t* __restrict__ csc_perm,
+                                  const int32_t* __restrict__ inv, int32_t* __restrict__ out) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= slots) return;
+  const int p = tperm_csc[q];
+  out[q] = p >= 0 ? inv[csc_perm[p]] : -1;
+}
+
+void destroy(mbx_colblock* cb);
+
+void drop_adjoint(mbx_colblock* cb) {
+  if (cb->t) destroy(cb->t);
+  if (cb->tperm) (void)hipFree(cb->tperm);
+  if (cb->tval) (void)hipFree(cb->tval);
+  cb->t = nullptr;
+  cb->tperm = nullptr;
+  cb->tval = nullptr;
+}
+
 void destroy(mbx_colblock* cb) {
   if (!cb) return;
   (void)hipDeviceSynchronize();
+  drop_adjoint(cb);
   for (int j = 0; j < kMaxBlocks; ++j)
     if (cb->plan[j]) mbx_plan_destroy(cb->plan[j]);
   if (cb->row_ptr) (void)hipFree(cb->row_ptr);
@@ -260,13 +305,85 @@ int mbx_colblock_info(const mbx_colblock* cb, int32_t* nblocks, int64_t* value_s
 int mbx_colblock_set_stream(mbx_colblock* cb, int32_t mode) {
   if (!cb || mode < MBX_STREAM_AUTO || mode > MBX_STREAM_NONTEMPORAL) return MBX_EINVAL;
   cb->stream_mode = mode;
+  if (cb->t) cb->t->stream_mode = mode;
   return MBX_OK;
 }
 
 int mbx_colblock_set_residency(mbx_colblock* cb, int32_t wg_per_cu) {
   if (!cb || wg_per_cu < 0 || wg_per_cu > 16) return MBX_EINVAL;
   for (int j = 0; j < cb->nb; ++j) mbx_plan_set_residency(cb->plan[j], wg_per_cu);
+  if (cb->t) mbx_colblock_set_residency(cb->t, wg_per_cu);
   return MBX_OK;
+}
+
+int mbx_colblock_adjoint_prepare(mbx_colblock* cb, const int32_t* d_row_ptr,
+                                 const int32_t* d_col, int32_t nblocks,
+                                 int32_t items_per_thread, void* stream, int32_t* nblocks_out) {
+  if (!cb || nblocks < 0 || nblocks > kMaxBlocks) return MBX_EINVAL;
+  if (!d_row_ptr || (cb->nnz > 0 && !d_col)) return MBX_EINVAL;
+  hipStream_t s = (hipStream_t)stream;
+  (void)hipStreamSynchronize(s);
+  drop_adjoint(cb);
+  mbx_adjoint* adj = nullptr;
+  int st = mbx_adjoint_create(d_row_ptr, d_col, cb->nrow, cb->ncol, cb->nnz, items_per_thread,
+                              stream, &adj);
+  if (st == MBX_OK)
+    st = mbx_colblock_create(adj->col_ptr, adj->row_idx, cb->ncol, cb->nrow, cb->nnz, nblocks,
+                             items_per_thread, stream, &cb->t);
+  int32_t* inv = nullptr;
+  const long long slots = cb->base[cb->nb];
+  const long long tslots = cb->t ? cb->t->base[cb->t->nb] : 0;
+  if (st == MBX_OK &&
+      (hipMalloc((void**)&inv, sizeof(int32_t) * (cb->nnz > 0 ? cb->nnz : 1)) != hipSuccess ||
+       hipMalloc((void**)&cb->tperm, sizeof(int32_t) * (tslots > 0 ? tslots : 1)) != hipSuccess ||
+       hipMalloc(&cb->tval, 8 * (tslots > 0 ? tslots : 1)) != hipSuccess))
+    st = MBX_ENOMEM;
+  // entries no slot claims (a pattern other than the one the form was built
+  // from) read as padding rather than out of bounds
+  if (st == MBX_OK &&
+      hipMemsetAsync(inv, 0xFF, sizeof(int32_t) * (cb->nnz > 0 ? cb->nnz : 1), s) != hipSuccess)
+    st = MBX_EHIP;
+  if (st == MBX_OK && slots > 0)
+    cb_inverse_kernel<<<(int)((slots + 255) / 256), 256, 0, s>>>(cb->perm, slots, inv);
+  if (st == MBX_OK && tslots > 0)
+    cb_compose_kernel<<<(int)((tslots + 255) / 256), 256, 0, s>>>(cb->t->perm, tslots,
+                                                                   adj->perm, inv, cb->tperm);
+  if (st == MBX_OK && hipGetLastError() != hipSuccess) st = MBX_EHIP;
+  if (hipStreamSynchronize(s) != hipSuccess && st == MBX_OK) st = MBX_EHIP;
+  if (inv) (void)hipFree(inv);
+  if (adj) mbx_adjoint_destroy(adj);
+  if (st != MBX_OK) {
+    drop_adjoint(cb);
+    return st;
+  }
+  cb->t->stream_mode = cb->stream_mode;
+  if (cb->nb > 0 && cb->plan[0]) mbx_colblock_set_residency(cb->t, cb->plan[0]->wg_per_cu);
+  if (nblocks_out) *nblocks_out = cb->t->nb;
+  return MBX_OK;
+}
+
+int mbx_colblock_dx(const mbx_colblock* cb, int32_t value_kind, const void* d_val_blocked,
+                    double alpha, const void* d_g, double beta, void* d_dx, void* stream) {
+  if (!cb || !cb->t) return MBX_EINVAL;
+  const int vb = value_bytes(value_kind);
+  if (vb == 0) return MBX_EINVAL;
+  if (cb->nnz > 0 && !d_val_blocked) return MBX_EINVAL;
+  const long long tslots = cb->t->base[cb->t->nb];
+  hipStream_t s = (hipStream_t)stream;
+  if (tslots > 0) {
+    const int grid = (int)((tslots + 255) / 256);
+    if (vb == 8)
+      cb_values_kernel<<<grid, 256, 0, s>>>(cb->tperm, tslots, (const uint64_t*)d_val_blocked,
+                                            (uint64_t*)cb->tval);
+    else if (vb == 4)
+      cb_values_kernel<<<grid, 256, 0, s>>>(cb->tperm, tslots, (const uint32_t*)d_val_blocked,
+                                            (uint32_t*)cb->tval);
+    else
+      cb_values_kernel<<<grid, 256, 0, s>>>(cb->tperm, tslots, (const uint16_t*)d_val_blocked,
+                                            (uint16_t*)cb->tval);
+    if (hipGetLastError() != hipSuccess) return MBX_EHIP;
+  }
+  return mbx_colblock_spmv(cb->t, value_kind, cb->tval, alpha, d_g, beta, d_dx, stream);
 }
 
 int mbx_colblock_values(const mbx_colblock* cb, int32_t value_kind, const void* d_val,

@@ -26,6 +26,22 @@ struct mbx_plan {
   mutable hipEvent_t ev_begin, ev_end;
 };
 
+// Adjoint object (mbx_adjoint_*): the CSC of A and the schedule that moves
+// values into CSC order; also read by the column-blocked form's adjoint.
+struct mbx_adjoint {
+  int32_t nrow, ncol, nnz;
+  int32_t* col_ptr;   // ncol + 1: row pointer of A^T
+  int32_t* row_idx;   // nnz: column indices of A^T
+  int32_t* perm;      // nnz: CSR position of every CSC entry
+  void* vt;           // nnz values in CSC order (8 bytes each)
+  mbx_plan* plan_t;   // merge-path plan of A^T
+  // bucketed permutation schedule (built and described in merbit_adjoint.hip)
+  uint16_t* lslot;    // nnz: LDS slot of CSR entry k within its chunk
+  int32_t* spos;      // nnz: staging index of pass-1 slot i (chunk-major)
+  int32_t* lidx;      // nnz: staging index of CSC position p
+  void* stage;        // nnz values, bucket after bucket (8 bytes each)
+};
+
 namespace mbx_detail {
 
 constexpr int kXcds = 8;

@@ -344,6 +344,43 @@ class ColumnBlockedCsr:
         check(lib().mbx_colblock_set_residency(self.handle, wg_per_cu),
               "mbx_colblock_set_residency")
 
+    def prepare_adjoint(self, row_ptr: torch.Tensor, col: torch.Tensor, nblocks: int = 0,
+                        items_per_thread: int = 8, stream=None) -> int:
+        """Build the column-blocked form of A^T (blocks over A's rows) from the
+        CSR pattern this form was made from; returns its block count."""
+        nb = ctypes.c_int32()
+        check(lib().mbx_colblock_adjoint_prepare(self.handle,
+                                                 _dptr(row_ptr, torch.int32, "row_ptr"),
+                                                 _dptr(col, torch.int32, "col"), nblocks,
+                                                 items_per_thread, _stream_handle(stream),
+                                                 ctypes.byref(nb)),
+              "mbx_colblock_adjoint_prepare")
+        self.adjoint_blocks = nb.value
+        return nb.value
+
+    def adjoint_x(self, val_blocked: torch.Tensor, g: torch.Tensor,
+                  dx: torch.Tensor | None = None, alpha: float = 1.0, beta: float = 0.0,
+                  stream=None) -> torch.Tensor:
+        """dx = alpha*A^T*g + beta*dx from values in block-major order."""
+        kind = _VALUE_KINDS.get(val_blocked.dtype)
+        if kind is None:
+            raise TypeError(f"unsupported value dtype {val_blocked.dtype}")
+        if val_blocked.numel() < self.value_slots:
+            raise ValueError("val_blocked is shorter than the blocked pattern")
+        gdt = torch.float64 if kind == "f64" else torch.float32
+        if g.numel() != self.nrow:
+            raise ValueError("g has the wrong length")
+        if dx is None:
+            dx = torch.empty(self.ncol, dtype=gdt, device=g.device)
+            beta = 0.0
+        if dx.numel() != self.ncol:
+            raise ValueError("dx has the wrong length")
+        check(lib().mbx_colblock_dx(self.handle, _KIND_CODES[kind],
+                                    _dptr(val_blocked, val_blocked.dtype, "val_blocked"),
+                                    alpha, _dptr(g, gdt, "g"), beta, _dptr(dx, gdt, "dx"),
+                                    _stream_handle(stream)), "mbx_colblock_dx")
+        return dx
+
     def sddmm(self, g: torch.Tensor, x: torch.Tensor, out: torch.Tensor | None = None,
               stream=None) -> torch.Tensor:
         """dval = g[row]*x[col] in block-major order (FP64 or FP32)."""
