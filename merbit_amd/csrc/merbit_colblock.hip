@@ -127,6 +127,20 @@ __global__ void cb_values_kernel(const int32_t* __restrict__ perm, long long slo
   out[i] = k >= 0 ? val[k] : T(0);
 }
 
+// The dx gather: slot q of the A^T form reads slot map[q] of this form.  A
+// (row block, column block) pair's entries are contiguous on both sides, so
+// consecutive workgroups go to one XCD and each L2 serves the source window
+// of the destination range its XCD writes.
+template <class T>
+__global__ void cb_gather_xcd_kernel(const int32_t* __restrict__ map, long long slots,
+                                     const T* __restrict__ val, T* __restrict__ out) {
+  const long long i =
+      (long long)mbx_detail::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (i >= slots) return;
+  const int k = map[i];
+  out[i] = k >= 0 ? val[k] : T(0);
+}
+
 // Inverse of the slot -> CSR position map over the real slots.
 __global__ void cb_inverse_kernel(const int32_t* __restrict__ perm, long long slots,
                                   int32_t* __restrict__ inv) {
@@ -373,14 +387,17 @@ int mbx_colblock_dx(const mbx_colblock* cb, int32_t value_kind, const void* d_va
   if (tslots > 0) {
     const int grid = (int)((tslots + 255) / 256);
     if (vb == 8)
-      cb_values_kernel<<<grid, 256, 0, s>>>(cb->tperm, tslots, (const uint64_t*)d_val_blocked,
-                                            (uint64_t*)cb->tval);
+      cb_gather_xcd_kernel<<<grid, 256, 0, s>>>(cb->tperm, tslots,
+                                                (const uint64_t*)d_val_blocked,
+                                                (uint64_t*)cb->tval);
     else if (vb == 4)
-      cb_values_kernel<<<grid, 256, 0, s>>>(cb->tperm, tslots, (const uint32_t*)d_val_blocked,
-                                            (uint32_t*)cb->tval);
+      cb_gather_xcd_kernel<<<grid, 256, 0, s>>>(cb->tperm, tslots,
+                                                (const uint32_t*)d_val_blocked,
+                                                (uint32_t*)cb->tval);
     else
-      cb_values_kernel<<<grid, 256, 0, s>>>(cb->tperm, tslots, (const uint16_t*)d_val_blocked,
-                                            (uint16_t*)cb->tval);
+      cb_gather_xcd_kernel<<<grid, 256, 0, s>>>(cb->tperm, tslots,
+                                                (const uint16_t*)d_val_blocked,
+                                                (uint16_t*)cb->tval);
     if (hipGetLastError() != hipSuccess) return MBX_EHIP;
   }
   return mbx_colblock_spmv(cb->t, value_kind, cb->tval, alpha, d_g, beta, d_dx, stream);
