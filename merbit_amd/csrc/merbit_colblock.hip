@@ -216,13 +216,17 @@ int build(mbx_colblock* cb, const int32_t* rp, const int32_t* col, int ipt, hipS
   if (st == MBX_OK &&
       hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, cnt, scanned, (int)n, s) != hipSuccess)
     st = MBX_EHIP;
-  // block starts in the unpadded order, read back to size and pad the blocks
-  int32_t h_start[kMaxBlocks] = {};
-  for (int j = 0; j < nb && st == MBX_OK; ++j)
-    if (hipMemcpyAsync(&h_start[j], scanned + (long long)j * stride, sizeof(int32_t),
-                       hipMemcpyDeviceToHost, s) != hipSuccess)
+  // block starts in the unpadded order, read back to size and pad the blocks,
+  // and the total, which must be the nnz the caller gave (row_ptr[nrow])
+  int32_t h_start[kMaxBlocks + 1] = {};
+  for (int j = 0; j <= nb && st == MBX_OK; ++j) {
+    const long long at = j < nb ? (long long)j * stride : (long long)(nb - 1) * stride + nrow;
+    if (hipMemcpyAsync(&h_start[j], scanned + at, sizeof(int32_t), hipMemcpyDeviceToHost, s) !=
+        hipSuccess)
       st = MBX_EHIP;
+  }
   if (st == MBX_OK && hipStreamSynchronize(s) != hipSuccess) st = MBX_EHIP;
+  if (st == MBX_OK && h_start[nb] != cb->nnz) st = MBX_EINVAL;
   Bounds start{}, lo{}, base{};
   int32_t nnz_b[kMaxBlocks] = {};
   if (st == MBX_OK) {

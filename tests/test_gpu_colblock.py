@@ -228,10 +228,32 @@ def test_colblock_autograd_matches_csr_operator(device, dtype):
     tol = 1e-12 if dtype == torch.float64 else 1e-6
 
     def rel(a, b):
-        return float((a.double() - b.double()).norm() / b.double().norm())
+        a, b = a.detach().double(), b.detach().double()
+        return float((a - b).norm() / b.norm())
     assert rel(y2, y1) <= tol
     assert rel(x2.grad, x1.grad) <= tol
     # the blocked dval is the CSR dval moved into block order, bit for bit
     assert torch.equal(vb.grad[:blk.value_slots], blk.to_blocked(v1.grad)[:blk.value_slots])
     blk.close()
     csr.close()
+
+
+def test_colblock_rejects_nnz_other_than_row_ptr(device):
+    """nnz must equal row_ptr[nrow]: the block counts are checked against it
+    (the arrays themselves are whole, so every read stays in bounds)."""
+    import ctypes
+    from merbit_amd._lib import lib
+    say("colblock nnz mismatch")
+    rng = np.random.default_rng(5)
+    rp, col = random_csr(rng, 500, 400, density=6)
+    d_rp, d_col = dev(rp, device), dev(col, device)
+    h = ctypes.c_void_p()
+    nnz = int(rp[-1])
+    for bad in (nnz - 1, nnz + 5):
+        st = lib().mbx_colblock_create(d_rp.data_ptr(), d_col.data_ptr(), 500, 400, bad, 3, 8,
+                                       None, ctypes.byref(h))
+        assert st == -1 and not h.value
+    st = lib().mbx_colblock_create(d_rp.data_ptr(), d_col.data_ptr(), 500, 400, nnz, 3, 8, None,
+                                   ctypes.byref(h))
+    assert st == 0 and h.value
+    lib().mbx_colblock_destroy(h)
