@@ -213,6 +213,17 @@ def bench_mixed(name, A, x, dev, args, f64_ms=None, copies=1):
     v32 = np.ascontiguousarray(A.val, dtype=np.float32)
     x32 = np.ascontiguousarray(x, dtype=np.float32)
     out = {}
+
+    def compressed(plan, calls, ref_ms):
+        """The same products with 16-bit column offsets (banded only: its
+        tiles span a few hundred columns); None when the plan refuses."""
+        if name != "banded" or not plan.compress_cols(cols[0]):
+            return None
+        ms_c, kern_c = time_steps(calls, args.steps, args.warmup, plan.set_events)
+        plan.compress_cols(None)
+        return {"us_per_step": round(ms_c * 1e3, 2), "tile_kernel_us": round(kern_c * 1e3, 2),
+                "speedup_vs_f64": round(ref_ms / ms_c, 3), "column_bytes_per_nnz": 2}
+
     if f64_ms is None:  # FP64 values = the FP32 values widened
         vals = [torch.from_numpy(v32.astype(np.float64)).to(dev) for _ in range(copies)]
         xx = torch.from_numpy(x32.astype(np.float64)).to(dev)
@@ -231,6 +242,11 @@ def bench_mixed(name, A, x, dev, args, f64_ms=None, copies=1):
                       "tile_kernel_us": round(kern * 1e3, 2), "items_per_thread": ipt,
                       "stream": stream, "tiles_per_cu": resid,
                       "pct_hbm_peak": round(100 * b64 / (f64_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 1)}
+        c = compressed(plan, calls, f64_ms)
+        if c:
+            out["f64"]["compressed_cols"] = c
+            log(f"{name} f64, 16-bit columns: {c['us_per_step']:.1f} us/step "
+                f"(x{c['speedup_vs_f64']:.2f})")
         plan.close()
         del vals, y
     for kind in ("f32", "bf16"):
@@ -262,6 +278,26 @@ def bench_mixed(name, A, x, dev, args, f64_ms=None, copies=1):
             f"x{f64_ms / ms:.2f} "
             f"vs fp64, "
             f"err {out[kind]['rel_err_vs_f64_same_values']:.2e}")
+        c = compressed(plan, calls, f64_ms)
+        if c:
+            out[kind]["compressed_cols"] = c
+            log(f"{name} {kind}, 16-bit columns: {c['us_per_step']:.1f} us/step "
+                f"(x{c['speedup_vs_f64']:.2f} vs plain fp64)")
+        # the same values with exact FP64 products and FP64 sums
+        plan.set_accumulation("f64")
+        ms_a, kern_a = time_steps(calls, args.steps, args.warmup, plan.set_events)
+        plan.spmv(rp, cols[0], vals[0], xx, y)
+        out[kind]["acc_f64"] = {"us_per_step": round(ms_a * 1e3, 2),
+                                "tile_kernel_us": round(kern_a * 1e3, 2),
+                                "gbs": gbs(b, ms_a), "speedup_vs_f64": round(f64_ms / ms_a, 3),
+                                "rel_err_vs_f64_same_values": rel_err(y, ref)}
+        log(f"{name} {kind}, FP64 sums: {ms_a * 1e3:.1f} us/step (x{f64_ms / ms_a:.2f} vs fp64, "
+            f"err {out[kind]['acc_f64']['rel_err_vs_f64_same_values']:.2e})")
+        c = compressed(plan, calls, f64_ms)
+        if c:
+            out[kind]["acc_f64"]["compressed_cols"] = c
+            log(f"{name} {kind}, FP64 sums, 16-bit columns: {c['us_per_step']:.1f} us/step "
+                f"(x{c['speedup_vs_f64']:.2f} vs plain fp64)")
         plan.close()
         del vals, y, ref
     return out

@@ -93,6 +93,31 @@ int mbx_plan_set_stream(mbx_plan* plan, int32_t mode);
  * (a timing knob for the autotuner; results are identical). */
 int mbx_plan_set_residency(mbx_plan* plan, int32_t wg_per_cu);
 
+/* Accumulation of the FP32 / bfloat16 value forms (no reference
+ * counterpart).  MBX_ACC_COMPENSATED_F32 (the default): TwoProduct + TwoSum
+ * compensated FP32 sums per thread, FP64 partials across threads and tiles.
+ * MBX_ACC_F64: each product formed exactly in FP64 (FP32 x FP32 fits its
+ * significand) and the rows summed in FP64; x and y stay FP32 and y is
+ * rounded once.  With packed FP32 off an FP64 add issues at the FP32 rate,
+ * so MBX_ACC_F64 spends one VALU add per nonzero where TwoSum spends about
+ * six; the compensated form is the VALU-bound one on HBM-bound matrices. */
+enum { MBX_ACC_COMPENSATED_F32 = 0, MBX_ACC_F64 = 1 };
+int mbx_plan_set_accumulation(mbx_plan* plan, int32_t mode);
+
+/* Compressed column indices (no reference counterpart).  Stores every
+ * nonzero's column as a 16-bit offset from its tile's smallest column, so
+ * products on this plan read 2 bytes of column index per nonzero instead of
+ * 4 (banded and other local patterns: a tile of 2048 merge items spans a few
+ * hundred columns).  Applies to plans with items_per_thread 8 or 12 and to
+ * products on the vector staging path (16-byte aligned row_ptr, col and
+ * values); other products read d_col as before, so d_col must still be the
+ * pattern's column array.  Results are bit-identical either way.
+ * MBX_EUNSUPPORTED (nothing kept) when a tile spans more than 65536 columns
+ * or the plan's items_per_thread is not 8 or 12; d_col NULL drops the
+ * compressed columns.  Plan clones read the caller's columns.  Synchronises
+ * `stream`. */
+int mbx_plan_compress_cols(mbx_plan* plan, const int32_t* d_col, void* stream);
+
 /* Optional timing hook: when ev_begin/ev_end (hipEvent_t) are non-NULL, the
  * next spmv call on this plan records them immediately before and after its
  * tile kernel on the call's stream, then clears them. */

@@ -16,7 +16,8 @@ LIB_PATH = os.path.join(_PKG, "libmerbit_amd.so")
 ABI_SYMBOLS = (
     "mbx_plan_create", "mbx_plan_destroy", "mbx_plan_clone", "mbx_plan_info",
     "mbx_plan_partition", "mbx_plan_set_stream", "mbx_batch_set_stream",
-    "mbx_plan_set_residency", "mbx_batch_set_residency",
+    "mbx_plan_set_residency", "mbx_batch_set_residency", "mbx_plan_compress_cols",
+    "mbx_plan_set_accumulation",
     "mbx_plan_set_events", "mbx_spmv_f64", "mbx_spmv_f32", "mbx_spmv_bf16",
     "mbx_spmm_workspace_bytes", "mbx_spmm_f32",
     "mbx_batch_create", "mbx_batch_destroy", "mbx_batch_info", "mbx_batch_spmv",
@@ -126,6 +127,8 @@ def lib() -> ctypes.CDLL:
                                     ctypes.POINTER(f32)]
     L.mbx_autotune_batch.argtypes = [vp, i32, i32, i32, vp, ctypes.POINTER(i32),
                                      ctypes.POINTER(f32)]
+    L.mbx_plan_compress_cols.argtypes = [vp, vp, vp]
+    L.mbx_plan_set_accumulation.argtypes = [vp, i32]
     L.mbx_colblock_create.argtypes = [vp, vp, i32, i32, i32, i32, i32, vp, ctypes.POINTER(vp)]
     L.mbx_colblock_destroy.argtypes = [vp]
     L.mbx_colblock_destroy.restype = None

@@ -24,6 +24,9 @@ struct mbx_plan {
   int32_t stream_mode;           // MBX_STREAM_*
   int32_t wg_per_cu;             // resident tiles per CU; 0 = as many as fit
   mutable hipEvent_t ev_begin, ev_end;
+  // mbx_plan_compress_cols: column index minus its tile's base (d_coord[t].w)
+  uint16_t* d_col16;
+  int32_t accumulation;  // MBX_ACC_*: sums of the FP32 / bfloat16 value forms
 };
 
 // Adjoint object (mbx_adjoint_*): the CSC of A and the schedule that moves

@@ -62,6 +62,49 @@ __global__ void partition_kernel(const int32_t* __restrict__ row_ptr, int nrow, 
   coord[t] = mbx_detail::tile_start_coord(row_ptr, nrow, nnz, tile_items, t);
 }
 
+// Column span of one tile (one workgroup per tile): base = smallest column,
+// written to coord[t].w; *wide is set when the span needs more than 16 bits.
+__global__ __launch_bounds__(256) void col_span_kernel(const int32_t* __restrict__ col,
+                                                       int4* __restrict__ coord,
+                                                       int* __restrict__ wide) {
+  __shared__ int s_lo[4], s_hi[4];
+  const int t = blockIdx.x;
+  const int k0 = coord[t].y, k1 = coord[t + 1].y;
+  int lo = INT_MAX, hi = INT_MIN;
+  for (int k = k0 + (int)threadIdx.x; k < k1; k += 256) {
+    const int c = col[k];
+    lo = min(lo, c);
+    hi = max(hi, c);
+  }
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    lo = min(lo, __shfl_xor(lo, d));
+    hi = max(hi, __shfl_xor(hi, d));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    s_lo[threadIdx.x >> 6] = lo;
+    s_hi[threadIdx.x >> 6] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    lo = min(min(s_lo[0], s_lo[1]), min(s_lo[2], s_lo[3]));
+    hi = max(max(s_hi[0], s_hi[1]), max(s_hi[2], s_hi[3]));
+    const bool any = k1 > k0;
+    coord[t].w = any ? lo : 0;
+    if (any && (long long)hi - lo > 0xFFFF) atomicOr(wide, 1);
+  }
+}
+
+// 16-bit offsets of every nonzero of a tile from the tile's base column.
+__global__ __launch_bounds__(256) void col16_kernel(const int32_t* __restrict__ col,
+                                                    const int4* __restrict__ coord,
+                                                    uint16_t* __restrict__ col16) {
+  const int t = blockIdx.x;
+  const int k0 = coord[t].y, k1 = coord[t + 1].y, base = coord[t].w;
+  for (int k = k0 + (int)threadIdx.x; k < k1; k += 256)
+    col16[k] = (uint16_t)(col[k] - base);
+}
+
 template <int WG, int IPT, bool VEC, class P>
 __global__ __launch_bounds__(WG) void tile_spmv_kernel(
     const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
@@ -116,7 +159,29 @@ hipError_t launch_tile(const mbx_plan* p, hipStream_t s, const int32_t* rp,
   const bool nt = mbx_detail::stream_nontemporal(p->stream_mode, footprint);
   if (p->ev_begin) (void)hipEventRecord(p->ev_begin, s);
   const size_t lds = mbx_detail::residency_lds(TileSmem<256, IPT, P>::bytes(), p->wg_per_cu);
-  if (vec && nt) {
+  // the plan's 16-bit columns, instantiated for the tile shapes the banded
+  // and graph products use
+  constexpr bool kC16 = IPT == 8 || IPT == 12;
+  bool launched = false;
+  if constexpr (kC16) {
+    if (vec && p->d_col16) {
+      const int32_t* c16 = reinterpret_cast<const int32_t*>(p->d_col16);
+      if (nt) {
+        using Q = mbx_detail::Col16<mbx_detail::Streamed<P>>;
+        mbx_detail::allow_lds(tile_spmv_kernel<256, IPT, true, Q>, lds);
+        tile_spmv_kernel<256, IPT, true, Q><<<p->ntiles, 256, lds, s>>>(
+            rp, c16, val, x, y, p->d_coord, p->d_carry, p->ntiles, p->ncol, alpha, beta, bz);
+      } else {
+        using Q = mbx_detail::Col16<P>;
+        mbx_detail::allow_lds(tile_spmv_kernel<256, IPT, true, Q>, lds);
+        tile_spmv_kernel<256, IPT, true, Q><<<p->ntiles, 256, lds, s>>>(
+            rp, c16, val, x, y, p->d_coord, p->d_carry, p->ntiles, p->ncol, alpha, beta, bz);
+      }
+      launched = true;
+    }
+  }
+  if (launched) {
+  } else if (vec && nt) {
     mbx_detail::allow_lds(tile_spmv_kernel<256, IPT, true, mbx_detail::Streamed<P>>, lds);
     tile_spmv_kernel<256, IPT, true, mbx_detail::Streamed<P>><<<p->ntiles, 256, lds, s>>>(
         rp, col, val, x, y, p->d_coord, p->d_carry, p->ntiles, p->ncol, alpha, beta, bz);
@@ -243,6 +308,7 @@ void mbx_plan_destroy(mbx_plan* p) {
     delete p->coord_refs;
   }
   if (p->d_carry) (void)hipFree(p->d_carry);
+  if (p->d_col16) (void)hipFree(p->d_col16);
   delete p;
 }
 
@@ -251,6 +317,7 @@ int mbx_plan_clone(const mbx_plan* p, mbx_plan** out) {
   *out = nullptr;
   mbx_plan* c = new (std::nothrow) mbx_plan(*p);
   if (!c) return MBX_ENOMEM;
+  c->d_col16 = nullptr;  // the clone reads the caller's columns
   c->ev_begin = c->ev_end = nullptr;
   c->d_carry = nullptr;
   if (hipMalloc((void**)&c->d_carry, sizeof(double) * 2 * ((size_t)p->ntiles + 1)) !=
@@ -302,6 +369,53 @@ int mbx_plan_set_residency(mbx_plan* p, int32_t wg_per_cu) {
   return MBX_OK;
 }
 
+int mbx_plan_set_accumulation(mbx_plan* p, int32_t mode) {
+  if (!p || (mode != MBX_ACC_COMPENSATED_F32 && mode != MBX_ACC_F64)) return MBX_EINVAL;
+  p->accumulation = mode;
+  return MBX_OK;
+}
+
+int mbx_plan_compress_cols(mbx_plan* p, const int32_t* d_col, void* stream) {
+  if (!p) return MBX_EINVAL;
+  hipStream_t s = (hipStream_t)stream;
+  if (p->d_col16) {
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(p->d_col16);
+    p->d_col16 = nullptr;
+  }
+  if (!d_col) return MBX_OK;  // back to the caller's 32-bit columns
+  if (p->ipt != 8 && p->ipt != 12) return MBX_EUNSUPPORTED;
+  if (p->ntiles == 0 || p->nnz == 0) return MBX_OK;
+  int* d_wide = nullptr;
+  uint16_t* c16 = nullptr;
+  const size_t n16 = ((size_t)p->nnz + 7) / 8 * 8;  // whole 8-byte groups
+  if (hipMalloc((void**)&d_wide, sizeof(int)) != hipSuccess) return MBX_ENOMEM;
+  if (hipMalloc((void**)&c16, sizeof(uint16_t) * n16) != hipSuccess) {
+    (void)hipFree(d_wide);
+    return MBX_ENOMEM;
+  }
+  int st = MBX_OK, wide = 0;
+  if (hipMemsetAsync(d_wide, 0, sizeof(int), s) != hipSuccess ||
+      hipMemsetAsync(c16, 0, sizeof(uint16_t) * n16, s) != hipSuccess)
+    st = MBX_EHIP;
+  if (st == MBX_OK) {
+    col_span_kernel<<<p->ntiles, 256, 0, s>>>(d_col, p->d_coord, d_wide);
+    col16_kernel<<<p->ntiles, 256, 0, s>>>(d_col, p->d_coord, c16);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(&wide, d_wide, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      st = MBX_EHIP;
+  }
+  (void)hipFree(d_wide);
+  if (st == MBX_OK && wide) st = MBX_EUNSUPPORTED;
+  if (st != MBX_OK) {
+    (void)hipFree(c16);
+    return st;
+  }
+  p->d_col16 = c16;
+  return MBX_OK;
+}
+
 int mbx_plan_set_events(mbx_plan* p, void* ev_begin, void* ev_end) {
   if (!p) return MBX_EINVAL;
   p->ev_begin = (hipEvent_t)ev_begin;
@@ -318,12 +432,16 @@ int mbx_spmv_f64(const mbx_plan* p, const int32_t* rp, const int32_t* col,
 int mbx_spmv_f32(const mbx_plan* p, const int32_t* rp, const int32_t* col,
                  const float* val, float alpha, const float* x, float beta, float* y,
                  void* stream) {
+  if (p && p->accumulation == MBX_ACC_F64)
+    return run_spmv<mbx_detail::F32A>(p, rp, col, val, alpha, x, beta, y, stream);
   return run_spmv<F32C>(p, rp, col, val, alpha, x, beta, y, stream);
 }
 
 int mbx_spmv_bf16(const mbx_plan* p, const int32_t* rp, const int32_t* col,
                   const uint16_t* val, float alpha, const float* x, float beta, float* y,
                   void* stream) {
+  if (p && p->accumulation == MBX_ACC_F64)
+    return run_spmv<mbx_detail::BF16A>(p, rp, col, val, alpha, x, beta, y, stream);
   return run_spmv<BF16C>(p, rp, col, val, alpha, x, beta, y, stream);
 }
 

@@ -30,6 +30,7 @@ struct F64 {
   static constexpr bool kValueNT = false;
   static constexpr bool kStreamedIndexNT = true;
   static constexpr bool kStreamedValueNT = false;
+  static constexpr bool kCol16 = false;
   using val_t = double;
   using x_t = double;
   using y_t = double;
@@ -69,6 +70,7 @@ struct CompF32 {
   static constexpr bool kValueNT = false;
   static constexpr bool kStreamedIndexNT = true;
   static constexpr bool kStreamedValueNT = true;
+  static constexpr bool kCol16 = false;
   using val_t = V;
   using x_t = float;
   using y_t = float;
@@ -121,6 +123,48 @@ struct CompF32 {
 using F32C = CompF32<float>;
 using BF16C = CompF32<uint16_t>;
 
+// FP32 or bfloat16 values and FP32 x, y with FP64 products and sums: the
+// product of two FP32 values (24-bit significands) is exact in FP64, so each
+// nonzero costs one FP64 add in the walk instead of a TwoSum.  With packed
+// FP32 off, an FP64 add issues at the FP32 rate, so this form needs far
+// fewer VALU cycles than CompF32 for the same bytes; its rows carry FP64
+// rounding until the final rounding of y to FP32.
+template <class V>
+struct Acc64 {
+  static constexpr bool kIndexNT = false;
+  static constexpr bool kValueNT = false;
+  static constexpr bool kStreamedIndexNT = true;
+  static constexpr bool kStreamedValueNT = true;
+  static constexpr bool kCol16 = false;
+  using val_t = V;
+  using x_t = float;
+  using y_t = float;
+  using scal_t = float;
+  using lds_t = double;
+  using acc_t = double;
+  __device__ static lds_t product(val_t v, x_t x) {
+    return (double)CompF32<V>::widen(v) * (double)x;
+  }
+  __device__ static acc_t zero() { return 0.0; }
+  __device__ static void add(acc_t& a, lds_t p) { a += p; }
+  __device__ static acc_t combine(acc_t a, acc_t b) { return a + b; }
+  __device__ static acc_t shfl_up(acc_t a, int d) { return __shfl_up(a, d); }
+  __device__ static acc_t shfl_xor(acc_t a, int d) { return __shfl_xor(a, d); }
+  __device__ static lds_t to_lds(acc_t a) { return a; }
+  __device__ static double to_f64(lds_t a) { return a; }
+  __device__ static lds_t finish(double c, acc_t h) { return c + h; }
+  __device__ static void store(y_t* y, scal_t alpha, lds_t v, scal_t beta, bool beta_zero) {
+    store_f64(y, alpha, v, beta, beta_zero);
+  }
+  __device__ static void store_f64(y_t* y, scal_t alpha, double v, scal_t beta, bool beta_zero) {
+    *y = beta_zero ? (float)((double)alpha * v)
+                   : (float)((double)alpha * v + (double)beta * (double)*y);
+  }
+};
+
+using F32A = Acc64<float>;
+using BF16A = Acc64<uint16_t>;
+
 // P in its streamed mode: the stream loads P marks are non-temporal, so that
 // data read once per product does not displace the x lines the gathers
 // reuse.  For products whose data does not stay cache-resident from one
@@ -130,6 +174,14 @@ template <class P>
 struct Streamed : P {
   static constexpr bool kIndexNT = P::kStreamedIndexNT;
   static constexpr bool kValueNT = P::kStreamedValueNT;
+};
+
+// P reading 16-bit column offsets from its tile's base column (the plan's
+// compressed columns, mbx_plan_compress_cols) instead of 32-bit columns:
+// 2 bytes less per nonzero in the stream.  Vector staging path only.
+template <class P>
+struct Col16 : P {
+  static constexpr bool kCol16 = true;
 };
 
 // out[0] = most rows ending in one tile, out[1] = tiles whose first row
@@ -189,6 +241,15 @@ template <bool NT = false>
 __device__ __forceinline__ void load_group(const float* p, int g, float (&o)[4]) {
   const f32x4_t a = block_load<NT>(reinterpret_cast<const f32x4_t*>(p) + g);
   o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+}
+// Four 16-bit column offsets (an 8-byte aligned group) plus the tile's base.
+template <bool NT = false>
+__device__ __forceinline__ void load_col16(const uint16_t* p, int g, int base, int (&o)[4]) {
+  const u32x2_t a = block_load<NT>(reinterpret_cast<const u32x2_t*>(p) + g);
+  o[0] = base + (int)(a.x & 0xFFFFu);
+  o[1] = base + (int)(a.x >> 16);
+  o[2] = base + (int)(a.y & 0xFFFFu);
+  o[3] = base + (int)(a.y >> 16);
 }
 // bfloat16 groups stay packed (two values per 32-bit word) until their use.
 struct Bf16x4 {
@@ -467,7 +528,10 @@ __device__ __forceinline__ void tile_issue_loads(TileRegs<WG, IPT, P>& R,
     for (int j = 0; j < NG; ++j) {
       if (j == 0 || j * WG < nkg) {
         const int g = min(kb + tid + j * WG, klast);
-        load_group<P::kIndexNT>(col, g, R.cg[j]);
+        if constexpr (P::kCol16)
+          load_col16<P::kIndexNT>(reinterpret_cast<const uint16_t*>(col), g, R.c0.w, R.cg[j]);
+        else
+          load_group<P::kIndexNT>(col, g, R.cg[j]);
         load_group_clamped<P::kValueNT>(val, g, R.vg[j], k0, k1);
       }
     }
@@ -698,6 +762,7 @@ __device__ __forceinline__ void spmv_tile(
     const int4* __restrict__ coord, double* __restrict__ carry, typename P::scal_t alpha,
     typename P::scal_t beta, int beta_zero) {
   static_assert(WG % 64 == 0 && WG / 64 <= 4, "scan scratch holds up to 4 waves");
+  static_assert(VEC || !P::kCol16, "16-bit columns are read by the vector staging only");
   typename P::lds_t* s_p = sm.v;
   int4 c0, c1;
   // Stage: products to LDS and the row-end marks, every global load of the

@@ -98,6 +98,25 @@ class MerbitPlan:
         check(lib().mbx_plan_set_residency(self.handle, int(wg_per_cu)),
               "mbx_plan_set_residency")
 
+    def set_accumulation(self, mode: str) -> None:
+        """Sums of the FP32 / bfloat16 value forms: "compensated" (TwoSum
+        compensated FP32, the default) or "f64" (exact FP64 products, FP64
+        sums)."""
+        check(lib().mbx_plan_set_accumulation(self.handle,
+                                              {"compensated": 0, "f64": 1}[mode]),
+              "mbx_plan_set_accumulation")
+
+    def compress_cols(self, col: torch.Tensor | None, stream=None) -> bool:
+        """Keep the columns as 16-bit offsets from each tile's base
+        (mbx_plan_compress_cols); False when a tile spans too many columns
+        or the plan's items per thread is not 8 or 12.  None drops them."""
+        ptr = None if col is None else _dptr(col, torch.int32, "col")
+        st = lib().mbx_plan_compress_cols(self.handle, ptr, _stream_handle(stream))
+        if st == -3:  # MBX_EUNSUPPORTED
+            return False
+        check(st, "mbx_plan_compress_cols")
+        return col is not None
+
     def set_events(self, begin, end) -> None:
         """Record torch.cuda.Event pair around the next call's tile kernel."""
         check(lib().mbx_plan_set_events(self.handle, begin.cuda_event, end.cuda_event),
