@@ -598,6 +598,37 @@ def cpu_baseline(A, x, b_alg):
             "one_thread": {"value": round(b_alg / t1 / 1e9, 3), "ms_per_spmv": round(t1 * 1e3, 3)}}
 
 
+def north_star_summary(extras):
+    """The ratios the targets are stated in, gathered from the records above:
+    mixed and batched forms against the plain FP64 product of the same
+    matrix, and the differentiable step on the column-blocked forms."""
+    ns = {}
+    mb = extras.get("mixed_banded")
+    if mb:
+        m = {"bf16": mb["bf16"]["speedup_vs_f64"], "f32": mb["f32"]["speedup_vs_f64"]}
+        for kind in ("f32", "bf16"):
+            acc = mb[kind].get("acc_f64", {})
+            if "compressed_cols" in acc:
+                m[f"{kind}_fp64_sums_16bit_cols"] = acc["compressed_cols"]["speedup_vs_f64"]
+        m["rel_err"] = max(mb[k]["rel_err_vs_f64_same_values"] for k in ("f32", "bf16"))
+        ns["mixed_vs_fp64_banded"] = m
+    mr = extras.get("mixed_rmat")
+    if mr:
+        ns["mixed_vs_fp64_rmat"] = {k: mr[k]["speedup_vs_f64"] for k in ("f32", "bf16")}
+    sp = extras.get("spmm_rmat")
+    f64_us = extras.get("tile_kernel_us")
+    if sp and f64_us:
+        ns["spmm_per_rhs_vs_fp64_spmv"] = {k: round(f64_us / v["us_per_rhs"], 2)
+                                           for k, v in sp.items()}
+    bt = extras.get("batch_uneven")
+    if bt:
+        ns["batch_vs_separate_launches"] = bt["speedup_vs_separate_launches"]
+    ad = extras.get("adjoint_graph", {}).get("forward_column_blocked")
+    if ad:
+        ns["graph_forward_backward_colblocked_vs_csr"] = ad["step_speedup"]
+    return ns
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -677,6 +708,7 @@ def main():
         extras["rocsparse_rmat_f64"] = bench_rocsparse(A, x, dev, args, ms)
     if args.only == "all" or args.copy_probe:
         extras["copy_ceiling"] = bench_copy(dev)
+    extras["north_star"] = north_star_summary(extras)
 
     out = {
         "metric": METRIC,
