@@ -115,7 +115,7 @@ int mbx_plan_set_accumulation(mbx_plan* plan, int32_t mode);
  * MBX_EUNSUPPORTED (nothing kept) when a tile spans more than 65536 columns
  * or the plan's items_per_thread is not 8 or 12; d_col NULL drops the
  * compressed columns.  Plan clones read the caller's columns.  Synchronises
- * `stream`. */
+ * `stream` (and the device, when it replaces compressed columns). */
 int mbx_plan_compress_cols(mbx_plan* plan, const int32_t* d_col, void* stream);
 
 /* Optional timing hook: when ev_begin/ev_end (hipEvent_t) are non-NULL, the

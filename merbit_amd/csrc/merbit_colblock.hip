@@ -340,7 +340,7 @@ int mbx_colblock_adjoint_prepare(mbx_colblock* cb, const int32_t* d_row_ptr,
   if (!cb || nblocks < 0 || nblocks > kMaxBlocks) return MBX_EINVAL;
   if (!d_row_ptr || (cb->nnz > 0 && !d_col)) return MBX_EINVAL;
   hipStream_t s = (hipStream_t)stream;
-  (void)hipStreamSynchronize(s);
+  if (cb->t) (void)hipDeviceSynchronize();  // a dx on any stream may still read them
   drop_adjoint(cb);
   mbx_adjoint* adj = nullptr;
   int st = mbx_adjoint_create(d_row_ptr, d_col, cb->nrow, cb->ncol, cb->nnz, items_per_thread,

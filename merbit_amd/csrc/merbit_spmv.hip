@@ -378,8 +378,8 @@ int mbx_plan_set_accumulation(mbx_plan* p, int32_t mode) {
 int mbx_plan_compress_cols(mbx_plan* p, const int32_t* d_col, void* stream) {
   if (!p) return MBX_EINVAL;
   hipStream_t s = (hipStream_t)stream;
-  if (p->d_col16) {
-    (void)hipStreamSynchronize(s);
+  if (p->d_col16) {  // products on any stream may still read them
+    (void)hipDeviceSynchronize();
     (void)hipFree(p->d_col16);
     p->d_col16 = nullptr;
   }
