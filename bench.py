@@ -493,9 +493,8 @@ def bench_adjoint(dev, args):
     db = torch.zeros(max(cb.value_slots, 1), dtype=torch.float32, device=dev)
     dvb_ms, _ = time_steps([lambda: cb.sddmm(gg, xx, out=db)], steps, args.warmup)
     t0 = time.time()
-    # column-blocked A^T (blocks of A's rows); with 3 blocks of A, 16 of A^T
-    # keep the value gather into A^T's order the most local
-    nbt = cb.prepare_adjoint(rp, col, nblocks=16)
+    # column-blocked A^T (blocks of A's rows, g slices of at most 3.5 MiB)
+    nbt = cb.prepare_adjoint(rp, col)
     torch.cuda.synchronize()
     adj_build_ms = (time.time() - t0) * 1e3
     cb.set_stream(cb_mode)

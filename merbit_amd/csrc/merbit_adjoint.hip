@@ -368,6 +368,22 @@ void permute_values(const mbx_adjoint* a, const T* val, hipStream_t s) {
 
 }  // namespace
 
+namespace mbx_detail {
+
+int perm_schedule_build(mbx_adjoint* a, hipStream_t s) { return build_perm_schedule(a, s); }
+
+void perm_apply(const mbx_adjoint* a, int value_bytes, const void* val, hipStream_t s) {
+  if (a->nnz == 0) return;
+  if (value_bytes == 8)
+    permute_values<double>(a, (const double*)val, s);
+  else if (value_bytes == 4)
+    permute_values<float>(a, (const float*)val, s);
+  else
+    permute_values<uint16_t>(a, (const uint16_t*)val, s);
+}
+
+}  // namespace mbx_detail
+
 extern "C" {
 
 int mbx_adjoint_create(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow,

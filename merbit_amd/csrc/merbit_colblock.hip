@@ -24,16 +24,18 @@
 // The adjoint dx = A^T g gathers g by A's rows, so it is blocked the other
 // way: mbx_colblock_adjoint_prepare builds the CSC of A (the adjoint's
 // radix sort), a column-blocked form of A^T over it (blocks of A's rows) and
-// the composed map from that form's slots to this form's slots.  dx is then
-// one gather of the values into the A^T form's order and its chained
-// product.  The gather is local: a block of A^T reads the entries of one
-// range of A's rows, a contiguous part of this form's blocks.
+// the adjoint's two-pass bucketed permutation from this form's slots to CSC
+// order (a bijection once the padding slots are given destinations past
+// nnz).  dx moves the values into CSC order with it, gathers them into the
+// A^T form's order (each of its blocks reads CSC positions in increasing
+// order, so that gather streams) and runs the A^T form's chained product.
 
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 #include <stdint.h>
 
 #include <new>
+#include <vector>
 
 #include "merbit_amd.h"
 #include "merbit_internal.hpp"
@@ -51,10 +53,12 @@ struct mbx_colblock {
   int32_t* perm = nullptr;            // slots: CSR position of each slot, -1 for padding
   mbx_plan* plan[kMaxBlocks] = {};
   // adjoint (mbx_colblock_adjoint_prepare): the column-blocked form of A^T,
-  // the slot of this form each of its slots reads, and a value buffer
+  // the permutation that moves this form's values into CSC order, and the
+  // buffer of values in the A^T form's order
   mbx_colblock* t = nullptr;
-  int32_t* tperm = nullptr;
+  mbx_adjoint* tp = nullptr;  // bucketed permutation: this form's slots -> CSC order
   void* tval = nullptr;
+  int32_t nnz_b[kMaxBlocks] = {};
 };
 
 namespace {
@@ -127,20 +131,6 @@ __global__ void cb_values_kernel(const int32_t* __restrict__ perm, long long slo
   out[i] = k >= 0 ? val[k] : T(0);
 }
 
-// The dx gather: slot q of the A^T form reads slot map[q] of this form.  A
-// (row block, column block) pair's entries are contiguous on both sides, so
-// consecutive workgroups go to one XCD and each L2 serves the source window
-// of the destination range its XCD writes.
-template <class T>
-__global__ void cb_gather_xcd_kernel(const int32_t* __restrict__ map, long long slots,
-                                     const T* __restrict__ val, T* __restrict__ out) {
-  const long long i =
-      (long long)mbx_detail::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (i >= slots) return;
-  const int k = map[i];
-  out[i] = k >= 0 ? val[k] : T(0);
-}
-
 // Inverse of the slot -> CSR position map over the real slots.
 __global__ void cb_inverse_kernel(const int32_t* __restrict__ perm, long long slots,
                                   int32_t* __restrict__ inv) {
@@ -150,25 +140,38 @@ __global__ void cb_inverse_kernel(const int32_t* __restrict__ perm, long long sl
   if (k >= 0) inv[k] = (int32_t)i;
 }
 
-// Slot q of the blocked A^T reads CSC position tperm_csc[q], which holds CSR
-// entry csc_perm[.], which sits in slot inv[.] of the blocked A.
-__global__ void cb_compose_kernel(const int32_t* __restrict__ tperm_csc, long long slots,
-                                  const int32_t* __restrict__ csc_perm,
-                                  const int32_t* __restrict__ inv, int32_t* __restrict__ out) {
-  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= slots) return;
-  const int p = tperm_csc[q];
-  out[q] = p >= 0 ? inv[csc_perm[p]] : -1;
+// CSC position p of A holds CSR entry csc_perm[p], which sits in slot
+// inv[.] of this form; *bad is set if some entry has no slot (a pattern
+// other than the one the form was built from).
+__global__ void cb_csc_slot_kernel(const int32_t* __restrict__ csc_perm, int nnz,
+                                   const int32_t* __restrict__ inv, int32_t* __restrict__ out,
+                                   int* __restrict__ bad) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nnz) return;
+  const int v = inv[csc_perm[p]];
+  out[p] = v < 0 ? 0 : v;
+  if (v < 0) atomicOr(bad, 1);
 }
 
 void destroy(mbx_colblock* cb);
 
+void free_perm(mbx_adjoint* a) {
+  if (!a) return;
+  if (a->perm) (void)hipFree(a->perm);
+  if (a->lslot) (void)hipFree(a->lslot);
+  if (a->spos) (void)hipFree(a->spos);
+  if (a->lidx) (void)hipFree(a->lidx);
+  if (a->stage) (void)hipFree(a->stage);
+  if (a->vt) (void)hipFree(a->vt);
+  delete a;
+}
+
 void drop_adjoint(mbx_colblock* cb) {
   if (cb->t) destroy(cb->t);
-  if (cb->tperm) (void)hipFree(cb->tperm);
+  free_perm(cb->tp);
   if (cb->tval) (void)hipFree(cb->tval);
   cb->t = nullptr;
-  cb->tperm = nullptr;
+  cb->tp = nullptr;
   cb->tval = nullptr;
 }
 
@@ -237,6 +240,7 @@ int build(mbx_colblock* cb, const int32_t* rp, const int32_t* col, int ipt, hipS
     }
     cb->base[0] = 0;
     for (int j = 0; j < nb; ++j) cb->base[j + 1] = cb->base[j] + (nnz_b[j] + 3) / 4 * 4;
+    for (int j = 0; j < nb; ++j) cb->nnz_b[j] = nnz_b[j];
     if (cb->base[nb] > INT32_MAX) st = MBX_EOVERFLOW;
     for (int j = 0; j <= nb; ++j) base.v[j] = (int32_t)cb->base[j];
   }
@@ -348,24 +352,57 @@ int mbx_colblock_adjoint_prepare(mbx_colblock* cb, const int32_t* d_row_ptr,
   if (st == MBX_OK)
     st = mbx_colblock_create(adj->col_ptr, adj->row_idx, cb->ncol, cb->nrow, cb->nnz, nblocks,
                              items_per_thread, stream, &cb->t);
+  // The permutation over this form's slots: destination p < nnz is CSC
+  // position p (its entry's slot here), destinations past nnz take the
+  // padding slots in order, so it is a bijection of [0, slots).
   int32_t* inv = nullptr;
+  int* d_bad = nullptr;
+  const int nnz = cb->nnz;
   const long long slots = cb->base[cb->nb];
   const long long tslots = cb->t ? cb->t->base[cb->t->nb] : 0;
+  mbx_adjoint* tp = nullptr;
+  if (st == MBX_OK) {
+    tp = new (std::nothrow) mbx_adjoint();
+    if (!tp) st = MBX_ENOMEM;
+  }
+  const size_t m = (size_t)(slots > 0 ? slots : 1);
+  if (st == MBX_OK) {
+    tp->nnz = (int32_t)slots;
+    cb->tp = tp;
+    if (hipMalloc((void**)&inv, sizeof(int32_t) * (nnz > 0 ? nnz : 1)) != hipSuccess ||
+        hipMalloc((void**)&d_bad, sizeof(int)) != hipSuccess ||
+        hipMalloc((void**)&tp->perm, sizeof(int32_t) * m) != hipSuccess ||
+        hipMalloc((void**)&tp->lslot, sizeof(uint16_t) * m) != hipSuccess ||
+        hipMalloc((void**)&tp->spos, sizeof(int32_t) * m) != hipSuccess ||
+        hipMalloc((void**)&tp->lidx, sizeof(int32_t) * m) != hipSuccess ||
+        hipMalloc(&tp->stage, 8 * m) != hipSuccess || hipMalloc(&tp->vt, 8 * m) != hipSuccess ||
+        hipMalloc(&cb->tval, 8 * (tslots > 0 ? tslots : 1)) != hipSuccess)
+      st = MBX_ENOMEM;
+  }
   if (st == MBX_OK &&
-      (hipMalloc((void**)&inv, sizeof(int32_t) * (cb->nnz > 0 ? cb->nnz : 1)) != hipSuccess ||
-       hipMalloc((void**)&cb->tperm, sizeof(int32_t) * (tslots > 0 ? tslots : 1)) != hipSuccess ||
-       hipMalloc(&cb->tval, 8 * (tslots > 0 ? tslots : 1)) != hipSuccess))
-    st = MBX_ENOMEM;
-  // entries no slot claims (a pattern other than the one the form was built
-  // from) read as padding rather than out of bounds
-  if (st == MBX_OK &&
-      hipMemsetAsync(inv, 0xFF, sizeof(int32_t) * (cb->nnz > 0 ? cb->nnz : 1), s) != hipSuccess)
+      (hipMemsetAsync(inv, 0xFF, sizeof(int32_t) * (nnz > 0 ? nnz : 1), s) != hipSuccess ||
+       hipMemsetAsync(d_bad, 0, sizeof(int), s) != hipSuccess))
     st = MBX_EHIP;
   if (st == MBX_OK && slots > 0)
     cb_inverse_kernel<<<(int)((slots + 255) / 256), 256, 0, s>>>(cb->perm, slots, inv);
-  if (st == MBX_OK && tslots > 0)
-    cb_compose_kernel<<<(int)((tslots + 255) / 256), 256, 0, s>>>(cb->t->perm, tslots,
-                                                                   adj->perm, inv, cb->tperm);
+  if (st == MBX_OK && nnz > 0)
+    cb_csc_slot_kernel<<<(nnz + 255) / 256, 256, 0, s>>>(adj->perm, nnz, inv, tp->perm, d_bad);
+  std::vector<int32_t> pad;
+  for (int j = 0; j < cb->nb; ++j)
+    for (long long q = cb->base[j] + cb->nnz_b[j]; q < cb->base[j + 1]; ++q)
+      pad.push_back((int32_t)q);
+  if (st == MBX_OK && !pad.empty() &&
+      hipMemcpyAsync(tp->perm + nnz, pad.data(), sizeof(int32_t) * pad.size(),
+                     hipMemcpyHostToDevice, s) != hipSuccess)
+    st = MBX_EHIP;
+  int bad = 0;
+  if (st == MBX_OK &&
+      (hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+       hipStreamSynchronize(s) != hipSuccess))
+    st = MBX_EHIP;
+  if (st == MBX_OK && bad) st = MBX_EINVAL;
+  if (st == MBX_OK && slots > 0) st = mbx_detail::perm_schedule_build(tp, s);
+  if (d_bad) (void)hipFree(d_bad);
   if (st == MBX_OK && hipGetLastError() != hipSuccess) st = MBX_EHIP;
   if (hipStreamSynchronize(s) != hipSuccess && st == MBX_OK) st = MBX_EHIP;
   if (inv) (void)hipFree(inv);
@@ -389,19 +426,20 @@ int mbx_colblock_dx(const mbx_colblock* cb, int32_t value_kind, const void* d_va
   const long long tslots = cb->t->base[cb->t->nb];
   hipStream_t s = (hipStream_t)stream;
   if (tslots > 0) {
+    // into CSC order by the two-pass bucketed permutation, then a streaming
+    // gather into the A^T form's order (its blocks read CSC positions in
+    // increasing order)
+    mbx_detail::perm_apply(cb->tp, vb, d_val_blocked, s);
     const int grid = (int)((tslots + 255) / 256);
     if (vb == 8)
-      cb_gather_xcd_kernel<<<grid, 256, 0, s>>>(cb->tperm, tslots,
-                                                (const uint64_t*)d_val_blocked,
-                                                (uint64_t*)cb->tval);
+      cb_values_kernel<<<grid, 256, 0, s>>>(cb->t->perm, tslots, (const uint64_t*)cb->tp->vt,
+                                            (uint64_t*)cb->tval);
     else if (vb == 4)
-      cb_gather_xcd_kernel<<<grid, 256, 0, s>>>(cb->tperm, tslots,
-                                                (const uint32_t*)d_val_blocked,
-                                                (uint32_t*)cb->tval);
+      cb_values_kernel<<<grid, 256, 0, s>>>(cb->t->perm, tslots, (const uint32_t*)cb->tp->vt,
+                                            (uint32_t*)cb->tval);
     else
-      cb_gather_xcd_kernel<<<grid, 256, 0, s>>>(cb->tperm, tslots,
-                                                (const uint16_t*)d_val_blocked,
-                                                (uint16_t*)cb->tval);
+      cb_values_kernel<<<grid, 256, 0, s>>>(cb->t->perm, tslots, (const uint16_t*)cb->tp->vt,
+                                            (uint16_t*)cb->tval);
     if (hipGetLastError() != hipSuccess) return MBX_EHIP;
   }
   return mbx_colblock_spmv(cb->t, value_kind, cb->tval, alpha, d_g, beta, d_dx, stream);

@@ -47,6 +47,14 @@ struct mbx_adjoint {
 
 namespace mbx_detail {
 
+// The adjoint's two-pass bucketed permutation on any bijection: for an
+// object with nnz, perm (destination p reads source perm[p]) and the lslot,
+// spos, lidx, stage and vt buffers allocated (nnz entries; 8 bytes for stage
+// and vt), perm_schedule_build computes the schedule (synchronises s) and
+// perm_apply moves values: vt[p] = val[perm[p]].
+int perm_schedule_build(mbx_adjoint* a, hipStream_t s);
+void perm_apply(const mbx_adjoint* a, int value_bytes, const void* val, hipStream_t s);
+
 constexpr int kXcds = 8;
 
 // Workgroups are dispatched round-robin over the 8 XCDs; give XCD j the
