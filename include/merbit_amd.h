@@ -286,7 +286,8 @@ int mbx_autotune_batch(const mbx_batch_item* items, int32_t count, int32_t value
  *       4 slots) and one plan per block; nblocks 0..16 (0: slices of at most
  *       3.5 MiB of 4-byte x), clamped to ncol.  Synchronises `stream`.
  *   mbx_colblock_info: block count, value slots (the length of a blocked
- *       value array: nnz plus padding) and the nblocks+1 column bounds.
+ *       value array: nnz, the block padding and 64 spare slots after the
+ *       last block) and the nblocks+1 column bounds.
  *   mbx_colblock_values: moves values of the given kind from CSR order into
  *       a blocked array of value_slots entries (padding slots set to 0).
  *   mbx_colblock_spmv: the chained product on blocked values; x and y as
@@ -297,12 +298,13 @@ int mbx_autotune_batch(const mbx_batch_item* items, int32_t count, int32_t value
  *   mbx_colblock_adjoint_prepare: builds the column-blocked form of A^T
  *       (blocks over A's rows, nblocks 0..16 as above, reported in
  *       *nblocks_out when not NULL) from the same CSR pattern the form was
- *       made from, and the map from its slots to this form's slots.
- *       Synchronises `stream`.
+ *       made from (MBX_EINVAL when it is not), and a two-pass bucketed
+ *       permutation from this form's slots to that form's.  Synchronises
+ *       `stream`.
  *   mbx_colblock_dx: dx = alpha*A^T*g + beta*dx from values in this form's
- *       block-major order: one gather into the A^T form's order (into a
- *       buffer the form owns), then its chained product, whose g slices stay
- *       in L2.  MBX_EINVAL before mbx_colblock_adjoint_prepare.
+ *       block-major order: the permutation (into buffers the form owns),
+ *       then the A^T form's chained product, whose g slices stay in L2.
+ *       MBX_EINVAL before mbx_colblock_adjoint_prepare.
  *   mbx_colblock_set_stream / _set_residency: the stream cache policy and
  *       tiles per CU of every block's product (and of the A^T form's), as
  *       mbx_plan_set_stream / mbx_plan_set_residency; MBX_STREAM_AUTO (the

@@ -24,11 +24,10 @@
 // The adjoint dx = A^T g gathers g by A's rows, so it is blocked the other
 // way: mbx_colblock_adjoint_prepare builds the CSC of A (the adjoint's
 // radix sort), a column-blocked form of A^T over it (blocks of A's rows) and
-// the adjoint's two-pass bucketed permutation from this form's slots to CSC
-// order (a bijection once the padding slots are given destinations past
-// nnz).  dx moves the values into CSC order with it, gathers them into the
-// A^T form's order (each of its blocks reads CSC positions in increasing
-// order, so that gather streams) and runs the A^T form's chained product.
+// the adjoint's two-pass bucketed permutation from this form's slots to the
+// A^T form's slots: a bijection of [0, slots), because this form keeps spare
+// slots after its last block that, with the padding of both forms, pair up
+// in order.  dx is that one permutation and the A^T form's chained product.
 
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
@@ -41,6 +40,10 @@
 #include "merbit_internal.hpp"
 
 constexpr int kMaxBlocks = 16;
+// Spare slots after the last block: the A^T form of a pattern pads at most
+// 3 slots per block, so this form's slot count always covers it and one
+// bijection of [0, slots) maps this form's order onto the A^T form's.
+constexpr int kReserve = 4 * kMaxBlocks;
 
 struct mbx_colblock {
   int32_t nrow = 0, ncol = 0, nnz = 0, nb = 0;
@@ -48,16 +51,15 @@ struct mbx_colblock {
   int32_t stream_mode = MBX_STREAM_AUTO;
   int32_t lo[kMaxBlocks + 1] = {};    // column ranges
   int64_t base[kMaxBlocks + 1] = {};  // padded first slot of each block
+  int64_t slots = 0;                  // value slots: base[nb] plus kReserve spare ones
   int32_t* row_ptr = nullptr;         // nb * stride: block j's row pointer at j * stride
   int32_t* col = nullptr;             // slots: column minus lo_j, block-major
   int32_t* perm = nullptr;            // slots: CSR position of each slot, -1 for padding
   mbx_plan* plan[kMaxBlocks] = {};
-  // adjoint (mbx_colblock_adjoint_prepare): the column-blocked form of A^T,
-  // the permutation that moves this form's values into CSC order, and the
-  // buffer of values in the A^T form's order
+  // adjoint (mbx_colblock_adjoint_prepare): the column-blocked form of A^T
+  // and the permutation that moves this form's values into its order
   mbx_colblock* t = nullptr;
-  mbx_adjoint* tp = nullptr;  // bucketed permutation: this form's slots -> CSC order
-  void* tval = nullptr;
+  mbx_adjoint* tp = nullptr;  // bucketed permutation: this form's slots -> the A^T form's
   int32_t nnz_b[kMaxBlocks] = {};
 };
 
@@ -140,17 +142,29 @@ __global__ void cb_inverse_kernel(const int32_t* __restrict__ perm, long long sl
   if (k >= 0) inv[k] = (int32_t)i;
 }
 
-// CSC position p of A holds CSR entry csc_perm[p], which sits in slot
-// inv[.] of this form; *bad is set if some entry has no slot (a pattern
-// other than the one the form was built from).
-__global__ void cb_csc_slot_kernel(const int32_t* __restrict__ csc_perm, int nnz,
-                                   const int32_t* __restrict__ inv, int32_t* __restrict__ out,
-                                   int* __restrict__ bad) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= nnz) return;
+// Slot q of the A^T form holds CSC entry tperm[q] (or padding, -1), that is
+// CSR entry csc_perm[.], which sits in slot inv[.] of this form; *bad is set
+// if some entry has no slot (a pattern other than the one the form was built
+// from).  Padding slots are filled by cb_pair_kernel.
+__global__ void cb_real_slot_kernel(const int32_t* __restrict__ tperm, long long tslots,
+                                    const int32_t* __restrict__ csc_perm,
+                                    const int32_t* __restrict__ inv, int32_t* __restrict__ out,
+                                    int* __restrict__ bad) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= tslots) return;
+  const int p = tperm[q];
+  if (p < 0) return;
   const int v = inv[csc_perm[p]];
-  out[p] = v < 0 ? 0 : v;
+  out[q] = v < 0 ? 0 : v;
   if (v < 0) atomicOr(bad, 1);
+}
+
+// out[dst[i]] = src[i]: the spare and padding slots of both forms, paired in
+// order.
+__global__ void cb_pair_kernel(const int32_t* __restrict__ dst, const int32_t* __restrict__ src,
+                               int n, int32_t* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[dst[i]] = src[i];
 }
 
 void destroy(mbx_colblock* cb);
@@ -169,10 +183,8 @@ void free_perm(mbx_adjoint* a) {
 void drop_adjoint(mbx_colblock* cb) {
   if (cb->t) destroy(cb->t);
   free_perm(cb->tp);
-  if (cb->tval) (void)hipFree(cb->tval);
   cb->t = nullptr;
   cb->tp = nullptr;
-  cb->tval = nullptr;
 }
 
 void destroy(mbx_colblock* cb) {
@@ -241,10 +253,11 @@ int build(mbx_colblock* cb, const int32_t* rp, const int32_t* col, int ipt, hipS
     cb->base[0] = 0;
     for (int j = 0; j < nb; ++j) cb->base[j + 1] = cb->base[j] + (nnz_b[j] + 3) / 4 * 4;
     for (int j = 0; j < nb; ++j) cb->nnz_b[j] = nnz_b[j];
-    if (cb->base[nb] > INT32_MAX) st = MBX_EOVERFLOW;
+    cb->slots = cb->base[nb] + kReserve;
+    if (cb->slots > INT32_MAX) st = MBX_EOVERFLOW;
     for (int j = 0; j <= nb; ++j) base.v[j] = (int32_t)cb->base[j];
   }
-  const long long slots = cb->base[nb];
+  const long long slots = cb->slots;
   if (st == MBX_OK &&
       (hipMalloc(&cb->col, sizeof(int32_t) * (slots > 0 ? slots : 1)) != hipSuccess ||
        hipMalloc(&cb->perm, sizeof(int32_t) * (slots > 0 ? slots : 1)) != hipSuccess))
@@ -318,7 +331,7 @@ int mbx_colblock_info(const mbx_colblock* cb, int32_t* nblocks, int64_t* value_s
                       int32_t* col_bounds) {
   if (!cb) return MBX_EINVAL;
   if (nblocks) *nblocks = cb->nb;
-  if (value_slots) *value_slots = cb->base[cb->nb];
+  if (value_slots) *value_slots = cb->slots;
   if (col_bounds)
     for (int j = 0; j <= cb->nb; ++j) col_bounds[j] = cb->lo[j];
   return MBX_OK;
@@ -352,57 +365,78 @@ int mbx_colblock_adjoint_prepare(mbx_colblock* cb, const int32_t* d_row_ptr,
   if (st == MBX_OK)
     st = mbx_colblock_create(adj->col_ptr, adj->row_idx, cb->ncol, cb->nrow, cb->nnz, nblocks,
                              items_per_thread, stream, &cb->t);
-  // The permutation over this form's slots: destination p < nnz is CSC
-  // position p (its entry's slot here), destinations past nnz take the
-  // padding slots in order, so it is a bijection of [0, slots).
+  // The permutation over [0, slots): destination q (a slot of the A^T form,
+  // or one past its last block) reads source slot perm[q] of this form.
+  // Real entries pair through the CSC position; the A^T form's padding and
+  // the slots past its last block take this form's padding and spare slots
+  // in order (equal counts: slots - nnz on each side).
   int32_t* inv = nullptr;
   int* d_bad = nullptr;
+  int32_t *d_pd = nullptr, *d_ps = nullptr;
   const int nnz = cb->nnz;
-  const long long slots = cb->base[cb->nb];
-  const long long tslots = cb->t ? cb->t->base[cb->t->nb] : 0;
+  const long long slots = cb->slots;
   mbx_adjoint* tp = nullptr;
+  std::vector<int32_t> pd, ps;
   if (st == MBX_OK) {
+    const mbx_colblock* t = cb->t;
+    for (int j = 0; j < t->nb; ++j)
+      for (long long q = t->base[j] + t->nnz_b[j]; q < t->base[j + 1]; ++q) pd.push_back((int32_t)q);
+    for (long long q = t->base[t->nb]; q < slots; ++q) pd.push_back((int32_t)q);
+    for (int j = 0; j < cb->nb; ++j)
+      for (long long q = cb->base[j] + cb->nnz_b[j]; q < cb->base[j + 1]; ++q)
+        ps.push_back((int32_t)q);
+    for (long long q = cb->base[cb->nb]; q < slots; ++q) ps.push_back((int32_t)q);
+    if (t->base[t->nb] > slots || pd.size() != ps.size()) st = MBX_EOVERFLOW;  // cannot happen
     tp = new (std::nothrow) mbx_adjoint();
-    if (!tp) st = MBX_ENOMEM;
+    if (!tp && st == MBX_OK) st = MBX_ENOMEM;
   }
-  const size_t m = (size_t)(slots > 0 ? slots : 1);
+  const size_t m = (size_t)slots;
   if (st == MBX_OK) {
     tp->nnz = (int32_t)slots;
     cb->tp = tp;
     if (hipMalloc((void**)&inv, sizeof(int32_t) * (nnz > 0 ? nnz : 1)) != hipSuccess ||
         hipMalloc((void**)&d_bad, sizeof(int)) != hipSuccess ||
+        hipMalloc((void**)&d_pd, sizeof(int32_t) * (pd.size() + 1)) != hipSuccess ||
+        hipMalloc((void**)&d_ps, sizeof(int32_t) * (ps.size() + 1)) != hipSuccess ||
         hipMalloc((void**)&tp->perm, sizeof(int32_t) * m) != hipSuccess ||
         hipMalloc((void**)&tp->lslot, sizeof(uint16_t) * m) != hipSuccess ||
         hipMalloc((void**)&tp->spos, sizeof(int32_t) * m) != hipSuccess ||
         hipMalloc((void**)&tp->lidx, sizeof(int32_t) * m) != hipSuccess ||
-        hipMalloc(&tp->stage, 8 * m) != hipSuccess || hipMalloc(&tp->vt, 8 * m) != hipSuccess ||
-        hipMalloc(&cb->tval, 8 * (tslots > 0 ? tslots : 1)) != hipSuccess)
+        hipMalloc(&tp->stage, 8 * m) != hipSuccess || hipMalloc(&tp->vt, 8 * m) != hipSuccess)
       st = MBX_ENOMEM;
+  } else if (tp) {
+    delete tp;
+    tp = nullptr;
   }
   if (st == MBX_OK &&
       (hipMemsetAsync(inv, 0xFF, sizeof(int32_t) * (nnz > 0 ? nnz : 1), s) != hipSuccess ||
-       hipMemsetAsync(d_bad, 0, sizeof(int), s) != hipSuccess))
+       hipMemsetAsync(d_bad, 0, sizeof(int), s) != hipSuccess ||
+       hipMemsetAsync(tp->perm, 0, sizeof(int32_t) * m, s) != hipSuccess ||
+       hipMemcpyAsync(d_pd, pd.data(), sizeof(int32_t) * pd.size(), hipMemcpyHostToDevice, s) !=
+           hipSuccess ||
+       hipMemcpyAsync(d_ps, ps.data(), sizeof(int32_t) * ps.size(), hipMemcpyHostToDevice, s) !=
+           hipSuccess))
     st = MBX_EHIP;
-  if (st == MBX_OK && slots > 0)
+  if (st == MBX_OK) {
+    const long long treal = cb->t->base[cb->t->nb];
     cb_inverse_kernel<<<(int)((slots + 255) / 256), 256, 0, s>>>(cb->perm, slots, inv);
-  if (st == MBX_OK && nnz > 0)
-    cb_csc_slot_kernel<<<(nnz + 255) / 256, 256, 0, s>>>(adj->perm, nnz, inv, tp->perm, d_bad);
-  std::vector<int32_t> pad;
-  for (int j = 0; j < cb->nb; ++j)
-    for (long long q = cb->base[j] + cb->nnz_b[j]; q < cb->base[j + 1]; ++q)
-      pad.push_back((int32_t)q);
-  if (st == MBX_OK && !pad.empty() &&
-      hipMemcpyAsync(tp->perm + nnz, pad.data(), sizeof(int32_t) * pad.size(),
-                     hipMemcpyHostToDevice, s) != hipSuccess)
-    st = MBX_EHIP;
+    if (treal > 0)
+      cb_real_slot_kernel<<<(int)((treal + 255) / 256), 256, 0, s>>>(cb->t->perm, treal,
+                                                                     adj->perm, inv, tp->perm,
+                                                                     d_bad);
+    const int np = (int)pd.size();
+    cb_pair_kernel<<<(np + 255) / 256, 256, 0, s>>>(d_pd, d_ps, np, tp->perm);
+  }
   int bad = 0;
   if (st == MBX_OK &&
       (hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
        hipStreamSynchronize(s) != hipSuccess))
     st = MBX_EHIP;
   if (st == MBX_OK && bad) st = MBX_EINVAL;
-  if (st == MBX_OK && slots > 0) st = mbx_detail::perm_schedule_build(tp, s);
+  if (st == MBX_OK) st = mbx_detail::perm_schedule_build(tp, s);
   if (d_bad) (void)hipFree(d_bad);
+  if (d_pd) (void)hipFree(d_pd);
+  if (d_ps) (void)hipFree(d_ps);
   if (st == MBX_OK && hipGetLastError() != hipSuccess) st = MBX_EHIP;
   if (hipStreamSynchronize(s) != hipSuccess && st == MBX_OK) st = MBX_EHIP;
   if (inv) (void)hipFree(inv);
@@ -422,27 +456,10 @@ int mbx_colblock_dx(const mbx_colblock* cb, int32_t value_kind, const void* d_va
   if (!cb || !cb->t) return MBX_EINVAL;
   const int vb = value_bytes(value_kind);
   if (vb == 0) return MBX_EINVAL;
-  if (cb->nnz > 0 && !d_val_blocked) return MBX_EINVAL;
-  const long long tslots = cb->t->base[cb->t->nb];
-  hipStream_t s = (hipStream_t)stream;
-  if (tslots > 0) {
-    // into CSC order by the two-pass bucketed permutation, then a streaming
-    // gather into the A^T form's order (its blocks read CSC positions in
-    // increasing order)
-    mbx_detail::perm_apply(cb->tp, vb, d_val_blocked, s);
-    const int grid = (int)((tslots + 255) / 256);
-    if (vb == 8)
-      cb_values_kernel<<<grid, 256, 0, s>>>(cb->t->perm, tslots, (const uint64_t*)cb->tp->vt,
-                                            (uint64_t*)cb->tval);
-    else if (vb == 4)
-      cb_values_kernel<<<grid, 256, 0, s>>>(cb->t->perm, tslots, (const uint32_t*)cb->tp->vt,
-                                            (uint32_t*)cb->tval);
-    else
-      cb_values_kernel<<<grid, 256, 0, s>>>(cb->t->perm, tslots, (const uint16_t*)cb->tp->vt,
-                                            (uint16_t*)cb->tval);
-    if (hipGetLastError() != hipSuccess) return MBX_EHIP;
-  }
-  return mbx_colblock_spmv(cb->t, value_kind, cb->tval, alpha, d_g, beta, d_dx, stream);
+  if (!d_val_blocked || (cb->ncol > 0 && !d_dx) || (cb->nnz > 0 && !d_g)) return MBX_EINVAL;
+  // one two-pass bucketed permutation into the A^T form's order
+  mbx_detail::perm_apply(cb->tp, vb, d_val_blocked, (hipStream_t)stream);
+  return mbx_colblock_spmv(cb->t, value_kind, cb->tp->vt, alpha, d_g, beta, d_dx, stream);
 }
 
 int mbx_colblock_values(const mbx_colblock* cb, int32_t value_kind, const void* d_val,
@@ -450,8 +467,7 @@ int mbx_colblock_values(const mbx_colblock* cb, int32_t value_kind, const void* 
   if (!cb) return MBX_EINVAL;
   const int vb = value_bytes(value_kind);
   if (vb == 0) return MBX_EINVAL;
-  const long long slots = cb->base[cb->nb];
-  if (slots == 0) return MBX_OK;
+  const long long slots = cb->slots;
   if (!d_val_blocked || (cb->nnz > 0 && !d_val)) return MBX_EINVAL;
   hipStream_t s = (hipStream_t)stream;
   const int grid = (int)((slots + 255) / 256);

@@ -2,8 +2,8 @@
 
 * The block-major layout is checked bit for bit: values moved by
   mbx_colblock_values must equal a numpy stable sort of the CSR entries by
-  (column block, row, position), with block starts padded to 4 slots and
-  zeros in the padding.
+  (column block, row, position), with block starts padded to 4 slots, 64
+  spare slots after the last block and zeros in the padding.
 * The chained product is checked against the oracle's accurate CSR product
   with the bars of the plain product: condition-scaled error <= 1e-12 for
   FP64 and <= 1e-6 for FP32/BF16 values (rows are split over the blocks, so
@@ -38,7 +38,7 @@ def expected_blocked(rp, col, ncol, nb, val):
     order = np.lexsort((np.arange(len(col)), rows, blk))
     counts = np.bincount(blk, minlength=nb) if len(col) else np.zeros(nb, np.int64)
     base = np.concatenate([[0], np.cumsum((counts + 3) // 4 * 4)])
-    out = np.zeros(int(base[-1]), val.dtype)
+    out = np.zeros(int(base[-1]) + 64, val.dtype)  # 64 spare slots after the last block
     start = 0
     for j in range(nb):
         out[base[j]:base[j] + counts[j]] = val[order[start:start + counts[j]]]
