@@ -502,6 +502,16 @@ def bench_adjoint(dev, args):
     dxb = torch.empty(A.ncol, dtype=torch.float32, device=dev)
     dxb_ms, _ = time_steps([lambda: cb.adjoint_x(vb, gg, dxb)], steps, args.warmup)
     dx_ref = op.adjoint_x(val, gg)
+    # the CSR-order adjoint with a blocked A^T: values stay in CSR order
+    opb = MerbitOperator(rp, col, A.ncol, items_per_thread=args.ipt or 8, adjoint_blocks=-1)
+    opb.adjoint_handle
+    torch.cuda.synchronize()
+    dxc_ms, _ = time_steps([lambda: opb.adjoint_x(val, gg)], steps, args.warmup)
+    out["dx_blocked_transpose"] = {"us": round(dxc_ms * 1e3, 2), "gbs": gbs(b_dx, dxc_ms),
+                                   "speedup": round(dx_ms / dxc_ms, 2),
+                                   "rel_diff": rel_err(opb.adjoint_x(val, gg), dx_ref)}
+    log(f"csr adjoint dx through a blocked A^T: {dxc_ms * 1e3:.1f} us (x{dx_ms / dxc_ms:.2f})")
+    opb.close()
     out["forward_column_blocked"] = {
         "us": round(fwdb_ms * 1e3, 2), "gbs": gbs(b_fwd, fwdb_ms),
         "gflops": round(2 * nnz / (fwdb_ms * 1e-3) / 1e9, 1), "blocks": cb.nblocks,

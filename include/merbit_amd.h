@@ -351,6 +351,16 @@ int mbx_adjoint_transpose(const mbx_adjoint* adj, int32_t* h_col_ptr, int32_t* h
                           int32_t* h_perm);
 int mbx_adjoint_dx(const mbx_adjoint* adj, int32_t value_kind, const void* d_val, double alpha,
                    const void* d_g, double beta, void* d_dx, void* stream);
+/* Optional: dx through a column-blocked form of A^T (mbx_colblock_*, blocks
+ * of A's rows), for graphs whose g outgrows one XCD's L2.  One two-pass
+ * bucketed permutation moves the CSR-order values straight into that
+ * form's order (its padding slots read virtual zero sources), then its
+ * chained product sums dx.  nblocks: 1..16, -1 for slices of at most
+ * 3.5 MiB of 4-byte g, 0 to return to the single merge-path product on
+ * A^T.  Synchronises `stream` (and the device when it replaces a blocked
+ * form). */
+int mbx_adjoint_set_blocks(mbx_adjoint* a, int32_t nblocks, void* stream);
+
 int mbx_sddmm(const mbx_plan* plan, int32_t value_kind, const int32_t* d_row_ptr,
               const int32_t* d_col, const void* d_g, const void* d_x, void* d_dval,
               void* stream);

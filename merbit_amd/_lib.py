@@ -24,6 +24,7 @@ ABI_SYMBOLS = (
     "mbx_batch_graph_create", "mbx_spmv_graph_create", "mbx_graph_launch", "mbx_graph_destroy",
     "mbx_autotune_spmv", "mbx_autotune_spmv_launch", "mbx_autotune_spmm", "mbx_autotune_batch",
     "mbx_adjoint_create", "mbx_adjoint_destroy", "mbx_adjoint_transpose", "mbx_adjoint_dx",
+    "mbx_adjoint_set_blocks",
     "mbx_sddmm", "mbx_ingest_ref", "mbx_plan_stats",
     "mbx_pagerank_workspace_bytes", "mbx_pagerank_f64", "mbx_pagerank_graph_create",
     "mbx_colblock_create", "mbx_colblock_destroy", "mbx_colblock_info", "mbx_colblock_values",
@@ -145,6 +146,7 @@ def lib() -> ctypes.CDLL:
     L.mbx_adjoint_destroy.restype = None
     L.mbx_adjoint_transpose.argtypes = [vp, vp, vp, vp]
     L.mbx_adjoint_dx.argtypes = [vp, i32, vp, f64, vp, f64, vp, vp]
+    L.mbx_adjoint_set_blocks.argtypes = [vp, i32, vp]
     L.mbx_sddmm.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp]
     L.mbx_ingest_ref.argtypes = [i32, i32, vp, vp, i32, vp, vp, vp, vp]
     L.mbx_plan_stats.argtypes = [vp, vp, ctypes.POINTER(PatternStats)]

@@ -26,7 +26,11 @@ _KIND = {torch.float64: MBX_VALUE_F64, torch.float32: MBX_VALUE_F32}
 
 class MerbitOperator:
     def __init__(self, row_ptr: torch.Tensor, col: torch.Tensor, ncol: int,
-                 items_per_thread: int = 8):
+                 items_per_thread: int = 8, adjoint_blocks: int = 0):
+        """adjoint_blocks: 0 for dx by one merge-path product on A^T; -1 or
+        1..16 for dx through a column-blocked A^T (mbx_adjoint_set_blocks; -1
+        picks the count), for graphs whose g outgrows one XCD's L2."""
+        self.adjoint_blocks = adjoint_blocks
         self.row_ptr = row_ptr
         self.col = col
         self.nrow = row_ptr.numel() - 1
@@ -46,6 +50,9 @@ class MerbitOperator:
                                            _stream_handle(None), ctypes.byref(h)),
                   "mbx_adjoint_create")
             self._adj = h
+            if self.adjoint_blocks:
+                check(lib().mbx_adjoint_set_blocks(h, self.adjoint_blocks, _stream_handle(None)),
+                      "mbx_adjoint_set_blocks")
         return self._adj
 
     def transpose(self):

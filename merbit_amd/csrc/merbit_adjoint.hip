@@ -30,6 +30,7 @@
 
 #include <hipcub/hipcub.hpp>
 #include <new>
+#include <vector>
 
 #include "merbit_amd.h"
 #include "merbit_internal.hpp"
@@ -91,13 +92,15 @@ template <class T, bool VEC>
 __global__ __launch_bounds__(kPassWg) void perm_pass1_kernel(const T* __restrict__ val,
                                                              const uint16_t* __restrict__ lslot,
                                                              const int32_t* __restrict__ spos,
-                                                             int nnz, int nchunks,
+                                                             int nnz, int nsrc, int nchunks,
                                                              T* __restrict__ stage) {
   extern __shared__ __align__(16) char smem[];
   T* s_v = reinterpret_cast<T*>(smem);  // kChunk values
   const long long base = (long long)xcd_tile(blockIdx.x, nchunks) * kChunk;
   const int n = (int)min((long long)kChunk, (long long)nnz - base);
-  const int n4 = VEC ? (n & ~3) : 0;
+  // sources at or past nsrc are virtual (value 0): val holds nsrc values
+  const int nv = (int)max(0ll, min((long long)n, (long long)nsrc - base));
+  const int n4 = VEC ? (nv & ~3) : 0;
   if (VEC) {
 #pragma unroll 4
     for (int j = 4 * threadIdx.x; j < n4; j += 4 * kPassWg) {
@@ -111,7 +114,8 @@ __global__ __launch_bounds__(kPassWg) void perm_pass1_kernel(const T* __restrict
       s_v[l.w] = v.w;
     }
   }
-  for (int j = n4 + threadIdx.x; j < n; j += kPassWg) s_v[lslot[base + j]] = val[base + j];
+  for (int j = n4 + threadIdx.x; j < n; j += kPassWg)
+    s_v[lslot[base + j]] = j < nv ? val[base + j] : T(0);
   __syncthreads();
   // one slot per lane: a store instruction covers 64 consecutive slots, so
   // it touches as few bucket runs (staging lines) as possible
@@ -351,13 +355,13 @@ void permute_values(const mbx_adjoint* a, const T* val, hipStream_t s) {
   if ((reinterpret_cast<uintptr_t>(val) & (4 * sizeof(T) - 1)) == 0) {
     (void)hipFuncSetAttribute((const void*)perm_pass1_kernel<T, true>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1);
-    perm_pass1_kernel<T, true><<<g1, kPassWg, lds1, s>>>(val, a->lslot, a->spos, nnz, g1,
-                                                          (T*)a->stage);
+    perm_pass1_kernel<T, true><<<g1, kPassWg, lds1, s>>>(val, a->lslot, a->spos, nnz, a->nsrc,
+                                                          g1, (T*)a->stage);
   } else {
     (void)hipFuncSetAttribute((const void*)perm_pass1_kernel<T, false>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1);
-    perm_pass1_kernel<T, false><<<g1, kPassWg, lds1, s>>>(val, a->lslot, a->spos, nnz, g1,
-                                                           (T*)a->stage);
+    perm_pass1_kernel<T, false><<<g1, kPassWg, lds1, s>>>(val, a->lslot, a->spos, nnz,
+                                                           a->nsrc, g1, (T*)a->stage);
   }
   const int g2 = (int)(((long long)nnz + kBucket - 1) / kBucket);
   const size_t lds2 = sizeof(T) * kBucket;
@@ -366,11 +370,44 @@ void permute_values(const mbx_adjoint* a, const T* val, hipStream_t s) {
   perm_pass2_kernel<T><<<g2, kPassWg, lds2, s>>>((const T*)a->stage, a->lidx, nnz, (T*)a->vt);
 }
 
+// Slot q of the blocked A^T holds CSC entry tperm[q], CSR entry perm[.];
+// its padding slots (tperm -1) are given by tb_pad_kernel.
+__global__ void tb_real_kernel(const int32_t* __restrict__ tperm, int n,
+                               const int32_t* __restrict__ perm, int32_t* __restrict__ out) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n && tperm[q] >= 0) out[q] = perm[tperm[q]];
+}
+
+// The i-th padding slot reads virtual source nnz + i.
+__global__ void tb_pad_kernel(const int32_t* __restrict__ dst, int n, int nnz,
+                              int32_t* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[dst[i]] = nnz + i;
+}
+
+void drop_blocks(mbx_adjoint* a) {
+  if (a->tb) mbx_colblock_destroy(a->tb);
+  mbx_detail::perm_free(a->tbp);
+  a->tb = nullptr;
+  a->tbp = nullptr;
+}
+
 }  // namespace
 
 namespace mbx_detail {
 
 int perm_schedule_build(mbx_adjoint* a, hipStream_t s) { return build_perm_schedule(a, s); }
+
+void perm_free(mbx_adjoint* a) {
+  if (!a) return;
+  if (a->perm) (void)hipFree(a->perm);
+  if (a->lslot) (void)hipFree(a->lslot);
+  if (a->spos) (void)hipFree(a->spos);
+  if (a->lidx) (void)hipFree(a->lidx);
+  if (a->stage) (void)hipFree(a->stage);
+  if (a->vt) (void)hipFree(a->vt);
+  delete a;
+}
 
 void perm_apply(const mbx_adjoint* a, int value_bytes, const void* val, hipStream_t s) {
   if (a->nnz == 0) return;
@@ -400,6 +437,7 @@ int mbx_adjoint_create(const int32_t* d_row_ptr, const int32_t* d_col, int32_t n
   a->nrow = nrow;
   a->ncol = ncol;
   a->nnz = nnz;
+  a->nsrc = nnz;
   const size_t m = (size_t)(nnz > 0 ? nnz : 1);
   int32_t *row = nullptr, *iota = nullptr;
   uint32_t* keys = nullptr;
@@ -466,8 +504,62 @@ void mbx_adjoint_destroy(mbx_adjoint* a) {
   if (a->lslot) (void)hipFree(a->lslot);
   if (a->spos) (void)hipFree(a->spos);
   if (a->lidx) (void)hipFree(a->lidx);
+  drop_blocks(a);
   mbx_plan_destroy(a->plan_t);
   delete a;
+}
+
+int mbx_adjoint_set_blocks(mbx_adjoint* a, int32_t nblocks, void* stream) {
+  if (!a || nblocks < -1 || nblocks > 16) return MBX_EINVAL;
+  hipStream_t s = (hipStream_t)stream;
+  if (a->tb) {  // a dx on any stream may still read them
+    (void)hipDeviceSynchronize();
+    drop_blocks(a);
+  }
+  if (nblocks == 0) return MBX_OK;
+  int st = mbx_colblock_create(a->col_ptr, a->row_idx, a->ncol, a->nrow, a->nnz,
+                               nblocks < 0 ? 0 : nblocks, a->plan_t->ipt, stream, &a->tb);
+  if (st != MBX_OK) return st;
+  // destinations: the blocked form's slots up to its last block; sources:
+  // the nnz CSR values, then one virtual (zero) source per padding slot
+  int32_t nb = 0;
+  int32_t bounds[17];
+  (void)mbx_colblock_info(a->tb, &nb, nullptr, bounds);
+  std::vector<int32_t> pad = mbx_detail::colblock_padding(a->tb);
+  const long long n = (long long)a->nnz + (long long)pad.size();
+  mbx_adjoint* p = new (std::nothrow) mbx_adjoint();
+  if (!p) {
+    drop_blocks(a);
+    return MBX_ENOMEM;
+  }
+  a->tbp = p;
+  p->nnz = (int32_t)n;
+  p->nsrc = a->nnz;
+  const size_t m = (size_t)(n > 0 ? n : 1);
+  int32_t* d_pad = nullptr;
+  if (hipMalloc((void**)&p->perm, 4 * m) != hipSuccess ||
+      hipMalloc((void**)&p->lslot, 2 * m) != hipSuccess ||
+      hipMalloc((void**)&p->spos, 4 * m) != hipSuccess ||
+      hipMalloc((void**)&p->lidx, 4 * m) != hipSuccess || hipMalloc(&p->stage, 8 * m) != hipSuccess ||
+      hipMalloc(&p->vt, 8 * m) != hipSuccess ||
+      hipMalloc((void**)&d_pad, 4 * (pad.size() + 1)) != hipSuccess)
+    st = MBX_ENOMEM;
+  if (st == MBX_OK && !pad.empty() &&
+      hipMemcpyAsync(d_pad, pad.data(), 4 * pad.size(), hipMemcpyHostToDevice, s) != hipSuccess)
+    st = MBX_EHIP;
+  if (st == MBX_OK && n > 0) {
+    tb_real_kernel<<<blocks(n), kBlock, 0, s>>>(mbx_detail::colblock_slot_map(a->tb), (int)n,
+                                                a->perm, p->perm);
+    if (!pad.empty())
+      tb_pad_kernel<<<blocks((long long)pad.size()), kBlock, 0, s>>>(d_pad, (int)pad.size(),
+                                                                     a->nnz, p->perm);
+    if (hipGetLastError() != hipSuccess) st = MBX_EHIP;
+  }
+  if (st == MBX_OK && n > 0) st = build_perm_schedule(p, s);
+  if (hipStreamSynchronize(s) != hipSuccess && st == MBX_OK) st = MBX_EHIP;
+  if (d_pad) (void)hipFree(d_pad);
+  if (st != MBX_OK) drop_blocks(a);
+  return st;
 }
 
 int mbx_adjoint_transpose(const mbx_adjoint* a, int32_t* h_col_ptr, int32_t* h_row_idx,
@@ -489,6 +581,14 @@ int mbx_adjoint_dx(const mbx_adjoint* a, int32_t value_kind, const void* d_val, 
   if (!a || (a->nnz > 0 && (!d_val || !d_g)) || (a->ncol > 0 && !d_dx)) return MBX_EINVAL;
   hipStream_t s = (hipStream_t)stream;
   const int nnz = a->nnz;
+  if (a->tb) {  // one permutation from CSR order into the blocked A^T, then its products
+    const int vb = value_kind == MBX_VALUE_F64 ? 8 : value_kind == MBX_VALUE_F32 ? 4 : 2;
+    if (value_kind != MBX_VALUE_F64 && value_kind != MBX_VALUE_F32 &&
+        value_kind != MBX_VALUE_BF16)
+      return MBX_EINVAL;
+    if (a->tbp->nnz > 0) mbx_detail::perm_apply(a->tbp, vb, d_val, s);
+    return mbx_colblock_spmv(a->tb, value_kind, a->tbp->vt, alpha, d_g, beta, d_dx, stream);
+  }
   switch (value_kind) {
     case MBX_VALUE_F64:
       if (nnz > 0) permute_values<double>(a, (const double*)d_val, s);

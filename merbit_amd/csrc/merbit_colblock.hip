@@ -169,20 +169,9 @@ __global__ void cb_pair_kernel(const int32_t* __restrict__ dst, const int32_t* _
 
 void destroy(mbx_colblock* cb);
 
-void free_perm(mbx_adjoint* a) {
-  if (!a) return;
-  if (a->perm) (void)hipFree(a->perm);
-  if (a->lslot) (void)hipFree(a->lslot);
-  if (a->spos) (void)hipFree(a->spos);
-  if (a->lidx) (void)hipFree(a->lidx);
-  if (a->stage) (void)hipFree(a->stage);
-  if (a->vt) (void)hipFree(a->vt);
-  delete a;
-}
-
 void drop_adjoint(mbx_colblock* cb) {
   if (cb->t) destroy(cb->t);
-  free_perm(cb->tp);
+  mbx_detail::perm_free(cb->tp);
   cb->t = nullptr;
   cb->tp = nullptr;
 }
@@ -286,6 +275,20 @@ int build(mbx_colblock* cb, const int32_t* rp, const int32_t* col, int ipt, hipS
 }
 
 }  // namespace
+
+namespace mbx_detail {
+
+std::vector<int32_t> colblock_padding(const mbx_colblock* cb) {
+  std::vector<int32_t> pad;
+  for (int j = 0; j < cb->nb; ++j)
+    for (long long q = cb->base[j] + cb->nnz_b[j]; q < cb->base[j + 1]; ++q)
+      pad.push_back((int32_t)q);
+  return pad;
+}
+
+const int32_t* colblock_slot_map(const mbx_colblock* cb) { return cb->perm; }
+
+}  // namespace mbx_detail
 
 extern "C" {
 
@@ -393,6 +396,7 @@ int mbx_colblock_adjoint_prepare(mbx_colblock* cb, const int32_t* d_row_ptr,
   const size_t m = (size_t)slots;
   if (st == MBX_OK) {
     tp->nnz = (int32_t)slots;
+    tp->nsrc = (int32_t)slots;
     cb->tp = tp;
     if (hipMalloc((void**)&inv, sizeof(int32_t) * (nnz > 0 ? nnz : 1)) != hipSuccess ||
         hipMalloc((void**)&d_bad, sizeof(int)) != hipSuccess ||

@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include <atomic>
+#include <vector>
 
 #include "merbit_amd.h"
 
@@ -29,6 +30,8 @@ struct mbx_plan {
   int32_t accumulation;  // MBX_ACC_*: sums of the FP32 / bfloat16 value forms
 };
 
+struct mbx_colblock;
+
 // Adjoint object (mbx_adjoint_*): the CSC of A and the schedule that moves
 // values into CSC order; also read by the column-blocked form's adjoint.
 struct mbx_adjoint {
@@ -43,6 +46,11 @@ struct mbx_adjoint {
   int32_t* spos;      // nnz: staging index of pass-1 slot i (chunk-major)
   int32_t* lidx;      // nnz: staging index of CSC position p
   void* stage;        // nnz values, bucket after bucket (8 bytes each)
+  int32_t nsrc;       // sources held by the value array; those past it read 0
+  // optional (mbx_adjoint_set_blocks): a column-blocked form of A^T and the
+  // permutation from CSR order straight into its order
+  mbx_colblock* tb;
+  mbx_adjoint* tbp;
 };
 
 namespace mbx_detail {
@@ -54,6 +62,15 @@ namespace mbx_detail {
 // perm_apply moves values: vt[p] = val[perm[p]].
 int perm_schedule_build(mbx_adjoint* a, hipStream_t s);
 void perm_apply(const mbx_adjoint* a, int value_bytes, const void* val, hipStream_t s);
+// Frees a permutation object's buffers (perm, lslot, spos, lidx, stage, vt)
+// and the object.
+void perm_free(mbx_adjoint* a);
+
+// Column-blocked form internals read by the adjoint: the padding slots
+// before the end of its last block, and its slot -> source position map
+// (device array, -1 on padding).
+std::vector<int32_t> colblock_padding(const mbx_colblock* cb);
+const int32_t* colblock_slot_map(const mbx_colblock* cb);
 
 constexpr int kXcds = 8;
 

@@ -257,3 +257,50 @@ def test_colblock_rejects_nnz_other_than_row_ptr(device):
                                    ctypes.byref(h))
     assert st == 0 and h.value
     lib().mbx_colblock_destroy(h)
+
+
+@pytest.mark.parametrize("kind", ["f64", "f32", "bf16"])
+@pytest.mark.parametrize("blocks", [-1, 1, 3, 16])
+def test_csr_adjoint_through_blocked_transpose(orc, device, kind, blocks):
+    """mbx_adjoint_set_blocks: dx from CSR-order values through a blocked
+    A^T (one bucketed permutation whose padding slots read virtual zero
+    sources), against the oracle's accurate product on the transposed CSR;
+    back at 0 blocks the single merge-path product returns bit for bit."""
+    from merbit_amd import gen
+    from merbit_amd._lib import check, lib
+    from merbit_amd.autograd import MerbitOperator
+    from merbit_amd.spmv import _stream_handle
+    rng = np.random.default_rng(900 + blocks + 17)
+    for name, lens, ncol in cases(rng):
+        say(f"adjoint blocks={blocks} {kind} {name}")
+        rp, col = random_csr(rng, len(lens), ncol, lens=lens)
+        v = rng.standard_normal(int(rp[-1]))
+        if kind == "f64":
+            vals, g = v, rng.uniform(-1, 1, len(lens))
+        else:
+            v32 = v.astype(np.float32)
+            vals = v32 if kind == "f32" else gen.to_bf16(v32)
+            g = rng.uniform(-1, 1, len(lens)).astype(np.float32)
+        d_rp, d_col = dev(rp, device), dev(col, device)
+        op = MerbitOperator(d_rp, d_col, ncol)
+        d_val, d_g = dev(vals, device), dev(g, device)
+        kcode = {"f64": 0, "f32": 1, "bf16": 2}[kind]
+        gdt = torch.float64 if kind == "f64" else torch.float32
+
+        def dx_now():
+            out = torch.empty(ncol, dtype=gdt, device=device)
+            check(lib().mbx_adjoint_dx(op.adjoint_handle, kcode, d_val.data_ptr(), 1.0,
+                                       d_g.data_ptr(), 0.0, out.data_ptr(), _stream_handle(None)),
+                  "mbx_adjoint_dx")
+            return out.cpu().numpy()
+        plain = dx_now()
+        check(lib().mbx_adjoint_set_blocks(op.adjoint_handle, blocks, _stream_handle(None)),
+              "mbx_adjoint_set_blocks")
+        blocked = dx_now()
+        rpT, colT, valT = transpose_csr(rp, col, ncol, vals)
+        want, absr = orc.spmv_accurate(rpT, colT, valT, g)
+        assert cond_err(blocked, want, absr) <= (F64_TOL if kind == "f64" else MIXED_TOL), name
+        check(lib().mbx_adjoint_set_blocks(op.adjoint_handle, 0, _stream_handle(None)),
+              "mbx_adjoint_set_blocks")
+        assert np.array_equal(dx_now().view(np.uint8), plain.view(np.uint8)), name
+        op.close()
