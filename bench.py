@@ -632,9 +632,12 @@ def north_star_summary(extras):
     bt = extras.get("batch_uneven")
     if bt:
         ns["batch_vs_separate_launches"] = bt["speedup_vs_separate_launches"]
-    ad = extras.get("adjoint_graph", {}).get("forward_column_blocked")
+    ag = extras.get("adjoint_graph", {})
+    ad = ag.get("forward_column_blocked")
     if ad:
         ns["graph_forward_backward_colblocked_vs_csr"] = ad["step_speedup"]
+    if "dx_blocked_transpose" in ag:
+        ns["graph_dx_csr_values_blocked_transpose_vs_csr"] = ag["dx_blocked_transpose"]["speedup"]
     return ns
 
 
