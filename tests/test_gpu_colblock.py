@@ -304,3 +304,25 @@ def test_csr_adjoint_through_blocked_transpose(orc, device, kind, blocks):
               "mbx_adjoint_set_blocks")
         assert np.array_equal(dx_now().view(np.uint8), plain.view(np.uint8)), name
         op.close()
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_autograd_with_blocked_transpose(device, dtype):
+    """MerbitOperator(adjoint_blocks=3): the autograd x gradient through the
+    blocked A^T equals the single-product one within the bars."""
+    from merbit_amd import workloads
+    from merbit_amd.autograd import MerbitOperator, spmv
+    say(f"autograd blocked transpose {dtype}")
+    A, x, g = workloads.graph(150_000)
+    d_rp, d_col = dev(A.row_ptr, device), dev(A.col, device)
+    val = dev(A.val, device).to(dtype)
+    gg = dev(g, device).to(dtype)
+    grads = []
+    for blocks in (0, 3):
+        op = MerbitOperator(d_rp, d_col, A.ncol, adjoint_blocks=blocks)
+        xx = dev(x, device).to(dtype).requires_grad_(True)
+        spmv(op, val, xx).backward(gg)
+        grads.append(xx.grad.detach().double())
+        op.close()
+    tol = 1e-12 if dtype == torch.float64 else 1e-6
+    assert float((grads[1] - grads[0]).norm() / grads[0].norm()) <= tol
