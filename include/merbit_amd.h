@@ -289,7 +289,8 @@ int mbx_autotune_batch(const mbx_batch_item* items, int32_t count, int32_t value
  *       value array: nnz, the block padding and 64 spare slots after the
  *       last block) and the nblocks+1 column bounds.
  *   mbx_colblock_values: moves values of the given kind from CSR order into
- *       a blocked array of value_slots entries (padding slots set to 0).
+ *       a blocked array of value_slots entries (padding slots set to 0);
+ *       mbx_colblock_values_to_csr moves them back (nnz entries).
  *   mbx_colblock_spmv: the chained product on blocked values; x and y as
  *       for mbx_spmv_f64 (MBX_VALUE_F64) or mbx_spmv_f32/bf16 (FP32 x, y).
  *   mbx_colblock_sddmm: dval = g[row]*x[col] in block-major order (FP64 or
@@ -320,6 +321,8 @@ int mbx_colblock_info(const mbx_colblock* cb, int32_t* nblocks, int64_t* value_s
                       int32_t* col_bounds);
 int mbx_colblock_values(const mbx_colblock* cb, int32_t value_kind, const void* d_val,
                         void* d_val_blocked, void* stream);
+int mbx_colblock_values_to_csr(const mbx_colblock* cb, int32_t value_kind,
+                               const void* d_val_blocked, void* d_val, void* stream);
 int mbx_colblock_spmv(const mbx_colblock* cb, int32_t value_kind, const void* d_val_blocked,
                       double alpha, const void* d_x, double beta, void* d_y, void* stream);
 int mbx_colblock_sddmm(const mbx_colblock* cb, int32_t value_kind, const void* d_g,

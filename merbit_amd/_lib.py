@@ -28,7 +28,7 @@ ABI_SYMBOLS = (
     "mbx_sddmm", "mbx_ingest_ref", "mbx_plan_stats",
     "mbx_pagerank_workspace_bytes", "mbx_pagerank_f64", "mbx_pagerank_graph_create",
     "mbx_colblock_create", "mbx_colblock_destroy", "mbx_colblock_info", "mbx_colblock_values",
-    "mbx_colblock_spmv", "mbx_colblock_sddmm", "mbx_colblock_set_stream",
+    "mbx_colblock_spmv", "mbx_colblock_sddmm", "mbx_colblock_values_to_csr", "mbx_colblock_set_stream",
     "mbx_colblock_set_residency", "mbx_colblock_adjoint_prepare", "mbx_colblock_dx",
     "mbx_status_str",
 )
@@ -137,6 +137,7 @@ def lib() -> ctypes.CDLL:
     L.mbx_colblock_values.argtypes = [vp, i32, vp, vp, vp]
     L.mbx_colblock_spmv.argtypes = [vp, i32, vp, f64, vp, f64, vp, vp]
     L.mbx_colblock_sddmm.argtypes = [vp, i32, vp, vp, vp, vp]
+    L.mbx_colblock_values_to_csr.argtypes = [vp, i32, vp, vp, vp]
     L.mbx_colblock_set_stream.argtypes = [vp, i32]
     L.mbx_colblock_set_residency.argtypes = [vp, i32]
     L.mbx_colblock_adjoint_prepare.argtypes = [vp, vp, vp, i32, i32, vp, ctypes.POINTER(i32)]

@@ -26,10 +26,17 @@ _KIND = {torch.float64: MBX_VALUE_F64, torch.float32: MBX_VALUE_F32}
 
 class MerbitOperator:
     def __init__(self, row_ptr: torch.Tensor, col: torch.Tensor, ncol: int,
-                 items_per_thread: int = 8, adjoint_blocks: int = 0):
+                 items_per_thread: int = 8, adjoint_blocks: int = 0, blocks: int = 0):
         """adjoint_blocks: 0 for dx by one merge-path product on A^T; -1 or
         1..16 for dx through a column-blocked A^T (mbx_adjoint_set_blocks; -1
-        picks the count), for graphs whose g outgrows one XCD's L2."""
+        picks the count), for graphs whose g outgrows one XCD's L2.
+        blocks: -1 or 1..16 runs the forward product on a column-blocked
+        form too (values stay in CSR order and are moved into block order per
+        forward call), and sets adjoint_blocks to -1 unless it is given.  dval
+        stays the CSR SDDMM: moving a blocked dval back to CSR order costs
+        more than the blocked SDDMM saves."""
+        if blocks and not adjoint_blocks:
+            adjoint_blocks = -1
         self.adjoint_blocks = adjoint_blocks
         self.row_ptr = row_ptr
         self.col = col
@@ -39,6 +46,12 @@ class MerbitOperator:
         self.items_per_thread = items_per_thread
         self.plan = MerbitPlan(row_ptr, ncol, self.nnz, items_per_thread)
         self._adj = None
+        self._cb = None
+        if blocks:
+            from .spmv import ColumnBlockedCsr
+            self._cb = ColumnBlockedCsr(row_ptr, col, ncol, 0 if blocks < 0 else blocks,
+                                        items_per_thread)
+        self._vb = None
 
     @property
     def adjoint_handle(self) -> ctypes.c_void_p:
@@ -65,6 +78,11 @@ class MerbitOperator:
         return cp, ri[:self.nnz], pm[:self.nnz]
 
     def forward(self, val: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
+        if self._cb is not None:
+            if self._vb is None or self._vb.dtype != val.dtype:
+                self._vb = torch.empty(max(self._cb.value_slots, 1), dtype=val.dtype,
+                                       device=val.device)
+            return self._cb.spmv(self._cb.values(val, out=self._vb), x)
         return self.plan.spmv(self.row_ptr, self.col, val, x)
 
     def adjoint_x(self, val: torch.Tensor, g: torch.Tensor) -> torch.Tensor:
@@ -88,6 +106,9 @@ class MerbitOperator:
         if self._adj is not None:
             lib().mbx_adjoint_destroy(self._adj)
             self._adj = None
+        if self._cb is not None:
+            self._cb.close()
+            self._cb = None
         self.plan.close()
 
     def __del__(self):

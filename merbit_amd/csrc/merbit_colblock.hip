@@ -133,6 +133,16 @@ __global__ void cb_values_kernel(const int32_t* __restrict__ perm, long long slo
   out[i] = k >= 0 ? val[k] : T(0);
 }
 
+// Block-major values back to CSR order: out[perm[i]] = in[i] on real slots.
+template <class T>
+__global__ void cb_to_csr_kernel(const int32_t* __restrict__ perm, long long slots,
+                                 const T* __restrict__ in, T* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= slots) return;
+  const int k = perm[i];
+  if (k >= 0) out[k] = in[i];
+}
+
 // Inverse of the slot -> CSR position map over the real slots.
 __global__ void cb_inverse_kernel(const int32_t* __restrict__ perm, long long slots,
                                   int32_t* __restrict__ inv) {
@@ -484,6 +494,26 @@ int mbx_colblock_values(const mbx_colblock* cb, int32_t value_kind, const void* 
   else
     cb_values_kernel<<<grid, 256, 0, s>>>(cb->perm, slots, (const uint16_t*)d_val,
                                           (uint16_t*)d_val_blocked);
+  return hipGetLastError() == hipSuccess ? MBX_OK : MBX_EHIP;
+}
+
+int mbx_colblock_values_to_csr(const mbx_colblock* cb, int32_t value_kind,
+                               const void* d_val_blocked, void* d_val, void* stream) {
+  if (!cb) return MBX_EINVAL;
+  const int vb = value_bytes(value_kind);
+  if (vb == 0 || !d_val_blocked || (cb->nnz > 0 && !d_val)) return MBX_EINVAL;
+  hipStream_t s = (hipStream_t)stream;
+  const long long slots = cb->slots;
+  const int grid = (int)((slots + 255) / 256);
+  if (vb == 8)
+    cb_to_csr_kernel<<<grid, 256, 0, s>>>(cb->perm, slots, (const uint64_t*)d_val_blocked,
+                                          (uint64_t*)d_val);
+  else if (vb == 4)
+    cb_to_csr_kernel<<<grid, 256, 0, s>>>(cb->perm, slots, (const uint32_t*)d_val_blocked,
+                                          (uint32_t*)d_val);
+  else
+    cb_to_csr_kernel<<<grid, 256, 0, s>>>(cb->perm, slots, (const uint16_t*)d_val_blocked,
+                                          (uint16_t*)d_val);
   return hipGetLastError() == hipSuccess ? MBX_OK : MBX_EHIP;
 }
 

@@ -330,6 +330,24 @@ class ColumnBlockedCsr:
               "mbx_colblock_values")
         return out
 
+    def values_to_csr(self, val_blocked: torch.Tensor, out: torch.Tensor | None = None,
+                      stream=None) -> torch.Tensor:
+        """Block-major values moved back into CSR order (nnz entries)."""
+        kind = _VALUE_KINDS.get(val_blocked.dtype)
+        if kind is None:
+            raise TypeError(f"unsupported value dtype {val_blocked.dtype}")
+        if val_blocked.numel() < self.value_slots:
+            raise ValueError("val_blocked is shorter than the blocked pattern")
+        if out is None:
+            out = torch.empty(max(self.nnz, 1), dtype=val_blocked.dtype,
+                              device=val_blocked.device)
+        check(lib().mbx_colblock_values_to_csr(self.handle, _KIND_CODES[kind],
+                                               _dptr(val_blocked, val_blocked.dtype, "val_blocked"),
+                                               _dptr(out, out.dtype, "out"),
+                                               _stream_handle(stream)),
+              "mbx_colblock_values_to_csr")
+        return out[:self.nnz] if out.numel() != self.nnz else out
+
     def spmv(self, val_blocked: torch.Tensor, x: torch.Tensor, y: torch.Tensor | None = None,
              alpha: float = 1.0, beta: float = 0.0, stream=None) -> torch.Tensor:
         """y = alpha*A*x + beta*y on values in block-major order."""

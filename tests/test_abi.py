@@ -64,6 +64,7 @@ def test_host_side_argument_checks():
     assert L.mbx_colblock_values(None, 0, None, None, None) == -1
     assert L.mbx_colblock_spmv(None, 0, None, 1.0, None, 0.0, None, None) == -1
     assert L.mbx_colblock_sddmm(None, 0, None, None, None, None) == -1
+    assert L.mbx_colblock_values_to_csr(None, 0, None, None, None) == -1
     assert L.mbx_plan_compress_cols(None, None, None) == -1
     assert L.mbx_plan_set_accumulation(None, 0) == -1
     assert L.mbx_adjoint_set_blocks(None, 0, None) == -1
