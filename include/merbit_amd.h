@@ -309,7 +309,9 @@ int mbx_autotune_batch(const mbx_batch_item* items, int32_t count, int32_t value
  *   mbx_colblock_set_stream / _set_residency: the stream cache policy and
  *       tiles per CU of every block's product (and of the A^T form's), as
  *       mbx_plan_set_stream / mbx_plan_set_residency; MBX_STREAM_AUTO (the
- *       default) judges the bytes of the whole chained product.  Calls on one
+ *       default) judges the bytes of the whole chained product, and the
+ *       residency defaults to 4 tiles per CU (the gather-bound block
+ *       products' best).  Calls on one
  *       column-blocked form must not overlap (they share its plans' carry
  *       slots and its dx value buffer). */
 typedef struct mbx_colblock mbx_colblock;
