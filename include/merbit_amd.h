@@ -46,8 +46,9 @@ typedef struct mbx_plan mbx_plan;
  * merge items plus the per-tile carry slots.  items_per_thread plays the
  * reference's SIGMA (instantiated: 2, 4, 6, 8, 10, 12, 14, 16) and wg_size
  * its BLOCK (256); 0 selects the defaults (8, 256).
- * Synchronises `stream` once: the largest number of rows ending in one tile
- * is read back to size the tile kernel's LDS. */
+ * Synchronises `stream` once: it reads back whether any row crosses a tile
+ * boundary (the carry fixup launch is skipped when none does) and the ends
+ * row_ptr[0], row_ptr[nrow]; MBX_EINVAL unless they are 0 and nnz. */
 int mbx_plan_create(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol,
                     int32_t nnz, int32_t items_per_thread, int32_t wg_size,
                     void* stream, mbx_plan** out);
@@ -284,7 +285,9 @@ int mbx_autotune_batch(const mbx_batch_item* items, int32_t count, int32_t value
  *   mbx_colblock_create: builds the block-major pattern (block after block,
  *       rows in order, a row's entries in CSR order; block starts padded to
  *       4 slots) and one plan per block; nblocks 0..16 (0: slices of at most
- *       3.5 MiB of 4-byte x), clamped to ncol.  Synchronises `stream`.
+ *       3.5 MiB of 4-byte x, the FP32/BF16 value kinds; FP64 x is twice as
+ *       long, so FP64 callers pass twice that count), clamped to ncol.
+ *       Synchronises `stream`.
  *   mbx_colblock_info: block count, value slots (the length of a blocked
  *       value array: nnz, the block padding and 64 spare slots after the
  *       last block) and the nblocks+1 column bounds.
@@ -299,7 +302,8 @@ int mbx_autotune_batch(const mbx_batch_item* items, int32_t count, int32_t value
  *   mbx_colblock_adjoint_prepare: builds the column-blocked form of A^T
  *       (blocks over A's rows, nblocks 0..16 as above, reported in
  *       *nblocks_out when not NULL) from the same CSR pattern the form was
- *       made from (MBX_EINVAL when it is not), and a two-pass bucketed
+ *       made from (checked: MBX_EINVAL when the row lengths or the column
+ *       of any entry differ, or the pattern is malformed), and a two-pass bucketed
  *       permutation from this form's slots to that form's.  Synchronises
  *       `stream`.
  *   mbx_colblock_dx: dx = alpha*A^T*g + beta*dx from values in this form's
@@ -381,6 +385,36 @@ int mbx_sddmm(const mbx_plan* plan, int32_t value_kind, const int32_t* d_row_ptr
 int mbx_ingest_ref(int32_t n, int32_t nnz, const int32_t* d_row_ind, const int32_t* d_col_ind,
                    int32_t value_kind, int32_t* d_row_ptr_t, int32_t* d_col_t, void* d_val_t,
                    void* stream);
+
+/* Structural check of a device-resident CSR pattern (no reference
+ * counterpart: the reference builds its CSR itself and trusts it,
+ * src/class.cu:143-177).  The products read x through a bounded buffer
+ * resource, so a column index outside [0, ncol) reads 0 there instead of
+ * faulting; call this once on untrusted input to have such a pattern
+ * reported instead.  Counts:
+ *   row_ptr_errors  rows r with row_ptr[r] > row_ptr[r+1], or an end outside
+ *                   [0, nnz], plus row_ptr[0] != 0;
+ *   col_errors      entries whose column is outside [0, ncol)
+ *                   (first_bad_col: the first such position, -1 if none);
+ *   unsorted_pairs  adjacent entries of one row with descending columns (the
+ *                   reference's layout has ascending columns; the products
+ *                   do not need them, so this alone is not an error);
+ *   nnz_mismatch    1 when row_ptr[nrow] != nnz.
+ * Returns MBX_EINVAL when any count but unsorted_pairs is non-zero (the
+ * report is filled), MBX_OK otherwise.  Allocates nnz + 1 bytes of
+ * temporaries and synchronises `stream`.  mbx_adjoint_create and
+ * mbx_colblock_adjoint_prepare run the same check and refuse such input with
+ * MBX_EINVAL; mbx_plan_create refuses row_ptr[0] != 0 and
+ * row_ptr[nrow] != nnz. */
+typedef struct mbx_csr_report {
+  int64_t row_ptr_errors;
+  int64_t col_errors;
+  int64_t unsorted_pairs;
+  int32_t nnz_mismatch;
+  int32_t first_bad_col;
+} mbx_csr_report;
+int mbx_csr_validate(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow,
+                     int32_t ncol, int32_t nnz, void* stream, mbx_csr_report* out);
 
 /* Status text, like the CSR5 error codes (include/csr5/detail/common.h:13-22). */
 const char* mbx_status_str(int status);

@@ -30,7 +30,7 @@ ABI_SYMBOLS = (
     "mbx_colblock_create", "mbx_colblock_destroy", "mbx_colblock_info", "mbx_colblock_values",
     "mbx_colblock_spmv", "mbx_colblock_sddmm", "mbx_colblock_values_to_csr", "mbx_colblock_set_stream",
     "mbx_colblock_set_residency", "mbx_colblock_adjoint_prepare", "mbx_colblock_dx",
-    "mbx_status_str",
+    "mbx_csr_validate", "mbx_status_str",
 )
 GEN_SYMBOLS = (
     "mbg_fill_uniform_f64", "mbg_fill_uniform_f32", "mbg_fill_normal_f64",
@@ -65,6 +65,13 @@ class PatternStats(ctypes.Structure):
                 ("tile_items", ctypes.c_int32), ("single_row_tiles", ctypes.c_int32),
                 ("split_tiles", ctypes.c_int32), ("max_rows_per_tile", ctypes.c_int32),
                 ("long_rows", ctypes.c_int32)]
+
+
+class CsrReport(ctypes.Structure):
+    """mbx_csr_report of include/merbit_amd.h."""
+    _fields_ = [("row_ptr_errors", ctypes.c_int64), ("col_errors", ctypes.c_int64),
+                ("unsorted_pairs", ctypes.c_int64), ("nnz_mismatch", ctypes.c_int32),
+                ("first_bad_col", ctypes.c_int32)]
 
 
 _lib = None
@@ -156,6 +163,7 @@ def lib() -> ctypes.CDLL:
     L.mbx_pagerank_f64.argtypes = [vp, vp, vp, vp, f64, i32, vp, vp, vp, ctypes.c_size_t, vp]
     L.mbx_pagerank_graph_create.argtypes = [vp, vp, vp, vp, f64, i32, vp, vp, vp,
                                             ctypes.c_size_t, ctypes.POINTER(vp)]
+    L.mbx_csr_validate.argtypes = [vp, vp, i32, i32, i32, vp, ctypes.POINTER(CsrReport)]
     L.mbx_status_str.argtypes = [ctypes.c_int]
     L.mbx_status_str.restype = ctypes.c_char_p
     L.mbg_fill_uniform_f64.argtypes = [vp, i64, f64, f64, u64]

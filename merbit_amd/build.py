@@ -46,6 +46,7 @@ LIB_SOURCES = [
     os.path.join(CSRC, "merbit_ingest.hip"),
     os.path.join(CSRC, "merbit_pagerank.hip"),
     os.path.join(CSRC, "merbit_colblock.hip"),
+    os.path.join(CSRC, "merbit_validate.hip"),
     os.path.join(CSRC, "merbit_gen.cpp"),
     os.path.join(CSRC, "merbit_host.cpp"),
 ]
@@ -84,16 +85,21 @@ def build_library(verbose: bool = False) -> str:
     inc = os.path.join(ROOT, "include")
     obj_dir = os.path.join(ROOT, "_build")
     os.makedirs(obj_dir, exist_ok=True)
-    objs = []
+    objs, cmds = [], []
     # One hipcc call per source: mixing -x hip and -x c++ inputs in one call
-    # drops the requested --offload-arch.
+    # drops the requested --offload-arch.  The calls run in parallel.
     for src in LIB_SOURCES:
         # every .hip is a HIP translation unit; .cpp files are host C++
         obj = os.path.join(obj_dir, os.path.basename(src) + ".o")
         lang = "hip" if src.endswith(".hip") else "c++"
         flags = HIP_FLAGS if lang == "hip" else ["-O3", "-std=c++17", "-fPIC", "-Wall"]
-        _run([HIPCC, *flags, f"-I{inc}", "-x", lang, "-c", src, "-o", obj], verbose)
+        cmds.append([HIPCC, *flags, f"-I{inc}", "-x", lang, "-c", src, "-o", obj])
         objs.append(obj)
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(jobs) as ex:
+        for f in [ex.submit(_run, c, verbose) for c in cmds]:
+            f.result()
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", *objs, "-o", LIB + ".tmp"], verbose)
     targets = _fatbin_targets(LIB + ".tmp")
     if not any(t.endswith("--" + ARCH) for t in targets):

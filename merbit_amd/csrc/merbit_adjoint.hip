@@ -34,6 +34,7 @@
 
 #include "merbit_amd.h"
 #include "merbit_internal.hpp"
+#include "merbit_tile.hpp"
 
 using mbx_detail::lds_merge_search;
 using mbx_detail::xcd_tile;
@@ -204,12 +205,15 @@ __global__ void perm_final_kernel(const int32_t* __restrict__ order1,
   lidx[i] = sidx[perm[i]];
 }
 
-// dval[k] = g[row(k)] * x[col(k)] for the nonzeros of one forward tile.
+// dval[k] = g[row(k)] * x[col(k)] for the nonzeros of one forward tile.  x
+// is gathered through a buffer resource over its ncol entries when it spans
+// at most 4 GiB (x_rsrc), so a column index outside [0, ncol) reads 0, as in
+// the forward tile kernel, instead of faulting.
 template <int WG, int IPT, class T>
 __global__ __launch_bounds__(WG) void sddmm_tile_kernel(
     const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
     const T* __restrict__ g, const T* __restrict__ x, T* __restrict__ dval,
-    const int4* __restrict__ coord, int ntiles) {
+    const int4* __restrict__ coord, int ntiles, int ncol, int x_rsrc) {
   constexpr int TI = WG * IPT;
   __shared__ int s_re[TI + 1];
   __shared__ int s_row[TI];
@@ -262,14 +266,29 @@ __global__ __launch_bounds__(WG) void sddmm_tile_kernel(
   }
   __syncthreads();
   T xv[IPT], gv[IPT];
+  if (x_rsrc) {  // uniform
+    const __amdgpu_buffer_rsrc_t rx =
+        mbx_detail::buffer_rsrc(x, (uint32_t)ncol * (uint32_t)sizeof(T));
 #pragma unroll
-  for (int j = 0; j < IPT; ++j) {
-    const int i = tid + j * WG;
-    xv[j] = T(0);
-    gv[j] = T(0);
-    if (i < nnz_t) {
-      xv[j] = x[ci[j]];
-      gv[j] = g[r0 + s_row[i]];
+    for (int j = 0; j < IPT; ++j) {
+      const int i = tid + j * WG;
+      xv[j] = T(0);
+      gv[j] = T(0);
+      if (i < nnz_t) {
+        mbx_detail::buffer_gather(rx, ci[j], xv[j]);
+        gv[j] = g[r0 + s_row[i]];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const int i = tid + j * WG;
+      xv[j] = T(0);
+      gv[j] = T(0);
+      if (i < nnz_t) {
+        xv[j] = x[ci[j]];
+        gv[j] = g[r0 + s_row[i]];
+      }
     }
   }
 #pragma unroll
@@ -432,6 +451,13 @@ int mbx_adjoint_create(const int32_t* d_row_ptr, const int32_t* d_col, int32_t n
     return MBX_EINVAL;
   if (nnz > 0 && (nrow == 0 || ncol == 0)) return MBX_EINVAL;
   hipStream_t s = (hipStream_t)stream;
+  // The transposition indexes by column (radix-sort key width, column
+  // pointers): refuse columns outside [0, ncol) and a malformed row_ptr
+  // before any of it runs.
+  {
+    const int cst = mbx_detail::csr_check(d_row_ptr, d_col, nrow, ncol, nnz, s, nullptr);
+    if (cst != MBX_OK) return cst;
+  }
   mbx_adjoint* a = new (std::nothrow) mbx_adjoint();
   if (!a) return MBX_ENOMEM;
   a->nrow = nrow;
@@ -617,17 +643,18 @@ int mbx_sddmm(const mbx_plan* p, int32_t value_kind, const int32_t* d_row_ptr,
   if (p->ntiles == 0) return MBX_OK;
   hipStream_t s = (hipStream_t)stream;
   const bool f64 = value_kind == MBX_VALUE_F64;
+  const int x_rsrc = (uint64_t)p->ncol * (f64 ? 8u : 4u) <= 0xFFFFFFFFull;
   switch (p->ipt) {
 #define MBX_CASE(I)                                                                         \
   case I:                                                                                   \
     if (f64)                                                                                \
       sddmm_tile_kernel<256, I, double><<<p->ntiles, 256, 0, s>>>(                          \
           d_row_ptr, d_col, (const double*)d_g, (const double*)d_x, (double*)d_dval,        \
-          p->d_coord, p->ntiles);                                                           \
+          p->d_coord, p->ntiles, p->ncol, x_rsrc);                                          \
     else                                                                                    \
       sddmm_tile_kernel<256, I, float><<<p->ntiles, 256, 0, s>>>(                           \
           d_row_ptr, d_col, (const float*)d_g, (const float*)d_x, (float*)d_dval,           \
-          p->d_coord, p->ntiles);                                                           \
+          p->d_coord, p->ntiles, p->ncol, x_rsrc);                                          \
     break;
     MBX_CASE(2) MBX_CASE(4) MBX_CASE(6) MBX_CASE(8) MBX_CASE(10) MBX_CASE(12) MBX_CASE(14)
     MBX_CASE(16)

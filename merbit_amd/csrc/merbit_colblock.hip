@@ -170,6 +170,34 @@ __global__ void cb_real_slot_kernel(const int32_t* __restrict__ tperm, long long
   if (v < 0) atomicOr(bad, 1);
 }
 
+// Whether (rp, col) is the pattern the form was built from: every row has
+// the length of its blocks' rows together (with rp[0] == 0, so the row
+// pointers agree) ...
+__global__ void cb_same_rows_kernel(const int32_t* __restrict__ rp,
+                                    const int32_t* __restrict__ brp, int nrow, int nb,
+                                    int stride, int* __restrict__ bad) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrow) return;
+  int len = 0;
+  for (int j = 0; j < nb; ++j)
+    len += brp[(long long)j * stride + r + 1] - brp[(long long)j * stride + r];
+  if (rp[r + 1] - rp[r] != len || (r == 0 && rp[0] != 0)) atomicOr(bad, 1);
+}
+
+// ... and every real slot i of block j holds the column col[perm[i]] - lo_j.
+__global__ void cb_same_cols_kernel(const int32_t* __restrict__ col,
+                                    const int32_t* __restrict__ perm,
+                                    const int32_t* __restrict__ bcol, long long slots, int nb,
+                                    Bounds lo, Bounds base, int* __restrict__ bad) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= slots) return;
+  const int k = perm[i];
+  if (k < 0) return;
+  int j = 0;
+  while (j + 1 < nb && i >= base.v[j + 1]) ++j;
+  if (col[k] != bcol[i] + lo.v[j]) atomicOr(bad, 1);
+}
+
 // out[dst[i]] = src[i]: the spare and padding slots of both forms, paired in
 // order.
 __global__ void cb_pair_kernel(const int32_t* __restrict__ dst, const int32_t* __restrict__ src,
@@ -179,6 +207,35 @@ __global__ void cb_pair_kernel(const int32_t* __restrict__ dst, const int32_t* _
 }
 
 void destroy(mbx_colblock* cb);
+
+// MBX_EINVAL unless (rp, col) is the pattern cb was built from (the same
+// nnz is the caller's check); synchronises s.
+int same_pattern(const mbx_colblock* cb, const int32_t* rp, const int32_t* col, hipStream_t s) {
+  int* d_bad = nullptr;
+  if (hipMalloc((void**)&d_bad, sizeof(int)) != hipSuccess) return MBX_ENOMEM;
+  int st = MBX_OK, bad = 0;
+  if (hipMemsetAsync(d_bad, 0, sizeof(int), s) != hipSuccess) st = MBX_EHIP;
+  if (st == MBX_OK && cb->nrow > 0)
+    cb_same_rows_kernel<<<(cb->nrow + 255) / 256, 256, 0, s>>>(rp, cb->row_ptr, cb->nrow, cb->nb,
+                                                               cb->stride, d_bad);
+  if (st == MBX_OK && cb->nnz > 0) {
+    Bounds lo, base;
+    for (int j = 0; j <= kMaxBlocks; ++j) {
+      lo.v[j] = j <= cb->nb ? cb->lo[j] : 0;
+      base.v[j] = j <= cb->nb ? (int32_t)cb->base[j] : 0;
+    }
+    cb_same_cols_kernel<<<(int)((cb->slots + 255) / 256), 256, 0, s>>>(
+        col, cb->perm, cb->col, cb->slots, cb->nb, lo, base, d_bad);
+  }
+  if (st == MBX_OK &&
+      (hipGetLastError() != hipSuccess ||
+       hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+       hipStreamSynchronize(s) != hipSuccess))
+    st = MBX_EHIP;
+  (void)hipFree(d_bad);
+  if (st == MBX_OK && bad) st = MBX_EINVAL;
+  return st;
+}
 
 void drop_adjoint(mbx_colblock* cb) {
   if (cb->t) destroy(cb->t);
@@ -377,8 +434,10 @@ int mbx_colblock_adjoint_prepare(mbx_colblock* cb, const int32_t* d_row_ptr,
   if (cb->t) (void)hipDeviceSynchronize();  // a dx on any stream may still read them
   drop_adjoint(cb);
   mbx_adjoint* adj = nullptr;
+  // validates the pattern itself (row_ptr from 0 to nnz, columns in range)
   int st = mbx_adjoint_create(d_row_ptr, d_col, cb->nrow, cb->ncol, cb->nnz, items_per_thread,
                               stream, &adj);
+  if (st == MBX_OK) st = same_pattern(cb, d_row_ptr, d_col, s);
   if (st == MBX_OK)
     st = mbx_colblock_create(adj->col_ptr, adj->row_idx, cb->ncol, cb->nrow, cb->nnz, nblocks,
                              items_per_thread, stream, &cb->t);

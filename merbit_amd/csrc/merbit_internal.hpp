@@ -72,6 +72,13 @@ void perm_free(mbx_adjoint* a);
 std::vector<int32_t> colblock_padding(const mbx_colblock* cb);
 const int32_t* colblock_slot_map(const mbx_colblock* cb);
 
+// mbx_csr_validate's check (merbit_validate.hip), for entry points that must
+// refuse a malformed pattern: MBX_EINVAL when row_ptr is not a valid
+// non-decreasing pointer from 0 to nnz or a column lies outside [0, ncol).
+// Allocates temporaries and synchronises s.
+int csr_check(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow, int32_t ncol,
+              int32_t nnz, hipStream_t s, mbx_csr_report* out);
+
 constexpr int kXcds = 8;
 
 // Workgroups are dispatched round-robin over the 8 XCDs; give XCD j the

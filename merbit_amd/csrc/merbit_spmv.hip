@@ -55,6 +55,16 @@ __global__ void plan_stats_kernel(const int4* __restrict__ coord, int ntiles,
   if (t < ntiles) mbx_detail::plan_stats_update(coord, t, out);
 }
 
+// The ends of row_ptr (out[2] = row_ptr[0], out[3] = row_ptr[nrow]), read
+// back by plan creation to refuse a row_ptr that does not run from 0 to nnz.
+__global__ void row_ptr_ends_kernel(const int32_t* __restrict__ row_ptr, int nrow,
+                                    int* __restrict__ out) {
+  if (threadIdx.x == 0) {
+    out[2] = row_ptr[0];
+    out[3] = row_ptr[nrow];
+  }
+}
+
 __global__ void partition_kernel(const int32_t* __restrict__ row_ptr, int nrow, int nnz,
                                  int tile_items, int ntiles, int4* __restrict__ coord) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -239,6 +249,7 @@ int mbx_plan_create(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol,
   if (!out) return MBX_EINVAL;
   *out = nullptr;
   if (nrow < 0 || ncol < 0 || nnz < 0 || (nrow > 0 && !d_row_ptr)) return MBX_EINVAL;
+  if (nrow == 0 && nnz > 0) return MBX_EINVAL;
   const int ipt = items_per_thread ? items_per_thread : 8;
   const int wg = wg_size ? wg_size : 256;
   if (!supported_shape(wg, ipt)) return MBX_EUNSUPPORTED;
@@ -277,22 +288,29 @@ int mbx_plan_create(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol,
   const int grid = (p->ntiles + 1 + 255) / 256;
   partition_kernel<<<grid, 256, 0, st>>>(d_row_ptr, nrow, nnz, p->tile_items, p->ntiles,
                                          p->d_coord);
-  // the largest rows-per-tile count sizes the LDS of every launch: read it
-  // back once (plan creation synchronises the stream)
-  int* d_stats = reinterpret_cast<int*>(p->d_carry);  // carries are unused until a product
-  if (hipMemsetAsync(d_stats, 0, 2 * sizeof(int), st) != hipSuccess) {
+  // Read back once (plan creation synchronises the stream): whether any row
+  // crosses a tile boundary (products skip the fixup launch when none does),
+  // the most rows ending in one tile, and the ends of row_ptr.  The carry
+  // slots (at least 4 ints) are unused until a product.
+  int* d_stats = reinterpret_cast<int*>(p->d_carry);
+  if (hipMemsetAsync(d_stats, 0, 4 * sizeof(int), st) != hipSuccess) {
     mbx_plan_destroy(p);
     return MBX_EHIP;
   }
   if (p->ntiles > 0)
     plan_stats_kernel<<<(p->ntiles + 255) / 256, 256, 0, st>>>(p->d_coord, p->ntiles, d_stats);
-  int h_stats[2] = {0, 0};
+  if (nrow > 0) row_ptr_ends_kernel<<<1, 64, 0, st>>>(d_row_ptr, nrow, d_stats);
+  int h_stats[4] = {0, 0, 0, 0};
   if (hipGetLastError() != hipSuccess ||
-      hipMemcpyAsync(h_stats, d_stats, 2 * sizeof(int), hipMemcpyDeviceToHost, st) !=
+      hipMemcpyAsync(h_stats, d_stats, 4 * sizeof(int), hipMemcpyDeviceToHost, st) !=
           hipSuccess ||
       hipStreamSynchronize(st) != hipSuccess) {
     mbx_plan_destroy(p);
     return MBX_EHIP;
+  }
+  if (nrow > 0 && (h_stats[2] != 0 || h_stats[3] != nnz)) {
+    mbx_plan_destroy(p);
+    return MBX_EINVAL;
   }
   p->max_rows = h_stats[0];
   p->split_rows = h_stats[1];

@@ -41,6 +41,25 @@ def _dptr(t: torch.Tensor, dtype, name: str) -> int:
     return t.data_ptr()
 
 
+def validate_csr(row_ptr: torch.Tensor, col: torch.Tensor, ncol: int, stream=None) -> dict:
+    """Structural check of a device CSR pattern (mbx_csr_validate): counts of
+    row_ptr errors, out-of-range columns (with the first such position),
+    descending column pairs within rows and whether row_ptr[nrow] != nnz,
+    plus ``ok`` (no error but possibly unsorted columns)."""
+    from ._lib import CsrReport
+    nrow = int(row_ptr.numel()) - 1
+    nnz = int(col.numel())
+    rep = CsrReport()
+    st = lib().mbx_csr_validate(_dptr(row_ptr, torch.int32, "row_ptr"),
+                                _dptr(col, torch.int32, "col") if nnz else None, nrow,
+                                int(ncol), nnz, _stream_handle(stream), ctypes.byref(rep))
+    if st not in (0, -1):
+        check(st, "mbx_csr_validate")
+    out = {name: getattr(rep, name) for name, _ in CsrReport._fields_}
+    out["ok"] = st == 0
+    return out
+
+
 class MerbitPlan:
     """Merge-path partition of one CSR pattern (``TILE`` of the reference)."""
 
@@ -341,6 +360,8 @@ class ColumnBlockedCsr:
         if out is None:
             out = torch.empty(max(self.nnz, 1), dtype=val_blocked.dtype,
                               device=val_blocked.device)
+        if out.numel() < self.nnz or out.dtype != val_blocked.dtype:
+            raise ValueError("out is too short or of another dtype")
         check(lib().mbx_colblock_values_to_csr(self.handle, _KIND_CODES[kind],
                                                _dptr(val_blocked, val_blocked.dtype, "val_blocked"),
                                                _dptr(out, out.dtype, "out"),

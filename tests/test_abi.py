@@ -72,6 +72,15 @@ def test_host_side_argument_checks():
     assert L.mbx_colblock_set_residency(None, 0) == -1
     assert L.mbx_colblock_adjoint_prepare(None, None, None, 0, 8, None, None) == -1
     assert L.mbx_colblock_dx(None, 0, None, 1.0, None, 0.0, None, None) == -1
+    rep = _lib.CsrReport()
+    assert L.mbx_csr_validate(None, None, -1, 0, 0, None, ctypes.byref(rep)) == -1
+    assert L.mbx_csr_validate(None, None, 4, 4, 0, None, ctypes.byref(rep)) == -1
+    assert L.mbx_csr_validate(None, None, 0, 0, 0, None, ctypes.byref(rep)) == 0
+    assert rep.nnz_mismatch == 0 and rep.first_bad_col == -1
+    # no rows but entries: refused on the host with the mismatch reported
+    assert L.mbx_csr_validate(None, ctypes.c_void_p(16), 0, 4, 3, None, ctypes.byref(rep)) == -1
+    assert rep.nnz_mismatch == 1
+    assert L.mbx_plan_create(None, 0, 0, 3, 8, 256, None, ctypes.byref(h)) == -1
     assert _lib.status_str(0) == "ok"
     assert "overflow" not in _lib.status_str(-1)
     assert "int32" in _lib.status_str(-2)
@@ -171,3 +180,17 @@ int main(void) {
     assert r.returncode == 0, r.stderr
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0 and "abi ok" in r.stdout, (r.returncode, r.stdout, r.stderr)
+
+
+def test_values_to_csr_checks_out_before_any_launch():
+    """ColumnBlockedCsr.values_to_csr refuses a short or mistyped ``out``
+    before calling the library (host tensors: nothing reaches a device)."""
+    import torch
+    from merbit_amd.spmv import ColumnBlockedCsr
+    cb = ColumnBlockedCsr.__new__(ColumnBlockedCsr)
+    cb.nnz, cb.value_slots, cb._h = 10, 16, None
+    vb = torch.zeros(16, dtype=torch.float64)
+    with pytest.raises(ValueError):
+        cb.values_to_csr(vb, out=torch.zeros(9, dtype=torch.float64))
+    with pytest.raises(ValueError):
+        cb.values_to_csr(vb, out=torch.zeros(10, dtype=torch.float32))

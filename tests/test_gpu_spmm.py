@@ -82,21 +82,38 @@ def test_spmm_alpha_beta_and_ld(orc, device, ipt, k):
     plan.close()
 
 
+@pytest.fixture(scope="module")
+def rmat_spmm_case(orc):
+    """The batched workload's R-MAT at full size (2^20 rows, 2^24 entries,
+    FP32 values) with 32 right-hand sides and the oracle's accurate product
+    for each of them (computed once for every k and tile shape)."""
+    from merbit_amd import gen, workloads
+    A, _ = workloads.rmat()
+    val = A.val.astype(np.float32)
+    X = gen.uniform(A.ncol * 32, -1, 1, 77, np.float32).reshape(32, A.ncol).T
+    refs = [orc.spmv_accurate(A.row_ptr, A.col, val, np.ascontiguousarray(X[:, j]))
+            for j in range(32)]
+    return A, val, X, refs
+
+
 @pytest.mark.slow
 @pytest.mark.parametrize("ipt", [2, 4])
 @pytest.mark.parametrize("k", [8, 32])
-def test_spmm_rmat(orc, device, k, ipt):
-    from merbit_amd import gen, workloads
+def test_spmm_rmat(rmat_spmm_case, device, k, ipt):
+    """R-MAT (2^20 rows) times k = 8 and 32 column-major right-hand sides,
+    every column against the oracle."""
     from merbit_amd.spmv import MerbitPlan
-    say(f"spmm rmat scale 18 k={k} ipt={ipt}")
-    A, _ = workloads.rmat(18)
-    val = A.val.astype(np.float32)
-    X = gen.uniform(A.ncol * k, -1, 1, 77, np.float32).reshape(k, A.ncol).T
+    A, val, X32, refs = rmat_spmm_case
+    say(f"spmm rmat scale 20 k={k} ipt={ipt}")
+    X = X32[:, :k]
     d_rp, d_col, d_val = dev(A.row_ptr, device), dev(A.col, device), dev(val, device)
     plan = MerbitPlan(d_rp, A.ncol, A.nnz, items_per_thread=ipt)
     Y = plan.spmm(d_rp, d_col, d_val, dev(X.T.copy(), device).t()).cpu().numpy()
-    check_spmm(orc, A.row_ptr, A.col, val, X, Y)
     plan.close()
+    for j in range(k):
+        ya, absr = refs[j]
+        e = cond_err(Y[:, j].astype(np.float64), ya, absr)
+        assert e <= SPMM_TOL, (j, e)
 
 
 @pytest.mark.parametrize("ipt", [2, 4])

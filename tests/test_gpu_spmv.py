@@ -154,6 +154,36 @@ def test_f64_rmat_full(orc, device):
 
 
 @pytest.mark.slow
+@pytest.mark.parametrize("kind", ["f32", "bf16"])
+def test_rmat_mixed_full(orc, device, kind):
+    """The R-MAT half of the mixed-precision workload at full size (2^20
+    rows, 2^24 entries): values stored FP32 or bfloat16, x FP32, against the
+    oracle's accurate FP64 product on the same stored values, with both the
+    compensated FP32 sums and the FP64 sums.  Bar 1e-6, condition-scaled and
+    relative on the rows whose result is not a cancellation
+    (|y| > 1e-3 (|A||x|))."""
+    from merbit_amd import gen, workloads
+    from merbit_amd.spmv import MerbitPlan
+    say(f"rmat scale 20, 2^24 entries, {kind} values")
+    A, x = workloads.rmat()
+    v32 = A.val.astype(np.float32)
+    vals = v32 if kind == "f32" else gen.to_bf16(v32)
+    xs = x.astype(np.float32)
+    ya, absr = orc.spmv_accurate(A.row_ptr, A.col, vals, xs)
+    big = np.abs(ya) > 1e-3 * absr
+    d_rp, d_col, d_val, d_x = (dev(A.row_ptr, device), dev(A.col, device), dev(vals, device),
+                               dev(xs, device))
+    plan = MerbitPlan(d_rp, A.ncol, A.nnz)
+    for acc in ("compensated", "f64"):
+        plan.set_accumulation(acc)
+        y = plan.spmv(d_rp, d_col, d_val, d_x).cpu().numpy().astype(np.float64)
+        assert cond_err(y, ya, absr) <= MIXED_TOL, acc
+        rel = np.max(np.abs(y[big] - ya[big]) / np.abs(ya[big]))
+        assert rel <= MIXED_TOL, acc
+    plan.close()
+
+
+@pytest.mark.slow
 @pytest.mark.parametrize("kind", ["f64", "f32", "bf16"])
 def test_banded_full(orc, device, kind):
     """The banded workload at full size (4M rows, 27 diagonals, 108M entries)
