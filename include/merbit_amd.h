@@ -34,7 +34,8 @@ enum {
   MBX_EOVERFLOW = -2,    /* nrow + nnz (+ one tile) does not fit int32 */
   MBX_EUNSUPPORTED = -3, /* items_per_thread / wg_size / k not instantiated */
   MBX_ENOMEM = -4,       /* device allocation failed */
-  MBX_EHIP = -5          /* a HIP runtime call or launch failed */
+  MBX_EHIP = -5,         /* a HIP runtime call or launch failed */
+  MBX_ESCRATCH = -6      /* larger than mbx_spmv_f64_noalloc's static scratch */
 };
 
 typedef struct mbx_plan mbx_plan;
@@ -131,6 +132,23 @@ int mbx_plan_set_events(mbx_plan* plan, void* ev_begin, void* ev_end);
 int mbx_spmv_f64(const mbx_plan* plan, const int32_t* d_row_ptr,
                  const int32_t* d_col, const double* d_val, double alpha,
                  const double* d_x, double beta, double* d_y, void* stream);
+
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  y = alpha*A*x + beta*y in FP64 without a plan: the partition
+ * search, the tile kernel and the carry fixup of mbx_plan_create(items
+ * per thread 8, wg 256) + mbx_spmv_f64 run back to back on `stream`, with the
+ * tile coordinates and carry slots in a static 4 MiB device array (one per
+ * GPU, held until the process ends), so the result is bit-identical to that
+ * pair.  Allocates nothing, never synchronises (capturable); row_ptr must run
+ * from 0 to nnz (not checked here: mbx_csr_validate checks it).  Limit: at
+ * most 131071 tiles, nrow + nnz <= 268,433,408; larger inputs return
+ * MBX_ESCRATCH.  Every call on one GPU shares that GPU's array: calls that
+ * overlap in time on one GPU (other streams, other host threads) race and
+ * must be serialised by the caller. */
+int mbx_spmv_f64_noalloc(const int32_t* d_row_ptr, const int32_t* d_col, const double* d_val,
+                         int32_t nrow, int32_t ncol, int32_t nnz, double alpha,
+                         const double* d_x, double beta, double* d_y, void* stream);
 
 /* Mixed precision forms of mbx_spmv_f64 (no reference counterpart: the
  * reference instantiates float and double only, src/class.cu:427-430).

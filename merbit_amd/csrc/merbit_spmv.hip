@@ -36,6 +36,8 @@
 #include <new>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "merbit_amd.h"
 #include "merbit_internal.hpp"
 #include "merbit_tile.hpp"
@@ -237,6 +239,49 @@ int run_spmv(const mbx_plan* p, const int32_t* rp, const int32_t* col,
     default: return MBX_EUNSUPPORTED;
   }
   return e == hipSuccess ? MBX_OK : MBX_EHIP;
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of the no-allocation entry point (mbx_spmv_f64_noalloc): the tile
+// coordinates, then the FP64 carry slots, of at most kNoAllocTiles tiles of
+// the default shape.  One copy per GPU, held until the process ends.  Host
+// code reaches a GPU's copy only through the address hipGetSymbolAddress
+// returns with that GPU current (noalloc_scratch); the array's name is not a
+// device address on the host.
+// ---------------------------------------------------------------------------
+
+constexpr int kNoAllocIpt = 8;
+constexpr int kNoAllocTileItems = 256 * kNoAllocIpt;
+constexpr size_t kNoAllocBytes = size_t(4) << 20;
+// coordinate (int4) + two carry slots (FP64) per tile boundary
+constexpr int kNoAllocTiles = (int)(kNoAllocBytes / (sizeof(int4) + 2 * sizeof(double))) - 1;
+constexpr size_t kNoAllocCarryOffset = sizeof(int4) * ((size_t)kNoAllocTiles + 1);
+
+__device__ __align__(256) char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current GPU's scratch, looked up once per device.
+int noalloc_scratch(void** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return MBX_EHIP;
+  }
+  if (dev < 0 || dev >= kMaxDevices) return MBX_EUNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+      (void)hipGetLastError();
+      return MBX_EHIP;
+    }
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return MBX_OK;
 }
 
 }  // namespace
@@ -463,6 +508,46 @@ int mbx_spmv_bf16(const mbx_plan* p, const int32_t* rp, const int32_t* col,
   return run_spmv<BF16C>(p, rp, col, val, alpha, x, beta, y, stream);
 }
 
+int mbx_spmv_f64_noalloc(const int32_t* d_row_ptr, const int32_t* d_col, const double* d_val,
+                         int32_t nrow, int32_t ncol, int32_t nnz, double alpha,
+                         const double* d_x, double beta, double* d_y, void* stream) {
+  if (nrow < 0 || ncol < 0 || nnz < 0 || (nrow == 0 && nnz > 0)) return MBX_EINVAL;
+  const long long path = (long long)nrow + nnz;
+  const long long ntiles = (path + kNoAllocTileItems - 1) / kNoAllocTileItems;
+  if (ntiles > kNoAllocTiles) return MBX_ESCRATCH;
+  if (nrow > 0 && (!d_row_ptr || !d_y)) return MBX_EINVAL;
+  if (nnz > 0 && (!d_col || !d_val || !d_x)) return MBX_EINVAL;
+  if (ntiles == 0) return MBX_OK;
+  void* base = nullptr;
+  const int sst = noalloc_scratch(&base);
+  if (sst != MBX_OK) return sst;
+  // A plan of the default shape over the scratch: every tile coordinate is
+  // written by the partition kernel and every carry slot the fixup reads is
+  // written by its tile first, so nothing needs clearing between calls.  The
+  // fixup launch is not skipped (whether a row crosses a tile boundary is not
+  // read back: that would synchronise).
+  mbx_plan p{};
+  p.nrow = nrow;
+  p.ncol = ncol;
+  p.nnz = nnz;
+  p.wg = 256;
+  p.ipt = kNoAllocIpt;
+  p.tile_items = kNoAllocTileItems;
+  p.ntiles = (int)ntiles;
+  p.max_rows = kNoAllocTileItems;
+  p.split_rows = 1;
+  p.d_coord = static_cast<int4*>(base);
+  p.d_carry = reinterpret_cast<double*>(static_cast<char*>(base) + kNoAllocCarryOffset);
+  p.stream_mode = MBX_STREAM_AUTO;
+  hipStream_t s = (hipStream_t)stream;
+  partition_kernel<<<(p.ntiles + 1 + 255) / 256, 256, 0, s>>>(d_row_ptr, nrow, nnz,
+                                                               p.tile_items, p.ntiles, p.d_coord);
+  if (hipGetLastError() != hipSuccess) return MBX_EHIP;
+  const hipError_t e =
+      launch_tile<F64, kNoAllocIpt>(&p, s, d_row_ptr, d_col, d_val, d_x, d_y, alpha, beta);
+  return e == hipSuccess ? MBX_OK : MBX_EHIP;
+}
+
 const char* mbx_status_str(int st) {
   switch (st) {
     case MBX_OK: return "ok";
@@ -471,6 +556,8 @@ const char* mbx_status_str(int st) {
     case MBX_EUNSUPPORTED: return "unsupported items_per_thread / wg_size / k";
     case MBX_ENOMEM: return "device allocation failed";
     case MBX_EHIP: return "HIP runtime or launch error";
+    case MBX_ESCRATCH:
+      return "nrow + nnz exceeds the no-allocation scratch (131071 tiles of 2048 merge items)";
     default: return "unknown status";
   }
 }

@@ -81,6 +81,12 @@ def test_host_side_argument_checks():
     assert L.mbx_csr_validate(None, ctypes.c_void_p(16), 0, 4, 3, None, ctypes.byref(rep)) == -1
     assert rep.nnz_mismatch == 1
     assert L.mbx_plan_create(None, 0, 0, 3, 8, 256, None, ctypes.byref(h)) == -1
+    # the no-allocation product: bad sizes and the scratch limit, on the host
+    assert L.mbx_spmv_f64_noalloc(None, None, None, -1, 0, 0, 1.0, None, 0.0, None, None) == -1
+    assert L.mbx_spmv_f64_noalloc(None, None, None, 100, 100, 268433408, 1.0, None, 0.0,
+                                  None, None) == -6
+    assert L.mbx_spmv_f64_noalloc(None, None, None, 5, 5, 0, 1.0, None, 0.0, None, None) == -1
+    assert "scratch" in _lib.status_str(-6)
     assert _lib.status_str(0) == "ok"
     assert "overflow" not in _lib.status_str(-1)
     assert "int32" in _lib.status_str(-2)
