@@ -120,6 +120,26 @@ int mbx_plan_set_accumulation(mbx_plan* plan, int32_t mode);
  * `stream` (and the device, when it replaces compressed columns). */
 int mbx_plan_compress_cols(mbx_plan* plan, const int32_t* d_col, void* stream);
 
+/* Hot-column split (no reference counterpart; the reference gathers
+ * x[col] for every nonzero, include/tile.cuh:293-305).  Ranks the columns of
+ * d_col by nonzero count and keeps the max_hot most frequent (ties by column
+ * index; at most 32768) with a copy of the column array in which their
+ * entries name a slot instead.  Products on this plan with items_per_thread
+ * 8 or 12 on the vector staging path then gather the hot columns' x values
+ * once per product into the plan (nhot values) and run a persistent tile
+ * kernel that keeps them in each workgroup's LDS; every other x value is
+ * gathered as before.  For power-law patterns, whose hot columns take a
+ * large share of the gathers.  Results are bit-identical to the plain
+ * product; the hot values must fit one workgroup's LDS beside the tile
+ * (FP64 x: about 16000 columns), else the product runs the plain kernel.
+ * *covered (may be NULL) receives the share of the nonzeros in the kept
+ * columns.  max_hot 0 or d_col NULL drops the split.  The plan's products
+ * then read its own column copy, so d_col must stay the pattern's column
+ * array.  Allocates 4*nnz + 12*max_hot bytes and synchronises `stream`
+ * (and the device, when it replaces a split); clones do not inherit it. */
+int mbx_plan_split_hot(mbx_plan* plan, const int32_t* d_col, int32_t max_hot, void* stream,
+                       double* covered);
+
 /* Optional timing hook: when ev_begin/ev_end (hipEvent_t) are non-NULL, the
  * next spmv call on this plan records them immediately before and after its
  * tile kernel on the call's stream, then clears them. */

@@ -30,7 +30,7 @@ ABI_SYMBOLS = (
     "mbx_colblock_create", "mbx_colblock_destroy", "mbx_colblock_info", "mbx_colblock_values",
     "mbx_colblock_spmv", "mbx_colblock_sddmm", "mbx_colblock_values_to_csr", "mbx_colblock_set_stream",
     "mbx_colblock_set_residency", "mbx_colblock_adjoint_prepare", "mbx_colblock_dx",
-    "mbx_csr_validate", "mbx_spmv_f64_noalloc", "mbx_status_str",
+    "mbx_csr_validate", "mbx_spmv_f64_noalloc", "mbx_plan_split_hot", "mbx_status_str",
 )
 GEN_SYMBOLS = (
     "mbg_fill_uniform_f64", "mbg_fill_uniform_f32", "mbg_fill_normal_f64",
@@ -164,6 +164,7 @@ def lib() -> ctypes.CDLL:
     L.mbx_pagerank_graph_create.argtypes = [vp, vp, vp, vp, f64, i32, vp, vp, vp,
                                             ctypes.c_size_t, ctypes.POINTER(vp)]
     L.mbx_csr_validate.argtypes = [vp, vp, i32, i32, i32, vp, ctypes.POINTER(CsrReport)]
+    L.mbx_plan_split_hot.argtypes = [vp, vp, i32, vp, ctypes.POINTER(f64)]
     L.mbx_spmv_f64_noalloc.argtypes = [vp, vp, vp, i32, i32, i32, f64, vp, f64, vp, vp]
     L.mbx_status_str.argtypes = [ctypes.c_int]
     L.mbx_status_str.restype = ctypes.c_char_p

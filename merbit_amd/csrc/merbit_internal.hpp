@@ -28,7 +28,18 @@ struct mbx_plan {
   // mbx_plan_compress_cols: column index minus its tile's base (d_coord[t].w)
   uint16_t* d_col16;
   int32_t accumulation;  // MBX_ACC_*: sums of the FP32 / bfloat16 value forms
+  // mbx_plan_split_hot: the nhot most frequent columns (hot), the column
+  // array with hot entries as -1 - slot (colh, padded to a multiple of 4) and
+  // the per-product x values of the hot columns (xh, nhot * 8 bytes)
+  int32_t* d_colh;
+  int32_t* d_hot;
+  void* d_xh;
+  int32_t nhot;
+  int32_t cus;  // compute units of the plan's device (persistent grids)
 };
+
+// Most hot columns a plan keeps (mbx_plan_split_hot): 128 KB of FP32 x.
+constexpr int kMaxHotColumns = 32768;
 
 struct mbx_colblock;
 
@@ -78,6 +89,9 @@ const int32_t* colblock_slot_map(const mbx_colblock* cb);
 // Allocates temporaries and synchronises s.
 int csr_check(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow, int32_t ncol,
               int32_t nnz, hipStream_t s, mbx_csr_report* out);
+
+// Frees a plan's hot-column split (merbit_hot.hip), waiting for the device.
+void plan_drop_hot(mbx_plan* p);
 
 constexpr int kXcds = 8;
 

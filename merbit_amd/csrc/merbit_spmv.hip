@@ -130,6 +130,53 @@ __global__ __launch_bounds__(WG) void tile_spmv_kernel(
                                          val, x, y, coord, carry, alpha, beta, beta_zero);
 }
 
+// x values of the hot columns for one product: xh[h] = x[hot[h]].
+template <class X>
+__global__ __launch_bounds__(256) void hot_gather_kernel(const X* __restrict__ x,
+                                                         const int32_t* __restrict__ hot,
+                                                         int nhot, X* __restrict__ xh) {
+  const int h = blockIdx.x * 256 + threadIdx.x;
+  if (h < nhot) xh[h] = x[hot[h]];
+}
+
+// Persistent tile kernel of a hot-column plan (P = HotX<...>): each
+// workgroup copies the hot x values into LDS once (16-byte loads), then runs
+// the tiles of its share: each XCD takes a contiguous range of tiles, its
+// workgroups take every stride-th tile of it.
+template <int WG, int IPT, class P>
+__global__ __launch_bounds__(WG) void tile_spmv_hot_kernel(
+    const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ colh,
+    const typename P::val_t* __restrict__ val, const typename P::x_t* __restrict__ x,
+    typename P::y_t* __restrict__ y, const int4* __restrict__ coord,
+    double* __restrict__ carry, int ntiles, int ncol, typename P::scal_t alpha,
+    typename P::scal_t beta, int beta_zero, const typename P::x_t* __restrict__ xh, int nhot) {
+  extern __shared__ __align__(16) char smem[];
+  TileSmem<WG, IPT, P> sm(smem);
+  using X = typename P::x_t;
+  X* s_xh = reinterpret_cast<X*>(smem + TileSmem<WG, IPT, P>::bytes());
+  const int nv = (nhot * (int)sizeof(X) + 15) / 16;
+  for (int i = threadIdx.x; i < nv; i += WG)
+    reinterpret_cast<int4*>(s_xh)[i] = reinterpret_cast<const int4*>(xh)[i];
+  sm.xh = s_xh;
+  __syncthreads();
+  constexpr int XC = mbx_detail::kXcds;
+  const int b = blockIdx.x, nwg = gridDim.x;
+  int first = 0, len = ntiles, w = b, stride = nwg;
+  if (nwg >= XC) {
+    const int xcd = b % XC;
+    w = b / XC;
+    stride = (nwg - xcd + XC - 1) / XC;
+    const int q = ntiles / XC, r = ntiles % XC;
+    first = xcd * q + min(xcd, r);
+    len = q + (xcd < r ? 1 : 0);
+  }
+  for (int i = w; i < len; i += stride) {
+    mbx_detail::spmv_tile<WG, IPT, true, P>(sm, first + i, ncol, row_ptr, colh, val, x, y, coord,
+                                            carry, alpha, beta, beta_zero);
+    __syncthreads();  // the next tile reuses the LDS
+  }
+}
+
 // Carry fixup, one thread per tile (see fixup_tile).
 template <class P>
 __global__ __launch_bounds__(256) void carry_fixup_kernel(
@@ -155,6 +202,40 @@ bool supported_shape(int wg, int ipt) {
     case 2: case 4: case 6: case 8: case 10: case 12: case 14: case 16: return true;
     default: return false;
   }
+}
+
+// The hot-column launch (mbx_plan_split_hot) of P at IPT: the hot x gather,
+// then a persistent grid of as many workgroups as fit with the hot values in
+// LDS (at most wg_per_cu per CU when the plan caps residency).  False when
+// the hot values do not fit a workgroup's LDS for this value kind: the
+// caller then runs the plain product on the caller's columns.
+template <class P, int IPT>
+bool launch_hot(const mbx_plan* p, hipStream_t s, const int32_t* rp,
+                const typename P::val_t* val, const typename P::x_t* x, typename P::y_t* y,
+                typename P::scal_t alpha, typename P::scal_t beta, int bz) {
+  using Q = mbx_detail::HotX<P>;
+  using X = typename P::x_t;
+  const size_t xh_bytes = ((size_t)p->nhot * sizeof(X) + 15) / 16 * 16;
+  const size_t lds =
+      mbx_detail::residency_lds(TileSmem<256, IPT, Q>::bytes() + xh_bytes, p->wg_per_cu);
+  if (lds > mbx_detail::kLdsPerCu || p->cus <= 0) return false;
+  auto* kernel = tile_spmv_hot_kernel<256, IPT, Q>;
+  mbx_detail::allow_lds(kernel, lds);
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, lds) != hipSuccess ||
+      per_cu < 1) {
+    (void)hipGetLastError();
+    return false;
+  }
+  if (p->wg_per_cu > 0 && per_cu > p->wg_per_cu) per_cu = p->wg_per_cu;
+  const long long g = (long long)p->cus * per_cu;
+  const int grid = (int)(g < p->ntiles ? g : p->ntiles);
+  hot_gather_kernel<X><<<(p->nhot + 255) / 256, 256, 0, s>>>(x, p->d_hot, p->nhot,
+                                                               static_cast<X*>(p->d_xh));
+  kernel<<<grid, 256, lds, s>>>(rp, p->d_colh, val, x, y, p->d_coord, p->d_carry, p->ntiles,
+                                p->ncol, alpha, beta, bz, static_cast<const X*>(p->d_xh),
+                                p->nhot);
+  return true;
 }
 
 template <class P, int IPT>
@@ -190,6 +271,13 @@ hipError_t launch_tile(const mbx_plan* p, hipStream_t s, const int32_t* rp,
             rp, c16, val, x, y, p->d_coord, p->d_carry, p->ntiles, p->ncol, alpha, beta, bz);
       }
       launched = true;
+    }
+  }
+  if constexpr (kC16) {  // hot-column plans: the tile shapes of the power-law products
+    if (!launched && vec && p->d_colh && p->nhot > 0) {
+      launched = nt ? launch_hot<mbx_detail::Streamed<P>, IPT>(p, s, rp, val, x, y, alpha, beta,
+                                                               bz)
+                    : launch_hot<P, IPT>(p, s, rp, val, x, y, alpha, beta, bz);
     }
   }
   if (launched) {
@@ -318,6 +406,13 @@ int mbx_plan_create(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol,
   p->stream_mode = MBX_STREAM_AUTO;
   p->wg_per_cu = 0;
   p->ev_begin = p->ev_end = nullptr;
+  {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      p->cus = 0;
+    (void)hipGetLastError();
+  }
   if (!p->coord_refs) {
     mbx_plan_destroy(p);
     return MBX_ENOMEM;
@@ -372,6 +467,7 @@ void mbx_plan_destroy(mbx_plan* p) {
   }
   if (p->d_carry) (void)hipFree(p->d_carry);
   if (p->d_col16) (void)hipFree(p->d_col16);
+  if (p->d_colh || p->d_hot || p->d_xh) mbx_detail::plan_drop_hot(p);
   delete p;
 }
 
@@ -381,6 +477,10 @@ int mbx_plan_clone(const mbx_plan* p, mbx_plan** out) {
   mbx_plan* c = new (std::nothrow) mbx_plan(*p);
   if (!c) return MBX_ENOMEM;
   c->d_col16 = nullptr;  // the clone reads the caller's columns
+  c->d_colh = nullptr;
+  c->d_hot = nullptr;
+  c->d_xh = nullptr;
+  c->nhot = 0;
   c->ev_begin = c->ev_end = nullptr;
   c->d_carry = nullptr;
   if (hipMalloc((void**)&c->d_carry, sizeof(double) * 2 * ((size_t)p->ntiles + 1)) !=

@@ -31,6 +31,7 @@ struct F64 {
   static constexpr bool kStreamedIndexNT = true;
   static constexpr bool kStreamedValueNT = false;
   static constexpr bool kCol16 = false;
+  static constexpr bool kHotX = false;
   using val_t = double;
   using x_t = double;
   using y_t = double;
@@ -71,6 +72,7 @@ struct CompF32 {
   static constexpr bool kStreamedIndexNT = true;
   static constexpr bool kStreamedValueNT = true;
   static constexpr bool kCol16 = false;
+  static constexpr bool kHotX = false;
   using val_t = V;
   using x_t = float;
   using y_t = float;
@@ -136,6 +138,7 @@ struct Acc64 {
   static constexpr bool kStreamedIndexNT = true;
   static constexpr bool kStreamedValueNT = true;
   static constexpr bool kCol16 = false;
+  static constexpr bool kHotX = false;
   using val_t = V;
   using x_t = float;
   using y_t = float;
@@ -182,6 +185,15 @@ struct Streamed : P {
 template <class P>
 struct Col16 : P {
   static constexpr bool kCol16 = true;
+};
+
+// P reading x of the plan's hot columns from LDS (mbx_plan_split_hot): the
+// column array holds -1 - slot for a hot column, whose x value sits in the
+// workgroup's copy of the hot values (TileSmem::xh); the buffer gather of a
+// negative index reads 0 without touching memory.  Vector staging path only.
+template <class P>
+struct HotX : P {
+  static constexpr bool kHotX = true;
 };
 
 // out[0] = most rows ending in one tile, out[1] = tiles whose first row
@@ -606,6 +618,31 @@ __device__ __forceinline__ void tile_write_stage(const TileRegs<WG, IPT, P>& R, 
   }
 }
 
+// Hot columns (P::kHotX): x of every nonzero whose column index is -1 - h is
+// the LDS value s_xh[h].  Run after the gathers' results are needed anyway
+// (before the products): an LDS read per element, the gathered value kept
+// for cold columns.
+template <int WG, int IPT, class P>
+__device__ __forceinline__ void hot_substitute(TileRegs<WG, IPT, P>& R,
+                                               const typename P::x_t* s_xh) {
+  constexpr int NG = TileRegs<WG, IPT, P>::NG;
+  const int nnz_t = R.nnz();
+  if (nnz_t > 0) {
+    const int kb = R.k0() >> 2, nkg = ((R.k0() + nnz_t - 1) >> 2) - kb + 1;
+#pragma unroll
+    for (int j = 0; j < NG; ++j) {
+      if (j == 0 || j * WG < nkg) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int c = R.cg[j][v];
+          const typename P::x_t h = s_xh[c < 0 ? -1 - c : 0];
+          if (c < 0) R.xv[j][v] = h;
+        }
+      }
+    }
+  }
+}
+
 // LDS of one tile, carved from dynamic shared memory: the product / row-sum
 // area (products of the tile's nonzeros first, sums of the rows ending in the
 // tile after them: together at most WG*IPT entries, the merge-path identity,
@@ -623,6 +660,7 @@ struct TileSmem {
   double* tail;
   unsigned* mask;
   int* wcnt;    // NW
+  const typename P::x_t* xh = nullptr;  // hot x values (P::kHotX), set by the kernel
   __device__ TileSmem(char* base)
       : v(reinterpret_cast<typename P::lds_t*>(base)),
         wtail(reinterpret_cast<double*>(base + kV)),
@@ -763,6 +801,7 @@ __device__ __forceinline__ void spmv_tile(
     typename P::scal_t beta, int beta_zero) {
   static_assert(WG % 64 == 0 && WG / 64 <= 4, "scan scratch holds up to 4 waves");
   static_assert(VEC || !P::kCol16, "16-bit columns are read by the vector staging only");
+  static_assert(VEC || !P::kHotX, "hot columns are read by the vector staging only");
   typename P::lds_t* s_p = sm.v;
   int4 c0, c1;
   // Stage: products to LDS and the row-end marks, every global load of the
@@ -774,6 +813,7 @@ __device__ __forceinline__ void spmv_tile(
     tile_issue_gathers(R, x, ncol);
     clear_marks(sm);
     __syncthreads();
+    if constexpr (P::kHotX) hot_substitute(R, sm.xh);
     tile_write_stage(R, sm.mask, sm.wcnt, sm.v);
     s_p += R.k0() & 3;
     c0 = R.c0;

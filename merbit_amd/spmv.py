@@ -136,6 +136,16 @@ class MerbitPlan:
         check(st, "mbx_plan_compress_cols")
         return col is not None
 
+    def split_hot(self, col: torch.Tensor | None, max_hot: int, stream=None) -> float:
+        """Keep the max_hot most frequent columns' x values in LDS during
+        products (mbx_plan_split_hot); returns the share of the nonzeros in
+        those columns.  None or 0 drops the split."""
+        cov = ctypes.c_double()
+        ptr = None if col is None else _dptr(col, torch.int32, "col")
+        check(lib().mbx_plan_split_hot(self.handle, ptr, int(max_hot), _stream_handle(stream),
+                                       ctypes.byref(cov)), "mbx_plan_split_hot")
+        return cov.value
+
     def set_events(self, begin, end) -> None:
         """Record torch.cuda.Event pair around the next call's tile kernel."""
         check(lib().mbx_plan_set_events(self.handle, begin.cuda_event, end.cuda_event),

@@ -1,0 +1,151 @@
+"""The hot-column split (mbx_plan_split_hot): products on a split plan run a
+persistent tile kernel that reads the hot columns' x values from LDS; they
+must be bit-identical to the plain product of the same plan shape, and the
+plain product is itself checked against the oracle elsewhere (and here on
+the full R-MAT)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import cond_err, edge_case_patterns, random_csr
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def dev(a, device):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device)
+
+
+def values(rng, kind, nnz, ncol):
+    from merbit_amd import gen
+    v = rng.standard_normal(nnz)
+    x = rng.uniform(-1, 1, ncol)
+    if kind == "f64":
+        return v, x
+    v32 = v.astype(np.float32)
+    return (v32 if kind == "f32" else gen.to_bf16(v32)), x.astype(np.float32)
+
+
+def bits(t):
+    a = t.cpu().numpy()
+    return a.view(np.uint64 if a.dtype == np.float64 else np.uint32)
+
+
+@pytest.mark.parametrize("kind", ["f64", "f32", "bf16"])
+@pytest.mark.parametrize("ipt", [8, 12])
+def test_hot_split_bit_identical(device, kind, ipt):
+    from merbit_amd.spmv import MerbitPlan
+    rng = np.random.default_rng(11 + ipt)
+    T = 256 * ipt
+    cases = edge_case_patterns(T) + [
+        ("pareto", (rng.pareto(1.05, 40000) * 4).astype(int), 30000)]
+    for name, lens, ncol in cases:
+        rp, col = random_csr(rng, len(lens), ncol, lens=lens)
+        nnz = int(rp[-1])
+        vals, x = values(rng, kind, nnz, ncol)
+        d_rp, d_col, d_val, d_x = dev(rp, device), dev(col, device), dev(vals, device), \
+            dev(x, device)
+        plan = MerbitPlan(d_rp, ncol, nnz, items_per_thread=ipt)
+        y0 = rng.uniform(-1, 1, len(lens)).astype(np.float64 if kind == "f64" else np.float32)
+        ref = plan.spmv(d_rp, d_col, d_val, d_x)
+        ref_ab = plan.spmv(d_rp, d_col, d_val, d_x, dev(y0, device), -1.5, 0.5)
+        for nhot in (1, 64, 2048):
+            say(f"hot {kind} ipt {ipt} {name} nhot {nhot}")
+            plan.split_hot(d_col, nhot)
+            for mode in ("cached", "nontemporal"):
+                plan.set_stream(mode)
+                for wg in (0, 3):
+                    plan.set_residency(wg)
+                    y = plan.spmv(d_rp, d_col, d_val, d_x)
+                    assert np.array_equal(bits(y), bits(ref)), (name, nhot, mode, wg)
+            y = plan.spmv(d_rp, d_col, d_val, d_x, dev(y0, device), -1.5, 0.5)
+            assert np.array_equal(bits(y), bits(ref_ab)), (name, nhot)
+            plan.set_stream("auto")
+            plan.set_residency(0)
+        plan.split_hot(None, 0)
+        assert np.array_equal(bits(plan.spmv(d_rp, d_col, d_val, d_x)), bits(ref))
+        plan.close()
+
+
+def test_hot_split_coverage_and_fallback(device):
+    """The reported share of the entries in the kept columns matches a count
+    on the host (most frequent first, ties by column index); a hot set too
+    large for one workgroup's LDS with FP64 x falls back to the plain
+    kernel (same bits); clones do not inherit the split."""
+    from merbit_amd.spmv import MerbitPlan
+    rng = np.random.default_rng(5)
+    rp, col = random_csr(rng, 20000, 40000, lens=(rng.pareto(1.1, 20000) * 5).astype(int))
+    nnz = int(rp[-1])
+    cnt = np.bincount(col, minlength=40000)
+    order = np.lexsort((np.arange(40000), -cnt))  # count descending, column ascending
+    d_rp, d_col = dev(rp, device), dev(col, device)
+    plan = MerbitPlan(d_rp, 40000, nnz)
+    for nhot in (1, 100, 5000):
+        cov = plan.split_hot(d_col, nhot)
+        assert cov == pytest.approx(cnt[order[:nhot]].sum() / nnz, rel=0, abs=1e-12)
+    v = rng.standard_normal(nnz)
+    x = rng.uniform(-1, 1, 40000)
+    d_val, d_x = dev(v, device), dev(x, device)
+    plan.split_hot(None, 0)
+    ref = plan.spmv(d_rp, d_col, d_val, d_x)
+    plan.split_hot(d_col, 32768)  # 256 KB of FP64 x: does not fit LDS
+    assert np.array_equal(bits(plan.spmv(d_rp, d_col, d_val, d_x)), bits(ref))
+    plan.split_hot(d_col, 4096)
+    clone = plan.clone()
+    assert np.array_equal(bits(clone.spmv(d_rp, d_col, d_val, d_x)), bits(ref))
+    clone.close()
+    plan.close()
+
+
+def test_hot_split_graph_replays(device):
+    """A captured product of a split plan replays bit-identically."""
+    from merbit_amd.batch import spmv_graph
+    from merbit_amd.spmv import MerbitPlan
+    rng = np.random.default_rng(9)
+    rp, col = random_csr(rng, 30000, 30000, lens=(rng.pareto(1.1, 30000) * 6).astype(int))
+    nnz = int(rp[-1])
+    v, x = values(rng, "f64", nnz, 30000)
+    d_rp, d_col, d_val, d_x = dev(rp, device), dev(col, device), dev(v, device), dev(x, device)
+    plan = MerbitPlan(d_rp, 30000, nnz)
+    ref = plan.spmv(d_rp, d_col, d_val, d_x)
+    plan.split_hot(d_col, 4096)
+    y = torch.zeros(30000, dtype=torch.float64, device=device)
+    g = spmv_graph(plan, d_rp, d_col, d_val, d_x, y, repeats=3)
+    for _ in range(3):
+        y.zero_()
+        torch.cuda.synchronize()
+        g.launch()
+        torch.cuda.synchronize()
+        assert np.array_equal(bits(y), bits(ref))
+    g.close()
+    plan.close()
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("kind", ["f64", "f32"])
+def test_hot_split_rmat_full(orc, device, kind):
+    from merbit_amd import workloads
+    from merbit_amd.spmv import MerbitPlan
+    say(f"hot rmat scale 20 {kind}")
+    A, x = workloads.rmat()
+    vals = A.val if kind == "f64" else A.val.astype(np.float32)
+    xs = x if kind == "f64" else x.astype(np.float32)
+    d_rp, d_col, d_val, d_x = dev(A.row_ptr, device), dev(A.col, device), dev(vals, device), \
+        dev(xs, device)
+    plan = MerbitPlan(d_rp, A.ncol, A.nnz)
+    ref = plan.spmv(d_rp, d_col, d_val, d_x)
+    cov = plan.split_hot(d_col, 4096 if kind == "f64" else 5800)
+    assert cov > 0.35
+    plan.set_stream("nontemporal")
+    y = plan.spmv(d_rp, d_col, d_val, d_x)
+    assert np.array_equal(bits(y), bits(ref))
+    ya, absr = orc.spmv_accurate(A.row_ptr, A.col, vals, xs)
+    assert cond_err(y.cpu().numpy().astype(np.float64), ya, absr) <= \
+        (1e-12 if kind == "f64" else 1e-6)
+    plan.close()
