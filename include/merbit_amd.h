@@ -286,6 +286,55 @@ typedef struct mbx_pattern_stats {
 } mbx_pattern_stats;
 int mbx_plan_stats(const mbx_plan* plan, const int32_t* d_row_ptr, mbx_pattern_stats* out);
 
+/* Launch rule: the statistics of a pattern choose its launch without a
+ * timing sweep.  The reference computes degree and tile statistics
+ * (GRAPH::desc, src/class.cu:223-381; TILE::desc, include/tile.cuh:177-233)
+ * but compiles one SIGMA and BLOCK (CMakeLists.txt:4-6) for every matrix.
+ * Features: tiles at 8 items per thread, the widest column range of one such
+ * tile, x bytes, and the share of the entries in the 4096 (FP64 x) or 5800
+ * (FP32 x) most frequent columns.  Rules, first match:
+ *   MBX_RULE_FEW_TILES       fewer than 16 tiles per CU: 4 items per thread;
+ *   MBX_RULE_LOCAL           every tile spans < 65536 columns (banded):
+ *                            8 (FP64) or 12 items per thread, all tiles
+ *                            resident, 16-bit columns, FP64 sums for
+ *                            FP32/BF16 values;
+ *   MBX_RULE_X_FITS          x of at most 2 MiB: 8, at most 4 tiles per CU;
+ *   MBX_RULE_HOT_COLUMNS     the most frequent columns hold >= 1/4 of the
+ *                            entries: 8, the hot-column split;
+ *   MBX_RULE_SPREAD_COLUMNS  otherwise: 8, 3 tiles per CU, and the
+ *                            column-blocked form recommended (column_blocks
+ *                            for mbx_colblock_create).
+ * The stream policy stays MBX_STREAM_AUTO.  Synchronises `stream` (reads the
+ * features back).  mbx_plan_create_auto makes the plan with the choice
+ * applied (items per thread, residency, accumulation, 16-bit columns, hot
+ * split); column_blocks is a recommendation for the caller (a plan is a
+ * CSR product). */
+enum {
+  MBX_RULE_FEW_TILES = 1,
+  MBX_RULE_LOCAL = 2,
+  MBX_RULE_X_FITS = 3,
+  MBX_RULE_HOT_COLUMNS = 4,
+  MBX_RULE_SPREAD_COLUMNS = 5
+};
+typedef struct mbx_launch_choice {
+  int32_t rule;              /* MBX_RULE_* that decided */
+  int32_t items_per_thread;
+  int32_t stream_mode;       /* MBX_STREAM_* */
+  int32_t wg_per_cu;         /* tiles per CU (0: as many as fit) */
+  int32_t accumulation;      /* MBX_ACC_* (FP32 / bfloat16 values) */
+  int32_t compress_cols;     /* 1: mbx_plan_compress_cols */
+  int32_t hot_columns;       /* mbx_plan_split_hot max_hot (0: none) */
+  int32_t column_blocks;     /* > 0: the column-blocked form with this many blocks */
+  /* the features */
+  int32_t cus, ntiles8, max_tile_span;
+  double x_bytes, hot_share;
+} mbx_launch_choice;
+int mbx_launch_rule(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow, int32_t ncol,
+                    int32_t nnz, int32_t value_kind, void* stream, mbx_launch_choice* out);
+int mbx_plan_create_auto(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow,
+                         int32_t ncol, int32_t nnz, int32_t value_kind, void* stream,
+                         mbx_plan** out, mbx_launch_choice* choice);
+
 /* Timing autotuner (no reference counterpart: SIGMA and BLOCK are
  * compile-time in the reference, CMakeLists.txt:4-6): times every
  * instantiated items_per_thread on the given operands (which it overwrites in

@@ -30,7 +30,8 @@ ABI_SYMBOLS = (
     "mbx_colblock_create", "mbx_colblock_destroy", "mbx_colblock_info", "mbx_colblock_values",
     "mbx_colblock_spmv", "mbx_colblock_sddmm", "mbx_colblock_values_to_csr", "mbx_colblock_set_stream",
     "mbx_colblock_set_residency", "mbx_colblock_adjoint_prepare", "mbx_colblock_dx",
-    "mbx_csr_validate", "mbx_spmv_f64_noalloc", "mbx_plan_split_hot", "mbx_status_str",
+    "mbx_csr_validate", "mbx_spmv_f64_noalloc", "mbx_plan_split_hot",
+    "mbx_launch_rule", "mbx_plan_create_auto", "mbx_status_str",
 )
 GEN_SYMBOLS = (
     "mbg_fill_uniform_f64", "mbg_fill_uniform_f32", "mbg_fill_normal_f64",
@@ -65,6 +66,20 @@ class PatternStats(ctypes.Structure):
                 ("tile_items", ctypes.c_int32), ("single_row_tiles", ctypes.c_int32),
                 ("split_tiles", ctypes.c_int32), ("max_rows_per_tile", ctypes.c_int32),
                 ("long_rows", ctypes.c_int32)]
+
+
+class LaunchChoice(ctypes.Structure):
+    """mbx_launch_choice of include/merbit_amd.h."""
+    _fields_ = [("rule", ctypes.c_int32), ("items_per_thread", ctypes.c_int32),
+                ("stream_mode", ctypes.c_int32), ("wg_per_cu", ctypes.c_int32),
+                ("accumulation", ctypes.c_int32), ("compress_cols", ctypes.c_int32),
+                ("hot_columns", ctypes.c_int32), ("column_blocks", ctypes.c_int32),
+                ("cus", ctypes.c_int32), ("ntiles8", ctypes.c_int32),
+                ("max_tile_span", ctypes.c_int32), ("x_bytes", ctypes.c_double),
+                ("hot_share", ctypes.c_double)]
+
+
+RULE_NAMES = {1: "few_tiles", 2: "local", 3: "x_fits", 4: "hot_columns", 5: "spread_columns"}
 
 
 class CsrReport(ctypes.Structure):
@@ -164,6 +179,9 @@ def lib() -> ctypes.CDLL:
     L.mbx_pagerank_graph_create.argtypes = [vp, vp, vp, vp, f64, i32, vp, vp, vp,
                                             ctypes.c_size_t, ctypes.POINTER(vp)]
     L.mbx_csr_validate.argtypes = [vp, vp, i32, i32, i32, vp, ctypes.POINTER(CsrReport)]
+    L.mbx_launch_rule.argtypes = [vp, vp, i32, i32, i32, i32, vp, ctypes.POINTER(LaunchChoice)]
+    L.mbx_plan_create_auto.argtypes = [vp, vp, i32, i32, i32, i32, vp, ctypes.POINTER(vp),
+                                       ctypes.POINTER(LaunchChoice)]
     L.mbx_plan_split_hot.argtypes = [vp, vp, i32, vp, ctypes.POINTER(f64)]
     L.mbx_spmv_f64_noalloc.argtypes = [vp, vp, vp, i32, i32, i32, f64, vp, f64, vp, vp]
     L.mbx_status_str.argtypes = [ctypes.c_int]

@@ -26,7 +26,8 @@ _KIND = {torch.float64: MBX_VALUE_F64, torch.float32: MBX_VALUE_F32}
 
 class MerbitOperator:
     def __init__(self, row_ptr: torch.Tensor, col: torch.Tensor, ncol: int,
-                 items_per_thread: int = 8, adjoint_blocks: int = 0, blocks: int = 0):
+                 items_per_thread: int | None = None, adjoint_blocks: int | None = None,
+                 blocks: int | None = None, value_dtype=torch.float32):
         """adjoint_blocks: 0 for dx by one merge-path product on A^T; -1 or
         1..16 for dx through a column-blocked A^T (mbx_adjoint_set_blocks; -1
         picks the count), for graphs whose g outgrows one XCD's L2.
@@ -34,7 +35,18 @@ class MerbitOperator:
         form too (values stay in CSR order and are moved into block order per
         forward call), and sets adjoint_blocks to -1 unless it is given.  dval
         stays the CSR SDDMM: moving a blocked dval back to CSR order costs
-        more than the blocked SDDMM saves."""
+        more than the blocked SDDMM saves.
+        None (the defaults): the launch rule (mbx_launch_rule, for values of
+        value_dtype) picks the forward plan, and both blocked forms when it
+        finds spread columns over an x larger than L2 (random graphs)."""
+        spread = False
+        if blocks is None or adjoint_blocks is None:
+            from .spmv import launch_rule
+            spread = launch_rule(row_ptr, col, ncol, value_dtype)["rule"] == "spread_columns"
+        if blocks is None:
+            blocks = -1 if spread else 0
+        if adjoint_blocks is None:
+            adjoint_blocks = -1 if (blocks or spread) else 0
         if blocks and not adjoint_blocks:
             adjoint_blocks = -1
         self.adjoint_blocks = adjoint_blocks
@@ -43,14 +55,16 @@ class MerbitOperator:
         self.nrow = row_ptr.numel() - 1
         self.ncol = int(ncol)
         self.nnz = col.numel()
-        self.items_per_thread = items_per_thread
-        self.plan = MerbitPlan(row_ptr, ncol, self.nnz, items_per_thread)
+        self.plan = MerbitPlan(row_ptr, ncol, self.nnz, items_per_thread, col=col,
+                               value_dtype=value_dtype)
+        self.items_per_thread = self.plan.items_per_thread
         self._adj = None
         self._cb = None
         if blocks:
             from .spmv import ColumnBlockedCsr
             self._cb = ColumnBlockedCsr(row_ptr, col, ncol, 0 if blocks < 0 else blocks,
-                                        items_per_thread)
+                                        8 if self.items_per_thread not in (8, 12) else
+                                        self.items_per_thread)
         self._vb = None
 
     @property

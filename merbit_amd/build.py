@@ -48,6 +48,7 @@ LIB_SOURCES = [
     os.path.join(CSRC, "merbit_colblock.hip"),
     os.path.join(CSRC, "merbit_validate.hip"),
     os.path.join(CSRC, "merbit_hot.hip"),
+    os.path.join(CSRC, "merbit_rule.hip"),
     os.path.join(CSRC, "merbit_gen.cpp"),
     os.path.join(CSRC, "merbit_host.cpp"),
 ]

@@ -1,0 +1,251 @@
+// merbit_rule.hip -- the launch rule: pattern statistics -> launch shape
+// (mbx_launch_rule, mbx_plan_create_auto).
+//
+// The reference computes pattern statistics (degree moments, empty rows,
+// src/class.cu:223-381; fast and normal tiles per block, include/tile.cuh:
+// 177-233) but compiles one SIGMA and BLOCK for every matrix.  Here the
+// statistics choose the launch, without a timing sweep, from what bounds
+// the product (measured on MI355X, warm and rotated copies):
+//   * few tiles (under 16 per CU at 8 items per thread): the launch is
+//     short and unevenly filled, 4 items per thread give twice the tiles;
+//   * local columns (every tile of 2048 merge items spans < 65536
+//     columns, e.g. banded): the gathers hit L2 and the stream is the bound:
+//     every tile resident, 16-bit column offsets, and FP64 sums for FP32 /
+//     bfloat16 values (the compensated sums are VALU-bound once the stream
+//     is cut); 8 items per thread for FP64 values, 12 for the 4- and 2-byte
+//     values;
+//   * x of at most 2 MiB: the gathers stay in L2; 8 items per thread, at
+//     most 4 tiles per CU;
+//   * larger x with a hot column set (the kHot* most frequent columns hold
+//     at least a quarter of the entries, power-law patterns): the product is
+//     bound by the vector L1's miss slots; the hot values go to LDS
+//     (mbx_plan_split_hot) and the tiles per CU follow from that LDS;
+//   * larger x without one (uniform random graphs): 3 tiles per CU, and the
+//     column-blocked form (mbx_colblock_*) is recommended with slices of at
+//     most 3.5 MiB of x.
+// The stream policy stays MBX_STREAM_AUTO (by the product's bytes).
+
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+#include <limits.h>
+#include <stdint.h>
+
+#include "merbit_amd.h"
+#include "merbit_internal.hpp"
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kHotF64 = 4096;   // 32 KB of FP64 x: 3 tiles per CU with it in LDS
+constexpr int kHotF32 = 5800;   // 23 KB of FP32 x: 4 tiles per CU
+constexpr double kHotShare = 0.25;
+constexpr long long kXFitsBytes = 2ll << 20;
+constexpr long long kSliceBytes = (7ll << 20) / 2;  // column-blocked form's x slice
+constexpr int kFewTilesPerCu = 16;
+
+inline int blocks(long long n) { return (int)((n + kBlock - 1) / kBlock); }
+
+__global__ void rule_count_kernel(const int32_t* __restrict__ col, int nnz, int ncol,
+                                  int32_t* __restrict__ cnt) {
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k < nnz) {
+    const int c = col[k];
+    if (c >= 0 && c < ncol) atomicAdd(cnt + c, 1);
+  }
+}
+
+// Entries in the k most frequent columns (sorted counts, descending).
+__global__ void rule_top_kernel(const int32_t* __restrict__ scnt, int k,
+                                unsigned long long* __restrict__ out) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < k) atomicAdd(out, (unsigned long long)scnt[i]);
+}
+
+// Widest column range of one tile (one workgroup per tile).
+__global__ __launch_bounds__(256) void rule_span_kernel(const int32_t* __restrict__ col,
+                                                        const int4* __restrict__ coord,
+                                                        int* __restrict__ span) {
+  __shared__ int s_lo[4], s_hi[4];
+  const int t = blockIdx.x;
+  const int k0 = coord[t].y, k1 = coord[t + 1].y;
+  int lo = INT_MAX, hi = INT_MIN;
+  for (int k = k0 + (int)threadIdx.x; k < k1; k += 256) {
+    const int c = col[k];
+    lo = min(lo, c);
+    hi = max(hi, c);
+  }
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    lo = min(lo, __shfl_xor(lo, d));
+    hi = max(hi, __shfl_xor(hi, d));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    s_lo[threadIdx.x >> 6] = lo;
+    s_hi[threadIdx.x >> 6] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && k1 > k0) {
+    lo = min(min(s_lo[0], s_lo[1]), min(s_lo[2], s_lo[3]));
+    hi = max(max(s_hi[0], s_hi[1]), max(s_hi[2], s_hi[3]));
+    const long long w = (long long)hi - lo;
+    atomicMax(span, w > INT_MAX ? INT_MAX : (int)w);
+  }
+}
+
+// The column features: the widest tile span at 8 items per thread and the
+// share of the entries in the `hot` most frequent columns.
+int column_features(const int32_t* rp, const int32_t* col, int nrow, int ncol, int nnz, int hot,
+                    hipStream_t s, int* span_out, double* share_out) {
+  *span_out = 0;
+  *share_out = 0.0;
+  if (nnz == 0 || ncol == 0) return MBX_OK;
+  mbx_plan* p8 = nullptr;
+  int st = mbx_plan_create(rp, nrow, ncol, nnz, 8, 256, s, &p8);
+  if (st != MBX_OK) return st;
+  int32_t *cnt = nullptr, *scnt = nullptr;
+  int* d_span = nullptr;
+  unsigned long long* d_top = nullptr;
+  void* tmp = nullptr;
+  size_t tmp_bytes = 0;
+  if (hipMalloc((void**)&cnt, 4 * (size_t)ncol) != hipSuccess ||
+      hipMalloc((void**)&scnt, 4 * (size_t)ncol) != hipSuccess ||
+      hipMalloc((void**)&d_span, sizeof(int)) != hipSuccess ||
+      hipMalloc((void**)&d_top, sizeof(unsigned long long)) != hipSuccess)
+    st = MBX_ENOMEM;
+  if (st == MBX_OK && (hipMemsetAsync(cnt, 0, 4 * (size_t)ncol, s) != hipSuccess ||
+                       hipMemsetAsync(d_span, 0, sizeof(int), s) != hipSuccess ||
+                       hipMemsetAsync(d_top, 0, sizeof(unsigned long long), s) != hipSuccess))
+    st = MBX_EHIP;
+  if (st == MBX_OK) {
+    rule_span_kernel<<<p8->ntiles, 256, 0, s>>>(col, p8->d_coord, d_span);
+    rule_count_kernel<<<blocks(nnz), kBlock, 0, s>>>(col, nnz, ncol, cnt);
+    if (hipcub::DeviceRadixSort::SortKeysDescending(nullptr, tmp_bytes, cnt, scnt, ncol, 0, 32,
+                                                    s) != hipSuccess ||
+        hipMalloc(&tmp, tmp_bytes ? tmp_bytes : 1) != hipSuccess ||
+        hipcub::DeviceRadixSort::SortKeysDescending(tmp, tmp_bytes, cnt, scnt, ncol, 0, 32, s) !=
+            hipSuccess)
+      st = MBX_EHIP;
+  }
+  const int k = hot < ncol ? hot : ncol;
+  unsigned long long top = 0;
+  int span = 0;
+  if (st == MBX_OK) {
+    rule_top_kernel<<<blocks(k), kBlock, 0, s>>>(scnt, k, d_top);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(&top, d_top, sizeof(top), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(&span, d_span, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess)
+      st = MBX_EHIP;
+  }
+  if (hipStreamSynchronize(s) != hipSuccess && st == MBX_OK) st = MBX_EHIP;
+  if (cnt) (void)hipFree(cnt);
+  if (scnt) (void)hipFree(scnt);
+  if (d_span) (void)hipFree(d_span);
+  if (d_top) (void)hipFree(d_top);
+  if (tmp) (void)hipFree(tmp);
+  mbx_plan_destroy(p8);
+  if (st != MBX_OK) {
+    (void)hipGetLastError();
+    return st;
+  }
+  *span_out = span;
+  *share_out = (double)top / (double)nnz;
+  return MBX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mbx_launch_rule(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow, int32_t ncol,
+                    int32_t nnz, int32_t value_kind, void* stream, mbx_launch_choice* out) {
+  if (!out) return MBX_EINVAL;
+  if (nrow < 0 || ncol < 0 || nnz < 0 || (nrow > 0 && !d_row_ptr) || (nnz > 0 && !d_col))
+    return MBX_EINVAL;
+  if (value_kind != MBX_VALUE_F64 && value_kind != MBX_VALUE_F32 &&
+      value_kind != MBX_VALUE_BF16)
+    return MBX_EINVAL;
+  mbx_launch_choice c = {};
+  c.stream_mode = MBX_STREAM_AUTO;
+  c.accumulation = MBX_ACC_COMPENSATED_F32;
+  const bool f64 = value_kind == MBX_VALUE_F64;
+  const int sx = f64 ? 8 : 4;
+  const int hot = f64 ? kHotF64 : kHotF32;
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return MBX_EHIP;
+  }
+  c.cus = cus;
+  c.x_bytes = (double)ncol * sx;
+  c.ntiles8 = (int32_t)(((long long)nrow + nnz + 2047) / 2048);
+  if (nnz > 0) {
+    int span = 0;
+    double share = 0.0;
+    const int st = column_features(d_row_ptr, d_col, nrow, ncol, nnz, hot, (hipStream_t)stream,
+                                   &span, &share);
+    if (st != MBX_OK) return st;
+    c.max_tile_span = span;
+    c.hot_share = share;
+  }
+  const bool local = nnz > 0 && c.max_tile_span < 65536;
+  if (c.ntiles8 < kFewTilesPerCu * cus) {
+    c.items_per_thread = 4;
+    c.rule = MBX_RULE_FEW_TILES;
+  } else if (local) {
+    c.items_per_thread = f64 ? 8 : 12;
+    c.compress_cols = 1;
+    if (!f64) c.accumulation = MBX_ACC_F64;
+    c.rule = MBX_RULE_LOCAL;
+  } else if ((long long)ncol * sx <= kXFitsBytes) {
+    c.items_per_thread = 8;
+    c.wg_per_cu = 4;
+    c.rule = MBX_RULE_X_FITS;
+  } else if (c.hot_share >= kHotShare) {
+    c.items_per_thread = 8;
+    c.hot_columns = hot;
+    c.rule = MBX_RULE_HOT_COLUMNS;
+  } else {
+    c.items_per_thread = 8;
+    c.wg_per_cu = 3;
+    const long long b = ((long long)ncol * sx + kSliceBytes - 1) / kSliceBytes;
+    c.column_blocks = (int32_t)(b > 16 ? 16 : b);
+    c.rule = MBX_RULE_SPREAD_COLUMNS;
+  }
+  *out = c;
+  return MBX_OK;
+}
+
+int mbx_plan_create_auto(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow,
+                         int32_t ncol, int32_t nnz, int32_t value_kind, void* stream,
+                         mbx_plan** out, mbx_launch_choice* choice) {
+  if (!out) return MBX_EINVAL;
+  *out = nullptr;
+  mbx_launch_choice c;
+  int st = mbx_launch_rule(d_row_ptr, d_col, nrow, ncol, nnz, value_kind, stream, &c);
+  if (st != MBX_OK) return st;
+  mbx_plan* p = nullptr;
+  st = mbx_plan_create(d_row_ptr, nrow, ncol, nnz, c.items_per_thread, 256, stream, &p);
+  if (st != MBX_OK) return st;
+  (void)mbx_plan_set_stream(p, c.stream_mode);
+  (void)mbx_plan_set_residency(p, c.wg_per_cu);
+  (void)mbx_plan_set_accumulation(p, c.accumulation);
+  if (c.compress_cols) {
+    st = mbx_plan_compress_cols(p, d_col, stream);
+    if (st == MBX_EUNSUPPORTED) {  // cannot happen for a local pattern; keep 32-bit columns
+      c.compress_cols = 0;
+      st = MBX_OK;
+    }
+  }
+  if (st == MBX_OK && c.hot_columns > 0) st = mbx_plan_split_hot(p, d_col, c.hot_columns, stream,
+                                                                 nullptr);
+  if (st != MBX_OK) {
+    mbx_plan_destroy(p);
+    return st;
+  }
+  if (choice) *choice = c;
+  *out = p;
+  return MBX_OK;
+}
+
+}  // extern "C"

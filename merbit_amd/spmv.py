@@ -60,19 +60,62 @@ def validate_csr(row_ptr: torch.Tensor, col: torch.Tensor, ncol: int, stream=Non
     return out
 
 
+_KIND_OF_DTYPE = {torch.float64: 0, torch.float32: 1, torch.bfloat16: 2, torch.uint16: 2}
+
+
+def _choice_dict(ch) -> dict:
+    from ._lib import RULE_NAMES, LaunchChoice
+    out = {name: getattr(ch, name) for name, _ in LaunchChoice._fields_}
+    out["rule"] = RULE_NAMES.get(out["rule"], out["rule"])
+    return out
+
+
+def launch_rule(row_ptr: torch.Tensor, col: torch.Tensor, ncol: int,
+                value_dtype=torch.float64, stream=None) -> dict:
+    """The launch the pattern's statistics choose (mbx_launch_rule): items
+    per thread, tiles per CU, accumulation, 16-bit columns, hot columns and
+    a column-blocked recommendation, with the features that decided."""
+    from ._lib import LaunchChoice
+    nrow, nnz = int(row_ptr.numel()) - 1, int(col.numel())
+    ch = LaunchChoice()
+    check(lib().mbx_launch_rule(_dptr(row_ptr, torch.int32, "row_ptr"),
+                                _dptr(col, torch.int32, "col") if nnz else None, nrow, int(ncol),
+                                nnz, _KIND_OF_DTYPE[value_dtype], _stream_handle(stream),
+                                ctypes.byref(ch)), "mbx_launch_rule")
+    return _choice_dict(ch)
+
+
 class MerbitPlan:
-    """Merge-path partition of one CSR pattern (``TILE`` of the reference)."""
+    """Merge-path partition of one CSR pattern (``TILE`` of the reference).
+
+    items_per_thread None (the default) with the column array ``col``: the
+    launch rule (mbx_plan_create_auto) picks the plan's shape and options
+    from the pattern's statistics for values of ``value_dtype``; the choice
+    is kept in ``self.choice``.  Without ``col`` the default is 8 items per
+    thread."""
 
     def __init__(self, row_ptr: torch.Tensor, ncol: int, nnz: int | None = None,
-                 items_per_thread: int = 8, wg_size: int = 256, stream=None):
+                 items_per_thread: int | None = None, wg_size: int = 256, stream=None,
+                 col: torch.Tensor | None = None, value_dtype=torch.float64):
         self.nrow = int(row_ptr.numel()) - 1
         self.ncol = int(ncol)
         self.nnz = int(row_ptr[-1].item()) if nnz is None else int(nnz)
+        self.choice = None
         h = ctypes.c_void_p()
-        check(lib().mbx_plan_create(_dptr(row_ptr, torch.int32, "row_ptr"), self.nrow,
-                                    self.ncol, self.nnz, items_per_thread, wg_size,
-                                    _stream_handle(stream), ctypes.byref(h)),
-              "mbx_plan_create")
+        if items_per_thread is None and col is not None and wg_size == 256:
+            from ._lib import LaunchChoice
+            ch = LaunchChoice()
+            check(lib().mbx_plan_create_auto(
+                _dptr(row_ptr, torch.int32, "row_ptr"),
+                _dptr(col, torch.int32, "col") if self.nnz else None, self.nrow, self.ncol,
+                self.nnz, _KIND_OF_DTYPE[value_dtype], _stream_handle(stream), ctypes.byref(h),
+                ctypes.byref(ch)), "mbx_plan_create_auto")
+            self.choice = _choice_dict(ch)
+        else:
+            check(lib().mbx_plan_create(_dptr(row_ptr, torch.int32, "row_ptr"), self.nrow,
+                                        self.ncol, self.nnz, items_per_thread or 8, wg_size,
+                                        _stream_handle(stream), ctypes.byref(h)),
+                  "mbx_plan_create")
         self._h = h
         nt, ti, wg = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
         check(lib().mbx_plan_info(h, ctypes.byref(nt), ctypes.byref(ti), ctypes.byref(wg)),
@@ -289,8 +332,9 @@ class MerbitPlan:
 class DeviceCsr:
     """A CSR matrix resident in HBM with its plan (GRAPH + TILE together)."""
 
-    def __init__(self, A, device="cuda", value_dtype=None, items_per_thread: int = 8,
+    def __init__(self, A, device="cuda", value_dtype=None, items_per_thread: int | None = None,
                  wg_size: int = 256):
+        """items_per_thread None: the launch rule picks the plan (see MerbitPlan)."""
         self.nrow, self.ncol, self.nnz = A.nrow, A.ncol, A.nnz
         self.row_ptr = torch.from_numpy(np.ascontiguousarray(A.row_ptr)).to(device)
         self.col = torch.from_numpy(np.ascontiguousarray(A.col)).to(device)
@@ -298,7 +342,8 @@ class DeviceCsr:
         self.val = torch.from_numpy(val).to(device)
         if value_dtype is not None and self.val.dtype != value_dtype:
             self.val = self.val.to(value_dtype)
-        self.plan = MerbitPlan(self.row_ptr, self.ncol, self.nnz, items_per_thread, wg_size)
+        self.plan = MerbitPlan(self.row_ptr, self.ncol, self.nnz, items_per_thread, wg_size,
+                               col=self.col, value_dtype=self.val.dtype)
 
     def spmv(self, x, y=None, alpha=1.0, beta=0.0, stream=None):
         return self.plan.spmv(self.row_ptr, self.col, self.val, x, y, alpha, beta, stream)
