@@ -120,42 +120,88 @@ def load_traffic(name):
 # ---------------------------------------------------------------------------
 
 
-def pick_ipt(args, rp, ncol, nnz, make_calls):
+HOT_COLUMNS = {"f64": 4096, "f32": 5800, "bf16": 5800}
+
+
+def make_plans(copies, ncol, nnz, ipt, hot=0, kind="f64"):
+    """One plan per rotated copy (each keeps its own hot-column copy of the
+    columns when hot > 0, so no array is shared across the rotation)."""
+    from merbit_amd.spmv import MerbitPlan
+    plans = []
+    for c in copies:
+        p = MerbitPlan(c["rp"], ncol, nnz, items_per_thread=ipt)
+        if hot:
+            p.split_hot(c["col"], hot)
+        plans.append(p)
+    return plans
+
+
+def rule_plans(copies, ncol, nnz, value_dtype):
+    """The launch rule's plans (MerbitPlan's default), one per copy."""
+    from merbit_amd.spmv import MerbitPlan
+    return [MerbitPlan(c["rp"], ncol, nnz, col=c["col"], value_dtype=value_dtype)
+            for c in copies]
+
+
+def event_setter(plans):
+    """For time_steps' event pass: the i-th call's plan records the events
+    around its tile kernel (the calls cycle over the plans in order)."""
+    state = {"i": 0}
+
+    def setter(b, e):
+        plans[state["i"] % len(plans)].set_events(b, e)
+        state["i"] += 1
+    return setter
+
+
+def spmv_calls(plans, copies):
+    return [(lambda p=p, c=c: p.spmv(c["rp"], c["col"], c["val"], c["x"], c["y"]))
+            for p, c in zip(plans, copies)]
+
+
+def tune_shape(args, copies, ncol, nnz, kind, hot_ok):
     """Launch shape: --ipt / --stream / --residency when given, else the
-    (items per thread, stream policy, tiles per CU) triple whose plan runs
-    the timed workload (make_calls(plan): the rotated products) fastest over
-    a short timing of each (the first, in shape-policy-residency order, of
-    those within 1 % of the fastest).  Tuning on the rotated copies matters:
-    on one copy resident in the 256 MB cache the candidates 8, 12 and 16 time
-    alike on the R-MAT matrix, while rotated copies separate them by 5 %; a
-    matrix read once per product gains from non-temporal stream loads that
-    cost a cache-resident one 6 %; and fewer tiles per CU cut the L1 misses
-    of gather-bound products while HBM-bound ones want every tile resident."""
-    from merbit_amd.spmv import ITEMS_PER_THREAD, MerbitPlan
-    ipts = [args.ipt] if args.ipt else list(ITEMS_PER_THREAD)
+    (items per thread, stream policy, tiles per CU, hot columns) whose plans
+    run the timed workload (the rotated copies) fastest over a short timing
+    of each (the first, in candidate order, within 1 % of the fastest).
+    Tuning on the rotated copies matters: on one copy resident in the 256 MB
+    cache the shapes time alike on the R-MAT matrix, while rotated copies
+    separate them; a matrix read once per product gains from non-temporal
+    stream loads that cost a cache-resident one 6 %; fewer tiles per CU cut
+    the L1 misses of gather-bound products while HBM-bound ones want every
+    tile resident; the hot-column split (items per thread 8 and 12) pays on
+    power-law patterns."""
+    from merbit_amd.spmv import ITEMS_PER_THREAD
+    ipts = [args.ipt] if args.ipt else [i for i in ITEMS_PER_THREAD if i in (4, 8, 12, 16)]
     modes = [args.stream] if args.stream else ["cached", "nontemporal"]
     wgs = [args.residency] if args.residency is not None else [0, 3, 4, 5]
-    if len(ipts) == 1 and len(modes) == 1 and len(wgs) == 1:
-        return ipts[0], modes[0], wgs[0]
+    hots = [0, HOT_COLUMNS[kind]] if hot_ok else [0]
+    cands = [(i, m, w, h) for i in ipts for h in hots if not (h and i not in (8, 12))
+             for m in modes for w in (wgs if not h else [0])]
+    if len(cands) == 1:
+        return cands[0]
     times = {}
     for ipt in ipts:
-        plan = MerbitPlan(rp, ncol, nnz, items_per_thread=ipt)
-        calls = make_calls(plan)
-        for mode in modes:
-            plan.set_stream(mode)
-            for wg in wgs:
-                plan.set_residency(wg)
-                times[(ipt, mode, wg)] = min(
-                    time_steps(calls, 10 * len(calls), 2 * len(calls))[0] for _ in range(3))
-        plan.close()
-    # candidates within 1 % of the fastest time alike: take the first of
-    # them, so that run-to-run noise does not flip the choice
+        for hot in hots:
+            if hot and ipt not in (8, 12):
+                continue
+            plans = make_plans(copies, ncol, nnz, ipt, hot, kind)
+            calls = spmv_calls(plans, copies)
+            for mode in modes:
+                for p in plans:
+                    p.set_stream(mode)
+                for wg in (wgs if not hot else [0]):
+                    for p in plans:
+                        p.set_residency(wg)
+                    times[(ipt, mode, wg, hot)] = min(
+                        time_steps(calls, 10 * len(calls), 2 * len(calls))[0] for _ in range(3))
+            for p in plans:
+                p.close()
     fastest = min(times.values())
-    return min((k for k, ms in times.items() if ms <= 1.01 * fastest),
-               key=lambda k: (k[0], modes.index(k[1]), wgs.index(k[2])))
+    return next(k for k in cands if k in times and times[k] <= 1.01 * fastest)
 
 
-def bench_rmat_f64(A, x, dev, args):
+def bench_rmat_f64(A, x, dev, args, checks):
     import torch
     from merbit_amd import workloads
     from merbit_amd.spmv import MerbitPlan
@@ -166,155 +212,210 @@ def bench_rmat_f64(A, x, dev, args):
                    y=torch.empty(nrow, dtype=torch.float64, device=dev))
               for _ in range(max(1, args.copies))]
     c0 = copies[0]
-
-    def make_calls(plan):
-        return [(lambda c=c: plan.spmv(c["rp"], c["col"], c["val"], c["x"], c["y"]))
-                for c in copies]
-
-    ipt, stream, resid = pick_ipt(args, c0["rp"], ncol, nnz, make_calls)
+    ipt, stream, resid, hot = tune_shape(args, copies, ncol, nnz, "f64", not args.no_hot)
     plan_ms = []
     for _ in range(5):  # plan construction (the reference's tim1), host clock
         t0 = time.perf_counter()
         plan = MerbitPlan(c0["rp"], ncol, nnz, items_per_thread=ipt)
         plan_ms.append((time.perf_counter() - t0) * 1e3)
         plan.close()
-    plan = MerbitPlan(c0["rp"], ncol, nnz, items_per_thread=ipt)
-    plan.set_stream(stream)
-    plan.set_residency(resid)
-    calls = make_calls(plan)
-    ms, kern = time_steps(calls, args.steps, args.warmup, plan.set_events,
+    plans = make_plans(copies, ncol, nnz, ipt, hot)
+    for p in plans:
+        p.set_stream(stream)
+        p.set_residency(resid)
+    calls = spmv_calls(plans, copies)
+    # the tile kernel's own duration: events recorded by the library around
+    # it on the call's stream, for whichever copy's plan runs
+    ms, kern = time_steps(calls, args.steps, args.warmup, event_setter(plans),
                           graph=not args.no_graph)
+    checks.append(("headline_rmat_f64", A, A.val, x, c0["y"].cpu().numpy(), 1e-12))
     warm_ms = warm_kern = None
     if not args.no_warm:  # one copy over and over: cache-resident, plain loads
-        plan.set_stream("cached")
-        plan.set_residency(0)
-        warm_ms, warm_kern = time_steps(calls[:1], args.steps, args.warmup, plan.set_events)
-    log(f"rmat fp64 (ipt {ipt}, {stream} stream, {resid or 'all'} tiles/CU): "
+        plans[0].set_stream("cached")
+        warm_ms, warm_kern = time_steps(calls[:1], args.steps, args.warmup, plans[0].set_events)
+    ceilings = None
+    if not args.no_probes and ipt in (8, 12):
+        ceilings = bench_ceilings(A, copies, ncol, nnz, ipt, stream, resid, args)
+    # the launch rule's plans (the library default), on the same rotation
+    rp_rule = rule_plans(copies, ncol, nnz, torch.float64)
+    rule_ms, _ = time_steps(spmv_calls(rp_rule, copies), args.steps, args.warmup,
+                            graph=not args.no_graph)
+    rule = dict(rp_rule[0].choice)
+    for p in rp_rule:
+        p.close()
+    log(f"rmat fp64 (ipt {ipt}, {stream} stream, {resid or 'all'} tiles/CU, hot {hot}): "
         f"{ms * 1e3:.1f} us/step rotated ({gbs(b_alg, ms)} GB/s), tile kernel "
-        f"{kern * 1e3:.1f} us" + (f"; warm {warm_ms * 1e3:.1f} us" if warm_ms else ""))
+        f"{kern * 1e3:.1f} us; rule ({rule['rule']}) {rule_ms * 1e3:.1f} us"
+        + (f"; warm {warm_ms * 1e3:.1f} us" if warm_ms else ""))
     res = dict(b_alg=b_alg, ms=ms, kern=kern, warm_ms=warm_ms, warm_kern=warm_kern,
-               ntiles=plan.ntiles, tile_items=plan.tile_items, ipt=ipt, stream=stream,
-               resid=resid,
-               plan_ms=float(np.median(plan_ms)))
-    plan.close()
+               ntiles=plans[0].ntiles, tile_items=plans[0].tile_items, ipt=ipt, stream=stream,
+               resid=resid, hot=hot, plan_ms=float(np.median(plan_ms)), ceilings=ceilings,
+               rule={"us_per_step": round(rule_ms * 1e3, 2), "gbs": gbs(b_alg, rule_ms),
+                     "choice": {k: (round(v, 4) if isinstance(v, float) else v)
+                                for k, v in rule.items()}})
+    for p in plans:
+        p.close()
     del copies
     return res
 
 
-def bench_mixed(name, A, x, dev, args, f64_ms=None, copies=1):
-    """FP64 vs FP32 vs BF16 values on one pattern; errors against the FP64
-    path on the same stored values."""
+def bench_ceilings(A, copies, ncol, nnz, ipt, stream, resid, args):
+    """The headline tile kernel's two halves on their own (probes.hip, at
+    the same tile shape, residency and stream policy, rotated copies): the
+    stream probe reads row_ptr, col and the values as the tile kernel stages
+    them and writes y; the gather probe reads col and gathers x.  Beside
+    them the plain product (no hot split) of the same plans."""
+    import torch
+    from merbit_amd.compare import probe
+    plans = make_plans(copies, ncol, nnz, ipt, 0)
+    for p in plans:
+        p.set_stream(stream)
+        p.set_residency(resid)
+    outs = [torch.empty(max(A.nrow, plans[0].ntiles), dtype=torch.float64,
+                        device=copies[0]["x"].device) for _ in copies]
+    res = {}
+    for which in ("stream", "gather"):
+        calls = [(lambda p=p, c=c, o=o: probe(p, which, c["rp"], c["col"], c["val"], c["x"], o))
+                 for p, c, o in zip(plans, copies, outs)]
+        res[which] = time_steps(calls, args.steps, args.warmup, graph=not args.no_graph)[0]
+    res["plain"] = time_steps(spmv_calls(plans, copies), args.steps, args.warmup,
+                              graph=not args.no_graph)[0]
+    for p in plans:
+        p.close()
+    b_stream = 4 * (A.nrow + 1) + 12 * nnz + 8 * A.nrow
+    b_gather = 4 * nnz + 8 * ncol
+    out = {"stream_only_us": round(res["stream"] * 1e3, 2),
+           "stream_only_gbs": gbs(b_stream, res["stream"]),
+           "gather_only_us": round(res["gather"] * 1e3, 2),
+           "gather_only_gathers_per_s": round(nnz / (res["gather"] * 1e-3) / 1e9, 1),
+           "plain_product_us": round(res["plain"] * 1e3, 2),
+           "stream_plus_gather_us": round((res["stream"] + res["gather"]) * 1e3, 2),
+           "shape": {"items_per_thread": ipt, "stream": stream, "tiles_per_cu": resid},
+           "what": "probes.hip: the tile kernel's staging loads alone (stream) and its column "
+                   "loads plus x gathers alone (gather), G gathers/s"}
+    log(f"ceilings: stream only {out['stream_only_us']} us ({out['stream_only_gbs']} GB/s), "
+        f"gather only {out['gather_only_us']} us, plain product {out['plain_product_us']} us")
+    return out
+
+
+def bench_mixed(name, A, x, dev, args, checks, f64_ms=None, copies=1):
+    """FP64 vs FP32 vs BF16 values on one pattern: the tuned launch (with
+    the hot split as a candidate on the power-law pattern), the same plans
+    with FP64 sums and (banded) 16-bit columns, and the launch rule's plans
+    (the library default).  Outputs go to `checks` for the oracle leg."""
     import torch
     from merbit_amd import gen, workloads
-    from merbit_amd.spmv import MerbitPlan
     nrow, ncol, nnz = A.nrow, A.ncol, A.nnz
     rp = torch.from_numpy(A.row_ptr).to(dev)
-    cols = [torch.from_numpy(A.col).to(dev) for _ in range(copies)]
     v32 = np.ascontiguousarray(A.val, dtype=np.float32)
     x32 = np.ascontiguousarray(x, dtype=np.float32)
     out = {}
 
-    def compressed(plan, calls, ref_ms):
+    def make_copies(stored, xs, ydt):
+        xx = torch.from_numpy(xs).to(dev)
+        return [dict(rp=rp, col=torch.from_numpy(A.col).to(dev),
+                     val=torch.from_numpy(stored).to(dev), x=xx,
+                     y=torch.empty(nrow, dtype=ydt, device=dev)) for _ in range(copies)]
+
+    def timed(plans, cps):
+        return time_steps(spmv_calls(plans, cps), args.steps, args.warmup, event_setter(plans))
+
+    def compressed(plans, cps, ref_ms):
         """The same products with 16-bit column offsets (banded only: its
-        tiles span a few hundred columns); None when the plan refuses."""
-        if name != "banded" or not plan.compress_cols(cols[0]):
+        tiles span a few hundred columns); None when a plan refuses."""
+        if name != "banded" or not all(p.compress_cols(c["col"]) for p, c in zip(plans, cps)):
             return None
-        ms_c, kern_c = time_steps(calls, args.steps, args.warmup, plan.set_events)
-        plan.compress_cols(None)
-        return {"us_per_step": round(ms_c * 1e3, 2), "tile_kernel_us": round(kern_c * 1e3, 2),
+        ms_c, kern_c = timed(plans, cps)
+        for p in plans:
+            p.compress_cols(None)
+        return {"us_per_step": round(ms_c * 1e3, 2),
+                "tile_kernel_us": None if kern_c is None else round(kern_c * 1e3, 2),
                 "speedup_vs_f64": round(ref_ms / ms_c, 3), "column_bytes_per_nnz": 2}
 
-    if f64_ms is None:  # FP64 values = the FP32 values widened
-        vals = [torch.from_numpy(v32.astype(np.float64)).to(dev) for _ in range(copies)]
-        xx = torch.from_numpy(x32.astype(np.float64)).to(dev)
-        y = torch.empty(nrow, dtype=torch.float64, device=dev)
-        def make_calls(plan, vals=vals, xx=xx, y=y):
-            return [(lambda i=i: plan.spmv(rp, cols[i], vals[i], xx, y)) for i in range(copies)]
-
-        ipt, stream, resid = pick_ipt(args, rp, ncol, nnz, make_calls)
-        plan = MerbitPlan(rp, ncol, nnz, items_per_thread=ipt)
-        plan.set_stream(stream)
-        plan.set_residency(resid)
-        calls = make_calls(plan)
-        f64_ms, kern = time_steps(calls, args.steps, args.warmup, plan.set_events)
-        b64 = workloads.algorithmic_bytes(nrow, ncol, nnz, 8, 8, 8)
-        out["f64"] = {"gbs": gbs(b64, f64_ms), "us_per_step": round(f64_ms * 1e3, 2),
-                      "tile_kernel_us": round(kern * 1e3, 2), "items_per_thread": ipt,
-                      "stream": stream, "tiles_per_cu": resid,
-                      "pct_hbm_peak": round(100 * b64 / (f64_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 1)}
-        c = compressed(plan, calls, f64_ms)
+    kinds = (["f64"] if f64_ms is None else []) + ["f32", "bf16"]
+    for kind in kinds:
+        if kind == "f64":  # FP64 values = the FP32 values widened
+            stored, xs, s_val, s_x, ydt = v32.astype(np.float64), x32.astype(np.float64), 8, 8, \
+                torch.float64
+        else:
+            stored = v32 if kind == "f32" else gen.to_bf16(v32)
+            xs, s_val, s_x, ydt = x32, (4 if kind == "f32" else 2), 4, torch.float32
+        cps = make_copies(stored, xs, ydt)
+        b = workloads.algorithmic_bytes(nrow, ncol, nnz, s_val, s_x, s_x)
+        ipt, stream, resid, hot = tune_shape(args, cps, ncol, nnz, kind,
+                                             name == "rmat" and not args.no_hot)
+        plans = make_plans(cps, ncol, nnz, ipt, hot, kind)
+        for p in plans:
+            p.set_stream(stream)
+            p.set_residency(resid)
+        ms, kern = timed(plans, cps)
+        if kind == "f64":
+            f64_ms = ms
+        tol = 1e-12 if kind == "f64" else 1e-6
+        checks.append((f"{name}_{kind}", A, stored, xs, cps[0]["y"].cpu().numpy(), tol))
+        rec = {"gbs": gbs(b, ms), "gflops": round(2 * nnz / (ms * 1e-3) / 1e9, 1),
+               "us_per_step": round(ms * 1e3, 2),
+               "tile_kernel_us": None if kern is None else round(kern * 1e3, 2),
+               "items_per_thread": ipt, "stream": stream, "tiles_per_cu": resid,
+               "hot_columns": hot,
+               "pct_hbm_peak": round(100 * b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 1)}
+        if kind != "f64":
+            rec["speedup_vs_f64"] = round(f64_ms / ms, 3)
+        c = compressed(plans, cps, f64_ms)
         if c:
-            out["f64"]["compressed_cols"] = c
-            log(f"{name} f64, 16-bit columns: {c['us_per_step']:.1f} us/step "
-                f"(x{c['speedup_vs_f64']:.2f})")
-        plan.close()
-        del vals, y
-    for kind in ("f32", "bf16"):
-        stored = v32 if kind == "f32" else gen.to_bf16(v32)
-        wide = v32.astype(np.float64) if kind == "f32" else gen.bf16_to_f32(stored).astype(np.float64)
-        s_val = 4 if kind == "f32" else 2
-        vals = [torch.from_numpy(stored).to(dev) for _ in range(copies)]
-        xx = torch.from_numpy(x32).to(dev)
-        y = torch.empty(nrow, dtype=torch.float32, device=dev)
-        def make_calls(plan, vals=vals, xx=xx, y=y):
-            return [(lambda i=i: plan.spmv(rp, cols[i], vals[i], xx, y)) for i in range(copies)]
-
-        ipt, stream, resid = pick_ipt(args, rp, ncol, nnz, make_calls)
-        plan = MerbitPlan(rp, ncol, nnz, items_per_thread=ipt)
-        plan.set_stream(stream)
-        plan.set_residency(resid)
-        calls = make_calls(plan)
-        ms, kern = time_steps(calls, args.steps, args.warmup, plan.set_events)
-        ref = plan.spmv(rp, cols[0], torch.from_numpy(wide).to(dev),
-                        torch.from_numpy(x32.astype(np.float64)).to(dev))
-        plan.spmv(rp, cols[0], vals[0], xx, y)
-        b = workloads.algorithmic_bytes(nrow, ncol, nnz, s_val, 4, 4)
-        out[kind] = {"gbs": gbs(b, ms), "gflops": round(2 * nnz / (ms * 1e-3) / 1e9, 1),
-                     "us_per_step": round(ms * 1e3, 2), "tile_kernel_us": round(kern * 1e3, 2),
-                     "speedup_vs_f64": round(f64_ms / ms, 3), "items_per_thread": ipt,
-                     "stream": stream, "tiles_per_cu": resid,
-                     "rel_err_vs_f64_same_values": rel_err(y, ref)}
-        log(f"{name} {kind}: {ms * 1e3:.1f} us/step (ipt {ipt}, {stream}, {resid or 'all'}/CU), "
-            f"x{f64_ms / ms:.2f} "
-            f"vs fp64, "
-            f"err {out[kind]['rel_err_vs_f64_same_values']:.2e}")
-        c = compressed(plan, calls, f64_ms)
-        if c:
-            out[kind]["compressed_cols"] = c
-            log(f"{name} {kind}, 16-bit columns: {c['us_per_step']:.1f} us/step "
-                f"(x{c['speedup_vs_f64']:.2f} vs plain fp64)")
-        # the same values with exact FP64 products and FP64 sums
-        plan.set_accumulation("f64")
-        ms_a, kern_a = time_steps(calls, args.steps, args.warmup, plan.set_events)
-        plan.spmv(rp, cols[0], vals[0], xx, y)
-        out[kind]["acc_f64"] = {"us_per_step": round(ms_a * 1e3, 2),
-                                "tile_kernel_us": round(kern_a * 1e3, 2),
-                                "gbs": gbs(b, ms_a), "speedup_vs_f64": round(f64_ms / ms_a, 3),
-                                "rel_err_vs_f64_same_values": rel_err(y, ref)}
-        log(f"{name} {kind}, FP64 sums: {ms_a * 1e3:.1f} us/step (x{f64_ms / ms_a:.2f} vs fp64, "
-            f"err {out[kind]['acc_f64']['rel_err_vs_f64_same_values']:.2e})")
-        c = compressed(plan, calls, f64_ms)
-        if c:
-            out[kind]["acc_f64"]["compressed_cols"] = c
-            log(f"{name} {kind}, FP64 sums, 16-bit columns: {c['us_per_step']:.1f} us/step "
-                f"(x{c['speedup_vs_f64']:.2f} vs plain fp64)")
-        plan.close()
-        del vals, y, ref
+            rec["compressed_cols"] = c
+        if kind != "f64":  # the same plans with exact FP64 products and FP64 sums
+            for p in plans:
+                p.set_accumulation("f64")
+            ms_a, kern_a = timed(plans, cps)
+            checks.append((f"{name}_{kind}_fp64_sums", A, stored, xs,
+                           cps[0]["y"].cpu().numpy(), tol))
+            rec["acc_f64"] = {"us_per_step": round(ms_a * 1e3, 2),
+                              "tile_kernel_us": None if kern_a is None else round(kern_a * 1e3, 2),
+                              "gbs": gbs(b, ms_a), "speedup_vs_f64": round(f64_ms / ms_a, 3)}
+            c = compressed(plans, cps, f64_ms)
+            if c:
+                rec["acc_f64"]["compressed_cols"] = c
+        for p in plans:
+            p.close()
+        # the launch rule's plans: what MerbitPlan does by default
+        rps = rule_plans(cps, ncol, nnz, {"f64": torch.float64, "f32": torch.float32,
+                                          "bf16": torch.bfloat16}[kind])
+        ms_r, _ = timed(rps, cps)
+        checks.append((f"{name}_{kind}_rule", A, stored, xs, cps[0]["y"].cpu().numpy(), tol))
+        ch = rps[0].choice
+        rec["rule"] = {"us_per_step": round(ms_r * 1e3, 2), "gbs": gbs(b, ms_r),
+                       "speedup_vs_f64": round(f64_ms / ms_r, 3), "rule": ch["rule"],
+                       "items_per_thread": ch["items_per_thread"],
+                       "tiles_per_cu": ch["wg_per_cu"], "compress_cols": ch["compress_cols"],
+                       "accumulation": "f64" if ch["accumulation"] else "compensated",
+                       "hot_columns": ch["hot_columns"]}
+        for p in rps:
+            p.close()
+        out[kind] = rec
+        log(f"{name} {kind}: tuned {ms * 1e3:.1f} us/step (ipt {ipt}, {stream}, "
+            f"{resid or 'all'}/CU, hot {hot})"
+            + ("" if kind == "f64" else f" x{f64_ms / ms:.2f} vs fp64")
+            + f"; rule {ch['rule']} {ms_r * 1e3:.1f} us (x{f64_ms / ms_r:.2f})")
+        del cps
     return out
 
 
-def bench_spmm(A, dev, args):
+def bench_spmm(A, dev, args, checks):
     import torch
     from merbit_amd import gen, workloads
     from merbit_amd.spmv import MerbitPlan
     nrow, ncol, nnz = A.nrow, A.ncol, A.nnz
     rp = torch.from_numpy(A.row_ptr).to(dev)
     col = torch.from_numpy(A.col).to(dev)
-    val = torch.from_numpy(A.val.astype(np.float32)).to(dev)
+    v32 = A.val.astype(np.float32)
+    val = torch.from_numpy(v32).to(dev)
+    # 32 right-hand sides, column-major; k = 8 takes the first 8
+    X32 = gen.uniform(ncol * 32, -1, 1, workloads.SEED["spmm_x"], np.float32)
+    X32_dev = torch.from_numpy(X32).to(dev).view(32, ncol)
     out = {}
     for k in (8, 32):
-        X = torch.from_numpy(gen.uniform(ncol * k, -1, 1, workloads.SEED["spmm_x"],
-                                         np.float32)).to(dev).view(k, ncol).t()  # col-major
+        X = X32_dev[:k].t()  # col-major view, leading dimension ncol
         Y = torch.empty((k, nrow), dtype=torch.float32, device=dev).t()
         # items per thread: the faster of the two SpMM tile shapes on this k
         tuned = {}
@@ -330,18 +431,42 @@ def bench_spmm(A, dev, args):
         ms, kern = time_steps([lambda: plan.spmm(rp, col, val, X, Y, workspace=ws)],
                               max(20, args.steps // 4), args.warmup, plan.set_events)
         b = workloads.algorithmic_bytes(nrow, ncol, nnz, 4, 4, 4, k=k)
-        # error of column 0 against the FP64 SpMV of the same values
-        y0 = plan.spmv(rp, col, val.double(), X[:, 0].double().contiguous())
+        checks.append((f"spmm_k{k}", A, v32, X32.reshape(32, ncol)[:k],
+                       Y.t().contiguous().cpu().numpy(), 1e-5))
         out[f"k{k}"] = {"gbs": gbs(b, ms), "gflops": round(2 * nnz * k / (ms * 1e-3) / 1e9, 1),
                         "us_per_step": round(ms * 1e3, 2),
                         "tile_kernel_us": round(kern * 1e3, 2),
                         "us_per_rhs": round(ms * 1e3 / k, 2),
                         "items_per_thread": ipt,
-                        "rel_err_col0_vs_f64": rel_err(Y[:, 0], y0),
                         "layout": "column-major X and Y, repacked per call"}
         log(f"spmm k={k} (ipt {ipt}): {ms * 1e3:.1f} us/step ({out[f'k{k}']['gbs']} GB/s)")
         plan.close()
-        del X, Y, ws
+        del Y, ws
+    return out
+
+
+def bench_laplacian(dev, args, checks):
+    """The 256 x 256 5-point Laplacian (65,536 rows): FP64 products on the
+    launch rule's plan, the steps captured as one HIP graph (the product is
+    launch-bound: a few microseconds)."""
+    import torch
+    from merbit_amd import workloads
+    from merbit_amd.spmv import MerbitPlan
+    A, x = workloads.laplacian(256)
+    c = dict(rp=torch.from_numpy(A.row_ptr).to(dev), col=torch.from_numpy(A.col).to(dev),
+             val=torch.from_numpy(A.val).to(dev), x=torch.from_numpy(x).to(dev),
+             y=torch.empty(A.nrow, dtype=torch.float64, device=dev))
+    plan = MerbitPlan(c["rp"], A.ncol, A.nnz, col=c["col"], value_dtype=torch.float64)
+    calls = spmv_calls([plan], [c])
+    ms, _ = time_steps(calls, 10 * args.steps, args.warmup, graph=not args.no_graph)
+    checks.append(("laplacian_f64", A, A.val, x, c["y"].cpu().numpy(), 1e-12))
+    b = workloads.algorithmic_bytes(A.nrow, A.ncol, A.nnz, 8, 8, 8)
+    out = {"nrow": A.nrow, "nnz": A.nnz, "us_per_step": round(ms * 1e3, 3), "gbs": gbs(b, ms),
+           "gflops": round(2 * A.nnz / (ms * 1e-3) / 1e9, 1), "rule": plan.choice["rule"],
+           "items_per_thread": plan.items_per_thread,
+           "timed_as": "eager launches" if args.no_graph else "one HIP graph of the steps"}
+    log(f"laplacian 256^2 fp64: {ms * 1e3:.2f} us/step ({out['gbs']} GB/s)")
+    plan.close()
     return out
 
 
@@ -434,7 +559,7 @@ def bench_rocsparse(A, x, dev, args, f64_ms):
     return out
 
 
-def bench_adjoint(dev, args):
+def bench_adjoint(dev, args, checks):
     """The differentiable product on a random graph (2.4M nodes, Poisson(25)
     out-degree, FP32 values): forward y = A x, dx = A^T g through the
     transposed merge path, dval = g[row] x[col] (SDDMM)."""
@@ -449,7 +574,8 @@ def bench_adjoint(dev, args):
     val = torch.from_numpy(A.val).to(dev)
     xx = torch.from_numpy(x).to(dev)
     gg = torch.from_numpy(g).to(dev)
-    op = MerbitOperator(rp, col, A.ncol, items_per_thread=args.ipt or 8)
+    op = MerbitOperator(rp, col, A.ncol, items_per_thread=args.ipt or 8, blocks=0,
+                        adjoint_blocks=0)
     t0 = time.time()
     op.adjoint_handle  # builds the transposed pattern once
     torch.cuda.synchronize()
@@ -467,6 +593,7 @@ def bench_adjoint(dev, args):
         out[name] = {"us": round(ms * 1e3, 2), "gbs": gbs(b, ms),
                      "gflops": round(2 * nnz / (ms * 1e-3) / 1e9, 1)}
     out["backward_us"] = round((dx_ms + dv_ms) * 1e3, 2)
+    checks.append(("graph_forward_f32", A, A.val, x, op.forward(val, xx).cpu().numpy(), 1e-6))
     log(f"adjoint: fwd {fwd_ms * 1e3:.1f} us, dx {dx_ms * 1e3:.1f} us, "
         f"dval {dv_ms * 1e3:.1f} us")
     # the same forward product on the column-blocked form of the pattern
@@ -503,7 +630,8 @@ def bench_adjoint(dev, args):
     dxb_ms, _ = time_steps([lambda: cb.adjoint_x(vb, gg, dxb)], steps, args.warmup)
     dx_ref = op.adjoint_x(val, gg)
     # the CSR-order adjoint with a blocked A^T: values stay in CSR order
-    opb = MerbitOperator(rp, col, A.ncol, items_per_thread=args.ipt or 8, adjoint_blocks=-1)
+    opb = MerbitOperator(rp, col, A.ncol, items_per_thread=args.ipt or 8, blocks=0,
+                         adjoint_blocks=-1)
     opb.adjoint_handle
     torch.cuda.synchronize()
     dxc_ms, _ = time_steps([lambda: opb.adjoint_x(val, gg)], steps, args.warmup)
@@ -513,7 +641,9 @@ def bench_adjoint(dev, args):
     log(f"csr adjoint dx through a blocked A^T: {dxc_ms * 1e3:.1f} us (x{dx_ms / dxc_ms:.2f})")
     opb.close()
     # the CSR-order operator with every product on blocked forms
-    opc = MerbitOperator(rp, col, A.ncol, items_per_thread=args.ipt or 8, blocks=-1)
+    # the operator's defaults: the launch rule picks its plan and, for this
+    # spread-column graph, both blocked forms
+    opc = MerbitOperator(rp, col, A.ncol)
     opc.adjoint_handle
     torch.cuda.synchronize()
     f_ms, _ = time_steps([lambda: opc.forward(val, xx)], steps, args.warmup)
@@ -523,7 +653,9 @@ def bench_adjoint(dev, args):
         "forward_us": round(f_ms * 1e3, 2), "dval_us": round(v_ms * 1e3, 2),
         "dx_us": round(x_ms * 1e3, 2), "step_us": round((f_ms + v_ms + x_ms) * 1e3, 2),
         "step_speedup": round((fwd_ms + dv_ms + dx_ms) / (f_ms + v_ms + x_ms), 2),
-        "values": "CSR order (moved into block order per forward call); dval by the CSR SDDMM"}
+        "values": "CSR order (moved into block order per forward call); dval by the CSR SDDMM",
+        "operator": "MerbitOperator defaults (launch rule: spread columns -> blocked forms)",
+        "blocked": opc._cb is not None}
     log(f"csr operator on blocked forms: fwd {f_ms * 1e3:.1f}, dval {v_ms * 1e3:.1f}, "
         f"dx {x_ms * 1e3:.1f} us (step x{(fwd_ms + dv_ms + dx_ms) / (f_ms + v_ms + x_ms):.2f})")
     opc.close()
@@ -605,7 +737,24 @@ def bench_copy(dev):
     return {"gbs": gbs(2 * 8 * n, ms), "bytes": 2 * 8 * n, "what": "torch device copy of 2 GiB"}
 
 
-def cpu_baseline(A, x, b_alg):
+def _errors(y, ya, absr):
+    """(max condition-scaled error |y - ya| / (|A||x|), normwise relative
+    error); rows with (|A||x|) = 0 must be exactly 0 (else inf)."""
+    y = np.asarray(y, np.float64)
+    d = np.abs(y - ya)
+    zero = absr == 0
+    cond = float("inf") if np.any(d[zero] != 0) else \
+        (float(np.max(d[~zero] / absr[~zero])) if np.any(~zero) else 0.0)
+    den = float(np.linalg.norm(ya))
+    return cond, float(np.linalg.norm(y - ya) / (den if den > 0 else 1.0))
+
+
+def cpu_leg(A, x, b_alg, checks, laplacian, banded):
+    """The host leg: the oracle's OpenMP FP64 CSR product timed on the host
+    cores (the CPU baseline; the reference has no CPU SpMV) for the R-MAT,
+    the Laplacian and the banded matrix, at every thread and at one, and the
+    oracle as the checker of every GPU result gathered in `checks`: its
+    accurate (Dot2) product on the same stored values."""
     from oracle import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     threads = max(1, min(threads, os.cpu_count() or 1))
@@ -615,30 +764,74 @@ def cpu_baseline(A, x, b_alg):
     t1, _, _ = oracle.spmv_csr_omp_time(A.row_ptr, A.col, A.val, x, 1, warmup=3, reps=20)
     log(f"cpu baseline {t * 1e3:.2f} ms/SpMV on {used} threads, {t1 * 1e3:.2f} ms on 1 "
         f"({time.time() - t0:.1f}s)")
-    return {"value": round(b_alg / t / 1e9, 3), "unit": "GB/s", "cores": used,
+    base = {"value": round(b_alg / t / 1e9, 3), "unit": "GB/s", "cores": used,
             "kind": "port",
             "sample": "full R-MAT FP64 product (2^20 rows, 2^24 nnz), OpenMP row-parallel "
                       "CSR, median of 20 after 3 warm-ups", "ms_per_spmv": round(t * 1e3, 3),
             "one_thread": {"value": round(b_alg / t1 / 1e9, 3), "ms_per_spmv": round(t1 * 1e3, 3)}}
+    from merbit_amd import workloads
+    per = {"rmat_f64": {"threads": used, "ms": round(t * 1e3, 3), "gbs": base["value"],
+                        "one_thread_ms": round(t1 * 1e3, 3),
+                        "one_thread_gbs": base["one_thread"]["value"]}}
+    for name, (M, v, xm, reps) in (("laplacian_f64", laplacian), ("banded_f64", banded)):
+        if M is None:
+            continue
+        v64 = np.ascontiguousarray(v, dtype=np.float64)
+        x64 = np.ascontiguousarray(xm, dtype=np.float64)
+        b = workloads.algorithmic_bytes(M.nrow, M.ncol, M.nnz, 8, 8, 8)
+        tn, un, _ = oracle.spmv_csr_omp_time(M.row_ptr, M.col, v64, x64, threads, warmup=2,
+                                             reps=reps)
+        to, _, _ = oracle.spmv_csr_omp_time(M.row_ptr, M.col, v64, x64, 1, warmup=1,
+                                            reps=max(3, reps // 4))
+        per[name] = {"threads": un, "ms": round(tn * 1e3, 3), "gbs": round(b / tn / 1e9, 3),
+                     "one_thread_ms": round(to * 1e3, 3), "one_thread_gbs": round(b / to / 1e9, 3)}
+        log(f"cpu baseline {name}: {tn * 1e3:.2f} ms on {un} threads, {to * 1e3:.2f} ms on 1")
+    errs = {}
+    t0 = time.time()
+    for name, M, stored, xs, y, tol in checks:
+        if xs.ndim == 2:  # right-hand sides by rows (k x ncol), Y by rows (k x nrow)
+            worst_c = worst_n = 0.0
+            for j in range(xs.shape[0]):
+                ya, absr = oracle.spmv_accurate(M.row_ptr, M.col, stored,
+                                                np.ascontiguousarray(xs[j]))
+                c, n = _errors(y[j], ya, absr)
+                worst_c, worst_n = max(worst_c, c), max(worst_n, n)
+            errs[name] = {"cond_max": worst_c, "normwise_max": worst_n, "columns": xs.shape[0],
+                          "bar": tol, "within_bar": worst_c <= tol}
+        else:
+            ya, absr = oracle.spmv_accurate(M.row_ptr, M.col, stored, xs)
+            c, n = _errors(y, ya, absr)
+            errs[name] = {"cond_max": c, "normwise": n, "bar": tol, "within_bar": c <= tol}
+    log(f"oracle checks: {len(errs)} results, all within their bars: "
+        f"{all(e['within_bar'] for e in errs.values())} ({time.time() - t0:.1f}s)")
+    return base, per, errs
 
 
 def north_star_summary(extras):
     """The ratios the targets are stated in, gathered from the records above:
     mixed and batched forms against the plain FP64 product of the same
-    matrix, and the differentiable step on the column-blocked forms."""
+    matrix (its tuned launch with 32-bit columns), with the library's
+    default (the launch rule's plan) and the tuned launch, the largest
+    oracle error of the mixed results, and the differentiable step on the
+    column-blocked forms."""
     ns = {}
-    mb = extras.get("mixed_banded")
-    if mb:
-        m = {"bf16": mb["bf16"]["speedup_vs_f64"], "f32": mb["f32"]["speedup_vs_f64"]}
+    errs = extras.get("oracle_errors", {})
+    for name in ("banded", "rmat"):
+        mb = extras.get(f"mixed_{name}")
+        if not mb:
+            continue
+        m = {}
         for kind in ("f32", "bf16"):
+            m[kind] = mb[kind]["rule"]["speedup_vs_f64"]
+            m[f"{kind}_tuned_compensated"] = mb[kind]["speedup_vs_f64"]
             acc = mb[kind].get("acc_f64", {})
             if "compressed_cols" in acc:
-                m[f"{kind}_fp64_sums_16bit_cols"] = acc["compressed_cols"]["speedup_vs_f64"]
-        m["rel_err"] = max(mb[k]["rel_err_vs_f64_same_values"] for k in ("f32", "bf16"))
-        ns["mixed_vs_fp64_banded"] = m
-    mr = extras.get("mixed_rmat")
-    if mr:
-        ns["mixed_vs_fp64_rmat"] = {k: mr[k]["speedup_vs_f64"] for k in ("f32", "bf16")}
+                m[f"{kind}_tuned_fp64_sums_16bit_cols"] = acc["compressed_cols"]["speedup_vs_f64"]
+        e = [v["cond_max"] for k, v in errs.items()
+             if k.startswith(f"{name}_f32") or k.startswith(f"{name}_bf16")]
+        if e:
+            m["max_cond_err_vs_fp64_oracle"] = max(e)
+        ns[f"mixed_vs_fp64_{name}"] = m
     sp = extras.get("spmm_rmat")
     f64_us = extras.get("tile_kernel_us")
     if sp and f64_us:
@@ -688,6 +881,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true",
                     help="time the headline steps as eager launches instead of one graph replay")
+    ap.add_argument("--no-probes", action="store_true",
+                    help="skip the stream-only and gather-only probes of the headline kernel")
+    ap.add_argument("--no-hot", action="store_true",
+                    help="leave the hot-column split out of the tuned candidates")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -703,7 +900,8 @@ def main():
     t0 = time.time()
     A, x = workloads.rmat(args.scale)
     log(f"rmat: nrow={A.nrow} nnz={A.nnz} ({time.time() - t0:.1f}s)")
-    h = bench_rmat_f64(A, x, dev, args)
+    checks = []  # GPU results for the oracle leg: (name, A, values, x, y, bar)
+    h = bench_rmat_f64(A, x, dev, args, checks)
     b_alg, ms, kern = h["b_alg"], h["ms"], h["kern"]
     value = b_alg / (ms * 1e-3) / 1e9
     achieved = b_alg / (kern * 1e-3) / 1e9
@@ -717,28 +915,35 @@ def main():
             "tile_kernel_us": round(h["warm_kern"] * 1e3, 2)},
         "ntiles": h["ntiles"], "tile_items": h["tile_items"],
         "plan_create_ms": round(h["plan_ms"], 3),
+        "headline_rule": h["rule"],
+        "hot_columns": h["hot"],
+        "ceilings": h["ceilings"],
     }
+    lap = (None, None, None, 0)
+    band = (None, None, None, 0)
     if args.only == "all":
-        extras["mixed_rmat"] = bench_mixed("rmat", A, x, dev, args, f64_ms=ms, copies=3)
+        extras["laplacian"] = bench_laplacian(dev, args, checks)
+        La, lx = workloads.laplacian(256)
+        lap = (La, La.val, lx, 200)
+        extras["mixed_rmat"] = bench_mixed("rmat", A, x, dev, args, checks, f64_ms=ms, copies=3)
     if args.only in ("all", "spmm"):
-        extras["spmm_rmat"] = bench_spmm(A, dev, args)
+        extras["spmm_rmat"] = bench_spmm(A, dev, args, checks)
     if args.only in ("all", "banded"):
         t0 = time.time()
         B, xb = workloads.banded()
         log(f"banded: nrow={B.nrow} nnz={B.nnz} ({time.time() - t0:.1f}s)")
-        extras["mixed_banded"] = bench_mixed("banded", B, xb, dev, args)
-        del B, xb
+        extras["mixed_banded"] = bench_mixed("banded", B, xb, dev, args, checks)
+        band = (B, B.val, xb, 10)
     if args.only in ("all", "batch"):
         extras["batch_uneven"] = bench_batch(dev, args)
     if args.only in ("all", "adjoint"):
-        extras["adjoint_graph"] = bench_adjoint(dev, args)
+        extras["adjoint_graph"] = bench_adjoint(dev, args, checks)
     if args.only in ("all", "pagerank"):
         extras["pagerank_rmat"] = bench_pagerank(A, dev, args)
     if args.only == "all" or args.compare:
         extras["rocsparse_rmat_f64"] = bench_rocsparse(A, x, dev, args, ms)
     if args.only == "all" or args.copy_probe:
         extras["copy_ceiling"] = bench_copy(dev)
-    extras["north_star"] = north_star_summary(extras)
 
     out = {
         "metric": METRIC,
@@ -757,6 +962,7 @@ def main():
                                "(a=.57,b=c=.19), 3 rotated copies",
                    "nrow": A.nrow, "nnz": A.nnz, "items_per_thread": h["ipt"],
                    "stream_loads": h["stream"], "tiles_per_cu": h["resid"] or "all",
+                   "hot_columns": h["hot"],
                    "wg_size": 256, "copies_rotated": max(1, args.copies),
                    "timed_as": "eager launches" if args.no_graph else "one HIP graph of the steps"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -766,7 +972,11 @@ def main():
         "extras": extras,
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(A, x, b_alg)
+        base, per, errs = cpu_leg(A, x, b_alg, checks, lap, band)
+        out["cpu_baseline"] = base
+        extras["cpu_baselines"] = per
+        extras["oracle_errors"] = errs
+    extras["north_star"] = north_star_summary(extras)
     print(json.dumps(out), flush=True)
 
 

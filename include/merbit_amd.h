@@ -293,7 +293,8 @@ int mbx_plan_stats(const mbx_plan* plan, const int32_t* d_row_ptr, mbx_pattern_s
  * Features: tiles at 8 items per thread, the widest column range of one such
  * tile, x bytes, and the share of the entries in the 4096 (FP64 x) or 5800
  * (FP32 x) most frequent columns.  Rules, first match:
- *   MBX_RULE_FEW_TILES       fewer than 16 tiles per CU: 4 items per thread;
+ *   MBX_RULE_FEW_TILES       fewer than 16 tiles per CU: 4 items per thread
+ *                            (2 when that gives fewer tiles than CUs);
  *   MBX_RULE_LOCAL           every tile spans < 65536 columns (banded):
  *                            8 (FP64) or 12 items per thread, all tiles
  *                            resident, 16-bit columns, FP64 sums for

@@ -8,7 +8,9 @@ tree and every time it is called:
 * ``merbit_amd/bin/merbit_main`` -- the C++ config-ID driver (``./main <ID>``
   of the reference, src/main.cu:3-84) over that library;
 * ``merbit_amd/libmerbit_cmp.so`` -- rocSPARSE CSR SpMV behind a small C ABI,
-  a measured comparator for the benchmark (not used by the product path).
+  a measured comparator for the benchmark (not used by the product path);
+* ``merbit_amd/libmerbit_probe.so`` -- the benchmark's bandwidth probes of
+  the headline tile kernel's stream and gathers (measured only).
 """
 from __future__ import annotations
 
@@ -23,6 +25,7 @@ HOST = os.path.join(PKG, "host")
 LIB = os.path.join(PKG, "libmerbit_amd.so")
 BIN = os.path.join(PKG, "bin", "merbit_main")
 CMP_LIB = os.path.join(PKG, "libmerbit_cmp.so")
+PROBE_LIB = os.path.join(PKG, "libmerbit_probe.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -141,10 +144,23 @@ def build_comparator(verbose: bool = False) -> str | None:
     return CMP_LIB
 
 
+def build_probes(verbose: bool = False) -> str:
+    """libmerbit_probe.so: the benchmark's bandwidth probes (measured only)."""
+    src = os.path.join(PKG, "compare", "probes.hip")
+    inc = os.path.join(ROOT, "include")
+    obj = os.path.join(ROOT, "_build", "probes.hip.o")
+    os.makedirs(os.path.dirname(obj), exist_ok=True)
+    _run([HIPCC, *HIP_FLAGS, f"-I{inc}", f"-I{CSRC}", "-x", "hip", "-c", src, "-o", obj], verbose)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", obj, "-o", PROBE_LIB + ".tmp"], verbose)
+    os.replace(PROBE_LIB + ".tmp", PROBE_LIB)
+    return PROBE_LIB
+
+
 def build(verbose: bool = False) -> None:
     build_library(verbose)
     build_driver(verbose)
     build_comparator(verbose)
+    build_probes(verbose)
 
 
 if __name__ == "__main__":

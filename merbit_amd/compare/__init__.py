@@ -77,3 +77,30 @@ class RocsparseSpmv:
             self.close()
         except Exception:
             pass
+
+
+_PROBE_PATH = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                           "libmerbit_probe.so")
+_PL = None
+
+
+def probe(plan, which: str, row_ptr, col, val, x, y, stream=None) -> None:
+    """One bandwidth probe launch at the plan's tile shape (probes.hip):
+    which = "stream" (row_ptr, col and values as the tile kernel stages them;
+    y: nrow doubles) or "gather" (columns and x gathers; y: ntiles doubles)."""
+    global _PL
+    if _PL is None:
+        if not os.path.exists(_PROBE_PATH):
+            raise ImportError(f"{_PROBE_PATH} is not built; run `python -m merbit_amd.build`")
+        L = ctypes.CDLL(_PROBE_PATH)
+        vp = ctypes.c_void_p
+        L.mbp_probe.argtypes = [vp, ctypes.c_int32, vp, vp, vp, vp, vp, vp]
+        L.mbp_probe.restype = ctypes.c_int
+        _PL = L
+    s = torch.cuda.current_stream() if stream is None else stream
+    st = _PL.mbp_probe(plan.handle, {"stream": 0, "gather": 1}[which], row_ptr.data_ptr(),
+                       col.data_ptr(), val.data_ptr() if val is not None else None,
+                       x.data_ptr() if x is not None else None, y.data_ptr(),
+                       int(s.cuda_stream))
+    if st != 0:
+        raise MerbitError(st, "mbp_probe")

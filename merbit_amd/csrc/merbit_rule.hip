@@ -7,7 +7,8 @@
 // statistics choose the launch, without a timing sweep, from what bounds
 // the product (measured on MI355X, warm and rotated copies):
 //   * few tiles (under 16 per CU at 8 items per thread): the launch is
-//     short and unevenly filled, 4 items per thread give twice the tiles;
+//     short and unevenly filled, 4 items per thread give twice the tiles
+//     (2 when even that leaves CUs without a tile);
 //   * local columns (every tile of 2048 merge items spans < 65536
 //     columns, e.g. banded): the gathers hit L2 and the stream is the bound:
 //     every tile resident, 16-bit column offsets, and FP64 sums for FP32 /
@@ -190,7 +191,8 @@ int mbx_launch_rule(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow
   }
   const bool local = nnz > 0 && c.max_tile_span < 65536;
   if (c.ntiles8 < kFewTilesPerCu * cus) {
-    c.items_per_thread = 4;
+    // twice the tiles at 4 items per thread; 2 when even those leave CUs idle
+    c.items_per_thread = 2ll * c.ntiles8 < cus ? 2 : 4;
     c.rule = MBX_RULE_FEW_TILES;
   } else if (local) {
     c.items_per_thread = f64 ? 8 : 12;

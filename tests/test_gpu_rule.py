@@ -4,7 +4,7 @@ plain plan within the bars, and that its launch is within 5 % of the
 timing autotuner's best (mbx_autotune_spmv_launch, which sweeps items per
 thread x stream policy x tiles per CU) on the benchmark workloads.  The
 autotuner's pick is re-timed with the same method as the rule's plan (the
-minimum of its 64 single timings is biased low)."""
+minimum of its 64 single timings is biased low), both as graph replays."""
 from __future__ import annotations
 
 import ctypes
@@ -25,17 +25,25 @@ def dev(a, device):
 
 
 def time_us(fn, reps, rounds=5):
-    best = float("inf")
-    for _ in range(rounds):
-        fn()
-        torch.cuda.synchronize()
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record()
+    """Device time per call: `reps` calls captured as one HIP graph, the
+    best of `rounds` replays (host launch overhead left out, which would
+    dominate the microsecond products)."""
+    fn()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
         for _ in range(reps):
             fn()
+    torch.cuda.synchronize()
+    best = float("inf")
+    for _ in range(rounds):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        g.replay()
         b.record()
         torch.cuda.synchronize()
         best = min(best, a.elapsed_time(b) * 1e3 / reps)
+    del g
     return best
 
 
@@ -135,3 +143,7 @@ def test_rule_features_small(device):
     c3 = launch_rule(dev(np.zeros(11, np.int32), device),
                      torch.zeros(0, dtype=torch.int32, device=device), 7)
     assert c3["rule"] == "few_tiles" and c3["hot_share"] == 0.0
+    # fewer tiles at 4 items per thread than CUs: 2 items per thread
+    for ch in (c, c3):
+        assert ch["items_per_thread"] == (2 if 2 * ch["ntiles8"] < ch["cus"] else 4)
+    assert c3["items_per_thread"] == 2
