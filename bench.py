@@ -175,7 +175,7 @@ def tune_shape(args, copies, ncol, nnz, kind, hot_ok):
     ipts = [args.ipt] if args.ipt else [i for i in ITEMS_PER_THREAD if i in (4, 8, 12, 16)]
     modes = [args.stream] if args.stream else ["cached", "nontemporal"]
     wgs = [args.residency] if args.residency is not None else [0, 3, 4, 5]
-    hots = [0, HOT_COLUMNS[kind]] if hot_ok else [0]
+    hots = ([args.hot] if args.hot is not None else [0, HOT_COLUMNS[kind]]) if hot_ok else [0]
     cands = [(i, m, w, h) for i in ipts for h in hots if not (h and i not in (8, 12))
              for m in modes for w in (wgs if not h else [0])]
     if len(cands) == 1:
@@ -866,11 +866,11 @@ def main():
     ap.add_argument("--copies", type=int, default=3)
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--batch-count", type=int, default=64)
-    ap.add_argument("--only", choices=["headline", "banded", "spmm", "batch", "adjoint", "pagerank",
-                                       "all"],
+    ap.add_argument("--only", choices=["headline", "mixed", "banded", "spmm", "batch", "adjoint",
+                                       "pagerank", "all"],
                     default="all",
-                    help="headline: only the FP64 R-MAT line; banded / spmm / batch / adjoint "
-                         "/ pagerank: the headline plus that part (for profiling)")
+                    help="headline: only the FP64 R-MAT line; mixed / banded / spmm / batch / "
+                         "adjoint / pagerank: the headline plus that part (for profiling)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run the plain device copy (counter calibration)")
     ap.add_argument("--compare", action="store_true",
@@ -885,6 +885,8 @@ def main():
                     help="skip the stream-only and gather-only probes of the headline kernel")
     ap.add_argument("--no-hot", action="store_true",
                     help="leave the hot-column split out of the tuned candidates")
+    ap.add_argument("--hot", type=int, default=None,
+                    help="hot columns of the power-law products (0 = none); default: tuned")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -925,6 +927,7 @@ def main():
         extras["laplacian"] = bench_laplacian(dev, args, checks)
         La, lx = workloads.laplacian(256)
         lap = (La, La.val, lx, 200)
+    if args.only in ("all", "mixed"):
         extras["mixed_rmat"] = bench_mixed("rmat", A, x, dev, args, checks, f64_ms=ms, copies=3)
     if args.only in ("all", "spmm"):
         extras["spmm_rmat"] = bench_spmm(A, dev, args, checks)
