@@ -296,7 +296,7 @@ int mbx_plan_stats(const mbx_plan* plan, const int32_t* d_row_ptr, mbx_pattern_s
  *   MBX_RULE_FEW_TILES       fewer than 16 tiles per CU: 4 items per thread
  *                            (2 when that gives fewer tiles than CUs);
  *   MBX_RULE_LOCAL           every tile spans < 65536 columns (banded):
- *                            8 (FP64) or 12 items per thread, all tiles
+ *                            8 (FP64, FP32) or 12 (BF16) items per thread, all tiles
  *                            resident, 16-bit columns, FP64 sums for
  *                            FP32/BF16 values;
  *   MBX_RULE_X_FITS          x of at most 2 MiB: 8, at most 4 tiles per CU;

@@ -13,8 +13,8 @@
 //     columns, e.g. banded): the gathers hit L2 and the stream is the bound:
 //     every tile resident, 16-bit column offsets, and FP64 sums for FP32 /
 //     bfloat16 values (the compensated sums are VALU-bound once the stream
-//     is cut); 8 items per thread for FP64 values, 12 for the 4- and 2-byte
-//     values;
+//     is cut); 8 items per thread for FP64 and FP32 values, 12 for bfloat16
+//     (its 2-byte values leave the tile's loads short);
 //   * x of at most 2 MiB: the gathers stay in L2; 8 items per thread, at
 //     most 4 tiles per CU;
 //   * larger x with a hot column set (the kHot* most frequent columns hold
@@ -195,7 +195,7 @@ int mbx_launch_rule(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow
     c.items_per_thread = 2ll * c.ntiles8 < cus ? 2 : 4;
     c.rule = MBX_RULE_FEW_TILES;
   } else if (local) {
-    c.items_per_thread = f64 ? 8 : 12;
+    c.items_per_thread = value_kind == MBX_VALUE_BF16 ? 12 : 8;
     c.compress_cols = 1;
     if (!f64) c.accumulation = MBX_ACC_F64;
     c.rule = MBX_RULE_LOCAL;
