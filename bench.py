@@ -2,31 +2,38 @@
 
 Headline workload: FP64 merge SpMV y = A*x on the R-MAT power-law matrix
 (2^20 rows, 2^24 entries with duplicates kept, a=.57 b=c=.19, values N(0,1),
-x ~ U[-1,1)).  A step is one SpMV (tile kernel + carry fixup) with the matrix
-and x already resident in HBM; the step time is taken over the whole loop of
-steps, with no per-step events, the steps captured as one HIP graph and
-replayed once (--no-graph: eager launches).  Three copies of the matrix are
-rotated so consecutive steps cannot be served from the 256 MB MALL; the warm
-(single copy) rate is reported beside it.  The tile shape (items per thread) and the
-stream cache policy (plain or non-temporal matrix loads) are picked by timing
-every pair on the rotated workload, as a user would tune for their access
-pattern; --ipt / --stream pin them.
+x ~ U[-1,1)).  A step is one SpMV (hot gather, tile kernel and carry fixup)
+with the matrix and x already resident in HBM; the step time is taken over
+the whole loop of steps, with no per-step events, the steps captured as one
+HIP graph and replayed once (--no-graph: eager launches).  Three copies of
+the matrix, each with its own plan, are rotated so consecutive steps cannot
+be served from the 256 MB MALL; the warm (single copy) rate is reported
+beside it.  The launch shape (items per thread, stream cache policy, tiles
+per CU, hot columns) is picked by timing the candidates on the rotated
+workload, as a user would tune for their access pattern (--ipt / --stream /
+--residency / --hot pin them); the launch rule's plans (the library's
+default) are timed beside it.
 
 value = algorithmic GB/s = (4(nrow+1) + 12 nnz + 8 ncol + 8 nrow) / time per
 step.  The roofline object is for the dominant (tile) kernel, timed with HIP
 events recorded around it on its own stream.  `traffic` (measured HBM bytes
-per launch) is read from the committed counter summary for this workload when
-present.  The CPU baseline is the oracle's row-parallel FP64 CSR product,
-timed on the host cores.
+per launch) is read from the committed counter summary for this workload.
+`extras.ceilings` times the tile kernel's staging loads alone and its x
+gathers alone (probes at the same shape).
 
-Beside the headline, `extras` reports the other measured forms: mixed
-precision (FP32 / BF16 values, compensated FP32 sums) on the R-MAT and on a
-banded 4M-row matrix, multi-RHS SpMM (k = 8, 32), 64 uneven matrices in one
-persistent graph launch (autotuned), the adjoint on a 2.4M-node graph,
-PageRank iterations on the R-MAT edge list after device ingest, rocSPARSE's
-CSR and COO SpMV on the headline matrix (measured comparators), and a plain
-device copy as the measured bandwidth ceiling.  Every rate uses the same
-algorithmic-bytes definition.
+Beside the headline, `extras` reports the other measured forms: the 256^2
+Laplacian, mixed precision (FP32 / BF16 values) on the R-MAT and on a
+banded 4M-row matrix (tuned, with FP64 sums and 16-bit columns, and the
+launch rule's default), multi-RHS SpMM (k = 8, 32), 64 uneven matrices in
+one persistent graph launch (autotuned), the adjoint on a 2.4M-node graph
+(CSR, column-blocked and the operator's defaults), PageRank iterations on the
+R-MAT edge list after device ingest, rocSPARSE's CSR and COO SpMV on the
+headline matrix (measured comparators), and a plain device copy as the
+measured bandwidth ceiling.  Every rate uses the same algorithmic-bytes
+definition.  The CPU leg times the oracle's OpenMP FP64 CSR product (the
+CPU baseline) on the R-MAT, the Laplacian and the banded matrix, and checks
+every GPU result gathered by the other legs against the oracle's accurate
+product (`extras.oracle_errors`).
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
