@@ -2,7 +2,7 @@
 //
 //   merbit_main <ID> [--config FILE] [--log FILE] [--device N] [--type f64|f32]
 //               [--sigma 2|4|6|8|10|12|14|16] [--perrnd N] [--maxiter N] [--check]
-//               [--pagerank [--damp D]]
+//               [--pagerank [--damp D]] [--sort-rows]
 //   merbit_main --mtx FILE [...]
 //
 // Loads graph <ID> from the registry (lines "ID path [nrow ncol nnz]"), builds
@@ -30,7 +30,7 @@ struct Args {
   std::string type = "f64";
   int device = 0, sigma = 8, perrnd = 20, maxiter = 20;
   double damp = 0.85;
-  bool check = false, pagerank = false;
+  bool check = false, pagerank = false, sort_rows = false;
 };
 
 int usage() {
@@ -92,7 +92,12 @@ int run_sigma(const ENV& env, GRAPH<V>& graph, const Args& a) {
     return 2;
   }
   PARAMETER<V> para(&graph, a.damp, a.perrnd, a.maxiter);
-  const MerbitTimes t = Fun_Merbit<V, 256, S>(env, para);
+  // --sort-rows leaves the entries in (row, col) order, as after the
+  // reference's GRAPH::sort(true); Fun_Merbit sorts them back to (col, row)
+  // before it takes the CSR view, as the reference's does (tile.cuh:571)
+  if (a.sort_rows) graph.sort(true);
+  Fun_Merbit<V, 256, S>(env, para);
+  const MerbitTimes t = merbit::last_times();
   const double bytes = 4.0 * (graph.nrow + 1) + graph.nnz * (4.0 + sizeof(V)) +
                        2.0 * graph.nrow * sizeof(V);
   std::cout << "graph = " << graph.graph_id << ", block = 256, sigma = " << S
@@ -150,6 +155,7 @@ int main(int argc, char** argv) {
     else if (s == "--maxiter" && (v = next())) a.maxiter = std::atoi(v);
     else if (s == "--check") a.check = true;
     else if (s == "--pagerank") a.pagerank = true;
+    else if (s == "--sort-rows") a.sort_rows = true;
     else if (s == "--damp" && (v = next())) a.damp = std::atof(v);
     else if (!s.empty() && s[0] != '-' && a.gid < 0) a.gid = std::atoi(s.c_str());
     else return usage();

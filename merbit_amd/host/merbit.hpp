@@ -5,8 +5,9 @@
 //
 //   Timer       include/class.cuh:5-31    HIP events; start() syncs the device
 //   ENV         include/class.cuh:36-59   device + stream (no cuSPARSE/cuBLAS)
-//   GRAPH<V>    include/class.cuh:62-81   the operated CSR (A^T of the loaded
-//               src/class.cu:60-177       graph, values 1/out-degree), on device
+//   GRAPH<V>    include/class.cuh:62-81   the loaded graph as COO (row_ind,
+//               src/class.cu:60-177       col_ind, values 1/out-degree; sort)
+//                                         and as the operated CSR (A^T), on device
 //   PARAMETER   include/class.cuh:119-146 x (= 1) and y on device
 //   TILE        include/tile.cuh:6-25     the merge-path plan (mbx_plan)
 //   Fun_Merbit  include/tile.cuh:563-615  plan + maxiter*perrnd timed SpMVs
@@ -19,6 +20,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <fstream>
@@ -102,9 +104,16 @@ struct ENV {
   ENV& operator=(const ENV&) = delete;
 };
 
-// The operated matrix in HBM: CSR (offset, col, values) of A^T of the loaded
-// graph with values 1/out-degree, i.e. what the reference holds after
-// Fun_Values, GRAPH::sort(false) and cusparseXcoo2csr.  Host copies are kept
+// The loaded graph in HBM, in the reference's two views:
+//  * COO (include/class.cuh:62-81): entry k is (row_ind[k], col_ind[k],
+//    values[k]) -- an edge of the loaded graph with value 1/out-degree of its
+//    row (Fun_Values, src/class.cu:22-34); sort(false) orders the entries by
+//    (col_ind, row_ind), sort(true) by (row_ind, col_ind) (class.cu:143-177);
+//  * the operated CSR (offset, col, values): A^T after sort(false) and
+//    cusparseXcoo2csr over col_ind (include/tile.cuh:571-574), so col is
+//    row_ind (the same device array) and offset the row pointer of col_ind.
+// Entries stay in the sort(false) order unless sort(true) is called; the
+// CSR view is valid in that order only (csr_valid).  Host copies are kept
 // for verification.
 template <typename V>
 struct GRAPH {
@@ -112,10 +121,13 @@ struct GRAPH {
   int nrow = 0;
   int nnz = 0;
   bool owner = true;
-  int* offset = nullptr;
-  int* col = nullptr;
+  int* row_ind = nullptr;  // COO row (source) of each entry; the CSR's col
+  int* col_ind = nullptr;  // COO column (target) of each entry; the CSR's row
   V* values = nullptr;
-  std::vector<int32_t> h_offset, h_col;
+  int* offset = nullptr;   // CSR row pointer over col_ind (sort(false) order)
+  int* col = nullptr;      // == row_ind
+  bool csr_valid = true;
+  std::vector<int32_t> h_offset, h_col, h_col_ind;
   std::vector<V> h_values;
 
   GRAPH() = default;
@@ -150,11 +162,15 @@ struct GRAPH {
     std::swap(nrow, o.nrow);
     std::swap(nnz, o.nnz);
     std::swap(owner, o.owner);
+    std::swap(row_ind, o.row_ind);
+    std::swap(col_ind, o.col_ind);
+    std::swap(values, o.values);
     std::swap(offset, o.offset);
     std::swap(col, o.col);
-    std::swap(values, o.values);
+    std::swap(csr_valid, o.csr_valid);
     h_offset.swap(o.h_offset);
     h_col.swap(o.h_col);
+    h_col_ind.swap(o.h_col_ind);
     h_values.swap(o.h_values);
     return *this;
   }
@@ -164,9 +180,44 @@ struct GRAPH {
   ~GRAPH() {
     if (owner) {
       merbit::device_free(offset);
-      merbit::device_free(col);
+      merbit::device_free(row_ind);  // col is the same array
+      merbit::device_free(col_ind);
       merbit::device_free(values);
     }
+    col = nullptr;
+  }
+
+  // src/class.cu:143-177: dir false sorts the entries by (col_ind, row_ind)
+  // -- the order the CSR view needs --, dir true by (row_ind, col_ind).
+  // Stable, so duplicate entries (equal index pairs and values) keep their
+  // order.  A set-up step on the host copies, uploaded again.
+  void sort(const bool dir) {
+    std::vector<int32_t> perm((size_t)nnz);
+    for (int k = 0; k < nnz; ++k) perm[(size_t)k] = k;
+    const std::vector<int32_t>& a = dir ? h_col : h_col_ind;  // primary key
+    const std::vector<int32_t>& b = dir ? h_col_ind : h_col;  // secondary key
+    std::stable_sort(perm.begin(), perm.end(), [&](int32_t p, int32_t q) {
+      return a[(size_t)p] != a[(size_t)q] ? a[(size_t)p] < a[(size_t)q] : b[(size_t)p] < b[(size_t)q];
+    });
+    std::vector<int32_t> r((size_t)nnz), c((size_t)nnz);
+    std::vector<V> v((size_t)nnz);
+    for (int k = 0; k < nnz; ++k) {
+      r[(size_t)k] = h_col[(size_t)perm[(size_t)k]];
+      c[(size_t)k] = h_col_ind[(size_t)perm[(size_t)k]];
+      v[(size_t)k] = h_values[(size_t)perm[(size_t)k]];
+    }
+    h_col.swap(r);
+    h_col_ind.swap(c);
+    h_values.swap(v);
+    if (nnz > 0) {
+      merbit::hip_check(hipMemcpy(row_ind, h_col.data(), sizeof(int32_t) * nnz,
+                                  hipMemcpyHostToDevice), "hipMemcpy");
+      merbit::hip_check(hipMemcpy(col_ind, h_col_ind.data(), sizeof(int32_t) * nnz,
+                                  hipMemcpyHostToDevice), "hipMemcpy");
+      merbit::hip_check(hipMemcpy(values, h_values.data(), sizeof(V) * nnz,
+                                  hipMemcpyHostToDevice), "hipMemcpy");
+    }
+    csr_valid = !dir;
   }
 
  private:
@@ -191,9 +242,16 @@ struct GRAPH {
   }
 
   void upload() {
+    // the COO column of every entry: the CSR row it sits in
+    h_col_ind.assign((size_t)nnz, 0);
+    for (int r = 0; r < nrow; ++r)
+      for (int k = h_offset[(size_t)r]; k < h_offset[(size_t)r + 1]; ++k) h_col_ind[(size_t)k] = r;
     offset = merbit::device_copy(h_offset);
-    col = merbit::device_copy(h_col);
+    row_ind = merbit::device_copy(h_col);
+    col_ind = merbit::device_copy(h_col_ind);
     values = merbit::device_copy(h_values);
+    col = row_ind;
+    csr_valid = true;
   }
 };
 
@@ -265,15 +323,28 @@ struct MerbitTimes {
   float spmv_ms = 0.f;  // tim2 / (maxiter * perrnd)
 };
 
-// include/tile.cuh:563-615.  Builds the plan (tim1), runs maxiter*perrnd
-// SpMVs y = A*x on env.stream (tim2), leaves y in para.pagerank_real and
-// appends " gid&MERBIT&SIGMA&tim1&tim2/iters" to FSTR when it is open.
+namespace merbit {
+// The times of the calling thread's last Fun_Merbit / Fun_PageRank (the
+// reference writes them to its log only).
+inline MerbitTimes& last_times() {
+  static thread_local MerbitTimes t;
+  return t;
+}
+}  // namespace merbit
+
+// include/tile.cuh:563-615.  Sorts the graph's entries by (col_ind, row_ind)
+// and takes the CSR view as the reference does (tile.cuh:571-574), builds
+// the plan (tim1), runs maxiter*perrnd SpMVs y = A*x on env.stream (tim2),
+// leaves y in para.pagerank_real and appends " gid&MERBIT&SIGMA&tim1&
+// tim2/iters" to FSTR when it is open; merbit::last_times() holds the times.
 template <typename V, unsigned BLOCK, unsigned SIGMA>
-MerbitTimes Fun_Merbit(const ENV& env, const PARAMETER<V>& para) {
+void Fun_Merbit(const ENV& env, const PARAMETER<V>& para) {
   static_assert(std::is_same<V, double>::value || std::is_same<V, float>::value,
                 "Fun_Merbit is instantiated for float and double");
   Timer timer;
-  MerbitTimes t;
+  MerbitTimes& t = merbit::last_times();
+  t = MerbitTimes();
+  if (!para.graph->csr_valid) para.graph->sort(false);
   const GRAPH<V>& g = *para.graph;
   timer.start(env.stream);
   TILE<BLOCK, SIGMA> tile(g.offset, g.nrow, g.nnz, env.stream);
@@ -295,7 +366,6 @@ MerbitTimes Fun_Merbit(const ENV& env, const PARAMETER<V>& para) {
   if (FSTR.is_open())
     FSTR << " " << g.graph_id << "&MERBIT&" << SIGMA << "&" << t.plan_ms << "&" << t.spmv_ms
          << "\n";
-  return t;
 }
 
 // PageRank with the damping factor the reference carries in PARAMETER::damp
@@ -307,6 +377,7 @@ template <unsigned BLOCK, unsigned SIGMA>
 MerbitTimes Fun_PageRank(const ENV& env, const PARAMETER<double>& para) {
   Timer timer;
   MerbitTimes t;
+  if (!para.graph->csr_valid) para.graph->sort(false);
   const GRAPH<double>& g = *para.graph;
   timer.start(env.stream);
   TILE<BLOCK, SIGMA> tile(g.offset, g.nrow, g.nnz, env.stream);

@@ -200,3 +200,40 @@ def test_values_to_csr_checks_out_before_any_launch():
         cb.values_to_csr(vb, out=torch.zeros(9, dtype=torch.float64))
     with pytest.raises(ValueError):
         cb.values_to_csr(vb, out=torch.zeros(10, dtype=torch.float32))
+
+
+def test_reference_style_cpp_compiles(tmp_path):
+    """Host code written against the reference's API compiles against the
+    C++ mirror: GRAPH's COO fields and sort(bool) (include/class.cuh:62-81),
+    Fun_Merbit<V,BLOCK,SIGMA>(env, para) returning void (include/tile.cuh:
+    563-564), Fun_Check, FSTR.  Compile only (no device work)."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("no C++ compiler")
+    src = tmp_path / "ref_style.cpp"
+    src.write_text(r"""
+#include "merbit.hpp"
+int run(const std::string& config, int gid) {
+  ENV env(0, config, "log.txt");
+  GRAPH<double> graph(config, gid);
+  graph.sort(true);
+  const int* r = graph.row_ind;
+  const int* c = graph.col_ind;
+  double* v = graph.values;
+  (void)r; (void)c; (void)v;
+  PARAMETER<double> para(&graph, 0.85, 20, 20);
+  para.graph->sort(false);
+  Fun_Merbit<double, 256, 14>(env, para);
+  static_assert(std::is_same<decltype(Fun_Merbit<float, 256, 14>(env, *(PARAMETER<float>*)0)),
+                             void>::value, "Fun_Merbit returns void");
+  std::vector<double> y(graph.nrow), x(graph.nrow);
+  return Fun_Check<double>(x.data(), y.data(), graph.nrow);
+}
+int main() { return 0; }
+""")
+    rocm = "/opt/rocm"
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-D__HIP_PLATFORM_AMD__",
+                        f"-I{ROOT}/include", f"-I{ROOT}/merbit_amd/host", f"-I{rocm}/include",
+                        str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

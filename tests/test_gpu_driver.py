@@ -54,3 +54,19 @@ def test_driver_missing_id_errors(device, tmp_path):
     r = subprocess.run([BIN, "9", "--config", str(cfg)], stdout=subprocess.PIPE,
                        stderr=subprocess.STDOUT, text=True, timeout=60)
     assert r.returncode != 0 and "config ID 9" in r.stdout
+
+
+def test_driver_coo_sort_round_trip(device, tmp_path):
+    """--sort-rows puts the graph's COO entries in (row, col) order (the
+    reference's GRAPH::sort(true)); Fun_Merbit sorts them back before it
+    takes the CSR view, so the product still checks."""
+    write_rmat_mtx(tmp_path / "rmat.mtx", 12, 5)
+    cfg = tmp_path / "config.txt"
+    cfg.write_text(f"7 {tmp_path}/rmat.mtx\n")
+    for typ in ("f64", "f32"):
+        print(f"[case] driver sort-rows type={typ}", flush=True)
+        r = subprocess.run([BIN, "7", "--config", str(cfg), "--type", typ, "--sigma", "14",
+                            "--perrnd", "2", "--maxiter", "2", "--sort-rows", "--check"],
+                           stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                           timeout=120)
+        assert r.returncode == 0 and "check ok" in r.stdout, r.stdout
