@@ -16,7 +16,8 @@ default) are timed beside it.
 
 value = algorithmic GB/s = (4(nrow+1) + 12 nnz + 8 ncol + 8 nrow) / time per
 step.  The roofline object is for the dominant (tile) kernel, timed with HIP
-events recorded around it on its own stream.  `traffic` (measured HBM bytes
+events attached to its own dispatch (hipExtLaunchKernel) on the stream it is
+launched on, the interval a kernel trace reports.  `traffic` (measured HBM bytes
 per launch) is read from the committed counter summary for this workload.
 `extras.ceilings` times the tile kernel's staging loads alone and its x
 gathers alone (probes at the same shape).
@@ -65,7 +66,7 @@ def time_steps(calls, steps, warmup, plan_events=None, graph=False):
     whole loop only.  graph: the `steps` calls are captured once as one HIP
     graph (as the reference's timed loop of repeated launches) and the timed
     region is its single replay.  With plan_events the loop is run once more,
-    eagerly, with events recorded around every tile kernel (on the stream it
+    eagerly, with events attached to every tile kernel's dispatch (on the stream it
     is launched on), for the kernel's own duration."""
     import torch
     for i in range(warmup):
@@ -154,8 +155,8 @@ def rule_plans(copies, ncol, nnz, value_dtype):
 
 
 def event_setter(plans):
-    """For time_steps' event pass: the i-th call's plan records the events
-    around its tile kernel (the calls cycle over the plans in order)."""
+    """For time_steps' event pass: the i-th call's plan attaches the events
+    to its tile kernel (the calls cycle over the plans in order)."""
     state = {"i": 0}
 
     def setter(b, e):
@@ -240,8 +241,8 @@ def bench_rmat_f64(A, x, dev, args, checks):
         p.set_stream(stream)
         p.set_residency(resid)
     calls = spmv_calls(plans, copies)
-    # the tile kernel's own duration: events recorded by the library around
-    # it on the call's stream, for whichever copy's plan runs
+    # the tile kernel's own duration: events the library attaches to its
+    # dispatch on the call's stream, for whichever copy's plan runs
     ms, kern = time_steps(calls, args.steps, args.warmup, event_setter(plans),
                           graph=not args.no_graph)
     checks.append(("headline_rmat_f64", A, A.val, x, c0["y"].cpu().numpy(), 1e-12))

@@ -2,6 +2,7 @@
 // libmerbit_amd.so (not part of the C ABI).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -189,6 +190,23 @@ inline int capture_graph(int repeats, F fn, void** exec_out) {
   if (st == MBX_OK) *exec_out = exec;
   (void)hipGetLastError();
   return st;
+}
+
+// kernel<<<grid, block, lds, s>>>(args...).  When the plan carries timing
+// events (mbx_plan_set_events), they are attached to this dispatch instead
+// (hipExtLaunchKernel) and then cleared: their interval is the kernel's own
+// execution, the interval a kernel trace reports, without the marker packets
+// and dispatch latency that event records around the launch would add.
+template <class... KA, class... A>
+inline void timed_launch(const mbx_plan* p, void (*kernel)(KA...), unsigned grid,
+                         unsigned block, size_t lds, hipStream_t s, A... args) {
+  if (p->ev_begin && p->ev_end) {
+    hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), (uint32_t)lds, s, p->ev_begin,
+                          p->ev_end, 0u, args...);
+    p->ev_begin = p->ev_end = nullptr;
+  } else {
+    kernel<<<grid, block, lds, s>>>(args...);
+  }
 }
 
 }  // namespace mbx_detail

@@ -571,11 +571,9 @@ void launch_spmm(const mbx_plan* p, hipStream_t s, const int32_t* rp, const int3
                  const float* val, const float* Xr, int ldr, int k, float* Y, long long ldy,
                  int yrm, double* carry, float alpha, float beta) {
   const int bz = beta == 0.f;
-  if (p->ev_begin) (void)hipEventRecord(p->ev_begin, s);
-  tile_spmm_kernel<256, IPT, KC><<<p->ntiles, 256, 0, s>>>(
-      rp, col, val, Xr, ldr, k, Y, ldy, yrm, p->d_coord, carry, p->ntiles, alpha, beta, bz);
-  if (p->ev_end) (void)hipEventRecord(p->ev_end, s);
-  p->ev_begin = p->ev_end = nullptr;
+  mbx_detail::timed_launch(p, tile_spmm_kernel<256, IPT, KC>, p->ntiles, 256, 0, s, rp, col, val,
+                           Xr, ldr, k, Y, ldy, yrm, p->d_coord, carry, p->ntiles, alpha, beta,
+                           bz);
   if (p->ntiles > 1)
     spmm_fixup_kernel<<<(int)(((long long)p->ntiles * k + 255) / 256), 256, 0, s>>>(
         p->d_coord, carry, k, Y, ldy, yrm, p->ntiles, p->tile_items, alpha, beta, bz);
@@ -589,11 +587,8 @@ void launch_spmm_rows(const mbx_plan* p, hipStream_t s, const int32_t* rp, const
   const size_t lds = sizeof(float4) * (size_t)TI * G;
   auto kern = tile_spmm_rows_kernel<WG, TI, G>;
   mbx_detail::allow_lds(kern, lds + sizeof(int) * (TI + 1) + 4096);
-  if (p->ev_begin) (void)hipEventRecord(p->ev_begin, s);
-  kern<<<p->ntiles, WG, lds, s>>>(rp, col, val, Xr, ldr, k, Y, ldy, yrm, p->d_coord, carry,
-                                  p->ntiles, alpha, beta, bz);
-  if (p->ev_end) (void)hipEventRecord(p->ev_end, s);
-  p->ev_begin = p->ev_end = nullptr;
+  mbx_detail::timed_launch(p, kern, p->ntiles, WG, lds, s, rp, col, val, Xr, ldr, k, Y, ldy, yrm,
+                           p->d_coord, carry, p->ntiles, alpha, beta, bz);
   if (p->ntiles > 1)
     spmm_fixup_kernel<<<(int)(((long long)p->ntiles * k + 255) / 256), 256, 0, s>>>(
         p->d_coord, carry, k, Y, ldy, yrm, p->ntiles, p->tile_items, alpha, beta, bz);

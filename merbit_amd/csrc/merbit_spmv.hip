@@ -42,6 +42,7 @@
 #include "merbit_internal.hpp"
 #include "merbit_tile.hpp"
 
+using mbx_detail::timed_launch;
 using mbx_detail::xcd_tile;
 
 namespace {
@@ -232,9 +233,8 @@ bool launch_hot(const mbx_plan* p, hipStream_t s, const int32_t* rp,
   const int grid = (int)(g < p->ntiles ? g : p->ntiles);
   hot_gather_kernel<X><<<(p->nhot + 255) / 256, 256, 0, s>>>(x, p->d_hot, p->nhot,
                                                                static_cast<X*>(p->d_xh));
-  kernel<<<grid, 256, lds, s>>>(rp, p->d_colh, val, x, y, p->d_coord, p->d_carry, p->ntiles,
-                                p->ncol, alpha, beta, bz, static_cast<const X*>(p->d_xh),
-                                p->nhot);
+  timed_launch(p, kernel, grid, 256, lds, s, rp, p->d_colh, val, x, y, p->d_coord, p->d_carry,
+               p->ntiles, p->ncol, alpha, beta, bz, static_cast<const X*>(p->d_xh), p->nhot);
   return true;
 }
 
@@ -250,7 +250,6 @@ hipError_t launch_tile(const mbx_plan* p, hipStream_t s, const int32_t* rp,
                               (long long)p->ncol * sizeof(typename P::x_t) +
                               (long long)p->nrow * sizeof(typename P::y_t);
   const bool nt = mbx_detail::stream_nontemporal(p->stream_mode, footprint);
-  if (p->ev_begin) (void)hipEventRecord(p->ev_begin, s);
   const size_t lds = mbx_detail::residency_lds(TileSmem<256, IPT, P>::bytes(), p->wg_per_cu);
   // the plan's 16-bit columns, instantiated for the tile shapes the banded
   // and graph products use
@@ -262,13 +261,13 @@ hipError_t launch_tile(const mbx_plan* p, hipStream_t s, const int32_t* rp,
       if (nt) {
         using Q = mbx_detail::Col16<mbx_detail::Streamed<P>>;
         mbx_detail::allow_lds(tile_spmv_kernel<256, IPT, true, Q>, lds);
-        tile_spmv_kernel<256, IPT, true, Q><<<p->ntiles, 256, lds, s>>>(
-            rp, c16, val, x, y, p->d_coord, p->d_carry, p->ntiles, p->ncol, alpha, beta, bz);
+        timed_launch(p, tile_spmv_kernel<256, IPT, true, Q>, p->ntiles, 256, lds, s, rp, c16, val,
+                     x, y, p->d_coord, p->d_carry, p->ntiles, p->ncol, alpha, beta, bz);
       } else {
         using Q = mbx_detail::Col16<P>;
         mbx_detail::allow_lds(tile_spmv_kernel<256, IPT, true, Q>, lds);
-        tile_spmv_kernel<256, IPT, true, Q><<<p->ntiles, 256, lds, s>>>(
-            rp, c16, val, x, y, p->d_coord, p->d_carry, p->ntiles, p->ncol, alpha, beta, bz);
+        timed_launch(p, tile_spmv_kernel<256, IPT, true, Q>, p->ntiles, 256, lds, s, rp, c16, val,
+                     x, y, p->d_coord, p->d_carry, p->ntiles, p->ncol, alpha, beta, bz);
       }
       launched = true;
     }
@@ -283,18 +282,18 @@ hipError_t launch_tile(const mbx_plan* p, hipStream_t s, const int32_t* rp,
   if (launched) {
   } else if (vec && nt) {
     mbx_detail::allow_lds(tile_spmv_kernel<256, IPT, true, mbx_detail::Streamed<P>>, lds);
-    tile_spmv_kernel<256, IPT, true, mbx_detail::Streamed<P>><<<p->ntiles, 256, lds, s>>>(
-        rp, col, val, x, y, p->d_coord, p->d_carry, p->ntiles, p->ncol, alpha, beta, bz);
+    timed_launch(p, tile_spmv_kernel<256, IPT, true, mbx_detail::Streamed<P>>, p->ntiles, 256,
+                 lds, s, rp, col, val, x, y, p->d_coord, p->d_carry, p->ntiles, p->ncol, alpha,
+                 beta, bz);
   } else if (vec) {
     mbx_detail::allow_lds(tile_spmv_kernel<256, IPT, true, P>, lds);
-    tile_spmv_kernel<256, IPT, true, P><<<p->ntiles, 256, lds, s>>>(
-        rp, col, val, x, y, p->d_coord, p->d_carry, p->ntiles, p->ncol, alpha, beta, bz);
+    timed_launch(p, tile_spmv_kernel<256, IPT, true, P>, p->ntiles, 256, lds, s, rp, col, val, x,
+                 y, p->d_coord, p->d_carry, p->ntiles, p->ncol, alpha, beta, bz);
   } else {
     mbx_detail::allow_lds(tile_spmv_kernel<256, IPT, false, P>, lds);
-    tile_spmv_kernel<256, IPT, false, P><<<p->ntiles, 256, lds, s>>>(
-        rp, col, val, x, y, p->d_coord, p->d_carry, p->ntiles, p->ncol, alpha, beta, bz);
+    timed_launch(p, tile_spmv_kernel<256, IPT, false, P>, p->ntiles, 256, lds, s, rp, col, val,
+                 x, y, p->d_coord, p->d_carry, p->ntiles, p->ncol, alpha, beta, bz);
   }
-  if (p->ev_end) (void)hipEventRecord(p->ev_end, s);
   p->ev_begin = p->ev_end = nullptr;
   if (p->split_rows > 0) {  // some row crosses a tile boundary
     const int grid = (p->ntiles + 255) / 256;

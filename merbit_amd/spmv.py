@@ -190,7 +190,9 @@ class MerbitPlan:
         return cov.value
 
     def set_events(self, begin, end) -> None:
-        """Record torch.cuda.Event pair around the next call's tile kernel."""
+        """Attach a torch.cuda.Event pair (timing enabled, recorded once so
+        that the underlying events exist) to the next eager call's tile
+        kernel: their elapsed time is that kernel's own execution."""
         check(lib().mbx_plan_set_events(self.handle, begin.cuda_event, end.cuda_event),
               "mbx_plan_set_events")
 

@@ -127,6 +127,63 @@ def test_hot_split_graph_replays(device):
     plan.close()
 
 
+def _timed_call(plan, fn):
+    b, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    b.record()  # the underlying events exist once recorded
+    e.record()
+    torch.cuda.synchronize()
+    plan.set_events(b, e)
+    out = fn()
+    torch.cuda.synchronize()
+    return out, b.elapsed_time(e)
+
+
+def test_timing_events_on_tile_kernel(device):
+    """Events set with set_events are attached to the next call's tile kernel
+    (plain and hot-column FP64/FP32 plans, and an SpMM): that call's result is
+    bit-identical to a call without them, the interval is positive, and the
+    events are cleared after the one call (the next call leaves them as they
+    were)."""
+    from merbit_amd.spmv import MerbitPlan
+    rng = np.random.default_rng(21)
+    n = 40000
+    rp, col = random_csr(rng, n, n, lens=(rng.pareto(1.1, n) * 5).astype(int))
+    nnz = int(rp[-1])
+    d_rp, d_col = dev(rp, device), dev(col, device)
+    for kind in ("f64", "f32"):
+        v, x = values(rng, kind, nnz, n)
+        d_val, d_x = dev(v, device), dev(x, device)
+        for nhot in (0, 2048):
+            say(f"events {kind} nhot {nhot}")
+            plan = MerbitPlan(d_rp, n, nnz, items_per_thread=12)
+            if nhot:
+                plan.split_hot(d_col, nhot)
+            ref = plan.spmv(d_rp, d_col, d_val, d_x)
+            y, ms = _timed_call(plan, lambda: plan.spmv(d_rp, d_col, d_val, d_x))
+            assert np.array_equal(bits(y), bits(ref))
+            assert 0.0 < ms < 1000.0
+            b2, e2 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            b2.record()
+            e2.record()
+            torch.cuda.synchronize()
+            before = b2.elapsed_time(e2)
+            y2 = plan.spmv(d_rp, d_col, d_val, d_x)
+            torch.cuda.synchronize()
+            assert np.array_equal(bits(y2), bits(ref))
+            assert b2.elapsed_time(e2) == before  # not attached to this call
+            plan.close()
+    say("events spmm k 8")
+    v, _ = values(rng, "f32", nnz, n)
+    d_val = dev(v, device)
+    X = torch.from_numpy(rng.uniform(-1, 1, (n, 8)).astype(np.float32)).to(device)
+    plan = MerbitPlan(d_rp, n, nnz, items_per_thread=4)
+    ref = plan.spmm(d_rp, d_col, d_val, X)
+    Y, ms = _timed_call(plan, lambda: plan.spmm(d_rp, d_col, d_val, X))
+    assert np.array_equal(bits(Y.contiguous()), bits(ref.contiguous()))
+    assert 0.0 < ms < 1000.0
+    plan.close()
+
+
 @pytest.mark.slow
 @pytest.mark.parametrize("kind", ["f64", "f32"])
 def test_hot_split_rmat_full(orc, device, kind):
