@@ -31,14 +31,58 @@ constexpr int kBlock = 256;
 
 inline int blocks(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 
-__global__ void col_count_kernel(const int32_t* __restrict__ col, int nnz, int ncol,
-                                 int32_t* __restrict__ cnt, int32_t* __restrict__ iota) {
-  const int k = blockIdx.x * kBlock + threadIdx.x;
-  if (k < ncol) iota[k] = k;
-  if (k < nnz) {
-    const int c = col[k];
-    if (c >= 0 && c < ncol) atomicAdd(cnt + c, 1);
+// Column counts, aggregated per workgroup: the workgroup's 4096 column
+// indices are sorted in LDS and each run of one column adds its length to
+// cnt with one atomic at its last entry and one at its first (a single
+// entry: one), so a column held by many nonzeros costs two atomics per
+// workgroup instead of one per nonzero (the hottest R-MAT columns made the
+// per-entry form serialise on their counters).  Indices outside [0, ncol)
+// and the padding past nnz sort last as the key ncol and are not counted.
+constexpr int kCountIpt = 16;
+constexpr int kCountPerBlock = kBlock * kCountIpt;
+
+__global__ __launch_bounds__(kBlock) void col_count_kernel(const int32_t* __restrict__ col,
+                                                           int nnz, int ncol, int end_bit,
+                                                           int32_t* __restrict__ cnt) {
+  using Sort = hipcub::BlockRadixSort<int, kBlock, kCountIpt>;
+  __shared__ typename Sort::TempStorage ts;
+  __shared__ int s_first[kBlock];
+  __shared__ int s_last[kBlock];
+  const int t = threadIdx.x;
+  const long long base = (long long)blockIdx.x * kCountPerBlock;
+  int key[kCountIpt];
+#pragma unroll
+  for (int i = 0; i < kCountIpt; ++i) {  // striped: coalesced loads
+    const long long k = base + (long long)i * kBlock + t;
+    int c = ncol;
+    if (k < nnz) {
+      const int v = col[k];
+      if (v >= 0 && v < ncol) c = v;
+    }
+    key[i] = c;
   }
+  Sort(ts).Sort(key, 0, end_bit);  // blocked: thread t holds sorted positions 16t .. 16t+15
+  s_first[t] = key[0];
+  s_last[t] = key[kCountIpt - 1];
+  __syncthreads();
+  const int before = t > 0 ? s_last[t - 1] : -1;
+  const int after = t < kBlock - 1 ? s_first[t + 1] : -1;
+#pragma unroll
+  for (int i = 0; i < kCountIpt; ++i) {
+    const int c = key[i];
+    if (c == ncol) continue;
+    const int pos = t * kCountIpt + i;
+    const bool head = c != (i > 0 ? key[i - 1] : before);
+    const bool tail = c != (i < kCountIpt - 1 ? key[i + 1] : after);
+    if (head && tail) atomicAdd(cnt + c, 1);
+    else if (head) atomicAdd(cnt + c, -pos);
+    else if (tail) atomicAdd(cnt + c, pos + 1);
+  }
+}
+
+__global__ void iota_kernel(int n, int32_t* __restrict__ iota) {
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k < n) iota[k] = k;
 }
 
 // The nhot most frequent columns: slots, the hot list and the entries they
@@ -86,16 +130,27 @@ void drop_hot(mbx_plan* p) {
 
 namespace mbx_detail {
 void plan_drop_hot(mbx_plan* p) { drop_hot(p); }
+
+int count_columns(const int32_t* d_col, int nnz, int ncol, int32_t* cnt, hipStream_t s) {
+  if (ncol <= 0) return MBX_OK;
+  if (hipMemsetAsync(cnt, 0, 4 * (size_t)ncol, s) != hipSuccess) return MBX_EHIP;
+  if (nnz <= 0) return MBX_OK;
+  const int end_bit = 32 - __builtin_clz((unsigned)ncol);  // keys 0 .. ncol
+  const long long nb = ((long long)nnz + kCountPerBlock - 1) / kCountPerBlock;
+  col_count_kernel<<<(int)nb, kBlock, 0, s>>>(d_col, nnz, ncol, end_bit, cnt);
+  return hipGetLastError() == hipSuccess ? MBX_OK : MBX_EHIP;
+}
 }  // namespace mbx_detail
 
-extern "C" int mbx_plan_split_hot(mbx_plan* p, const int32_t* d_col, int32_t max_hot,
-                                  void* stream, double* covered) {
+namespace mbx_detail {
+
+int split_hot_counted(mbx_plan* p, const int32_t* d_col, int32_t max_hot,
+                      const int32_t* cnt_given, hipStream_t s, double* covered) {
   if (!p || max_hot < 0) return MBX_EINVAL;
   if (covered) *covered = 0.0;
   drop_hot(p);
   if (!d_col || max_hot == 0 || p->nnz == 0 || p->ncol == 0) return MBX_OK;
   if (max_hot > kMaxHotColumns) return MBX_EUNSUPPORTED;
-  hipStream_t s = (hipStream_t)stream;
   const int ncol = p->ncol, nnz = p->nnz;
   const int nhot = max_hot < ncol ? max_hot : ncol;
   const int padded = (nnz + 3) / 4 * 4;
@@ -104,7 +159,7 @@ extern "C" int mbx_plan_split_hot(mbx_plan* p, const int32_t* d_col, int32_t max
   void* tmp = nullptr;
   size_t tmp_bytes = 0;
   int st = MBX_OK;
-  if (hipMalloc((void**)&cnt, 4 * (size_t)ncol) != hipSuccess ||
+  if ((!cnt_given && hipMalloc((void**)&cnt, 4 * (size_t)ncol) != hipSuccess) ||
       hipMalloc((void**)&iota, 4 * (size_t)ncol) != hipSuccess ||
       hipMalloc((void**)&scnt, 4 * (size_t)ncol) != hipSuccess ||
       hipMalloc((void**)&scol, 4 * (size_t)ncol) != hipSuccess ||
@@ -115,20 +170,24 @@ extern "C" int mbx_plan_split_hot(mbx_plan* p, const int32_t* d_col, int32_t max
       hipMalloc(&p->d_xh, ((size_t)nhot * 8 + 15) / 16 * 16) != hipSuccess)
     st = MBX_ENOMEM;
   if (st == MBX_OK &&
-      (hipMemsetAsync(cnt, 0, 4 * (size_t)ncol, s) != hipSuccess ||
-       hipMemsetAsync(slot, 0xFF, 4 * (size_t)ncol, s) != hipSuccess ||
+      (hipMemsetAsync(slot, 0xFF, 4 * (size_t)ncol, s) != hipSuccess ||
        hipMemsetAsync(d_cov, 0, sizeof(unsigned long long), s) != hipSuccess ||
        hipMemsetAsync(p->d_xh, 0, ((size_t)nhot * 8 + 15) / 16 * 16, s) != hipSuccess))
     st = MBX_EHIP;
   if (st == MBX_OK) {
-    col_count_kernel<<<blocks(nnz > ncol ? nnz : ncol), kBlock, 0, s>>>(d_col, nnz, ncol, cnt,
-                                                                        iota);
-    // most frequent first; equal counts keep ascending column order (stable)
-    if (hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tmp_bytes, cnt, scnt, iota, scol,
-                                                     ncol, 0, 32, s) != hipSuccess ||
+    iota_kernel<<<blocks(ncol), kBlock, 0, s>>>(ncol, iota);
+    if (!cnt_given) st = count_columns(d_col, nnz, ncol, cnt, s);
+  }
+  const int32_t* counts = cnt_given ? cnt_given : cnt;
+  if (st == MBX_OK) {
+    // most frequent first; equal counts keep ascending column order (the
+    // sort is stable); counts are at most nnz, so cbits bits order them
+    const int cbits = 32 - __builtin_clz((unsigned)nnz);
+    if (hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tmp_bytes, counts, scnt, iota, scol,
+                                                     ncol, 0, cbits, s) != hipSuccess ||
         hipMalloc(&tmp, tmp_bytes ? tmp_bytes : 1) != hipSuccess ||
-        hipcub::DeviceRadixSort::SortPairsDescending(tmp, tmp_bytes, cnt, scnt, iota, scol, ncol,
-                                                     0, 32, s) != hipSuccess)
+        hipcub::DeviceRadixSort::SortPairsDescending(tmp, tmp_bytes, counts, scnt, iota, scol, ncol,
+                                                     0, cbits, s) != hipSuccess)
       st = MBX_EHIP;
   }
   unsigned long long h_cov = 0;
@@ -155,4 +214,12 @@ extern "C" int mbx_plan_split_hot(mbx_plan* p, const int32_t* d_col, int32_t max
   p->nhot = nhot;
   if (covered) *covered = (double)h_cov / (double)nnz;
   return MBX_OK;
+}
+
+}  // namespace mbx_detail
+
+extern "C" int mbx_plan_split_hot(mbx_plan* p, const int32_t* d_col, int32_t max_hot,
+                                  void* stream, double* covered) {
+  return mbx_detail::split_hot_counted(p, d_col, max_hot, nullptr, (hipStream_t)stream,
+                                       covered);
 }

@@ -93,6 +93,13 @@ int csr_check(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow, int3
 
 // Frees a plan's hot-column split (merbit_hot.hip), waiting for the device.
 void plan_drop_hot(mbx_plan* p);
+// cnt[c] = entries of d_col equal to c, for c < ncol (entries outside
+// [0, ncol) are not counted); cnt is cleared first.  Stream-ordered.
+int count_columns(const int32_t* d_col, int nnz, int ncol, int32_t* cnt, hipStream_t s);
+// mbx_plan_split_hot with the column counts of d_col given (cnt_given, as
+// count_columns makes them) or, when it is null, made here.
+int split_hot_counted(mbx_plan* p, const int32_t* d_col, int32_t max_hot,
+                      const int32_t* cnt_given, hipStream_t s, double* covered);
 
 constexpr int kXcds = 8;
 

@@ -46,15 +46,6 @@ constexpr int kFewTilesPerCu = 16;
 
 inline int blocks(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 
-__global__ void rule_count_kernel(const int32_t* __restrict__ col, int nnz, int ncol,
-                                  int32_t* __restrict__ cnt) {
-  const int k = blockIdx.x * kBlock + threadIdx.x;
-  if (k < nnz) {
-    const int c = col[k];
-    if (c >= 0 && c < ncol) atomicAdd(cnt + c, 1);
-  }
-}
-
 // Entries in the k most frequent columns (sorted counts, descending).
 __global__ void rule_top_kernel(const int32_t* __restrict__ scnt, int k,
                                 unsigned long long* __restrict__ out) {
@@ -94,9 +85,10 @@ __global__ __launch_bounds__(256) void rule_span_kernel(const int32_t* __restric
 }
 
 // The column features: the widest tile span at 8 items per thread and the
-// share of the entries in the `hot` most frequent columns.
+// share of the entries in the `hot` most frequent columns.  With cnt_keep
+// (ncol counters, the caller's), the column counts are left there.
 int column_features(const int32_t* rp, const int32_t* col, int nrow, int ncol, int nnz, int hot,
-                    hipStream_t s, int* span_out, double* share_out) {
+                    hipStream_t s, int* span_out, double* share_out, int32_t* cnt_keep) {
   *span_out = 0;
   *share_out = 0.0;
   if (nnz == 0 || ncol == 0) return MBX_OK;
@@ -108,22 +100,25 @@ int column_features(const int32_t* rp, const int32_t* col, int nrow, int ncol, i
   unsigned long long* d_top = nullptr;
   void* tmp = nullptr;
   size_t tmp_bytes = 0;
-  if (hipMalloc((void**)&cnt, 4 * (size_t)ncol) != hipSuccess ||
+  if (cnt_keep) cnt = cnt_keep;
+  if ((!cnt_keep && hipMalloc((void**)&cnt, 4 * (size_t)ncol) != hipSuccess) ||
       hipMalloc((void**)&scnt, 4 * (size_t)ncol) != hipSuccess ||
       hipMalloc((void**)&d_span, sizeof(int)) != hipSuccess ||
       hipMalloc((void**)&d_top, sizeof(unsigned long long)) != hipSuccess)
     st = MBX_ENOMEM;
-  if (st == MBX_OK && (hipMemsetAsync(cnt, 0, 4 * (size_t)ncol, s) != hipSuccess ||
-                       hipMemsetAsync(d_span, 0, sizeof(int), s) != hipSuccess ||
+  if (st == MBX_OK && (hipMemsetAsync(d_span, 0, sizeof(int), s) != hipSuccess ||
                        hipMemsetAsync(d_top, 0, sizeof(unsigned long long), s) != hipSuccess))
     st = MBX_EHIP;
   if (st == MBX_OK) {
     rule_span_kernel<<<p8->ntiles, 256, 0, s>>>(col, p8->d_coord, d_span);
-    rule_count_kernel<<<blocks(nnz), kBlock, 0, s>>>(col, nnz, ncol, cnt);
-    if (hipcub::DeviceRadixSort::SortKeysDescending(nullptr, tmp_bytes, cnt, scnt, ncol, 0, 32,
+    st = mbx_detail::count_columns(col, nnz, ncol, cnt, s);
+  }
+  const int cbits = 32 - __builtin_clz((unsigned)nnz);  // counts are at most nnz
+  if (st == MBX_OK) {
+    if (hipcub::DeviceRadixSort::SortKeysDescending(nullptr, tmp_bytes, cnt, scnt, ncol, 0, cbits,
                                                     s) != hipSuccess ||
         hipMalloc(&tmp, tmp_bytes ? tmp_bytes : 1) != hipSuccess ||
-        hipcub::DeviceRadixSort::SortKeysDescending(tmp, tmp_bytes, cnt, scnt, ncol, 0, 32, s) !=
+        hipcub::DeviceRadixSort::SortKeysDescending(tmp, tmp_bytes, cnt, scnt, ncol, 0, cbits, s) !=
             hipSuccess)
       st = MBX_EHIP;
   }
@@ -138,7 +133,7 @@ int column_features(const int32_t* rp, const int32_t* col, int nrow, int ncol, i
       st = MBX_EHIP;
   }
   if (hipStreamSynchronize(s) != hipSuccess && st == MBX_OK) st = MBX_EHIP;
-  if (cnt) (void)hipFree(cnt);
+  if (cnt && !cnt_keep) (void)hipFree(cnt);
   if (scnt) (void)hipFree(scnt);
   if (d_span) (void)hipFree(d_span);
   if (d_top) (void)hipFree(d_top);
@@ -153,12 +148,10 @@ int column_features(const int32_t* rp, const int32_t* col, int nrow, int ncol, i
   return MBX_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int mbx_launch_rule(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow, int32_t ncol,
-                    int32_t nnz, int32_t value_kind, void* stream, mbx_launch_choice* out) {
+// mbx_launch_rule, leaving the column counts in cnt_keep when it is given.
+int launch_rule(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow, int32_t ncol,
+                int32_t nnz, int32_t value_kind, void* stream, mbx_launch_choice* out,
+                int32_t* cnt_keep) {
   if (!out) return MBX_EINVAL;
   if (nrow < 0 || ncol < 0 || nnz < 0 || (nrow > 0 && !d_row_ptr) || (nnz > 0 && !d_col))
     return MBX_EINVAL;
@@ -184,7 +177,7 @@ int mbx_launch_rule(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow
     int span = 0;
     double share = 0.0;
     const int st = column_features(d_row_ptr, d_col, nrow, ncol, nnz, hot, (hipStream_t)stream,
-                                   &span, &share);
+                                   &span, &share, cnt_keep);
     if (st != MBX_OK) return st;
     c.max_tile_span = span;
     c.hot_share = share;
@@ -218,17 +211,35 @@ int mbx_launch_rule(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow
   return MBX_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int mbx_launch_rule(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow, int32_t ncol,
+                    int32_t nnz, int32_t value_kind, void* stream, mbx_launch_choice* out) {
+  return launch_rule(d_row_ptr, d_col, nrow, ncol, nnz, value_kind, stream, out, nullptr);
+}
+
 int mbx_plan_create_auto(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow,
                          int32_t ncol, int32_t nnz, int32_t value_kind, void* stream,
                          mbx_plan** out, mbx_launch_choice* choice) {
   if (!out) return MBX_EINVAL;
   *out = nullptr;
+  // the column counts are made once, for the rule and the hot split
+  int32_t* cnt = nullptr;
+  if (nnz > 0 && ncol > 0 && hipMalloc((void**)&cnt, 4 * (size_t)ncol) != hipSuccess) {
+    (void)hipGetLastError();
+    return MBX_ENOMEM;
+  }
   mbx_launch_choice c;
-  int st = mbx_launch_rule(d_row_ptr, d_col, nrow, ncol, nnz, value_kind, stream, &c);
-  if (st != MBX_OK) return st;
   mbx_plan* p = nullptr;
-  st = mbx_plan_create(d_row_ptr, nrow, ncol, nnz, c.items_per_thread, 256, stream, &p);
-  if (st != MBX_OK) return st;
+  int st = launch_rule(d_row_ptr, d_col, nrow, ncol, nnz, value_kind, stream, &c, cnt);
+  if (st == MBX_OK)
+    st = mbx_plan_create(d_row_ptr, nrow, ncol, nnz, c.items_per_thread, 256, stream, &p);
+  if (st != MBX_OK) {
+    if (cnt) (void)hipFree(cnt);
+    return st;
+  }
   (void)mbx_plan_set_stream(p, c.stream_mode);
   (void)mbx_plan_set_residency(p, c.wg_per_cu);
   (void)mbx_plan_set_accumulation(p, c.accumulation);
@@ -239,8 +250,10 @@ int mbx_plan_create_auto(const int32_t* d_row_ptr, const int32_t* d_col, int32_t
       st = MBX_OK;
     }
   }
-  if (st == MBX_OK && c.hot_columns > 0) st = mbx_plan_split_hot(p, d_col, c.hot_columns, stream,
-                                                                 nullptr);
+  if (st == MBX_OK && c.hot_columns > 0)
+    st = mbx_detail::split_hot_counted(p, d_col, c.hot_columns, cnt, (hipStream_t)stream,
+                                       nullptr);
+  if (cnt) (void)hipFree(cnt);
   if (st != MBX_OK) {
     mbx_plan_destroy(p);
     return st;

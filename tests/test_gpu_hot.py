@@ -103,6 +103,35 @@ def test_hot_split_coverage_and_fallback(device):
     plan.close()
 
 
+@pytest.mark.parametrize("nnz,ncol", [(4095, 50000), (4096, 50000), (4097, 50000),
+                                      (3 * 4096 - 1, 7), (3 * 4096 + 1, 1), (40000, 200000)])
+def test_hot_split_counts_at_block_edges(device, nnz, ncol):
+    """The column counts behind the split are aggregated per 4096 entries on
+    the device (sorted runs, one atomic pair per run): one column spanning
+    whole count blocks, runs across thread boundaries, nnz at and around
+    multiples of 4096, one-column and few-column patterns.  The covered
+    share for every nhot prefix must match a host count."""
+    from merbit_amd.spmv import MerbitPlan
+    rng = np.random.default_rng(nnz + ncol)
+    col = rng.integers(0, ncol, nnz)
+    col[: nnz * 2 // 5] = 0                      # one column over whole blocks
+    col[nnz // 2: nnz // 2 + 300] = min(3, ncol - 1)  # a run straddling thread ranges
+    rng.shuffle(col[nnz // 3:])
+    col = col.astype(np.int32)
+    rp = np.minimum(np.arange(0, nnz + 8, 8), nnz).astype(np.int32)
+    if rp[-1] != nnz:
+        rp = np.append(rp, nnz).astype(np.int32)
+    nrow = len(rp) - 1
+    cnt = np.sort(np.bincount(col, minlength=ncol))[::-1]
+    d_rp, d_col = dev(rp, device), dev(col, device)
+    plan = MerbitPlan(d_rp, ncol, nnz)
+    for nhot in (1, 2, 3, 4, 10, 100, 1000):
+        say(f"counts nnz {nnz} ncol {ncol} nrow {nrow} nhot {nhot}")
+        cov = plan.split_hot(d_col, nhot)
+        assert cov == pytest.approx(cnt[:nhot].sum() / nnz, rel=0, abs=1e-12)
+    plan.close()
+
+
 def test_hot_split_graph_replays(device):
     """A captured product of a split plan replays bit-identically."""
     from merbit_amd.batch import spmv_graph
