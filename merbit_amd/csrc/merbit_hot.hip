@@ -31,53 +31,26 @@ constexpr int kBlock = 256;
 
 inline int blocks(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 
-// Column counts, aggregated per workgroup: the workgroup's 4096 column
-// indices are sorted in LDS and each run of one column adds its length to
-// cnt with one atomic at its last entry and one at its first (a single
-// entry: one), so a column held by many nonzeros costs two atomics per
-// workgroup instead of one per nonzero (the hottest R-MAT columns made the
-// per-entry form serialise on their counters).  Indices outside [0, ncol)
-// and the padding past nnz sort last as the key ncol and are not counted.
-constexpr int kCountIpt = 16;
-constexpr int kCountPerBlock = kBlock * kCountIpt;
+// Column counts by sorting: the column indices (those outside [0, ncol)
+// mapped to the key ncol) are radix-sorted on the bits of ncol, the sorted
+// keys run-length encoded, and each run's length stored at its column.  No
+// atomics: one atomic increment per nonzero took 2.2 ms on the 2^20-row
+// R-MAT, bound by the atomic rate and, on its hottest columns, serialised
+// on one counter.
+__global__ void col_keys_kernel(const int32_t* __restrict__ col, int nnz, int ncol,
+                                int32_t* __restrict__ keys) {
+  const long long k = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (k < nnz) {
+    const int c = col[k];
+    keys[k] = (c >= 0 && c < ncol) ? c : ncol;
+  }
+}
 
-__global__ __launch_bounds__(kBlock) void col_count_kernel(const int32_t* __restrict__ col,
-                                                           int nnz, int ncol, int end_bit,
-                                                           int32_t* __restrict__ cnt) {
-  using Sort = hipcub::BlockRadixSort<int, kBlock, kCountIpt>;
-  __shared__ typename Sort::TempStorage ts;
-  __shared__ int s_first[kBlock];
-  __shared__ int s_last[kBlock];
-  const int t = threadIdx.x;
-  const long long base = (long long)blockIdx.x * kCountPerBlock;
-  int key[kCountIpt];
-#pragma unroll
-  for (int i = 0; i < kCountIpt; ++i) {  // striped: coalesced loads
-    const long long k = base + (long long)i * kBlock + t;
-    int c = ncol;
-    if (k < nnz) {
-      const int v = col[k];
-      if (v >= 0 && v < ncol) c = v;
-    }
-    key[i] = c;
-  }
-  Sort(ts).Sort(key, 0, end_bit);  // blocked: thread t holds sorted positions 16t .. 16t+15
-  s_first[t] = key[0];
-  s_last[t] = key[kCountIpt - 1];
-  __syncthreads();
-  const int before = t > 0 ? s_last[t - 1] : -1;
-  const int after = t < kBlock - 1 ? s_first[t + 1] : -1;
-#pragma unroll
-  for (int i = 0; i < kCountIpt; ++i) {
-    const int c = key[i];
-    if (c == ncol) continue;
-    const int pos = t * kCountIpt + i;
-    const bool head = c != (i > 0 ? key[i - 1] : before);
-    const bool tail = c != (i < kCountIpt - 1 ? key[i + 1] : after);
-    if (head && tail) atomicAdd(cnt + c, 1);
-    else if (head) atomicAdd(cnt + c, -pos);
-    else if (tail) atomicAdd(cnt + c, pos + 1);
-  }
+__global__ void run_store_kernel(const int32_t* __restrict__ uniq,
+                                 const int32_t* __restrict__ len, const int32_t* __restrict__ nruns,
+                                 int max_runs, int ncol, int32_t* __restrict__ cnt) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < max_runs && i < *nruns && uniq[i] < ncol) cnt[uniq[i]] = len[i];
 }
 
 __global__ void iota_kernel(int n, int32_t* __restrict__ iota) {
@@ -136,9 +109,50 @@ int count_columns(const int32_t* d_col, int nnz, int ncol, int32_t* cnt, hipStre
   if (hipMemsetAsync(cnt, 0, 4 * (size_t)ncol, s) != hipSuccess) return MBX_EHIP;
   if (nnz <= 0) return MBX_OK;
   const int end_bit = 32 - __builtin_clz((unsigned)ncol);  // keys 0 .. ncol
-  const long long nb = ((long long)nnz + kCountPerBlock - 1) / kCountPerBlock;
-  col_count_kernel<<<(int)nb, kBlock, 0, s>>>(d_col, nnz, ncol, end_bit, cnt);
-  return hipGetLastError() == hipSuccess ? MBX_OK : MBX_EHIP;
+  const int max_runs = nnz < ncol ? nnz : ncol + 1;
+  int32_t *k0 = nullptr, *k1 = nullptr, *uniq = nullptr, *len = nullptr, *nruns = nullptr;
+  void* tmp = nullptr;
+  size_t sort_bytes = 0, rle_bytes = 0;
+  int st = MBX_OK;
+  hipcub::DoubleBuffer<int32_t> keys;
+  if (hipMalloc((void**)&k0, 4 * (size_t)nnz) != hipSuccess ||
+      hipMalloc((void**)&k1, 4 * (size_t)nnz) != hipSuccess ||
+      hipMalloc((void**)&uniq, 4 * (size_t)max_runs) != hipSuccess ||
+      hipMalloc((void**)&len, 4 * (size_t)max_runs) != hipSuccess ||
+      hipMalloc((void**)&nruns, sizeof(int32_t)) != hipSuccess)
+    st = MBX_ENOMEM;
+  if (st == MBX_OK) {
+    keys = hipcub::DoubleBuffer<int32_t>(k0, k1);
+    if (hipcub::DeviceRadixSort::SortKeys(nullptr, sort_bytes, keys, nnz, 0, end_bit, s) !=
+            hipSuccess ||
+        hipcub::DeviceRunLengthEncode::Encode(nullptr, rle_bytes, k0, uniq, len, nruns, nnz,
+                                              s) != hipSuccess ||
+        hipMalloc(&tmp, sort_bytes > rle_bytes ? sort_bytes : rle_bytes) != hipSuccess)
+      st = MBX_ENOMEM;
+  }
+  if (st == MBX_OK) {
+    col_keys_kernel<<<blocks(nnz), kBlock, 0, s>>>(d_col, nnz, ncol, k0);
+    if (hipGetLastError() != hipSuccess ||
+        hipcub::DeviceRadixSort::SortKeys(tmp, sort_bytes, keys, nnz, 0, end_bit, s) !=
+            hipSuccess ||
+        hipcub::DeviceRunLengthEncode::Encode(tmp, rle_bytes, keys.Current(), uniq, len, nruns,
+                                              nnz, s) != hipSuccess)
+      st = MBX_EHIP;
+  }
+  if (st == MBX_OK) {
+    run_store_kernel<<<blocks(max_runs), kBlock, 0, s>>>(uniq, len, nruns, max_runs, ncol, cnt);
+    if (hipGetLastError() != hipSuccess) st = MBX_EHIP;
+  }
+  // the buffers are released once the stream has used them
+  if (hipStreamSynchronize(s) != hipSuccess && st == MBX_OK) st = MBX_EHIP;
+  if (k0) (void)hipFree(k0);
+  if (k1) (void)hipFree(k1);
+  if (uniq) (void)hipFree(uniq);
+  if (len) (void)hipFree(len);
+  if (nruns) (void)hipFree(nruns);
+  if (tmp) (void)hipFree(tmp);
+  if (st != MBX_OK) (void)hipGetLastError();
+  return st;
 }
 }  // namespace mbx_detail
 

@@ -106,11 +106,11 @@ def test_hot_split_coverage_and_fallback(device):
 @pytest.mark.parametrize("nnz,ncol", [(4095, 50000), (4096, 50000), (4097, 50000),
                                       (3 * 4096 - 1, 7), (3 * 4096 + 1, 1), (40000, 200000)])
 def test_hot_split_counts_at_block_edges(device, nnz, ncol):
-    """The column counts behind the split are aggregated per 4096 entries on
-    the device (sorted runs, one atomic pair per run): one column spanning
-    whole count blocks, runs across thread boundaries, nnz at and around
-    multiples of 4096, one-column and few-column patterns.  The covered
-    share for every nhot prefix must match a host count."""
+    """The column counts behind the split (device radix sort of the column
+    indices, run-length encoded): one column holding 40 % of the entries,
+    a long run elsewhere, nnz at and around multiples of the sort's 4096-key
+    blocks, one-column and few-column patterns.  The covered share for
+    every nhot prefix must match a host count."""
     from merbit_amd.spmv import MerbitPlan
     rng = np.random.default_rng(nnz + ncol)
     col = rng.integers(0, ncol, nnz)
