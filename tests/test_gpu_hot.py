@@ -108,14 +108,14 @@ def test_hot_split_coverage_and_fallback(device):
 def test_hot_split_counts_at_block_edges(device, nnz, ncol):
     """The column counts behind the split (device radix sort of the column
     indices, run-length encoded): one column holding 40 % of the entries,
-    a long run elsewhere, nnz at and around multiples of the sort's 4096-key
-    blocks, one-column and few-column patterns.  The covered share for
-    every nhot prefix must match a host count."""
+    a long run of another, nnz at and around multiples of 4096, one-column
+    and few-column patterns.  The covered share for every nhot prefix must
+    match a host count."""
     from merbit_amd.spmv import MerbitPlan
     rng = np.random.default_rng(nnz + ncol)
     col = rng.integers(0, ncol, nnz)
-    col[: nnz * 2 // 5] = 0                      # one column over whole blocks
-    col[nnz // 2: nnz // 2 + 300] = min(3, ncol - 1)  # a run straddling thread ranges
+    col[: nnz * 2 // 5] = 0                      # one column with 40 % of the entries
+    col[nnz // 2: nnz // 2 + 300] = min(3, ncol - 1)  # a run of 300 entries
     rng.shuffle(col[nnz // 3:])
     col = col.astype(np.int32)
     rp = np.minimum(np.arange(0, nnz + 8, 8), nnz).astype(np.int32)
