@@ -253,21 +253,36 @@ def bench_rmat_f64(A, x, dev, args, checks):
     ceilings = None
     if not args.no_probes and ipt in (8, 12):
         ceilings = bench_ceilings(A, copies, ncol, nnz, ipt, stream, resid, args)
-    # the launch rule's plans (the library default), on the same rotation
+    # the launch rule's plans (the library default), on the same rotation,
+    # and on one copy over and over (the rule's own use: a matrix reused)
+    auto_ms = []
+    for _ in range(3):  # rule + hot split + plan, host clock
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        MerbitPlan(c0["rp"], ncol, nnz, col=c0["col"], value_dtype=torch.float64).close()
+        auto_ms.append((time.perf_counter() - t0) * 1e3)
     rp_rule = rule_plans(copies, ncol, nnz, torch.float64)
     rule_ms, _ = time_steps(spmv_calls(rp_rule, copies), args.steps, args.warmup,
                             graph=not args.no_graph)
+    rule_warm_ms = None
+    if not args.no_warm:
+        rule_warm_ms, _ = time_steps(spmv_calls(rp_rule, copies)[:1], args.steps, args.warmup,
+                                     graph=not args.no_graph)
     rule = dict(rp_rule[0].choice)
     for p in rp_rule:
         p.close()
     log(f"rmat fp64 (ipt {ipt}, {stream} stream, {resid or 'all'} tiles/CU, hot {hot}): "
         f"{ms * 1e3:.1f} us/step rotated ({gbs(b_alg, ms)} GB/s), tile kernel "
         f"{kern * 1e3:.1f} us; rule ({rule['rule']}) {rule_ms * 1e3:.1f} us"
-        + (f"; warm {warm_ms * 1e3:.1f} us" if warm_ms else ""))
+        + (f"; warm {warm_ms * 1e3:.1f} us, rule's warm {rule_warm_ms * 1e3:.1f} us"
+           if warm_ms else "") + f"; rule's plan creation {np.median(auto_ms):.2f} ms")
     res = dict(b_alg=b_alg, ms=ms, kern=kern, warm_ms=warm_ms, warm_kern=warm_kern,
                ntiles=plans[0].ntiles, tile_items=plans[0].tile_items, ipt=ipt, stream=stream,
                resid=resid, hot=hot, plan_ms=float(np.median(plan_ms)), ceilings=ceilings,
                rule={"us_per_step": round(rule_ms * 1e3, 2), "gbs": gbs(b_alg, rule_ms),
+                     "warm_us_per_step": (round(rule_warm_ms * 1e3, 2) if rule_warm_ms
+                                          else None),
+                     "plan_create_ms": round(float(np.median(auto_ms)), 3),
                      "choice": {k: (round(v, 4) if isinstance(v, float) else v)
                                 for k, v in rule.items()}})
     for p in plans:

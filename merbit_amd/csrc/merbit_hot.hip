@@ -115,11 +115,11 @@ int count_columns(const int32_t* d_col, int nnz, int ncol, int32_t* cnt, hipStre
   size_t sort_bytes = 0, rle_bytes = 0;
   int st = MBX_OK;
   hipcub::DoubleBuffer<int32_t> keys;
-  if (hipMalloc((void**)&k0, 4 * (size_t)nnz) != hipSuccess ||
-      hipMalloc((void**)&k1, 4 * (size_t)nnz) != hipSuccess ||
-      hipMalloc((void**)&uniq, 4 * (size_t)max_runs) != hipSuccess ||
-      hipMalloc((void**)&len, 4 * (size_t)max_runs) != hipSuccess ||
-      hipMalloc((void**)&nruns, sizeof(int32_t)) != hipSuccess)
+  if (hipMallocAsync((void**)&k0, 4 * (size_t)nnz, s) != hipSuccess ||
+      hipMallocAsync((void**)&k1, 4 * (size_t)nnz, s) != hipSuccess ||
+      hipMallocAsync((void**)&uniq, 4 * (size_t)max_runs, s) != hipSuccess ||
+      hipMallocAsync((void**)&len, 4 * (size_t)max_runs, s) != hipSuccess ||
+      hipMallocAsync((void**)&nruns, sizeof(int32_t), s) != hipSuccess)
     st = MBX_ENOMEM;
   if (st == MBX_OK) {
     keys = hipcub::DoubleBuffer<int32_t>(k0, k1);
@@ -127,7 +127,7 @@ int count_columns(const int32_t* d_col, int nnz, int ncol, int32_t* cnt, hipStre
             hipSuccess ||
         hipcub::DeviceRunLengthEncode::Encode(nullptr, rle_bytes, k0, uniq, len, nruns, nnz,
                                               s) != hipSuccess ||
-        hipMalloc(&tmp, sort_bytes > rle_bytes ? sort_bytes : rle_bytes) != hipSuccess)
+        hipMallocAsync(&tmp, sort_bytes > rle_bytes ? sort_bytes : rle_bytes, s) != hipSuccess)
       st = MBX_ENOMEM;
   }
   if (st == MBX_OK) {
@@ -143,14 +143,9 @@ int count_columns(const int32_t* d_col, int nnz, int ncol, int32_t* cnt, hipStre
     run_store_kernel<<<blocks(max_runs), kBlock, 0, s>>>(uniq, len, nruns, max_runs, ncol, cnt);
     if (hipGetLastError() != hipSuccess) st = MBX_EHIP;
   }
-  // the buffers are released once the stream has used them
-  if (hipStreamSynchronize(s) != hipSuccess && st == MBX_OK) st = MBX_EHIP;
-  if (k0) (void)hipFree(k0);
-  if (k1) (void)hipFree(k1);
-  if (uniq) (void)hipFree(uniq);
-  if (len) (void)hipFree(len);
-  if (nruns) (void)hipFree(nruns);
-  if (tmp) (void)hipFree(tmp);
+  // stream-ordered: released once the stream has used them
+  for (void* q : {(void*)k0, (void*)k1, (void*)uniq, (void*)len, (void*)nruns, tmp})
+    if (q) (void)hipFreeAsync(q, s);
   if (st != MBX_OK) (void)hipGetLastError();
   return st;
 }
@@ -173,12 +168,12 @@ int split_hot_counted(mbx_plan* p, const int32_t* d_col, int32_t max_hot,
   void* tmp = nullptr;
   size_t tmp_bytes = 0;
   int st = MBX_OK;
-  if ((!cnt_given && hipMalloc((void**)&cnt, 4 * (size_t)ncol) != hipSuccess) ||
-      hipMalloc((void**)&iota, 4 * (size_t)ncol) != hipSuccess ||
-      hipMalloc((void**)&scnt, 4 * (size_t)ncol) != hipSuccess ||
-      hipMalloc((void**)&scol, 4 * (size_t)ncol) != hipSuccess ||
-      hipMalloc((void**)&slot, 4 * (size_t)ncol) != hipSuccess ||
-      hipMalloc((void**)&d_cov, sizeof(unsigned long long)) != hipSuccess ||
+  if ((!cnt_given && hipMallocAsync((void**)&cnt, 4 * (size_t)ncol, s) != hipSuccess) ||
+      hipMallocAsync((void**)&iota, 4 * (size_t)ncol, s) != hipSuccess ||
+      hipMallocAsync((void**)&scnt, 4 * (size_t)ncol, s) != hipSuccess ||
+      hipMallocAsync((void**)&scol, 4 * (size_t)ncol, s) != hipSuccess ||
+      hipMallocAsync((void**)&slot, 4 * (size_t)ncol, s) != hipSuccess ||
+      hipMallocAsync((void**)&d_cov, sizeof(unsigned long long), s) != hipSuccess ||
       hipMalloc((void**)&p->d_colh, 4 * (size_t)padded) != hipSuccess ||
       hipMalloc((void**)&p->d_hot, 4 * (size_t)nhot) != hipSuccess ||
       hipMalloc(&p->d_xh, ((size_t)nhot * 8 + 15) / 16 * 16) != hipSuccess)
@@ -199,7 +194,7 @@ int split_hot_counted(mbx_plan* p, const int32_t* d_col, int32_t max_hot,
     const int cbits = 32 - __builtin_clz((unsigned)nnz);
     if (hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tmp_bytes, counts, scnt, iota, scol,
                                                      ncol, 0, cbits, s) != hipSuccess ||
-        hipMalloc(&tmp, tmp_bytes ? tmp_bytes : 1) != hipSuccess ||
+        hipMallocAsync(&tmp, tmp_bytes ? tmp_bytes : 1, s) != hipSuccess ||
         hipcub::DeviceRadixSort::SortPairsDescending(tmp, tmp_bytes, counts, scnt, iota, scol, ncol,
                                                      0, cbits, s) != hipSuccess)
       st = MBX_EHIP;
@@ -213,13 +208,9 @@ int split_hot_counted(mbx_plan* p, const int32_t* d_col, int32_t max_hot,
       st = MBX_EHIP;
   }
   if (hipStreamSynchronize(s) != hipSuccess && st == MBX_OK) st = MBX_EHIP;
-  if (cnt) (void)hipFree(cnt);
-  if (iota) (void)hipFree(iota);
-  if (scnt) (void)hipFree(scnt);
-  if (scol) (void)hipFree(scol);
-  if (slot) (void)hipFree(slot);
-  if (d_cov) (void)hipFree(d_cov);
-  if (tmp) (void)hipFree(tmp);
+  for (void* q : {(void*)cnt, (void*)iota, (void*)scnt, (void*)scol, (void*)slot,
+                  (void*)d_cov, tmp})
+    if (q) (void)hipFreeAsync(q, s);
   if (st != MBX_OK) {
     (void)hipGetLastError();
     drop_hot(p);

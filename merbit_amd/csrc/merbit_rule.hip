@@ -101,10 +101,10 @@ int column_features(const int32_t* rp, const int32_t* col, int nrow, int ncol, i
   void* tmp = nullptr;
   size_t tmp_bytes = 0;
   if (cnt_keep) cnt = cnt_keep;
-  if ((!cnt_keep && hipMalloc((void**)&cnt, 4 * (size_t)ncol) != hipSuccess) ||
-      hipMalloc((void**)&scnt, 4 * (size_t)ncol) != hipSuccess ||
-      hipMalloc((void**)&d_span, sizeof(int)) != hipSuccess ||
-      hipMalloc((void**)&d_top, sizeof(unsigned long long)) != hipSuccess)
+  if ((!cnt_keep && hipMallocAsync((void**)&cnt, 4 * (size_t)ncol, s) != hipSuccess) ||
+      hipMallocAsync((void**)&scnt, 4 * (size_t)ncol, s) != hipSuccess ||
+      hipMallocAsync((void**)&d_span, sizeof(int), s) != hipSuccess ||
+      hipMallocAsync((void**)&d_top, sizeof(unsigned long long), s) != hipSuccess)
     st = MBX_ENOMEM;
   if (st == MBX_OK && (hipMemsetAsync(d_span, 0, sizeof(int), s) != hipSuccess ||
                        hipMemsetAsync(d_top, 0, sizeof(unsigned long long), s) != hipSuccess))
@@ -117,7 +117,7 @@ int column_features(const int32_t* rp, const int32_t* col, int nrow, int ncol, i
   if (st == MBX_OK) {
     if (hipcub::DeviceRadixSort::SortKeysDescending(nullptr, tmp_bytes, cnt, scnt, ncol, 0, cbits,
                                                     s) != hipSuccess ||
-        hipMalloc(&tmp, tmp_bytes ? tmp_bytes : 1) != hipSuccess ||
+        hipMallocAsync(&tmp, tmp_bytes ? tmp_bytes : 1, s) != hipSuccess ||
         hipcub::DeviceRadixSort::SortKeysDescending(tmp, tmp_bytes, cnt, scnt, ncol, 0, cbits, s) !=
             hipSuccess)
       st = MBX_EHIP;
@@ -133,11 +133,8 @@ int column_features(const int32_t* rp, const int32_t* col, int nrow, int ncol, i
       st = MBX_EHIP;
   }
   if (hipStreamSynchronize(s) != hipSuccess && st == MBX_OK) st = MBX_EHIP;
-  if (cnt && !cnt_keep) (void)hipFree(cnt);
-  if (scnt) (void)hipFree(scnt);
-  if (d_span) (void)hipFree(d_span);
-  if (d_top) (void)hipFree(d_top);
-  if (tmp) (void)hipFree(tmp);
+  for (void* q : {(void*)(cnt_keep ? nullptr : cnt), (void*)scnt, (void*)d_span, (void*)d_top, tmp})
+    if (q) (void)hipFreeAsync(q, s);
   mbx_plan_destroy(p8);
   if (st != MBX_OK) {
     (void)hipGetLastError();
@@ -227,7 +224,8 @@ int mbx_plan_create_auto(const int32_t* d_row_ptr, const int32_t* d_col, int32_t
   *out = nullptr;
   // the column counts are made once, for the rule and the hot split
   int32_t* cnt = nullptr;
-  if (nnz > 0 && ncol > 0 && hipMalloc((void**)&cnt, 4 * (size_t)ncol) != hipSuccess) {
+  hipStream_t s = (hipStream_t)stream;
+  if (nnz > 0 && ncol > 0 && hipMallocAsync((void**)&cnt, 4 * (size_t)ncol, s) != hipSuccess) {
     (void)hipGetLastError();
     return MBX_ENOMEM;
   }
@@ -237,7 +235,7 @@ int mbx_plan_create_auto(const int32_t* d_row_ptr, const int32_t* d_col, int32_t
   if (st == MBX_OK)
     st = mbx_plan_create(d_row_ptr, nrow, ncol, nnz, c.items_per_thread, 256, stream, &p);
   if (st != MBX_OK) {
-    if (cnt) (void)hipFree(cnt);
+    if (cnt) (void)hipFreeAsync(cnt, s);
     return st;
   }
   (void)mbx_plan_set_stream(p, c.stream_mode);
@@ -253,7 +251,7 @@ int mbx_plan_create_auto(const int32_t* d_row_ptr, const int32_t* d_col, int32_t
   if (st == MBX_OK && c.hot_columns > 0)
     st = mbx_detail::split_hot_counted(p, d_col, c.hot_columns, cnt, (hipStream_t)stream,
                                        nullptr);
-  if (cnt) (void)hipFree(cnt);
+  if (cnt) (void)hipFreeAsync(cnt, s);
   if (st != MBX_OK) {
     mbx_plan_destroy(p);
     return st;
