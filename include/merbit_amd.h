@@ -142,9 +142,10 @@ int mbx_plan_split_hot(mbx_plan* plan, const int32_t* d_col, int32_t max_hot, vo
 
 /* Optional timing hook: when ev_begin and ev_end (hipEvent_t) are both
  * non-NULL, the next spmv or spmm call on this plan attaches them to its tile
- * kernel's dispatch (hipExtLaunchKernel), then clears them: their interval
- * is that kernel's own execution, as a kernel trace reports it.  For eager
- * calls; do not set them on a plan whose next call is captured into a graph. */
+ * kernel's dispatch (hipExtLaunchKernel): their interval is that kernel's
+ * own execution, as a kernel trace reports it.  The next call clears the
+ * plan's events whether or not both were set.  For eager calls; do not set
+ * them on a plan whose next call is captured into a graph. */
 int mbx_plan_set_events(mbx_plan* plan, void* ev_begin, void* ev_end);
 
 /* Replaces merbit_kernel<double,BLOCK,SIGMA>, include/tile.cuh:476-561, and the

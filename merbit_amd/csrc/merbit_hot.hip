@@ -109,7 +109,7 @@ int count_columns(const int32_t* d_col, int nnz, int ncol, int32_t* cnt, hipStre
   if (hipMemsetAsync(cnt, 0, 4 * (size_t)ncol, s) != hipSuccess) return MBX_EHIP;
   if (nnz <= 0) return MBX_OK;
   const int end_bit = 32 - __builtin_clz((unsigned)ncol);  // keys 0 .. ncol
-  const int max_runs = nnz < ncol ? nnz : ncol + 1;
+  const int max_runs = (int)(nnz < (long long)ncol + 1 ? nnz : (long long)ncol + 1);
   int32_t *k0 = nullptr, *k1 = nullptr, *uniq = nullptr, *len = nullptr, *nruns = nullptr;
   void* tmp = nullptr;
   size_t sort_bytes = 0, rle_bytes = 0;

@@ -203,17 +203,18 @@ inline int capture_graph(int repeats, F fn, void** exec_out) {
 // events (mbx_plan_set_events), they are attached to this dispatch instead
 // (hipExtLaunchKernel) and then cleared: their interval is the kernel's own
 // execution, the interval a kernel trace reports, without the marker packets
-// and dispatch latency that event records around the launch would add.
+// and dispatch latency that event records around the launch would add.  A
+// plan with only one of the two events set launches plainly; either way the
+// plan's events are cleared.
 template <class... KA, class... A>
 inline void timed_launch(const mbx_plan* p, void (*kernel)(KA...), unsigned grid,
                          unsigned block, size_t lds, hipStream_t s, A... args) {
-  if (p->ev_begin && p->ev_end) {
+  if (p->ev_begin && p->ev_end)
     hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), (uint32_t)lds, s, p->ev_begin,
                           p->ev_end, 0u, args...);
-    p->ev_begin = p->ev_end = nullptr;
-  } else {
+  else
     kernel<<<grid, block, lds, s>>>(args...);
-  }
+  p->ev_begin = p->ev_end = nullptr;
 }
 
 }  // namespace mbx_detail
