@@ -72,9 +72,11 @@ __global__ void hot_pick_kernel(const int32_t* __restrict__ sorted_col,
   atomicAdd(covered, (unsigned long long)sorted_cnt[h]);
 }
 
-// colh[k] = -1 - slot of a hot column, the column itself otherwise; the
-// entries past nnz up to the next multiple of 4 (read by the 16-byte column
-// groups of the staging) are 0.
+// colh[k] = -1 - slot of a hot column, the column itself otherwise; a
+// negative (invalid) column becomes ncol, which the bounded gathers read as
+// 0 as the plain product reads it, rather than a hot slot; the entries past
+// nnz up to the next multiple of 4 (read by the 16-byte column groups of the
+// staging) are 0.
 __global__ void colh_kernel(const int32_t* __restrict__ col, int nnz, int padded, int ncol,
                             const int32_t* __restrict__ slot, int32_t* __restrict__ colh) {
   const int k = blockIdx.x * kBlock + threadIdx.x;
@@ -85,7 +87,7 @@ __global__ void colh_kernel(const int32_t* __restrict__ col, int nnz, int padded
   }
   const int c = col[k];
   const int s = (c >= 0 && c < ncol) ? slot[c] : -1;
-  colh[k] = s >= 0 ? -1 - s : c;
+  colh[k] = s >= 0 ? -1 - s : (c >= 0 ? c : ncol);
 }
 
 void drop_hot(mbx_plan* p) {

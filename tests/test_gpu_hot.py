@@ -103,6 +103,43 @@ def test_hot_split_coverage_and_fallback(device):
     plan.close()
 
 
+@pytest.mark.parametrize("kind", ["f64", "f32"])
+def test_hot_split_out_of_range_columns_read_zero(device, kind):
+    """Column indices outside [0, ncol), negative ones included, read x as 0
+    in the plain product (bounded gathers); a hot-column plan must do the
+    same (a negative index must not be taken for a hot slot), so both
+    products are bit-identical and match a host product that drops those
+    entries."""
+    from merbit_amd.spmv import MerbitPlan
+    rng = np.random.default_rng(17)
+    n = 20000
+    rp, col = random_csr(rng, n, n, lens=(rng.pareto(1.1, n) * 5).astype(int))
+    nnz = int(rp[-1])
+    bad = col.astype(np.int64)
+    picks = rng.choice(nnz, 64, replace=False)
+    bad[picks[:32]] = -1 - rng.integers(0, 5000, 32)       # negative: would alias hot slots
+    bad[picks[32:]] = n + rng.integers(0, 1 << 20, 32)      # past the end
+    bad = bad.astype(np.int32)
+    v, x = values(rng, kind, nnz, n)
+    d_rp, d_bad, d_val, d_x = dev(rp, device), dev(bad, device), dev(v, device), dev(x, device)
+    plan = MerbitPlan(d_rp, n, nnz, items_per_thread=12)
+    ref = plan.spmv(d_rp, d_bad, d_val, d_x)
+    say(f"out-of-range columns {kind}")
+    plan.split_hot(d_bad, 4096)
+    y = plan.spmv(d_rp, d_bad, d_val, d_x)
+    assert np.array_equal(bits(y), bits(ref))
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    ok = (bad >= 0) & (bad < n)
+    vv = v.astype(np.float64) if kind == "f64" else v.astype(np.float32).astype(np.float64)
+    host = np.zeros(n)
+    np.add.at(host, rows[ok], vv[ok] * np.asarray(x, np.float64)[bad[ok]])
+    absrow = np.zeros(n)
+    np.add.at(absrow, rows[ok], np.abs(vv[ok] * np.asarray(x, np.float64)[bad[ok]]))
+    tol = 1e-12 if kind == "f64" else 1e-6
+    assert np.all(np.abs(y.double().cpu().numpy() - host) <= tol * absrow + 1e-300)
+    plan.close()
+
+
 @pytest.mark.parametrize("nnz,ncol", [(4095, 50000), (4096, 50000), (4097, 50000),
                                       (3 * 4096 - 1, 7), (3 * 4096 + 1, 1), (40000, 200000)])
 def test_hot_split_counts_at_block_edges(device, nnz, ncol):
