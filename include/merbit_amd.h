@@ -460,6 +460,16 @@ int mbx_adjoint_dx(const mbx_adjoint* adj, int32_t value_kind, const void* d_val
  * A^T.  Synchronises `stream` (and the device when it replaces a blocked
  * form). */
 int mbx_adjoint_set_blocks(mbx_adjoint* a, int32_t nblocks, void* stream);
+/* The launch rule (mbx_launch_rule) for the transposed pattern, for values
+ * of value_kind: the plan of A^T is rebuilt with the rule's shape and
+ * options (mbx_plan_create_auto on col_ptr / row_idx: items per thread,
+ * tiles per CU, 16-bit columns, FP64 sums, the hot-column split of A^T's
+ * most frequent columns, i.e. A's longest rows), and dx goes through the
+ * column-blocked A^T when the rule finds spread columns (its
+ * column_blocks), through the single product otherwise.  *choice (may be
+ * NULL) receives the rule's choice.  Synchronises `stream` (and the device
+ * when it replaces a blocked form). */
+int mbx_adjoint_auto(mbx_adjoint* a, int32_t value_kind, void* stream, mbx_launch_choice* choice);
 
 int mbx_sddmm(const mbx_plan* plan, int32_t value_kind, const int32_t* d_row_ptr,
               const int32_t* d_col, const void* d_g, const void* d_x, void* d_dval,

@@ -24,7 +24,7 @@ ABI_SYMBOLS = (
     "mbx_batch_graph_create", "mbx_spmv_graph_create", "mbx_graph_launch", "mbx_graph_destroy",
     "mbx_autotune_spmv", "mbx_autotune_spmv_launch", "mbx_autotune_spmm", "mbx_autotune_batch",
     "mbx_adjoint_create", "mbx_adjoint_destroy", "mbx_adjoint_transpose", "mbx_adjoint_dx",
-    "mbx_adjoint_set_blocks",
+    "mbx_adjoint_set_blocks", "mbx_adjoint_auto",
     "mbx_sddmm", "mbx_ingest_ref", "mbx_plan_stats",
     "mbx_pagerank_workspace_bytes", "mbx_pagerank_f64", "mbx_pagerank_graph_create",
     "mbx_colblock_create", "mbx_colblock_destroy", "mbx_colblock_info", "mbx_colblock_values",
@@ -180,6 +180,7 @@ def lib() -> ctypes.CDLL:
                                             ctypes.c_size_t, ctypes.POINTER(vp)]
     L.mbx_csr_validate.argtypes = [vp, vp, i32, i32, i32, vp, ctypes.POINTER(CsrReport)]
     L.mbx_launch_rule.argtypes = [vp, vp, i32, i32, i32, i32, vp, ctypes.POINTER(LaunchChoice)]
+    L.mbx_adjoint_auto.argtypes = [vp, i32, vp, ctypes.POINTER(LaunchChoice)]
     L.mbx_plan_create_auto.argtypes = [vp, vp, i32, i32, i32, i32, vp, ctypes.POINTER(vp),
                                        ctypes.POINTER(LaunchChoice)]
     L.mbx_plan_split_hot.argtypes = [vp, vp, i32, vp, ctypes.POINTER(f64)]

@@ -38,18 +38,26 @@ class MerbitOperator:
         more than the blocked SDDMM saves.
         None (the defaults): the launch rule (mbx_launch_rule, for values of
         value_dtype) picks the forward plan, and both blocked forms when it
-        finds spread columns over an x larger than L2 (random graphs)."""
+        finds spread columns over an x larger than L2 (random graphs).
+        Otherwise, with items_per_thread left to the rule as well, the rule
+        run on the transposed pattern (mbx_adjoint_auto) picks the plan of
+        A^T: its hot-column split when A has a few long rows, the blocked
+        A^T for spread rows; ``adjoint_choice`` holds that choice once the
+        adjoint exists.  A given items_per_thread is kept for A^T too."""
         spread = False
-        if blocks is None or adjoint_blocks is None:
+        if blocks is None or (adjoint_blocks is None and items_per_thread is not None):
             from .spmv import launch_rule
             spread = launch_rule(row_ptr, col, ncol, value_dtype)["rule"] == "spread_columns"
         if blocks is None:
             blocks = -1 if spread else 0
-        if adjoint_blocks is None:
+        if adjoint_blocks is None and (blocks or items_per_thread is not None):
             adjoint_blocks = -1 if (blocks or spread) else 0
         if blocks and not adjoint_blocks:
             adjoint_blocks = -1
-        self.adjoint_blocks = adjoint_blocks
+        self.adjoint_blocks = adjoint_blocks  # None: the rule on A^T (mbx_adjoint_auto)
+        self.adjoint_choice = None
+        from .spmv import _KIND_OF_DTYPE
+        self._kind = _KIND_OF_DTYPE[value_dtype]
         self.row_ptr = row_ptr
         self.col = col
         self.nrow = row_ptr.numel() - 1
@@ -77,7 +85,14 @@ class MerbitOperator:
                                            _stream_handle(None), ctypes.byref(h)),
                   "mbx_adjoint_create")
             self._adj = h
-            if self.adjoint_blocks:
+            if self.adjoint_blocks is None:
+                from ._lib import LaunchChoice
+                from .spmv import _choice_dict
+                c = LaunchChoice()
+                check(lib().mbx_adjoint_auto(h, self._kind, _stream_handle(None),
+                                             ctypes.byref(c)), "mbx_adjoint_auto")
+                self.adjoint_choice = _choice_dict(c)
+            elif self.adjoint_blocks:
                 check(lib().mbx_adjoint_set_blocks(h, self.adjoint_blocks, _stream_handle(None)),
                       "mbx_adjoint_set_blocks")
         return self._adj

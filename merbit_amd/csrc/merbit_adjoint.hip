@@ -588,6 +588,25 @@ int mbx_adjoint_set_blocks(mbx_adjoint* a, int32_t nblocks, void* stream) {
   return st;
 }
 
+int mbx_adjoint_auto(mbx_adjoint* a, int32_t value_kind, void* stream, mbx_launch_choice* out) {
+  if (!a) return MBX_EINVAL;
+  mbx_launch_choice c = {};
+  mbx_plan* p = nullptr;
+  // A^T is an ncol x nrow CSR: row pointer col_ptr, column indices row_idx
+  int st = mbx_plan_create_auto(a->col_ptr, a->row_idx, a->ncol, a->nrow, a->nnz, value_kind,
+                                stream, &p, &c);
+  if (st != MBX_OK) return st;
+  if (a->tb) (void)hipDeviceSynchronize();  // a dx on any stream may still use the old plan
+  mbx_plan_destroy(a->plan_t);
+  a->plan_t = p;
+  // the rule's column-blocked form for spread columns; the single product
+  // on A^T otherwise
+  st = mbx_adjoint_set_blocks(a, c.rule == MBX_RULE_SPREAD_COLUMNS ? c.column_blocks : 0,
+                              stream);
+  if (st == MBX_OK && out) *out = c;
+  return st;
+}
+
 int mbx_adjoint_transpose(const mbx_adjoint* a, int32_t* h_col_ptr, int32_t* h_row_idx,
                           int32_t* h_perm) {
   if (!a) return MBX_EINVAL;

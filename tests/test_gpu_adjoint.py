@@ -192,3 +192,72 @@ def test_adjoint_permutation_boundaries(orc, device, dtype):
         assert torch.equal(dx_u.view(torch.uint8), want.view(torch.uint8)), nnz
         plan_t.close()
         op.close()
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_operator_default_adjoint_rule_rmat(orc, device, dtype):
+    """A default MerbitOperator runs the launch rule on the transposed
+    pattern (mbx_adjoint_auto).  The R-MAT's rows are skewed like its
+    columns, so A^T gets the hot-column split; dx is bit-identical to the
+    plain single-product adjoint of the same shape (8 items per thread) and
+    matches the oracle's accurate A^T product."""
+    from merbit_amd import workloads
+    from merbit_amd.autograd import MerbitOperator
+    A, _ = workloads.rmat()
+    say(f"default operator adjoint rmat {dtype}")
+    d_rp, d_col = dev(A.row_ptr, device), dev(A.col, device)
+    rng = np.random.default_rng(31)
+    g = rng.uniform(-1, 1, A.nrow)
+    vals = A.val if dtype == torch.float64 else A.val.astype(np.float32)
+    d_val, d_g = dev(vals, device), dev(g, device).to(dtype)
+    op = MerbitOperator(d_rp, d_col, A.ncol, value_dtype=dtype)
+    assert op.adjoint_blocks is None
+    dx = op.adjoint_x(d_val, d_g)
+    c = op.adjoint_choice
+    assert c["rule"] == "hot_columns" and c["hot_columns"] > 0, c
+    ref_op = MerbitOperator(d_rp, d_col, A.ncol, items_per_thread=8, blocks=0,
+                            adjoint_blocks=0, value_dtype=dtype)
+    assert torch.equal(dx.view(torch.uint8), ref_op.adjoint_x(d_val, d_g).view(torch.uint8))
+    cp, ri, pm = orc.csr_transpose(A.row_ptr, A.col, A.ncol)
+    gh = d_g.double().cpu().numpy()
+    da, dabs = orc.spmv_accurate(cp, ri, np.ascontiguousarray(vals[pm]).astype(np.float64), gh)
+    tol = 1e-12 if dtype == torch.float64 else 1e-6
+    assert cond_err(dx.double().cpu().numpy(), da, dabs) <= tol
+    ref_op.close()
+    op.close()
+
+
+@pytest.mark.slow
+def test_operator_default_adjoint_rule_banded(orc, device):
+    """A banded pattern's transpose is banded: the rule on A^T picks the
+    local form (16-bit columns, FP64 sums for FP32 values); dx matches the
+    oracle and the plain adjoint within the FP32 bar."""
+    from merbit_amd.autograd import MerbitOperator
+    n, offs = 1_200_000, np.arange(-3, 4)
+    say("default operator adjoint banded")
+    rows = np.repeat(np.arange(n), len(offs))
+    cols = rows + np.tile(offs, n)
+    keep = (cols >= 0) & (cols < n)  # a clipped band: every tile spans few columns
+    rows, cols = rows[keep], cols[keep].astype(np.int32)
+    rp = np.zeros(n + 1, np.int32)
+    rp[1:] = np.cumsum(np.bincount(rows, minlength=n))
+    rng = np.random.default_rng(32)
+    vals = rng.uniform(-1, 1, cols.size).astype(np.float32)
+    g = rng.uniform(-1, 1, n).astype(np.float32)
+    d_rp, d_col, d_val, d_g = dev(rp, device), dev(cols, device), dev(vals, device), \
+        dev(g, device)
+    op = MerbitOperator(d_rp, d_col, n, value_dtype=torch.float32)
+    dx = op.adjoint_x(d_val, d_g)
+    c = op.adjoint_choice
+    assert c["rule"] == "local" and c["compress_cols"] == 1 and c["accumulation"] == 1, c
+    cp, ri, pm = orc.csr_transpose(rp, cols, n)
+    da, dabs = orc.spmv_accurate(cp, ri, np.ascontiguousarray(vals[pm]).astype(np.float64),
+                                 g.astype(np.float64))
+    assert cond_err(dx.double().cpu().numpy(), da, dabs) <= 1e-6
+    ref_op = MerbitOperator(d_rp, d_col, n, items_per_thread=8, blocks=0, adjoint_blocks=0,
+                            value_dtype=torch.float32)
+    ref = ref_op.adjoint_x(d_val, d_g).double()
+    assert float(torch.linalg.norm(dx.double() - ref) / torch.linalg.norm(ref)) <= 1e-6
+    ref_op.close()
+    op.close()
