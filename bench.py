@@ -27,7 +27,8 @@ Laplacian, mixed precision (FP32 / BF16 values) on the R-MAT and on a
 banded 4M-row matrix (tuned, with FP64 sums and 16-bit columns, and the
 launch rule's default), multi-RHS SpMM (k = 8, 32), 64 uneven matrices in
 one persistent graph launch (autotuned), the adjoint on a 2.4M-node graph
-(CSR, column-blocked and the operator's defaults), PageRank iterations on the
+(CSR, column-blocked and the operator's defaults) and on the R-MAT (the
+operator's defaults against a pinned shape), PageRank iterations on the
 R-MAT edge list after device ingest, rocSPARSE's CSR and COO SpMV on the
 headline matrix (measured comparators), and a plain device copy as the
 measured bandwidth ceiling.  Every rate uses the same algorithmic-bytes
@@ -714,6 +715,60 @@ def bench_adjoint(dev, args, checks):
     return out
 
 
+def bench_adjoint_rmat(A, x, dev, args, checks):
+    """The differentiable product on the headline R-MAT matrix (one copy, as
+    a training step reuses its matrix): forward y = A x, dx = A^T g and
+    dval = g[row] x[col] for the operator's defaults (the launch rule on A
+    and, through mbx_adjoint_auto, on A^T: hot-column splits of both) and
+    for a pinned operator (8 items per thread, single products, no split),
+    FP64 and FP32 values."""
+    import torch
+    from merbit_amd.autograd import MerbitOperator
+    from merbit_amd.gen import Csr
+    rp = torch.from_numpy(A.row_ptr).to(dev)
+    col = torch.from_numpy(A.col).to(dev)
+    g = np.random.default_rng(41).uniform(-1.0, 1.0, A.nrow)
+    steps = max(20, args.steps // 4)
+    out = {}
+    for name, dt in (("f64", torch.float64), ("f32", torch.float32)):
+        vals = A.val if name == "f64" else A.val.astype(np.float32)
+        gh = g if name == "f64" else g.astype(np.float32)
+        xh = x if name == "f64" else x.astype(np.float32)
+        val = torch.from_numpy(vals).to(dev)
+        xx, gg = torch.from_numpy(xh).to(dev), torch.from_numpy(gh).to(dev)
+        ops = {"defaults": MerbitOperator(rp, col, A.ncol, value_dtype=dt),
+               "pinned": MerbitOperator(rp, col, A.ncol, items_per_thread=8, blocks=0,
+                                        adjoint_blocks=0, value_dtype=dt)}
+        res = {}
+        for key, op in ops.items():
+            op.adjoint_handle  # the transposed pattern, built once
+            torch.cuda.synchronize()
+            f_ms, _ = time_steps([lambda: op.forward(val, xx)], steps, args.warmup)
+            x_ms, _ = time_steps([lambda: op.adjoint_x(val, gg)], steps, args.warmup)
+            v_ms, _ = time_steps([lambda: op.adjoint_val(gg, xx)], steps, args.warmup)
+            res[key] = {"forward_us": round(f_ms * 1e3, 2), "dx_us": round(x_ms * 1e3, 2),
+                        "dval_us": round(v_ms * 1e3, 2),
+                        "step_us": round((f_ms + x_ms + v_ms) * 1e3, 2)}
+        d = ops["defaults"]
+        res["defaults"]["forward_rule"] = {k: d.plan.choice[k] for k in ("rule", "hot_columns")}
+        res["defaults"]["adjoint_rule"] = {k: d.adjoint_choice[k]
+                                           for k in ("rule", "hot_columns")}
+        res["step_speedup"] = round(res["pinned"]["step_us"] / res["defaults"]["step_us"], 3)
+        res["dx_speedup"] = round(res["pinned"]["dx_us"] / res["defaults"]["dx_us"], 3)
+        cp, ri, pm = d.transpose()  # A^T from the device's own transpose
+        At = Csr(cp, ri, np.ascontiguousarray(vals[pm]), A.nrow)
+        checks.append((f"rmat_dx_{name}", At, At.val, gh, d.adjoint_x(val, gg).cpu().numpy(),
+                       1e-12 if name == "f64" else 1e-6))
+        for op in ops.values():
+            op.close()
+        out[name] = res
+        log(f"rmat adjoint {name}: defaults fwd {res['defaults']['forward_us']}, dx "
+            f"{res['defaults']['dx_us']}, dval {res['defaults']['dval_us']} us; pinned "
+            f"dx {res['pinned']['dx_us']} us (dx x{res['dx_speedup']}, step "
+            f"x{res['step_speedup']})")
+    return out
+
+
 def bench_pagerank(A, dev, args):
     """The reference's pipeline on the R-MAT edge list: device ingest (CSR of
     P^T, values 1/out-degree), then 20 damped PageRank iterations captured
@@ -973,6 +1028,7 @@ def main():
         extras["batch_uneven"] = bench_batch(dev, args)
     if args.only in ("all", "adjoint"):
         extras["adjoint_graph"] = bench_adjoint(dev, args, checks)
+        extras["adjoint_rmat"] = bench_adjoint_rmat(A, x, dev, args, checks)
     if args.only in ("all", "pagerank"):
         extras["pagerank_rmat"] = bench_pagerank(A, dev, args)
     if args.only == "all" or args.compare:
