@@ -68,6 +68,7 @@ def test_host_side_argument_checks():
     assert L.mbx_plan_compress_cols(None, None, None) == -1
     assert L.mbx_plan_set_accumulation(None, 0) == -1
     assert L.mbx_adjoint_set_blocks(None, 0, None) == -1
+    assert L.mbx_adjoint_auto(None, 0, None, None) == -1
     assert L.mbx_colblock_set_stream(None, 0) == -1
     assert L.mbx_colblock_set_residency(None, 0) == -1
     assert L.mbx_colblock_adjoint_prepare(None, None, None, 0, 8, None, None) == -1
