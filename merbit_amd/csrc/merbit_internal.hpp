@@ -112,6 +112,35 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
   return j * q + min(j, r) + i;
 }
 
+// The tiles of persistent workgroup b of nwg (a grid sized to stay resident):
+// each XCD takes a contiguous range of the tiles, its workgroups every
+// stride-th tile of it: first + w, first + w + stride, ... below first + len.
+struct PersistentShare {
+  int first, len, w, stride;
+};
+__device__ __forceinline__ PersistentShare persistent_share(int b, int nwg, int ntiles) {
+  PersistentShare s{0, ntiles, b, nwg};
+  if (nwg >= kXcds) {
+    const int xcd = b % kXcds;
+    s.w = b / kXcds;
+    s.stride = (nwg - xcd + kXcds - 1) / kXcds;
+    const int q = ntiles / kXcds, r = ntiles % kXcds;
+    s.first = xcd * q + min(xcd, r);
+    s.len = q + (xcd < r ? 1 : 0);
+  }
+  return s;
+}
+
+// x values of the hot columns for one product of a hot-column plan:
+// xh[h] = x[hot[h]] (mbx_plan_split_hot).
+template <class X>
+__global__ __launch_bounds__(256) void hot_gather_kernel(const X* __restrict__ x,
+                                                         const int32_t* __restrict__ hot,
+                                                         int nhot, X* __restrict__ xh) {
+  const int h = blockIdx.x * 256 + threadIdx.x;
+  if (h < nhot) xh[h] = x[hot[h]];
+}
+
 // Merge-path search in LDS (the SpMM and SDDMM tiles): first row r in
 // [max(0,d-nnz_t), min(d,rows_t)] whose local end s_re[r] exceeds d-r-1 (an
 // nnz index is consumed before an equal row end), include/utils.cuh:5-29.

@@ -131,15 +131,6 @@ __global__ __launch_bounds__(WG) void tile_spmv_kernel(
                                          val, x, y, coord, carry, alpha, beta, beta_zero);
 }
 
-// x values of the hot columns for one product: xh[h] = x[hot[h]].
-template <class X>
-__global__ __launch_bounds__(256) void hot_gather_kernel(const X* __restrict__ x,
-                                                         const int32_t* __restrict__ hot,
-                                                         int nhot, X* __restrict__ xh) {
-  const int h = blockIdx.x * 256 + threadIdx.x;
-  if (h < nhot) xh[h] = x[hot[h]];
-}
-
 // Persistent tile kernel of a hot-column plan (P = HotX<...>): each
 // workgroup copies the hot x values into LDS once (16-byte loads), then runs
 // the tiles of its share: each XCD takes a contiguous range of tiles, its
@@ -160,20 +151,11 @@ __global__ __launch_bounds__(WG) void tile_spmv_hot_kernel(
     reinterpret_cast<int4*>(s_xh)[i] = reinterpret_cast<const int4*>(xh)[i];
   sm.xh = s_xh;
   __syncthreads();
-  constexpr int XC = mbx_detail::kXcds;
-  const int b = blockIdx.x, nwg = gridDim.x;
-  int first = 0, len = ntiles, w = b, stride = nwg;
-  if (nwg >= XC) {
-    const int xcd = b % XC;
-    w = b / XC;
-    stride = (nwg - xcd + XC - 1) / XC;
-    const int q = ntiles / XC, r = ntiles % XC;
-    first = xcd * q + min(xcd, r);
-    len = q + (xcd < r ? 1 : 0);
-  }
-  for (int i = w; i < len; i += stride) {
-    mbx_detail::spmv_tile<WG, IPT, true, P>(sm, first + i, ncol, row_ptr, colh, val, x, y, coord,
-                                            carry, alpha, beta, beta_zero);
+  const mbx_detail::PersistentShare sh = mbx_detail::persistent_share(blockIdx.x, gridDim.x,
+                                                                      ntiles);
+  for (int i = sh.w; i < sh.len; i += sh.stride) {
+    mbx_detail::spmv_tile<WG, IPT, true, P>(sm, sh.first + i, ncol, row_ptr, colh, val, x, y,
+                                            coord, carry, alpha, beta, beta_zero);
     __syncthreads();  // the next tile reuses the LDS
   }
 }
@@ -231,7 +213,7 @@ bool launch_hot(const mbx_plan* p, hipStream_t s, const int32_t* rp,
   if (p->wg_per_cu > 0 && per_cu > p->wg_per_cu) per_cu = p->wg_per_cu;
   const long long g = (long long)p->cus * per_cu;
   const int grid = (int)(g < p->ntiles ? g : p->ntiles);
-  hot_gather_kernel<X><<<(p->nhot + 255) / 256, 256, 0, s>>>(x, p->d_hot, p->nhot,
+  mbx_detail::hot_gather_kernel<X><<<(p->nhot + 255) / 256, 256, 0, s>>>(x, p->d_hot, p->nhot,
                                                                static_cast<X*>(p->d_xh));
   timed_launch(p, kernel, grid, 256, lds, s, rp, p->d_colh, val, x, y, p->d_coord, p->d_carry,
                p->ntiles, p->ncol, alpha, beta, bz, static_cast<const X*>(p->d_xh), p->nhot);

@@ -272,3 +272,43 @@ def test_hot_split_rmat_full(orc, device, kind):
     assert cond_err(y.cpu().numpy().astype(np.float64), ya, absr) <= \
         (1e-12 if kind == "f64" else 1e-6)
     plan.close()
+
+
+@pytest.mark.parametrize("kind", ["f64", "f32"])
+@pytest.mark.parametrize("ipt", [8, 12])
+def test_sddmm_on_hot_split_plan_bit_identical(device, kind, ipt):
+    """mbx_sddmm on a plan with a hot-column split reads the plan's column
+    copy and the hot x values from LDS (a persistent kernel); dval must be
+    bit-identical to the same plan's SDDMM without the split, also with
+    column indices outside [0, ncol) (read as 0 by both)."""
+    from merbit_amd._lib import lib
+    from merbit_amd.spmv import MerbitPlan
+    rng = np.random.default_rng(51 + ipt)
+    T = 256 * ipt
+    cases = edge_case_patterns(T) + [
+        ("pareto", (rng.pareto(1.05, 40000) * 4).astype(int), 30000)]
+    dt = torch.float64 if kind == "f64" else torch.float32
+    code = 0 if kind == "f64" else 1
+    s = torch.cuda.current_stream().cuda_stream
+    for name, lens, ncol in cases:
+        rp, col = random_csr(rng, len(lens), ncol, lens=lens)
+        nnz = int(rp[-1])
+        if nnz > 8:
+            col[3] = -2
+            col[nnz - 2] = ncol + 5
+        d_rp, d_col = dev(rp, device), dev(col, device)
+        g = torch.from_numpy(rng.uniform(-1, 1, len(lens))).to(device, dt)
+        x = torch.from_numpy(rng.uniform(-1, 1, ncol)).to(device, dt)
+        plan = MerbitPlan(d_rp, ncol, nnz, items_per_thread=ipt)
+
+        def dval():
+            out = torch.zeros(max(nnz, 1), dtype=dt, device=device)
+            assert lib().mbx_sddmm(plan.handle, code, d_rp.data_ptr(), d_col.data_ptr(),
+                                   g.data_ptr(), x.data_ptr(), out.data_ptr(), s) == 0
+            return out
+        ref = dval()
+        for nhot in (1, 64, 2048):
+            say(f"sddmm hot {kind} ipt {ipt} {name} nhot {nhot}")
+            plan.split_hot(d_col, nhot)
+            assert torch.equal(dval().view(torch.uint8), ref.view(torch.uint8)), (name, nhot)
+        plan.close()
