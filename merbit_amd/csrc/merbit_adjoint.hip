@@ -208,16 +208,18 @@ __global__ void perm_final_kernel(const int32_t* __restrict__ order1,
 // dval[k] = g[row(k)] * x[col(k)] for the nonzeros of one forward tile.  x
 // is gathered through a buffer resource over its ncol entries when it spans
 // at most 4 GiB (x_rsrc), so a column index outside [0, ncol) reads 0, as in
-// the forward tile kernel, instead of faulting.  HOT: col is a hot-column
-// plan's column copy (mbx_plan_split_hot), whose entry -1 - h takes x from
-// s_xh[h] in LDS (requires x_rsrc); the products are the same.
-template <int WG, int IPT, class T, bool HOT>
+// the forward tile kernel, instead of faulting.  RSRC (x within the buffer
+// resource's 4 GiB) is a template parameter: with both gather paths in one
+// kernel the FP64 tile at 8 items per thread took 118 VGPRs (4 waves per
+// SIMD; 252 at 12 items), with one path 41 (62), and the SDDMM 13 % less
+// time on the R-MAT.
+template <int WG, int IPT, class T, bool RSRC>
 __device__ __forceinline__ void sddmm_tile(int tile, int* s_re, int* s_row,
                                            const int32_t* __restrict__ row_ptr,
                                            const int32_t* __restrict__ col,
                                            const T* __restrict__ g, const T* __restrict__ x,
                                            T* __restrict__ dval, const int4* __restrict__ coord,
-                                           int ncol, int x_rsrc, const T* s_xh) {
+                                           int ncol) {
   const int tid = threadIdx.x;
   const int4 c0 = coord[tile];
   const int4 c1 = coord[tile + 1];
@@ -266,7 +268,7 @@ __device__ __forceinline__ void sddmm_tile(int tile, int* s_re, int* s_row,
   }
   __syncthreads();
   T xv[IPT], gv[IPT];
-  if (HOT || x_rsrc) {  // uniform
+  if constexpr (RSRC) {
     const __amdgpu_buffer_rsrc_t rx =
         mbx_detail::buffer_rsrc(x, (uint32_t)ncol * (uint32_t)sizeof(T));
 #pragma unroll
@@ -275,10 +277,7 @@ __device__ __forceinline__ void sddmm_tile(int tile, int* s_re, int* s_row,
       xv[j] = T(0);
       gv[j] = T(0);
       if (i < nnz_t) {
-        if (HOT && ci[j] < 0)
-          xv[j] = s_xh[-1 - ci[j]];
-        else
-          mbx_detail::buffer_gather(rx, ci[j], xv[j]);
+        mbx_detail::buffer_gather(rx, ci[j], xv[j]);
         gv[j] = g[r0 + s_row[i]];
       }
     }
@@ -301,73 +300,29 @@ __device__ __forceinline__ void sddmm_tile(int tile, int* s_re, int* s_row,
   }
 }
 
-template <int WG, int IPT, class T>
+template <int WG, int IPT, class T, bool RSRC>
 __global__ __launch_bounds__(WG) void sddmm_tile_kernel(
     const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
     const T* __restrict__ g, const T* __restrict__ x, T* __restrict__ dval,
-    const int4* __restrict__ coord, int ntiles, int ncol, int x_rsrc) {
+    const int4* __restrict__ coord, int ntiles, int ncol) {
   constexpr int TI = WG * IPT;
   __shared__ int s_re[TI + 1];
   __shared__ int s_row[TI];
-  sddmm_tile<WG, IPT, T, false>(xcd_tile(blockIdx.x, ntiles), s_re, s_row, row_ptr, col, g, x,
-                                dval, coord, ncol, x_rsrc, nullptr);
+  sddmm_tile<WG, IPT, T, RSRC>(xcd_tile(blockIdx.x, ntiles), s_re, s_row, row_ptr, col, g, x,
+                               dval, coord, ncol);
 }
 
-// SDDMM of a hot-column plan: a persistent workgroup copies the hot x values
-// (xh, nhot of them) into LDS once and runs the tiles of its share
-// (persistent_share), reading the plan's column copy colh.
-template <int WG, int IPT, class T>
-struct SddmmHotLds {
-  static constexpr size_t kIdx = (sizeof(int) * (2 * WG * IPT + 1) + 15) / 16 * 16;
-  static size_t bytes(int nhot) { return kIdx + ((size_t)nhot * sizeof(T) + 15) / 16 * 16; }
-};
-
-template <int WG, int IPT, class T>
-__global__ __launch_bounds__(WG) void sddmm_hot_kernel(
-    const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ colh,
-    const T* __restrict__ g, const T* __restrict__ x, T* __restrict__ dval,
-    const int4* __restrict__ coord, int ntiles, int ncol, const T* __restrict__ xh, int nhot) {
-  constexpr int TI = WG * IPT;
-  extern __shared__ __align__(16) char smem[];
-  int* s_re = reinterpret_cast<int*>(smem);
-  int* s_row = s_re + TI + 1;
-  T* s_xh = reinterpret_cast<T*>(smem + SddmmHotLds<WG, IPT, T>::kIdx);
-  const int nv = (nhot * (int)sizeof(T) + 15) / 16;
-  for (int i = threadIdx.x; i < nv; i += WG)
-    reinterpret_cast<int4*>(s_xh)[i] = reinterpret_cast<const int4*>(xh)[i];
-  __syncthreads();
-  const mbx_detail::PersistentShare sh = mbx_detail::persistent_share(blockIdx.x, gridDim.x,
-                                                                      ntiles);
-  for (int i = sh.w; i < sh.len; i += sh.stride) {
-    sddmm_tile<WG, IPT, T, true>(sh.first + i, s_re, s_row, row_ptr, colh, g, x, dval, coord,
-                                 ncol, 1, s_xh);
-    __syncthreads();  // the next tile reuses s_re and s_row
-  }
-}
-
-// The SDDMM of a hot-column plan: the hot gather, then a persistent grid of
-// as many workgroups as fit with the hot values in LDS.  false: it does not
-// fit (the caller runs the plain kernel on the caller's columns).
+// The plain SDDMM, one workgroup per tile, on the bounded or the direct
+// gather path.
 template <int IPT, class T>
-bool launch_sddmm_hot(const mbx_plan* p, const int32_t* rp, const T* g, const T* x, T* dval,
-                      hipStream_t s) {
-  const size_t lds = SddmmHotLds<256, IPT, T>::bytes(p->nhot);
-  if (lds > mbx_detail::kLdsPerCu || p->cus <= 0) return false;
-  auto* kernel = sddmm_hot_kernel<256, IPT, T>;
-  mbx_detail::allow_lds(kernel, lds);
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, lds) != hipSuccess ||
-      per_cu < 1) {
-    (void)hipGetLastError();
-    return false;
-  }
-  const long long gsz = (long long)p->cus * per_cu;
-  const int grid = (int)(gsz < p->ntiles ? gsz : p->ntiles);
-  mbx_detail::hot_gather_kernel<T><<<(p->nhot + 255) / 256, 256, 0, s>>>(
-      x, p->d_hot, p->nhot, static_cast<T*>(p->d_xh));
-  kernel<<<grid, 256, lds, s>>>(rp, p->d_colh, g, x, dval, p->d_coord, p->ntiles, p->ncol,
-                                static_cast<const T*>(p->d_xh), p->nhot);
-  return true;
+void launch_sddmm(const mbx_plan* p, bool x_rsrc, const int32_t* rp, const int32_t* col,
+                  const T* g, const T* x, T* dval, hipStream_t s) {
+  if (x_rsrc)
+    sddmm_tile_kernel<256, IPT, T, true><<<p->ntiles, 256, 0, s>>>(rp, col, g, x, dval,
+                                                                   p->d_coord, p->ntiles, p->ncol);
+  else
+    sddmm_tile_kernel<256, IPT, T, false><<<p->ntiles, 256, 0, s>>>(
+        rp, col, g, x, dval, p->d_coord, p->ntiles, p->ncol);
 }
 
 constexpr int kBlock = 256;
@@ -735,33 +690,15 @@ int mbx_sddmm(const mbx_plan* p, int32_t value_kind, const int32_t* d_row_ptr,
   hipStream_t s = (hipStream_t)stream;
   const bool f64 = value_kind == MBX_VALUE_F64;
   const int x_rsrc = (uint64_t)p->ncol * (f64 ? 8u : 4u) <= 0xFFFFFFFFull;
-  // a hot-column plan reads its own column copy (x of its hot columns from
-  // LDS), as its products do
-  if (p->d_colh && p->nhot > 0 && x_rsrc && (p->ipt == 8 || p->ipt == 12)) {
-    bool done;
-    if (f64)
-      done = p->ipt == 8 ? launch_sddmm_hot<8>(p, d_row_ptr, (const double*)d_g,
-                                               (const double*)d_x, (double*)d_dval, s)
-                         : launch_sddmm_hot<12>(p, d_row_ptr, (const double*)d_g,
-                                                (const double*)d_x, (double*)d_dval, s);
-    else
-      done = p->ipt == 8 ? launch_sddmm_hot<8>(p, d_row_ptr, (const float*)d_g,
-                                               (const float*)d_x, (float*)d_dval, s)
-                         : launch_sddmm_hot<12>(p, d_row_ptr, (const float*)d_g,
-                                                (const float*)d_x, (float*)d_dval, s);
-    if (done) return hipGetLastError() == hipSuccess ? MBX_OK : MBX_EHIP;
-  }
   switch (p->ipt) {
 #define MBX_CASE(I)                                                                         \
   case I:                                                                                   \
     if (f64)                                                                                \
-      sddmm_tile_kernel<256, I, double><<<p->ntiles, 256, 0, s>>>(                          \
-          d_row_ptr, d_col, (const double*)d_g, (const double*)d_x, (double*)d_dval,        \
-          p->d_coord, p->ntiles, p->ncol, x_rsrc);                                          \
+      launch_sddmm<I>(p, x_rsrc, d_row_ptr, d_col, (const double*)d_g, (const double*)d_x,  \
+                      (double*)d_dval, s);                                                  \
     else                                                                                    \
-      sddmm_tile_kernel<256, I, float><<<p->ntiles, 256, 0, s>>>(                           \
-          d_row_ptr, d_col, (const float*)d_g, (const float*)d_x, (float*)d_dval,           \
-          p->d_coord, p->ntiles, p->ncol, x_rsrc);                                          \
+      launch_sddmm<I>(p, x_rsrc, d_row_ptr, d_col, (const float*)d_g, (const float*)d_x,    \
+                      (float*)d_dval, s);                                                   \
     break;
     MBX_CASE(2) MBX_CASE(4) MBX_CASE(6) MBX_CASE(8) MBX_CASE(10) MBX_CASE(12) MBX_CASE(14)
     MBX_CASE(16)

@@ -277,10 +277,11 @@ def test_hot_split_rmat_full(orc, device, kind):
 @pytest.mark.parametrize("kind", ["f64", "f32"])
 @pytest.mark.parametrize("ipt", [8, 12])
 def test_sddmm_on_hot_split_plan_bit_identical(device, kind, ipt):
-    """mbx_sddmm on a plan with a hot-column split reads the plan's column
-    copy and the hot x values from LDS (a persistent kernel); dval must be
+    """mbx_sddmm on a plan with a hot-column split gathers x for every entry
+    on the caller's columns, as without the split (an LDS copy of the hot
+    values cost the SDDMM more occupancy than it saved); dval must be
     bit-identical to the same plan's SDDMM without the split, also with
-    column indices outside [0, ncol) (read as 0 by both)."""
+    column indices outside [0, ncol) (read as 0)."""
     from merbit_amd._lib import lib
     from merbit_amd.spmv import MerbitPlan
     rng = np.random.default_rng(51 + ipt)
