@@ -1,0 +1,141 @@
+"""Randomised sweep over the product's options in combination (seeded, so a
+failure names a reproducible case): pattern shape and row-length
+distribution, items per thread, value kind, stream policy, residency cap,
+hot-column split, 16-bit columns, FP64 sums, alpha/beta and value arrays
+that are not 16-byte aligned (the scalar staging path).
+
+Every plain result is checked against the oracle's accurate product (FP64
+1e-12, FP32/BF16 1e-6, condition-scaled), the options that keep the
+arithmetic (stream policy, residency, hot split, 16-bit columns) must give
+the plan's plain product bit for bit, and the rest (FP64 sums, alpha/beta,
+the scalar path) must stay within the bars."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import cond_err, random_csr
+
+pytestmark = pytest.mark.gpu
+
+CASES = 160
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def dev(a, device):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device)
+
+
+def pattern(rng):
+    """(name, row_ptr, col, ncol): one of several row-length families."""
+    fam = int(rng.integers(0, 6))
+    nrow = int(rng.integers(1, 60000))
+    ncol = int(rng.integers(1, 80000))
+    if fam == 0:
+        lens = rng.poisson(rng.uniform(0.5, 12.0), nrow)
+        name = "poisson"
+    elif fam == 1:
+        lens = np.minimum((rng.pareto(1.1, nrow) * rng.uniform(1, 6)).astype(np.int64), 60000)
+        name = "pareto"
+    elif fam == 2:  # mostly empty rows
+        lens = np.where(rng.random(nrow) < 0.05, rng.integers(1, 40, nrow), 0)
+        name = "sparse_rows"
+    elif fam == 3:  # a few giant rows among short ones
+        lens = rng.integers(0, 4, nrow)
+        lens[rng.integers(0, nrow, 3)] = rng.integers(5000, 40000, 3)
+        name = "giant_rows"
+    elif fam == 4:  # banded: local columns
+        width = int(rng.integers(1, 9))
+        ncol = nrow
+        rows = np.repeat(np.arange(nrow), 2 * width + 1)
+        cols = rows + np.tile(np.arange(-width, width + 1), nrow)
+        keep = (cols >= 0) & (cols < ncol)
+        rp = np.zeros(nrow + 1, np.int32)
+        rp[1:] = np.cumsum(np.bincount(rows[keep], minlength=nrow))
+        return "banded", rp, cols[keep].astype(np.int32), ncol
+    else:  # one column holds a large share (power law in the columns)
+        lens = rng.poisson(6.0, nrow)
+        name = "hot_column"
+    rp, col = random_csr(rng, nrow, ncol, lens=lens)
+    if fam == 5 and col.size:
+        hot = rng.random(col.size) < 0.3
+        col[hot] = int(rng.integers(0, ncol))
+        # keep the columns ascending within rows, as the reference's layout
+        rows = np.repeat(np.arange(nrow), np.diff(rp))
+        col = col[np.lexsort((col, rows))]
+    return name, rp, col, ncol
+
+
+def values(rng, kind, nnz, ncol):
+    from merbit_amd import gen
+    v = rng.standard_normal(nnz)
+    x = rng.uniform(-1, 1, ncol)
+    if kind == "f64":
+        return v, x
+    v32 = v.astype(np.float32)
+    return (v32 if kind == "f32" else gen.to_bf16(v32)), x.astype(np.float32)
+
+
+def test_random_option_sweep(orc, device):
+    from merbit_amd.spmv import MerbitPlan
+    rng = np.random.default_rng(2026)
+    for case in range(CASES):
+        kind = ["f64", "f32", "bf16"][int(rng.integers(0, 3))]
+        ipt = int(rng.choice([2, 4, 6, 8, 10, 12, 14, 16]))
+        name, rp, col, ncol = pattern(rng)
+        nrow, nnz = len(rp) - 1, int(rp[-1])
+        vals, x = values(rng, kind, nnz, ncol)
+        tol = 1e-12 if kind == "f64" else 1e-6
+        ydt = torch.float64 if kind == "f64" else torch.float32
+        say(f"sweep {case}: {name} nrow {nrow} ncol {ncol} nnz {nnz} {kind} ipt {ipt}")
+        d_rp, d_col, d_val, d_x = dev(rp, device), dev(col, device), dev(vals, device), \
+            dev(x, device)
+        plan = MerbitPlan(d_rp, ncol, nnz, items_per_thread=ipt)
+        base = plan.spmv(d_rp, d_col, d_val, d_x)
+        ya, absr = orc.spmv_accurate(rp, col, vals, x)
+        assert cond_err(base.double().cpu().numpy(), ya, absr) <= tol, case
+        # options that keep the arithmetic: bit-identical to the plain product
+        mode = ["auto", "cached", "nontemporal"][int(rng.integers(0, 3))]
+        wg = int(rng.integers(0, 5))
+        plan.set_stream(mode)
+        plan.set_residency(wg)
+        opts = [mode, f"wg {wg}"]
+        if ipt in (8, 12) and rng.random() < 0.6:
+            nh = int(rng.choice([1, 64, 1000, 4096, 6900]))
+            plan.split_hot(d_col, nh)
+            opts.append(f"hot {nh}")
+        if ipt in (8, 12) and rng.random() < 0.4 and plan.compress_cols(d_col):
+            opts.append("col16")
+        y = plan.spmv(d_rp, d_col, d_val, d_x)
+        assert torch.equal(y.view(torch.uint8), base.view(torch.uint8)), (case, opts)
+        # alpha / beta against the host's combination of the oracle's product
+        alpha, beta = float(rng.uniform(-2, 2)), float(rng.choice([0.0, 1.0, -0.5]))
+        y0 = rng.uniform(-1, 1, nrow).astype(np.float64 if kind == "f64" else np.float32)
+        yab = plan.spmv(d_rp, d_col, d_val, d_x, dev(y0, device), alpha, beta)
+        want = alpha * ya + beta * y0.astype(np.float64)
+        scale = abs(alpha) * absr + abs(beta) * np.abs(y0.astype(np.float64))
+        got = yab.double().cpu().numpy()
+        bad = np.abs(got - want) > tol * scale + (4 * np.finfo(np.float32).eps
+                                                  if kind != "f64" else 0) * np.abs(want)
+        assert not np.any(bad & (scale > 0)), (case, opts, alpha, beta)
+        assert np.all(got[scale == 0] == 0), (case, opts)
+        # FP64 sums for the narrow value kinds
+        if kind != "f64" and rng.random() < 0.5:
+            plan.set_accumulation("f64")
+            yf = plan.spmv(d_rp, d_col, d_val, d_x)
+            assert cond_err(yf.double().cpu().numpy(), ya, absr) <= tol, (case, "acc f64")
+            plan.set_accumulation("compensated")
+        # values one element into a larger buffer: not 16-byte aligned, so the
+        # scalar staging path (the hot split and 16-bit columns need the
+        # vector path and step aside)
+        if nnz > 0 and rng.random() < 0.3:
+            holder = torch.zeros(nnz + 1, dtype=d_val.dtype, device=device)
+            holder[1:] = d_val
+            ys = plan.spmv(d_rp, d_col, holder[1:], d_x)
+            assert cond_err(ys.double().cpu().numpy(), ya, absr) <= tol, (case, "scalar path")
+        assert y.dtype == ydt
+        plan.close()
