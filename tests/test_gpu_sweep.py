@@ -139,3 +139,81 @@ def test_random_option_sweep(orc, device):
             assert cond_err(ys.double().cpu().numpy(), ya, absr) <= tol, (case, "scalar path")
         assert y.dtype == ydt
         plan.close()
+
+
+def test_random_spmm_sweep(orc, device):
+    """SpMM (FP32) over random patterns, k from 1 to 33, row- and
+    column-major X and Y, 2 or 4 items per thread and alpha/beta: every
+    column within 1e-5 (condition-scaled) of the oracle's accurate product."""
+    from merbit_amd.spmv import MerbitPlan
+    rng = np.random.default_rng(2027)
+    for case in range(60):
+        name, rp, col, ncol = pattern(rng)
+        nrow, nnz = len(rp) - 1, int(rp[-1])
+        k = int(rng.choice([1, 2, 3, 4, 5, 7, 8, 13, 16, 32, 33]))
+        ipt = int(rng.choice([2, 4]))
+        say(f"spmm sweep {case}: {name} nrow {nrow} ncol {ncol} nnz {nnz} k {k} ipt {ipt}")
+        vals = rng.standard_normal(nnz).astype(np.float32)
+        X = rng.uniform(-1, 1, (ncol, k)).astype(np.float32)
+        d_rp, d_col, d_val = dev(rp, device), dev(col, device), dev(vals, device)
+        dX = dev(X, device)
+        if rng.random() < 0.5:  # column-major X
+            dX = dX.t().contiguous().t()
+        y0 = rng.uniform(-1, 1, (nrow, k)).astype(np.float32)
+        dY = dev(y0, device)
+        if rng.random() < 0.5:  # column-major Y
+            dY = dY.t().contiguous().t()
+        alpha, beta = float(rng.uniform(-2, 2)), float(rng.choice([0.0, 1.0, -0.5]))
+        plan = MerbitPlan(d_rp, ncol, nnz, items_per_thread=ipt)
+        Y = plan.spmm(d_rp, d_col, d_val, dX, dY, alpha, beta).cpu().numpy().astype(np.float64)
+        for j in range(k):
+            ya, absr = orc.spmv_accurate(rp, col, vals, np.ascontiguousarray(X[:, j]))
+            want = alpha * ya + beta * y0[:, j].astype(np.float64)
+            scale = abs(alpha) * absr + abs(beta) * np.abs(y0[:, j].astype(np.float64))
+            bad = np.abs(Y[:, j] - want) > 1e-5 * scale
+            assert not np.any(bad & (scale > 0)), (case, j)
+            assert np.all(Y[scale == 0, j] == 0), (case, j)
+        plan.close()
+
+
+def test_random_batch_sweep(orc, device):
+    """Uneven batches (1 to 40 matrices of the families above, cut to at most
+    6000 rows, one value kind per batch) with random items per thread,
+    stream policy, residency and alpha/beta: every member within its bar of
+    the oracle."""
+    from merbit_amd.batch import MerbitBatch
+    rng = np.random.default_rng(2028)
+    for case in range(30):
+        kind = ["f64", "f32", "bf16"][int(rng.integers(0, 3))]
+        ipt = int(rng.choice([2, 4, 8, 12, 16]))
+        count = int(rng.integers(1, 41))
+        tol = 1e-12 if kind == "f64" else 1e-6
+        ydt = np.float64 if kind == "f64" else np.float32
+        alpha, beta = float(rng.uniform(-2, 2)), float(rng.choice([0.0, 1.0]))
+        say(f"batch sweep {case}: {count} matrices {kind} ipt {ipt}")
+        mats, host = [], []
+        for _ in range(count):
+            name, rp, col, ncol = pattern(rng)
+            r = min(len(rp) - 1, int(rng.integers(1, 6001)))
+            rp, nnz = rp[:r + 1].copy(), int(rp[r])
+            col = col[:nnz].copy()
+            vals, x = values(rng, kind, nnz, ncol)
+            y0 = rng.uniform(-1, 1, r).astype(ydt)
+            d = (dev(rp, device), dev(col, device), dev(vals, device), dev(x, device),
+                 dev(y0, device), ncol)
+            mats.append(d)
+            host.append((rp, col, vals, x, y0))
+        b = MerbitBatch(mats, items_per_thread=ipt)
+        b.set_stream(["auto", "cached", "nontemporal"][int(rng.integers(0, 3))])
+        b.set_residency(int(rng.integers(0, 5)))
+        b.spmv(alpha, beta)
+        torch.cuda.synchronize()
+        for i, ((rp, col, vals, x, y0), m) in enumerate(zip(host, mats)):
+            ya, absr = orc.spmv_accurate(rp, col, vals, x)
+            want = alpha * ya + beta * y0.astype(np.float64)
+            scale = abs(alpha) * absr + abs(beta) * np.abs(y0.astype(np.float64))
+            got = m[4].double().cpu().numpy()
+            bad = np.abs(got - want) > tol * scale
+            assert not np.any(bad & (scale > 0)), (case, i)
+            assert np.all(got[scale == 0] == 0), (case, i)
+        b.close()
