@@ -422,7 +422,7 @@ def bench_mixed(name, A, x, dev, args, checks, f64_ms=None, copies=1):
                        "items_per_thread": ch["items_per_thread"],
                        "tiles_per_cu": ch["wg_per_cu"], "compress_cols": ch["compress_cols"],
                        "accumulation": "f64" if ch["accumulation"] else "compensated",
-                       "hot_columns": ch["hot_columns"]}
+                       "hot_columns": ch["hot_columns"], "row_tiles": ch["row_tiles"]}
         for p in rps:
             p.close()
         out[kind] = rec
@@ -496,7 +496,7 @@ def bench_laplacian(dev, args, checks):
     b = workloads.algorithmic_bytes(A.nrow, A.ncol, A.nnz, 8, 8, 8)
     out = {"nrow": A.nrow, "nnz": A.nnz, "us_per_step": round(ms * 1e3, 3), "gbs": gbs(b, ms),
            "gflops": round(2 * A.nnz / (ms * 1e-3) / 1e9, 1), "rule": plan.choice["rule"],
-           "items_per_thread": plan.items_per_thread,
+           "items_per_thread": plan.items_per_thread, "row_tiles": plan.choice["row_tiles"],
            "timed_as": "eager launches" if args.no_graph else "one HIP graph of the steps"}
     log(f"laplacian 256^2 fp64: {ms * 1e3:.2f} us/step ({out['gbs']} GB/s)")
     plan.close()

@@ -54,6 +54,21 @@ int mbx_plan_create(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol,
                     int32_t nnz, int32_t items_per_thread, int32_t wg_size,
                     void* stream, mbx_plan** out);
 
+/* The plan of mbx_plan_create with row-aligned tiles (no reference
+ * counterpart: the reference's merge path always splits rows at tile
+ * boundaries, tile.cuh:28-104).  Tile t starts at the start of the row open
+ * at merge diagonal t*(T - L), T = wg_size*items_per_thread and L the
+ * longest row, so no tile holds more than T merge items and no row crosses
+ * a tile boundary: products need no carry fixup launch.  For patterns of
+ * short rows (banded, stencils, bounded degree): MBX_EUNSUPPORTED when
+ * L > T/16.  Rows are summed within one tile, so FP64 results may round
+ * differently from mbx_plan_create's (both within the parity bar).
+ * Synchronises `stream` twice (the longest row, then the plan statistics);
+ * products, clones and the plan options work on it as on any plan. */
+int mbx_plan_create_rows(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol,
+                         int32_t nnz, int32_t items_per_thread, int32_t wg_size,
+                         void* stream, mbx_plan** out);
+
 /* Replaces ~TILE, include/tile.cuh:19-24.  Waits for the device. */
 void mbx_plan_destroy(mbx_plan* plan);
 
@@ -308,11 +323,14 @@ int mbx_plan_stats(const mbx_plan* plan, const int32_t* d_row_ptr, mbx_pattern_s
  *   MBX_RULE_SPREAD_COLUMNS  otherwise: 8, 3 tiles per CU, and the
  *                            column-blocked form recommended (column_blocks
  *                            for mbx_colblock_create).
- * The stream policy stays MBX_STREAM_AUTO.  Synchronises `stream` (reads the
- * features back).  mbx_plan_create_auto makes the plan with the choice
- * applied (items per thread, residency, accumulation, 16-bit columns, hot
- * split); column_blocks is a recommendation for the caller (a plan is a
- * CSR product). */
+ * Whatever the rule, tiles are row-aligned (row_tiles, mbx_plan_create_rows:
+ * no carry fixup launch) when the longest row (feature max_row) is at most
+ * 1/16 of the chosen tile.  The stream policy stays MBX_STREAM_AUTO.
+ * Synchronises `stream` (reads the features back).  mbx_plan_create_auto
+ * makes the plan with the choice applied (row-aligned tiles, items per
+ * thread, residency, accumulation, 16-bit columns, hot split);
+ * column_blocks is a recommendation for the caller (a plan is a CSR
+ * product). */
 enum {
   MBX_RULE_FEW_TILES = 1,
   MBX_RULE_LOCAL = 2,
@@ -329,8 +347,9 @@ typedef struct mbx_launch_choice {
   int32_t compress_cols;     /* 1: mbx_plan_compress_cols */
   int32_t hot_columns;       /* mbx_plan_split_hot max_hot (0: none) */
   int32_t column_blocks;     /* > 0: the column-blocked form with this many blocks */
+  int32_t row_tiles;         /* 1: row-aligned tiles (mbx_plan_create_rows) */
   /* the features */
-  int32_t cus, ntiles8, max_tile_span;
+  int32_t cus, ntiles8, max_tile_span, max_row;
   double x_bytes, hot_share;
 } mbx_launch_choice;
 int mbx_launch_rule(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow, int32_t ncol,

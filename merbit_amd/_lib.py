@@ -14,7 +14,8 @@ LIB_PATH = os.path.join(_PKG, "libmerbit_amd.so")
 
 # Every symbol include/merbit_amd.h and include/merbit_gen.h declare.
 ABI_SYMBOLS = (
-    "mbx_plan_create", "mbx_plan_destroy", "mbx_plan_clone", "mbx_plan_info",
+    "mbx_plan_create", "mbx_plan_create_rows", "mbx_plan_destroy", "mbx_plan_clone",
+    "mbx_plan_info",
     "mbx_plan_partition", "mbx_plan_set_stream", "mbx_batch_set_stream",
     "mbx_plan_set_residency", "mbx_batch_set_residency", "mbx_plan_compress_cols",
     "mbx_plan_set_accumulation",
@@ -74,9 +75,10 @@ class LaunchChoice(ctypes.Structure):
                 ("stream_mode", ctypes.c_int32), ("wg_per_cu", ctypes.c_int32),
                 ("accumulation", ctypes.c_int32), ("compress_cols", ctypes.c_int32),
                 ("hot_columns", ctypes.c_int32), ("column_blocks", ctypes.c_int32),
+                ("row_tiles", ctypes.c_int32),
                 ("cus", ctypes.c_int32), ("ntiles8", ctypes.c_int32),
-                ("max_tile_span", ctypes.c_int32), ("x_bytes", ctypes.c_double),
-                ("hot_share", ctypes.c_double)]
+                ("max_tile_span", ctypes.c_int32), ("max_row", ctypes.c_int32),
+                ("x_bytes", ctypes.c_double), ("hot_share", ctypes.c_double)]
 
 
 RULE_NAMES = {1: "few_tiles", 2: "local", 3: "x_fits", 4: "hot_columns", 5: "spread_columns"}
@@ -112,6 +114,7 @@ def lib() -> ctypes.CDLL:
     vp, i32, i64, u64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64
     f64, f32 = ctypes.c_double, ctypes.c_float
     L.mbx_plan_create.argtypes = [vp, i32, i32, i32, i32, i32, vp, ctypes.POINTER(vp)]
+    L.mbx_plan_create_rows.argtypes = [vp, i32, i32, i32, i32, i32, vp, ctypes.POINTER(vp)]
     L.mbx_plan_destroy.argtypes = [vp]
     L.mbx_plan_clone.argtypes = [vp, ctypes.POINTER(vp)]
     L.mbx_plan_set_stream.argtypes = [vp, i32]

@@ -18,6 +18,9 @@
 struct mbx_plan {
   int32_t nrow, ncol, nnz;
   int32_t wg, ipt, tile_items, ntiles;
+  // merge items between tile starts: tile_items, or for row-aligned tiles
+  // (mbx_plan_create_rows) tile_items minus the longest row
+  int32_t tile_spacing;
   int32_t max_rows;    // most rows ending in one tile (pattern statistic)
   int32_t split_rows;  // tiles whose first row began earlier (fixup needed)
   int4* d_coord;  // per tile boundary: rows, nnz consumed, row_ptr[rows]
@@ -41,6 +44,13 @@ struct mbx_plan {
 
 // Most hot columns a plan keeps (mbx_plan_split_hot): 128 KB of FP32 x.
 constexpr int kMaxHotColumns = 32768;
+
+namespace mbx_detail {
+// Row-aligned tiles (mbx_plan_create_rows) need the longest row to be at
+// most tile_items / kRowTileDiv, so tiles are at most 1/kRowTileDiv fewer
+// items apart than plain ones.
+constexpr int kRowTileDiv = 16;
+}  // namespace mbx_detail
 
 struct mbx_colblock;
 

@@ -75,6 +75,34 @@ __global__ void partition_kernel(const int32_t* __restrict__ row_ptr, int nrow, 
   coord[t] = mbx_detail::tile_start_coord(row_ptr, nrow, nnz, tile_items, t);
 }
 
+// Row-aligned tiles: boundary t is the start of the row open at merge
+// diagonal t * spacing, i.e. the path point (i, row_ptr[i]) at or before it.
+// With spacing = tile_items - (longest row), every tile holds at most
+// tile_items merge items and none starts or ends inside a row.
+__global__ void partition_rows_kernel(const int32_t* __restrict__ row_ptr, int nrow, int nnz,
+                                      int spacing, int ntiles, int4* __restrict__ coord) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > ntiles) return;
+  int4 c = mbx_detail::tile_start_coord(row_ptr, nrow, nnz, spacing, t);
+  c.y = c.z;
+  coord[t] = c;
+}
+
+// Longest row (out[0], atomicMax) and the ends of row_ptr (out[2], out[3]).
+__global__ void max_row_kernel(const int32_t* __restrict__ row_ptr, int nrow,
+                               int* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int len = 0;
+  if (i < nrow) len = row_ptr[i + 1] - row_ptr[i];
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) len = max(len, __shfl_xor(len, d));
+  if ((threadIdx.x & 63) == 0 && len > 0) atomicMax(out, len);
+  if (i == 0) {
+    out[2] = row_ptr[0];
+    out[3] = row_ptr[nrow];
+  }
+}
+
 // Column span of one tile (one workgroup per tile): base = smallest column,
 // written to coord[t].w; *wide is set when the span needs more than 16 bits.
 __global__ __launch_bounds__(256) void col_span_kernel(const int32_t* __restrict__ col,
@@ -353,13 +381,10 @@ int noalloc_scratch(void** out) {
   return MBX_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int mbx_plan_create(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol,
-                    int32_t nnz, int32_t items_per_thread, int32_t wg_size,
-                    void* stream, mbx_plan** out) {
+// mbx_plan_create and mbx_plan_create_rows.
+int plan_create(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol, int32_t nnz,
+                int32_t items_per_thread, int32_t wg_size, void* stream, bool row_tiles,
+                mbx_plan** out) {
   if (!out) return MBX_EINVAL;
   *out = nullptr;
   if (nrow < 0 || ncol < 0 || nnz < 0 || (nrow > 0 && !d_row_ptr)) return MBX_EINVAL;
@@ -370,6 +395,35 @@ int mbx_plan_create(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol,
   const long long T = (long long)wg * ipt;
   const long long path = (long long)nrow + nnz;
   if (path + T >= (long long)INT_MAX) return MBX_EOVERFLOW;
+  // Row-aligned tiles are spaced by T minus the longest row, which must be
+  // at most T / kRowTileDiv (read back before the partition is sized).
+  long long spacing = T;
+  if (row_tiles && nrow > 0) {
+    hipStream_t st = (hipStream_t)stream;
+    int* d_m = nullptr;
+    int h_m[4] = {0, 0, 0, 0};
+    int rc = MBX_OK;
+    if (hipMallocAsync((void**)&d_m, 4 * sizeof(int), st) != hipSuccess) {
+      (void)hipGetLastError();
+      return MBX_ENOMEM;
+    }
+    if (hipMemsetAsync(d_m, 0, 4 * sizeof(int), st) != hipSuccess) rc = MBX_EHIP;
+    if (rc == MBX_OK) {
+      max_row_kernel<<<(nrow + 255) / 256, 256, 0, st>>>(d_row_ptr, nrow, d_m);
+      if (hipGetLastError() != hipSuccess ||
+          hipMemcpyAsync(h_m, d_m, 4 * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess)
+        rc = MBX_EHIP;
+    }
+    (void)hipFreeAsync(d_m, st);
+    if (hipStreamSynchronize(st) != hipSuccess && rc == MBX_OK) rc = MBX_EHIP;
+    if (rc != MBX_OK) {
+      (void)hipGetLastError();
+      return rc;
+    }
+    if (h_m[2] != 0 || h_m[3] != nnz) return MBX_EINVAL;
+    if ((long long)h_m[0] * mbx_detail::kRowTileDiv > T) return MBX_EUNSUPPORTED;
+    spacing = T - h_m[0];
+  }
   mbx_plan* p = new (std::nothrow) mbx_plan();
   if (!p) return MBX_ENOMEM;
   p->nrow = nrow;
@@ -378,9 +432,10 @@ int mbx_plan_create(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol,
   p->wg = wg;
   p->ipt = ipt;
   p->tile_items = (int)T;
+  p->tile_spacing = (int)spacing;
   p->max_rows = (int)T;
   p->split_rows = 0;
-  p->ntiles = (int)((path + T - 1) / T);
+  p->ntiles = (int)((path + spacing - 1) / spacing);
   p->d_coord = nullptr;
   p->d_carry = nullptr;
   p->coord_refs = new (std::nothrow) std::atomic<int>(1);
@@ -407,8 +462,12 @@ int mbx_plan_create(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol,
   }
   hipStream_t st = (hipStream_t)stream;
   const int grid = (p->ntiles + 1 + 255) / 256;
-  partition_kernel<<<grid, 256, 0, st>>>(d_row_ptr, nrow, nnz, p->tile_items, p->ntiles,
-                                         p->d_coord);
+  if (spacing != T)
+    partition_rows_kernel<<<grid, 256, 0, st>>>(d_row_ptr, nrow, nnz, (int)spacing, p->ntiles,
+                                                p->d_coord);
+  else
+    partition_kernel<<<grid, 256, 0, st>>>(d_row_ptr, nrow, nnz, p->tile_items, p->ntiles,
+                                           p->d_coord);
   // Read back once (plan creation synchronises the stream): whether any row
   // crosses a tile boundary (products skip the fixup launch when none does),
   // the most rows ending in one tile, and the ends of row_ptr.  The carry
@@ -437,6 +496,22 @@ int mbx_plan_create(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol,
   p->split_rows = h_stats[1];
   *out = p;
   return MBX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mbx_plan_create(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol,
+                    int32_t nnz, int32_t items_per_thread, int32_t wg_size,
+                    void* stream, mbx_plan** out) {
+  return plan_create(d_row_ptr, nrow, ncol, nnz, items_per_thread, wg_size, stream, false, out);
+}
+
+int mbx_plan_create_rows(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol,
+                         int32_t nnz, int32_t items_per_thread, int32_t wg_size,
+                         void* stream, mbx_plan** out) {
+  return plan_create(d_row_ptr, nrow, ncol, nnz, items_per_thread, wg_size, stream, true, out);
 }
 
 void mbx_plan_destroy(mbx_plan* p) {
@@ -614,6 +689,7 @@ int mbx_spmv_f64_noalloc(const int32_t* d_row_ptr, const int32_t* d_col, const d
   p.wg = 256;
   p.ipt = kNoAllocIpt;
   p.tile_items = kNoAllocTileItems;
+  p.tile_spacing = kNoAllocTileItems;
   p.ntiles = (int)ntiles;
   p.max_rows = kNoAllocTileItems;
   p.split_rows = 1;

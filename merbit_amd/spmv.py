@@ -92,11 +92,13 @@ class MerbitPlan:
     launch rule (mbx_plan_create_auto) picks the plan's shape and options
     from the pattern's statistics for values of ``value_dtype``; the choice
     is kept in ``self.choice``.  Without ``col`` the default is 8 items per
-    thread."""
+    thread.  row_tiles=True with an explicit items_per_thread: row-aligned
+    tiles (mbx_plan_create_rows; no row crosses a tile boundary)."""
 
     def __init__(self, row_ptr: torch.Tensor, ncol: int, nnz: int | None = None,
                  items_per_thread: int | None = None, wg_size: int = 256, stream=None,
-                 col: torch.Tensor | None = None, value_dtype=torch.float64):
+                 col: torch.Tensor | None = None, value_dtype=torch.float64,
+                 row_tiles: bool = False):
         self.nrow = int(row_ptr.numel()) - 1
         self.ncol = int(ncol)
         self.nnz = int(row_ptr[-1].item()) if nnz is None else int(nnz)
@@ -112,10 +114,11 @@ class MerbitPlan:
                 ctypes.byref(ch)), "mbx_plan_create_auto")
             self.choice = _choice_dict(ch)
         else:
-            check(lib().mbx_plan_create(_dptr(row_ptr, torch.int32, "row_ptr"), self.nrow,
-                                        self.ncol, self.nnz, items_per_thread or 8, wg_size,
-                                        _stream_handle(stream), ctypes.byref(h)),
-                  "mbx_plan_create")
+            create = lib().mbx_plan_create_rows if row_tiles else lib().mbx_plan_create
+            check(create(_dptr(row_ptr, torch.int32, "row_ptr"), self.nrow, self.ncol,
+                         self.nnz, items_per_thread or 8, wg_size, _stream_handle(stream),
+                         ctypes.byref(h)),
+                  "mbx_plan_create_rows" if row_tiles else "mbx_plan_create")
         self._h = h
         nt, ti, wg = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
         check(lib().mbx_plan_info(h, ctypes.byref(nt), ctypes.byref(ti), ctypes.byref(wg)),
