@@ -325,7 +325,7 @@ int mbx_plan_stats(const mbx_plan* plan, const int32_t* d_row_ptr, mbx_pattern_s
  *                            for mbx_colblock_create).
  * Whatever the rule, tiles are row-aligned (row_tiles, mbx_plan_create_rows:
  * no carry fixup launch) when the longest row (feature max_row) is at most
- * 1/16 of the chosen tile.  The stream policy stays MBX_STREAM_AUTO.
+ * 1/16 of the chosen tile and items per thread at most 8.  The stream policy stays MBX_STREAM_AUTO.
  * Synchronises `stream` (reads the features back).  mbx_plan_create_auto
  * makes the plan with the choice applied (row-aligned tiles, items per
  * thread, residency, accumulation, 16-bit columns, hot split);

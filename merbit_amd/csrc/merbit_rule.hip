@@ -25,10 +25,10 @@
 //     column-blocked form (mbx_colblock_*) is recommended with slices of at
 //     most 3.5 MiB of x.
 // Whatever the rule, a pattern whose longest row is at most 1/16 of the
-// chosen tile gets row-aligned tiles (mbx_plan_create_rows): no row crosses
-// a tile boundary, so the carry fixup launch goes (the 256^2 Laplacian
-// 6.1 -> 4.2 us per product in a graph, banded 2-5 % per product) for at
-// most 1/16 more tiles.
+// chosen tile gets row-aligned tiles (mbx_plan_create_rows) at up to 8
+// items per thread: no row crosses a tile boundary, so the carry fixup
+// launch goes (the 256^2 Laplacian 6.1 -> 4.2 us per product in a graph,
+// banded FP32 161 -> 156 us) for at most 1/16 more tiles.
 // The stream policy stays MBX_STREAM_AUTO (by the product's bytes).
 
 #include <hip/hip_runtime.h>
@@ -48,6 +48,9 @@ constexpr double kHotShare = 0.25;
 constexpr long long kXFitsBytes = 2ll << 20;
 constexpr long long kSliceBytes = (7ll << 20) / 2;  // column-blocked form's x slice
 constexpr int kFewTilesPerCu = 16;
+// Row-aligned tiles at 12 items per thread measured slower (banded BF16 /
+// FP32 tile kernel +8 %, against +-1 % at 8): the rule keeps them to <= 8.
+constexpr int kRowTilesMaxIpt = 8;
 
 inline int blocks(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 
@@ -224,8 +227,8 @@ int launch_rule(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow, in
     c.column_blocks = (int32_t)(b > 16 ? 16 : b);
     c.rule = MBX_RULE_SPREAD_COLUMNS;
   }
-  c.row_tiles = nnz > 0 && (long long)c.max_row * mbx_detail::kRowTileDiv <=
-                               256ll * c.items_per_thread;
+  c.row_tiles = nnz > 0 && c.items_per_thread <= kRowTilesMaxIpt &&
+                (long long)c.max_row * mbx_detail::kRowTileDiv <= 256ll * c.items_per_thread;
   *out = c;
   return MBX_OK;
 }

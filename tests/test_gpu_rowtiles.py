@@ -253,6 +253,16 @@ def test_rule_picks_row_tiles(orc, device):
     y = plan.spmv(d_rp, d_col, dev(A.val, device), dev(x, device)).cpu().numpy()
     assert cond_err(y, ya, absr) <= 1e-12
     plan.close()
+    # a local (banded) pattern: row tiles at 8 items per thread (FP32
+    # values), not at the 12 that BF16 values get
+    from merbit_amd import gen
+    B = gen.banded(320000, 13)
+    d_rp, d_col = dev(B.row_ptr, device), dev(B.col, device)
+    c32 = launch_rule(d_rp, d_col, B.ncol, value_dtype=torch.float32)
+    c16 = launch_rule(d_rp, d_col, B.ncol, value_dtype=torch.bfloat16)
+    assert c32["rule"] == "local" and c32["items_per_thread"] == 8 and c32["row_tiles"] == 1
+    assert c16["rule"] == "local" and c16["items_per_thread"] == 12 and c16["row_tiles"] == 0
+    assert c32["max_row"] == 27
     # one row longer than 1/16 of the chosen tile: merge-path tiles
     rng = np.random.default_rng(5)
     lens = np.full(20000, 3)
