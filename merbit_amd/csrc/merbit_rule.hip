@@ -257,9 +257,15 @@ int mbx_plan_create_auto(const int32_t* d_row_ptr, const int32_t* d_col, int32_t
   mbx_launch_choice c;
   mbx_plan* p = nullptr;
   int st = launch_rule(d_row_ptr, d_col, nrow, ncol, nnz, value_kind, stream, &c, cnt);
-  if (st == MBX_OK)
-    st = (c.row_tiles ? mbx_plan_create_rows : mbx_plan_create)(
-        d_row_ptr, nrow, ncol, nnz, c.items_per_thread, 256, stream, &p);
+  if (st == MBX_OK && c.row_tiles) {
+    st = mbx_plan_create_rows(d_row_ptr, nrow, ncol, nnz, c.items_per_thread, 256, stream, &p);
+    if (st == MBX_EUNSUPPORTED) {  // a row longer than the features saw (no columns)
+      c.row_tiles = 0;
+      st = MBX_OK;
+    }
+  }
+  if (st == MBX_OK && !c.row_tiles)
+    st = mbx_plan_create(d_row_ptr, nrow, ncol, nnz, c.items_per_thread, 256, stream, &p);
   if (st != MBX_OK) {
     if (cnt) (void)hipFreeAsync(cnt, s);
     return st;

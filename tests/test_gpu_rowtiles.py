@@ -270,3 +270,8 @@ def test_rule_picks_row_tiles(orc, device):
     rp, col = random_csr(rng, 20000, 20000, lens=lens)
     c = launch_rule(dev(rp, device), dev(col, device), 20000)
     assert c["max_row"] == 5000 and c["row_tiles"] == 0
+    # no columns at all (every index out of range): the features see no
+    # rows, plan creation falls back to merge-path tiles for the long row
+    p0 = MerbitPlan(dev(rp, device), 0, int(rp[-1]), col=dev(col, device))
+    assert p0.choice["row_tiles"] == 0 and p0.stats(dev(rp, device))["split_tiles"] > 0
+    p0.close()
