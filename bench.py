@@ -953,11 +953,11 @@ def main():
     ap.add_argument("--copies", type=int, default=3)
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--batch-count", type=int, default=64)
-    ap.add_argument("--only", choices=["headline", "mixed", "banded", "spmm", "batch", "adjoint",
-                                       "pagerank", "all"],
+    ap.add_argument("--only", choices=["headline", "laplacian", "mixed", "banded", "spmm", "batch",
+                                       "adjoint", "pagerank", "all"],
                     default="all",
-                    help="headline: only the FP64 R-MAT line; mixed / banded / spmm / batch / "
-                         "adjoint / pagerank: the headline plus that part (for profiling)")
+                    help="headline: only the FP64 R-MAT line; laplacian / mixed / banded / spmm / "
+                         "batch / adjoint / pagerank: the headline plus that part (for profiling)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run the plain device copy (counter calibration)")
     ap.add_argument("--compare", action="store_true",
@@ -1010,7 +1010,7 @@ def main():
     }
     lap = (None, None, None, 0)
     band = (None, None, None, 0)
-    if args.only == "all":
+    if args.only in ("all", "laplacian"):
         extras["laplacian"] = bench_laplacian(dev, args, checks)
         La, lx = workloads.laplacian(256)
         lap = (La, La.val, lx, 200)
