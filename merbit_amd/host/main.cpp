@@ -1,7 +1,7 @@
 // main.cpp -- config-ID driver: ./merbit_main <ID> (src/main.cu:3-84).
 //
 //   merbit_main <ID> [--config FILE] [--log FILE] [--device N] [--type f64|f32]
-//               [--sigma 2|4|6|8|10|12|14|16] [--perrnd N] [--maxiter N] [--check]
+//               [--sigma 2|4|6|8|10|12|14|16|auto] [--perrnd N] [--maxiter N] [--check]
 //               [--pagerank [--damp D]] [--sort-rows]
 //   merbit_main --mtx FILE [...]
 //
@@ -11,7 +11,9 @@
 // " gid&MERBIT&sigma&tim1&tim2" to the log.  --check verifies y against a
 // sequential FP64 CSR product on the host (Fun_Check).  --pagerank runs
 // Fun_PageRank instead: maxiter damped power iterations (FP64), checked
-// against the same iterations on the host.
+// against the same iterations on the host.  --sigma auto builds the plan with
+// the launch rule (Fun_Merbit_Auto: shape and options from the pattern's
+// statistics) instead of a fixed SIGMA.
 
 #include <cstdlib>
 #include <cstring>
@@ -35,7 +37,8 @@ struct Args {
 
 int usage() {
   std::cerr << "usage: merbit_main <ID> [--config FILE] [--log FILE] [--device N] "
-               "[--type f64|f32] [--sigma 2|4|...|16] [--perrnd N] [--maxiter N] [--check]\n"
+               "[--type f64|f32] [--sigma 2|4|...|16|auto] [--perrnd N] [--maxiter N] "
+               "[--check]\n"
                "                   [--pagerank [--damp D]]\n"
                "       merbit_main --mtx FILE [same options]\n";
   return 2;
@@ -83,24 +86,14 @@ int run_pagerank(const ENV& env, GRAPH<double>& graph, const Args& a) {
   return bad ? 1 : 0;
 }
 
-template <typename V, unsigned S>
-int run_sigma(const ENV& env, GRAPH<V>& graph, const Args& a) {
-  if constexpr (std::is_same<V, double>::value) {
-    if (a.pagerank) return run_pagerank<S>(env, graph, a);
-  } else if (a.pagerank) {
-    std::cerr << "--pagerank is FP64 only\n";
-    return 2;
-  }
-  PARAMETER<V> para(&graph, a.damp, a.perrnd, a.maxiter);
-  // --sort-rows leaves the entries in (row, col) order, as after the
-  // reference's GRAPH::sort(true); Fun_Merbit sorts them back to (col, row)
-  // before it takes the CSR view, as the reference's does (tile.cuh:571)
-  if (a.sort_rows) graph.sort(true);
-  Fun_Merbit<V, 256, S>(env, para);
+// The report line and --check of one Fun_Merbit / Fun_Merbit_Auto run.
+template <typename V>
+int report(const GRAPH<V>& graph, const PARAMETER<V>& para, const Args& a,
+           const std::string& sigma) {
   const MerbitTimes t = merbit::last_times();
   const double bytes = 4.0 * (graph.nrow + 1) + graph.nnz * (4.0 + sizeof(V)) +
                        2.0 * graph.nrow * sizeof(V);
-  std::cout << "graph = " << graph.graph_id << ", block = 256, sigma = " << S
+  std::cout << "graph = " << graph.graph_id << ", block = 256, sigma = " << sigma
             << ", nrow = " << graph.nrow << ", nnz = " << graph.nnz << "\n"
             << "tim1 = " << t.plan_ms << " ms, tim2 = " << t.spmv_ms << " ms/SpMV, "
             << bytes / (t.spmv_ms * 1e-3) / 1e9 << " GB/s\n";
@@ -120,11 +113,45 @@ int run_sigma(const ENV& env, GRAPH<V>& graph, const Args& a) {
   return bad ? 1 : 0;
 }
 
+template <typename V, unsigned S>
+int run_sigma(const ENV& env, GRAPH<V>& graph, const Args& a) {
+  if constexpr (std::is_same<V, double>::value) {
+    if (a.pagerank) return run_pagerank<S>(env, graph, a);
+  } else if (a.pagerank) {
+    std::cerr << "--pagerank is FP64 only\n";
+    return 2;
+  }
+  PARAMETER<V> para(&graph, a.damp, a.perrnd, a.maxiter);
+  // --sort-rows leaves the entries in (row, col) order, as after the
+  // reference's GRAPH::sort(true); Fun_Merbit sorts them back to (col, row)
+  // before it takes the CSR view, as the reference's does (tile.cuh:571)
+  if (a.sort_rows) graph.sort(true);
+  Fun_Merbit<V, 256, S>(env, para);
+  return report(graph, para, a, std::to_string(S));
+}
+
+// --sigma auto: the launch rule's plan.
+template <typename V>
+int run_auto(const ENV& env, GRAPH<V>& graph, const Args& a) {
+  if (a.pagerank) {
+    std::cerr << "--pagerank takes a fixed --sigma\n";
+    return 2;
+  }
+  PARAMETER<V> para(&graph, a.damp, a.perrnd, a.maxiter);
+  if (a.sort_rows) graph.sort(true);
+  const mbx_launch_choice c = Fun_Merbit_Auto<V>(env, para);
+  std::cout << "launch rule " << c.rule << ": items per thread " << c.items_per_thread
+            << ", row tiles " << c.row_tiles << ", hot columns " << c.hot_columns
+            << ", 16-bit columns " << c.compress_cols << "\n";
+  return report(graph, para, a, "auto:" + std::to_string(c.items_per_thread));
+}
+
 template <typename V>
 int run(const Args& a) {
   ENV env(a.device, a.config, a.log);
   GRAPH<V> graph = a.mtx.empty() ? GRAPH<V>(a.config, a.gid) : GRAPH<V>::from_mtx(a.mtx, a.gid);
   switch (a.sigma) {
+    case -1: return run_auto<V>(env, graph, a);
     case 2: return run_sigma<V, 2>(env, graph, a);
     case 4: return run_sigma<V, 4>(env, graph, a);
     case 6: return run_sigma<V, 6>(env, graph, a);
@@ -150,7 +177,7 @@ int main(int argc, char** argv) {
     else if (s == "--mtx" && (v = next())) a.mtx = v;
     else if (s == "--type" && (v = next())) a.type = v;
     else if (s == "--device" && (v = next())) a.device = std::atoi(v);
-    else if (s == "--sigma" && (v = next())) a.sigma = std::atoi(v);
+    else if (s == "--sigma" && (v = next())) a.sigma = std::strcmp(v, "auto") ? std::atoi(v) : -1;
     else if (s == "--perrnd" && (v = next())) a.perrnd = std::atoi(v);
     else if (s == "--maxiter" && (v = next())) a.maxiter = std::atoi(v);
     else if (s == "--check") a.check = true;

@@ -299,6 +299,24 @@ struct TILE {
   TILE& operator=(const TILE&) = delete;
 };
 
+// The launch rule's plan (mbx_plan_create_auto) in place of a compile-time
+// SIGMA and BLOCK: the pattern's statistics choose the shape and options
+// (no reference counterpart; the reference compiles one SIGMA for every
+// matrix, CMakeLists.txt:4-6).
+struct AUTO_TILE {
+  mbx_plan* plan = nullptr;
+  mbx_launch_choice choice{};
+  AUTO_TILE(const int* offset, const int* col, int nrow, int nnz, int32_t value_kind,
+            hipStream_t stream = nullptr) {
+    merbit::check(mbx_plan_create_auto(offset, col, nrow, nrow, nnz, value_kind, stream, &plan,
+                                       &choice),
+                  "mbx_plan_create_auto");
+  }
+  ~AUTO_TILE() { mbx_plan_destroy(plan); }
+  AUTO_TILE(const AUTO_TILE&) = delete;
+  AUTO_TILE& operator=(const AUTO_TILE&) = delete;
+};
+
 namespace merbit {
 
 template <typename V>
@@ -337,6 +355,29 @@ inline MerbitTimes& last_times() {
 // the plan (tim1), runs maxiter*perrnd SpMVs y = A*x on env.stream (tim2),
 // leaves y in para.pagerank_real and appends " gid&MERBIT&SIGMA&tim1&
 // tim2/iters" to FSTR when it is open; merbit::last_times() holds the times.
+namespace merbit {
+// The timed loop of products (include/tile.cuh:596-605) on a built plan,
+// captured once as a HIP graph and replayed in one launch; y lands in
+// para.pagerank_real and the time per product in t.spmv_ms.
+template <typename V>
+void timed_products(const ENV& env, const PARAMETER<V>& para, const mbx_plan* plan,
+                    MerbitTimes& t) {
+  const GRAPH<V>& g = *para.graph;
+  Timer timer;
+  const int iters = para.maxiter * para.perrnd;
+  void* graph = nullptr;
+  if (iters > 0)
+    merbit::check(mbx_spmv_graph_create(plan, merbit::value_kind<V>(), g.offset, g.col, g.values,
+                                        1.0, para.person, 0.0, para.pagerank_real, iters, &graph),
+                  "graph capture");
+  timer.start(env.stream);
+  if (graph) merbit::check(mbx_graph_launch(graph, env.stream), "graph launch");
+  const float tim2 = timer.stop(env.stream);
+  mbx_graph_destroy(graph);
+  t.spmv_ms = iters > 0 ? tim2 / (float)iters : 0.f;
+}
+}  // namespace merbit
+
 template <typename V, unsigned BLOCK, unsigned SIGMA>
 void Fun_Merbit(const ENV& env, const PARAMETER<V>& para) {
   static_assert(std::is_same<V, double>::value || std::is_same<V, float>::value,
@@ -349,23 +390,32 @@ void Fun_Merbit(const ENV& env, const PARAMETER<V>& para) {
   timer.start(env.stream);
   TILE<BLOCK, SIGMA> tile(g.offset, g.nrow, g.nnz, env.stream);
   t.plan_ms = timer.stop(env.stream);
-  const int iters = para.maxiter * para.perrnd;
-  // the timed loop of products (include/tile.cuh:596-605) captured once as
-  // a HIP graph and replayed in one launch
-  void* graph = nullptr;
-  if (iters > 0)
-    merbit::check(mbx_spmv_graph_create(tile.plan, merbit::value_kind<V>(), g.offset, g.col,
-                                        g.values, 1.0, para.person, 0.0, para.pagerank_real,
-                                        iters, &graph),
-                  "graph capture");
-  timer.start(env.stream);
-  if (graph) merbit::check(mbx_graph_launch(graph, env.stream), "graph launch");
-  const float tim2 = timer.stop(env.stream);
-  mbx_graph_destroy(graph);
-  t.spmv_ms = iters > 0 ? tim2 / (float)iters : 0.f;
+  merbit::timed_products(env, para, tile.plan, t);
   if (FSTR.is_open())
     FSTR << " " << g.graph_id << "&MERBIT&" << SIGMA << "&" << t.plan_ms << "&" << t.spmv_ms
          << "\n";
+}
+
+// Fun_Merbit with the launch rule's plan (AUTO_TILE) instead of a fixed
+// SIGMA: tim1 includes the rule's statistics; the log's sigma field reads
+// "auto:<items per thread>".  Returns the rule's choice.
+template <typename V>
+mbx_launch_choice Fun_Merbit_Auto(const ENV& env, const PARAMETER<V>& para) {
+  static_assert(std::is_same<V, double>::value || std::is_same<V, float>::value,
+                "Fun_Merbit_Auto is instantiated for float and double");
+  Timer timer;
+  MerbitTimes& t = merbit::last_times();
+  t = MerbitTimes();
+  if (!para.graph->csr_valid) para.graph->sort(false);
+  const GRAPH<V>& g = *para.graph;
+  timer.start(env.stream);
+  AUTO_TILE tile(g.offset, g.col, g.nrow, g.nnz, merbit::value_kind<V>(), env.stream);
+  t.plan_ms = timer.stop(env.stream);
+  merbit::timed_products(env, para, tile.plan, t);
+  if (FSTR.is_open())
+    FSTR << " " << g.graph_id << "&MERBIT&auto:" << tile.choice.items_per_thread << "&"
+         << t.plan_ms << "&" << t.spmv_ms << "\n";
+  return tile.choice;
 }
 
 // PageRank with the damping factor the reference carries in PARAMETER::damp

@@ -70,3 +70,33 @@ def test_driver_coo_sort_round_trip(device, tmp_path):
                            stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
                            timeout=120)
         assert r.returncode == 0 and "check ok" in r.stdout, r.stdout
+
+
+@pytest.mark.parametrize("typ", ["f64", "f32"])
+def test_driver_sigma_auto(device, tmp_path, typ):
+    """--sigma auto: Fun_Merbit_Auto builds the plan with the launch rule
+    (a power-law R-MAT gets the hot-column split, a banded-like stencil
+    row-aligned tiles) and the product still checks against the host."""
+    write_rmat_mtx(tmp_path / "rmat.mtx", 16, 7)
+    n = 300
+    with open(tmp_path / "band.mtx", "w") as f:
+        ent = [(r, c) for r in range(n) for c in range(max(0, r - 2), min(n, r + 3))]
+        f.write("%%MatrixMarket matrix coordinate real general\n")
+        f.write(f"{n} {n} {len(ent)}\n")
+        f.writelines(f"{r + 1} {c + 1} 1\n" for r, c in ent)
+    cfg = tmp_path / "config.txt"
+    cfg.write_text(f"1 {tmp_path}/rmat.mtx\n2 {tmp_path}/band.mtx\n")
+    log = tmp_path / "log.txt"
+    for gid in (1, 2):
+        print(f"[case] driver auto gid={gid} type={typ}", flush=True)
+        r = subprocess.run([BIN, str(gid), "--config", str(cfg), "--log", str(log), "--type", typ,
+                            "--sigma", "auto", "--perrnd", "3", "--maxiter", "2", "--check"],
+                           stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                           timeout=120)
+        assert r.returncode == 0 and "check ok" in r.stdout, r.stdout
+        assert "launch rule" in r.stdout and "sigma = auto:" in r.stdout, r.stdout
+    lines = log.read_text().splitlines()
+    assert len(lines) == 2 and all("&MERBIT&auto:" in l for l in lines)
+    r = subprocess.run([BIN, "1", "--config", str(cfg), "--sigma", "auto", "--pagerank"],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=60)
+    assert r.returncode == 2 and "fixed --sigma" in r.stdout
