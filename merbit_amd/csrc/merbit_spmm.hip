@@ -341,7 +341,10 @@ __global__ __launch_bounds__(WG) void tile_spmm_rows_kernel(
   constexpr int NW = WG / 64;
   constexpr int PW = 64 / G;         // path threads per wave
   constexpr int IPP = TI * G / WG;   // path steps per path thread
-  constexpr int U = 8;               // product-stage loads in flight per lane
+  // product-stage loads in flight per lane: 2 at G = 2 (k <= 8, fewer
+  // registers, twice the resident workgroups: k = 8 162 against 177 us on
+  // the R-MAT), 8 at G = 4 (k = 32 586 against 597 at 4, 646 at 2)
+  constexpr int U = G <= 2 ? 2 : 8;
   static_assert((TI * G) % WG == 0 && 64 % G == 0, "tile shape");
   __shared__ int s_re[TI + 1];
   __shared__ float s_wtail[NW][KC];
