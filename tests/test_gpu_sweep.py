@@ -141,6 +141,59 @@ def test_random_option_sweep(orc, device):
         plan.close()
 
 
+def test_random_row_tiles_sweep(orc, device):
+    """Row-aligned plans (mbx_plan_create_rows) over random short-row
+    patterns (every row at most 1/16 of a tile): against the oracle, the
+    options that keep the arithmetic bit-identical to the row plan's plain
+    product, the scalar staging path within the bar."""
+    from merbit_amd.spmv import MerbitPlan
+    rng = np.random.default_rng(77)
+    for case in range(80):
+        kind = ["f64", "f32", "bf16"][int(rng.integers(0, 3))]
+        ipt = int(rng.choice([2, 4, 6, 8, 10, 12, 14, 16]))
+        L = 16 * ipt  # longest row allowed: T / 16
+        nrow = int(rng.integers(1, 50000))
+        ncol = int(rng.integers(1, 60000))
+        fam = int(rng.integers(0, 3))
+        if fam == 0:
+            lens = np.minimum(rng.poisson(rng.uniform(0.5, 20.0), nrow), L)
+        elif fam == 1:
+            lens = np.where(rng.random(nrow) < 0.1, rng.integers(1, L + 1, nrow), 0)
+        else:
+            lens = np.full(nrow, int(rng.integers(0, L + 1)))
+        rp, col = random_csr(rng, nrow, ncol, lens=lens)
+        nnz = int(rp[-1])
+        vals, x = values(rng, kind, nnz, ncol)
+        tol = 1e-12 if kind == "f64" else 1e-6
+        say(f"row sweep {case}: fam {fam} nrow {nrow} ncol {ncol} nnz {nnz} {kind} ipt {ipt}")
+        d_rp, d_col, d_val, d_x = dev(rp, device), dev(col, device), dev(vals, device), \
+            dev(x, device)
+        plan = MerbitPlan(d_rp, ncol, nnz, items_per_thread=ipt, row_tiles=True)
+        base = plan.spmv(d_rp, d_col, d_val, d_x)
+        ya, absr = orc.spmv_accurate(rp, col, vals, x)
+        assert cond_err(base.double().cpu().numpy(), ya, absr) <= tol, case
+        mode = ["auto", "cached", "nontemporal"][int(rng.integers(0, 3))]
+        wg = int(rng.integers(0, 5))
+        plan.set_stream(mode)
+        plan.set_residency(wg)
+        opts = [mode, f"wg {wg}"]
+        if ipt in (8, 12) and rng.random() < 0.5:
+            nh = int(rng.choice([1, 64, 1000, 4096]))
+            plan.split_hot(d_col, nh)
+            opts.append(f"hot {nh}")
+        if ipt in (8, 12) and rng.random() < 0.5 and plan.compress_cols(d_col):
+            opts.append("col16")
+        y = plan.spmv(d_rp, d_col, d_val, d_x)
+        assert torch.equal(y.view(torch.uint8), base.view(torch.uint8)), (case, opts)
+        if nnz > 0 and rng.random() < 0.3:
+            holder = torch.zeros(nnz + 1, dtype=d_val.dtype, device=device)
+            holder[1:] = d_val
+            ys = plan.spmv(d_rp, d_col, holder[1:], d_x)
+            assert cond_err(ys.double().cpu().numpy(), ya, absr) <= tol, (case, "scalar path")
+        assert plan.stats(d_rp)["split_tiles"] == 0, case
+        plan.close()
+
+
 def test_random_spmm_sweep(orc, device):
     """SpMM (FP32) over random patterns, k from 1 to 33, row- and
     column-major X and Y, 2 or 4 items per thread and alpha/beta: every
