@@ -338,6 +338,7 @@ def bench_mixed(name, A, x, dev, args, checks, f64_ms=None, copies=1):
     (the library default).  Outputs go to `checks` for the oracle leg."""
     import torch
     from merbit_amd import gen, workloads
+    from merbit_amd.spmv import ACCUMULATION_NAMES
     nrow, ncol, nnz = A.nrow, A.ncol, A.nnz
     rp = torch.from_numpy(A.row_ptr).to(dev)
     v32 = np.ascontiguousarray(A.val, dtype=np.float32)
@@ -397,18 +398,19 @@ def bench_mixed(name, A, x, dev, args, checks, f64_ms=None, copies=1):
         c = compressed(plans, cps, f64_ms)
         if c:
             rec["compressed_cols"] = c
-        if kind != "f64":  # the same plans with exact FP64 products and FP64 sums
-            for p in plans:
-                p.set_accumulation("f64")
-            ms_a, kern_a = timed(plans, cps)
-            checks.append((f"{name}_{kind}_fp64_sums", A, stored, xs,
-                           cps[0]["y"].cpu().numpy(), tol))
-            rec["acc_f64"] = {"us_per_step": round(ms_a * 1e3, 2),
-                              "tile_kernel_us": None if kern_a is None else round(kern_a * 1e3, 2),
-                              "gbs": gbs(b, ms_a), "speedup_vs_f64": round(f64_ms / ms_a, 3)}
-            c = compressed(plans, cps, f64_ms)
-            if c:
-                rec["acc_f64"]["compressed_cols"] = c
+        if kind != "f64":  # the same plans with FP64 sums: exact or FP32-rounded products
+            for acc, key in (("f64", "acc_f64"), ("f32prod", "acc_f32prod")):
+                for p in plans:
+                    p.set_accumulation(acc)
+                ms_a, kern_a = timed(plans, cps)
+                checks.append((f"{name}_{kind}_{'fp64_sums' if acc == 'f64' else 'f32_products'}",
+                               A, stored, xs, cps[0]["y"].cpu().numpy(), tol))
+                rec[key] = {"us_per_step": round(ms_a * 1e3, 2),
+                            "tile_kernel_us": None if kern_a is None else round(kern_a * 1e3, 2),
+                            "gbs": gbs(b, ms_a), "speedup_vs_f64": round(f64_ms / ms_a, 3)}
+                c = compressed(plans, cps, f64_ms)
+                if c:
+                    rec[key]["compressed_cols"] = c
         for p in plans:
             p.close()
         # the launch rule's plans: what MerbitPlan does by default
@@ -421,7 +423,7 @@ def bench_mixed(name, A, x, dev, args, checks, f64_ms=None, copies=1):
                        "speedup_vs_f64": round(f64_ms / ms_r, 3), "rule": ch["rule"],
                        "items_per_thread": ch["items_per_thread"],
                        "tiles_per_cu": ch["wg_per_cu"], "compress_cols": ch["compress_cols"],
-                       "accumulation": "f64" if ch["accumulation"] else "compensated",
+                       "accumulation": ACCUMULATION_NAMES[ch["accumulation"]],
                        "hot_columns": ch["hot_columns"], "row_tiles": ch["row_tiles"]}
         for p in rps:
             p.close()
@@ -914,6 +916,10 @@ def north_star_summary(extras):
             acc = mb[kind].get("acc_f64", {})
             if "compressed_cols" in acc:
                 m[f"{kind}_tuned_fp64_sums_16bit_cols"] = acc["compressed_cols"]["speedup_vs_f64"]
+            acc = mb[kind].get("acc_f32prod", {})
+            if "compressed_cols" in acc:
+                m[f"{kind}_tuned_f32_products_16bit_cols"] = \
+                    acc["compressed_cols"]["speedup_vs_f64"]
         e = [v["cond_max"] for k, v in errs.items()
              if k.startswith(f"{name}_f32") or k.startswith(f"{name}_bf16")]
         if e:

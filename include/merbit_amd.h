@@ -117,8 +117,13 @@ int mbx_plan_set_residency(mbx_plan* plan, int32_t wg_per_cu);
  * significand) and the rows summed in FP64; x and y stay FP32 and y is
  * rounded once.  With packed FP32 off an FP64 add issues at the FP32 rate,
  * so MBX_ACC_F64 spends one VALU add per nonzero where TwoSum spends about
- * six; the compensated form is the VALU-bound one on HBM-bound matrices. */
-enum { MBX_ACC_COMPENSATED_F32 = 0, MBX_ACC_F64 = 1 };
+ * six; the compensated form is the VALU-bound one on HBM-bound matrices.
+ * MBX_ACC_F32_PRODUCTS: each product rounded once to FP32 and the rows
+ * summed in FP64, a row's partial rounded to FP32 where it is kept in LDS
+ * (error at most 3 * 2^-24, about 1.8e-7, of sum |val*x|, y's rounding
+ * included); the products take half the LDS of MBX_ACC_F64, so more tiles
+ * and their loads fit a CU (the launch rule's choice for local patterns). */
+enum { MBX_ACC_COMPENSATED_F32 = 0, MBX_ACC_F64 = 1, MBX_ACC_F32_PRODUCTS = 2 };
 int mbx_plan_set_accumulation(mbx_plan* plan, int32_t mode);
 
 /* Compressed column indices (no reference counterpart).  Stores every
@@ -314,9 +319,9 @@ int mbx_plan_stats(const mbx_plan* plan, const int32_t* d_row_ptr, mbx_pattern_s
  *   MBX_RULE_FEW_TILES       fewer than 16 tiles per CU: 4 items per thread
  *                            (2 when that gives fewer tiles than CUs);
  *   MBX_RULE_LOCAL           every tile spans < 65536 columns (banded):
- *                            8 (FP64, FP32) or 12 (BF16) items per thread, all tiles
- *                            resident, 16-bit columns, FP64 sums for
- *                            FP32/BF16 values;
+ *                            8 (FP64) or 12 (FP32, BF16) items per thread, all tiles
+ *                            resident, 16-bit columns, FP32 products and
+ *                            FP64 sums for FP32/BF16 values;
  *   MBX_RULE_X_FITS          x of at most 2 MiB: 8, at most 4 tiles per CU;
  *   MBX_RULE_HOT_COLUMNS     the most frequent columns hold >= 1/4 of the
  *                            entries: 8, the hot-column split;

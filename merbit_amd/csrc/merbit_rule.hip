@@ -11,10 +11,13 @@
 //     (2 when even that leaves CUs without a tile);
 //   * local columns (every tile of 2048 merge items spans < 65536
 //     columns, e.g. banded): the gathers hit L2 and the stream is the bound:
-//     every tile resident, 16-bit column offsets, and FP64 sums for FP32 /
-//     bfloat16 values (the compensated sums are VALU-bound once the stream
-//     is cut); 8 items per thread for FP64 and FP32 values, 12 for bfloat16
-//     (its 2-byte values leave the tile's loads short);
+//     every tile resident, 16-bit column offsets, and for FP32 / bfloat16
+//     values FP32 products with FP64 sums (the compensated sums are
+//     VALU-bound once the stream is cut; 4-byte products in LDS let more
+//     tiles' loads share a CU: banded BF16 155 -> 137 us, FP32 171 -> 164);
+//     8 items per thread for FP64 values, 12 for FP32 and bfloat16 values
+//     (their shorter values leave a tile's loads short; with 4-byte products
+//     the larger tile still fits: FP32 kernel 155.5 -> 151.9 us);
 //   * x of at most 2 MiB: the gathers stay in L2; 8 items per thread, at
 //     most 4 tiles per CU;
 //   * larger x with a hot column set (the kHot* most frequent columns hold
@@ -208,9 +211,9 @@ int launch_rule(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow, in
     c.items_per_thread = 2ll * c.ntiles8 < cus ? 2 : 4;
     c.rule = MBX_RULE_FEW_TILES;
   } else if (local) {
-    c.items_per_thread = value_kind == MBX_VALUE_BF16 ? 12 : 8;
+    c.items_per_thread = f64 ? 8 : 12;
     c.compress_cols = 1;
-    if (!f64) c.accumulation = MBX_ACC_F64;
+    if (!f64) c.accumulation = MBX_ACC_F32_PRODUCTS;
     c.rule = MBX_RULE_LOCAL;
   } else if ((long long)ncol * sx <= kXFitsBytes) {
     c.items_per_thread = 8;

@@ -589,7 +589,9 @@ int mbx_plan_set_residency(mbx_plan* p, int32_t wg_per_cu) {
 }
 
 int mbx_plan_set_accumulation(mbx_plan* p, int32_t mode) {
-  if (!p || (mode != MBX_ACC_COMPENSATED_F32 && mode != MBX_ACC_F64)) return MBX_EINVAL;
+  if (!p || (mode != MBX_ACC_COMPENSATED_F32 && mode != MBX_ACC_F64 &&
+             mode != MBX_ACC_F32_PRODUCTS))
+    return MBX_EINVAL;
   p->accumulation = mode;
   return MBX_OK;
 }
@@ -653,6 +655,8 @@ int mbx_spmv_f32(const mbx_plan* p, const int32_t* rp, const int32_t* col,
                  void* stream) {
   if (p && p->accumulation == MBX_ACC_F64)
     return run_spmv<mbx_detail::F32A>(p, rp, col, val, alpha, x, beta, y, stream);
+  if (p && p->accumulation == MBX_ACC_F32_PRODUCTS)
+    return run_spmv<mbx_detail::F32R>(p, rp, col, val, alpha, x, beta, y, stream);
   return run_spmv<F32C>(p, rp, col, val, alpha, x, beta, y, stream);
 }
 
@@ -661,6 +665,8 @@ int mbx_spmv_bf16(const mbx_plan* p, const int32_t* rp, const int32_t* col,
                   void* stream) {
   if (p && p->accumulation == MBX_ACC_F64)
     return run_spmv<mbx_detail::BF16A>(p, rp, col, val, alpha, x, beta, y, stream);
+  if (p && p->accumulation == MBX_ACC_F32_PRODUCTS)
+    return run_spmv<mbx_detail::BF16R>(p, rp, col, val, alpha, x, beta, y, stream);
   return run_spmv<BF16C>(p, rp, col, val, alpha, x, beta, y, stream);
 }
 

@@ -168,6 +168,34 @@ struct Acc64 {
 using F32A = Acc64<float>;
 using BF16A = Acc64<uint16_t>;
 
+// FP32 or bfloat16 values and FP32 x, y with each product rounded once to
+// FP32 and the rows summed in FP64: the products take 4 bytes of LDS
+// instead of Acc64's 8, so a tile needs half the product area and more tiles
+// (and their loads) fit a CU.  Each product's rounding adds at most 2^-24 of
+// |val*x| to a row's error, and a row's partial is rounded to FP32 where it
+// is kept in LDS and where a carry completes it: at most 3 * 2^-24 (about
+// 1.8e-7) of sum |val*x| in all, the FP32 rounding of y included.
+template <class V>
+struct Acc64R : Acc64<V> {
+  using typename Acc64<V>::val_t;
+  using typename Acc64<V>::x_t;
+  using typename Acc64<V>::y_t;
+  using typename Acc64<V>::scal_t;
+  using typename Acc64<V>::acc_t;
+  using lds_t = float;
+  __device__ static lds_t product(val_t v, x_t x) { return CompF32<V>::widen(v) * x; }
+  __device__ static void add(acc_t& a, lds_t p) { a += (double)p; }
+  __device__ static lds_t to_lds(acc_t a) { return (float)a; }
+  __device__ static double to_f64(lds_t a) { return (double)a; }
+  __device__ static lds_t finish(double c, acc_t h) { return (float)(c + h); }
+  __device__ static void store(y_t* y, scal_t alpha, lds_t v, scal_t beta, bool beta_zero) {
+    Acc64<V>::store_f64(y, alpha, (double)v, beta, beta_zero);
+  }
+};
+
+using F32R = Acc64R<float>;
+using BF16R = Acc64R<uint16_t>;
+
 // P in its streamed mode: the stream loads P marks are non-temporal, so that
 // data read once per product does not displace the x lines the gathers
 // reuse.  For products whose data does not stay cache-resident from one
@@ -482,6 +510,9 @@ __device__ __forceinline__ void stage_scalar(
 __device__ __forceinline__ void lds_store4(double* dst, const double (&v)[4]) {
   reinterpret_cast<double2*>(dst)[0] = make_double2(v[0], v[1]);
   reinterpret_cast<double2*>(dst)[1] = make_double2(v[2], v[3]);
+}
+__device__ __forceinline__ void lds_store4(float* dst, const float (&v)[4]) {
+  reinterpret_cast<float4*>(dst)[0] = make_float4(v[0], v[1], v[2], v[3]);
 }
 __device__ __forceinline__ void lds_store4(float2* dst, const float2 (&v)[4]) {
   reinterpret_cast<float4*>(dst)[0] = make_float4(v[0].x, v[0].y, v[1].x, v[1].y);

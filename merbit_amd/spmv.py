@@ -18,6 +18,10 @@ from ._lib import check, lib
 # Stream cache policies of mbx_plan_set_stream / mbx_batch_set_stream.
 STREAM_MODES = {"auto": 0, "cached": 1, "nontemporal": 2}
 
+# Accumulation modes of mbx_plan_set_accumulation (FP32 / bfloat16 values).
+ACCUMULATION_MODES = {"compensated": 0, "f64": 1, "f32prod": 2}
+ACCUMULATION_NAMES = {v: k for k, v in ACCUMULATION_MODES.items()}
+
 # Items per thread with an instantiated tile kernel (the reference's SIGMA).
 ITEMS_PER_THREAD = (2, 4, 6, 8, 10, 12, 14, 16)
 
@@ -165,10 +169,10 @@ class MerbitPlan:
 
     def set_accumulation(self, mode: str) -> None:
         """Sums of the FP32 / bfloat16 value forms: "compensated" (TwoSum
-        compensated FP32, the default) or "f64" (exact FP64 products, FP64
-        sums)."""
-        check(lib().mbx_plan_set_accumulation(self.handle,
-                                              {"compensated": 0, "f64": 1}[mode]),
+        compensated FP32, the default), "f64" (exact FP64 products, FP64
+        sums) or "f32prod" (products rounded once to FP32, FP64 sums: half
+        the LDS per tile)."""
+        check(lib().mbx_plan_set_accumulation(self.handle, ACCUMULATION_MODES[mode]),
               "mbx_plan_set_accumulation")
 
     def compress_cols(self, col: torch.Tensor | None, stream=None) -> bool:

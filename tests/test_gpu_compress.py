@@ -142,3 +142,42 @@ def test_f64_accumulation(orc, device, kind, ipt):
         yc = plan.spmv(d_rp, d_col, d_val, d_x).cpu().numpy()
         assert cond_err(yc, ya, absr) <= 1e-6, name
         plan.close()
+
+
+@pytest.mark.parametrize("ipt", [4, 8, 12])
+@pytest.mark.parametrize("kind", ["f32", "bf16"])
+def test_f32_products_accumulation(orc, device, kind, ipt):
+    """MBX_ACC_F32_PRODUCTS: each product rounded once to FP32, FP64 sums,
+    a row's partial rounded to FP32 where it is stored in LDS and again where
+    a carry completes it: at most 3 * 2^-24 (1.8e-7) of |A||x| in all; bar
+    2.5e-7.  Compressed columns and row-aligned tiles change nothing beyond
+    that; the launch rule's local plans use this mode."""
+    from merbit_amd.spmv import MerbitPlan
+    rng = np.random.default_rng(180 + ipt)
+    cases = edge_case_patterns(256 * ipt) + [("random", rng.poisson(8, 9000), 60000)]
+    for name, lens, ncol in cases:
+        say(f"acc f32prod {kind} ipt={ipt} {name}")
+        rp, col = random_csr(rng, len(lens), ncol, lens=lens)
+        vals = values(kind, rng, int(rp[-1]))
+        xs = rng.uniform(-1, 1, ncol).astype(np.float32)
+        d_rp, d_col, d_val, d_x = dev(rp, device), dev(col, device), dev(vals, device), dev(xs, device)
+        ya, absr = orc.spmv_accurate(rp, col, vals, xs)
+        plan = MerbitPlan(d_rp, ncol, int(rp[-1]), items_per_thread=ipt)
+        plan.set_accumulation("f32prod")
+        y = plan.spmv(d_rp, d_col, d_val, d_x).cpu().numpy()
+        assert cond_err(y, ya, absr) <= 2.5e-7, name
+        y0 = rng.uniform(-1, 1, len(lens)).astype(np.float32)
+        yb = plan.spmv(d_rp, d_col, d_val, d_x, dev(y0, device), alpha=-2.0, beta=0.5)
+        want = -2.0 * ya + 0.5 * y0.astype(np.float64)
+        den = 2.0 * absr + 0.5 * np.abs(y0) + 1e-300
+        assert np.max(np.abs(yb.cpu().numpy() - want) / den, initial=0.0) <= 2.5e-7, name
+        if ipt in (8, 12) and plan.compress_cols(d_col):
+            y1 = plan.spmv(d_rp, d_col, d_val, d_x).cpu().numpy()
+            assert np.array_equal(y1.view(np.uint8), y.view(np.uint8)), name
+        plan.close()
+        if max(lens) * 16 <= 256 * ipt:  # row-aligned tiles
+            rplan = MerbitPlan(d_rp, ncol, int(rp[-1]), items_per_thread=ipt, row_tiles=True)
+            rplan.set_accumulation("f32prod")
+            yr = rplan.spmv(d_rp, d_col, d_val, d_x).cpu().numpy()
+            assert cond_err(yr, ya, absr) <= 2.5e-7, name
+            rplan.close()

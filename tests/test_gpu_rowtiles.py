@@ -140,7 +140,7 @@ def test_row_plan_products(orc, device, kind, ipt):
         ref = plain.spmv(d_rp, d_col, d_val, d_x).double().cpu().numpy()
         y0 = rng.uniform(-1, 1, len(lens))
         y0 = y0 if kind == "f64" else y0.astype(np.float32)
-        accs = ["compensated"] if kind == "f64" else ["compensated", "f64"]
+        accs = ["compensated"] if kind == "f64" else ["compensated", "f64", "f32prod"]
         for acc in accs:
             plan.set_accumulation(acc)
             for c16 in (False, True):
@@ -253,16 +253,19 @@ def test_rule_picks_row_tiles(orc, device):
     y = plan.spmv(d_rp, d_col, dev(A.val, device), dev(x, device)).cpu().numpy()
     assert cond_err(y, ya, absr) <= 1e-12
     plan.close()
-    # a local (banded) pattern: row tiles at 8 items per thread (FP32
-    # values), not at the 12 that BF16 values get
+    # a local (banded) pattern: row tiles at the 8 items per thread of FP64
+    # values, not at the 12 that FP32 and BF16 values get
     from merbit_amd import gen
     B = gen.banded(320000, 13)
     d_rp, d_col = dev(B.row_ptr, device), dev(B.col, device)
+    c64 = launch_rule(d_rp, d_col, B.ncol, value_dtype=torch.float64)
     c32 = launch_rule(d_rp, d_col, B.ncol, value_dtype=torch.float32)
     c16 = launch_rule(d_rp, d_col, B.ncol, value_dtype=torch.bfloat16)
-    assert c32["rule"] == "local" and c32["items_per_thread"] == 8 and c32["row_tiles"] == 1
-    assert c16["rule"] == "local" and c16["items_per_thread"] == 12 and c16["row_tiles"] == 0
-    assert c32["max_row"] == 27
+    assert c64["rule"] == "local" and c64["items_per_thread"] == 8 and c64["row_tiles"] == 1
+    for c in (c32, c16):
+        assert c["rule"] == "local" and c["items_per_thread"] == 12 and c["row_tiles"] == 0
+        assert c["accumulation"] == 2  # FP32 products, FP64 sums
+    assert c64["max_row"] == 27
     # one row longer than 1/16 of the chosen tile: merge-path tiles
     rng = np.random.default_rng(5)
     lens = np.full(20000, 3)

@@ -123,11 +123,13 @@ def test_random_option_sweep(orc, device):
                                                   if kind != "f64" else 0) * np.abs(want)
         assert not np.any(bad & (scale > 0)), (case, opts, alpha, beta)
         assert np.all(got[scale == 0] == 0), (case, opts)
-        # FP64 sums for the narrow value kinds
-        if kind != "f64" and rng.random() < 0.5:
-            plan.set_accumulation("f64")
+        # FP64 sums for the narrow value kinds, with exact or FP32 products
+        r_acc = rng.random()
+        if kind != "f64" and r_acc < 0.5:
+            acc = "f64" if r_acc < 0.25 else "f32prod"
+            plan.set_accumulation(acc)
             yf = plan.spmv(d_rp, d_col, d_val, d_x)
-            assert cond_err(yf.double().cpu().numpy(), ya, absr) <= tol, (case, "acc f64")
+            assert cond_err(yf.double().cpu().numpy(), ya, absr) <= tol, (case, acc)
             plan.set_accumulation("compensated")
         # values one element into a larger buffer: not 16-byte aligned, so the
         # scalar staging path (the hot split and 16-bit columns need the
