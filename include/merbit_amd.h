@@ -314,7 +314,7 @@ int mbx_plan_stats(const mbx_plan* plan, const int32_t* d_row_ptr, mbx_pattern_s
  * (GRAPH::desc, src/class.cu:223-381; TILE::desc, include/tile.cuh:177-233)
  * but compiles one SIGMA and BLOCK (CMakeLists.txt:4-6) for every matrix.
  * Features: tiles at 8 items per thread, the widest column range of one such
- * tile, x bytes, and the share of the entries in the 4096 (FP64 x) or 5800
+ * tile, x bytes, and the share of the entries in the 4096 (FP64 x) or 7900
  * (FP32 x) most frequent columns.  Rules, first match:
  *   MBX_RULE_FEW_TILES       fewer than 16 tiles per CU: 4 items per thread
  *                            (2 when that gives fewer tiles than CUs);
@@ -324,7 +324,8 @@ int mbx_plan_stats(const mbx_plan* plan, const int32_t* d_row_ptr, mbx_pattern_s
  *                            FP64 sums for FP32/BF16 values;
  *   MBX_RULE_X_FITS          x of at most 2 MiB: 8, at most 4 tiles per CU;
  *   MBX_RULE_HOT_COLUMNS     the most frequent columns hold >= 1/4 of the
- *                            entries: 8, the hot-column split;
+ *                            entries: 8, the hot-column split (FP32/BF16
+ *                            values: FP32 products, FP64 sums);
  *   MBX_RULE_SPREAD_COLUMNS  otherwise: 8, 3 tiles per CU, and the
  *                            column-blocked form recommended (column_blocks
  *                            for mbx_colblock_create).

@@ -23,7 +23,9 @@
 //   * larger x with a hot column set (the kHot* most frequent columns hold
 //     at least a quarter of the entries, power-law patterns): the product is
 //     bound by the vector L1's miss slots; the hot values go to LDS
-//     (mbx_plan_split_hot) and the tiles per CU follow from that LDS;
+//     (mbx_plan_split_hot) and the tiles per CU follow from that LDS; FP32 /
+//     bfloat16 values with FP32 products, whose smaller tile LDS leaves room
+//     for more hot values at 3 tiles per CU;
 //   * larger x without one (uniform random graphs): 3 tiles per CU, and the
 //     column-blocked form (mbx_colblock_*) is recommended with slices of at
 //     most 3.5 MiB of x.
@@ -46,7 +48,10 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kHotF64 = 4096;   // 32 KB of FP64 x: 3 tiles per CU with it in LDS
-constexpr int kHotF32 = 5800;   // 23 KB of FP32 x: 4 tiles per CU
+// 31 KB of FP32 x: with 4-byte products (MBX_ACC_F32_PRODUCTS) 3 tiles per
+// CU at 8 items per thread (R-MAT FP32 kernel 64.2 us against 66.4 for the
+// compensated sums with 5800 hot values, 4 per CU; BF16 62.1 against 63.7)
+constexpr int kHotF32 = 7900;
 constexpr double kHotShare = 0.25;
 constexpr long long kXFitsBytes = 2ll << 20;
 constexpr long long kSliceBytes = (7ll << 20) / 2;  // column-blocked form's x slice
@@ -222,6 +227,7 @@ int launch_rule(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow, in
   } else if (c.hot_share >= kHotShare) {
     c.items_per_thread = 8;
     c.hot_columns = hot;
+    if (!f64) c.accumulation = MBX_ACC_F32_PRODUCTS;
     c.rule = MBX_RULE_HOT_COLUMNS;
   } else {
     c.items_per_thread = 8;

@@ -199,9 +199,11 @@ def test_adjoint_permutation_boundaries(orc, device, dtype):
 def test_operator_default_adjoint_rule_rmat(orc, device, dtype):
     """A default MerbitOperator runs the launch rule on the transposed
     pattern (mbx_adjoint_auto).  The R-MAT's rows are skewed like its
-    columns, so A^T gets the hot-column split; dx is bit-identical to the
-    plain single-product adjoint of the same shape (8 items per thread) and
-    matches the oracle's accurate A^T product."""
+    columns, so A^T gets the hot-column split; for FP64 dx is bit-identical
+    to the plain single-product adjoint of the same shape (8 items per
+    thread), for FP32 the rule also picks FP32 products with FP64 sums, so dx
+    matches the plain adjoint within the FP32 bar; both match the oracle's
+    accurate A^T product."""
     from merbit_amd import workloads
     from merbit_amd.autograd import MerbitOperator
     A, _ = workloads.rmat()
@@ -218,11 +220,17 @@ def test_operator_default_adjoint_rule_rmat(orc, device, dtype):
     assert c["rule"] == "hot_columns" and c["hot_columns"] > 0, c
     ref_op = MerbitOperator(d_rp, d_col, A.ncol, items_per_thread=8, blocks=0,
                             adjoint_blocks=0, value_dtype=dtype)
-    assert torch.equal(dx.view(torch.uint8), ref_op.adjoint_x(d_val, d_g).view(torch.uint8))
+    dx_ref = ref_op.adjoint_x(d_val, d_g)
     cp, ri, pm = orc.csr_transpose(A.row_ptr, A.col, A.ncol)
     gh = d_g.double().cpu().numpy()
     da, dabs = orc.spmv_accurate(cp, ri, np.ascontiguousarray(vals[pm]).astype(np.float64), gh)
     tol = 1e-12 if dtype == torch.float64 else 1e-6
+    if dtype == torch.float64:
+        assert c["accumulation"] == 0
+        assert torch.equal(dx.view(torch.uint8), dx_ref.view(torch.uint8))
+    else:
+        assert c["accumulation"] == 2  # FP32 products, FP64 sums
+        assert cond_err(dx.double().cpu().numpy(), dx_ref.double().cpu().numpy(), dabs) <= 2 * tol
     assert cond_err(dx.double().cpu().numpy(), da, dabs) <= tol
     ref_op.close()
     op.close()
@@ -231,8 +239,9 @@ def test_operator_default_adjoint_rule_rmat(orc, device, dtype):
 @pytest.mark.slow
 def test_operator_default_adjoint_rule_banded(orc, device):
     """A banded pattern's transpose is banded: the rule on A^T picks the
-    local form (16-bit columns, FP64 sums for FP32 values); dx matches the
-    oracle and the plain adjoint within the FP32 bar."""
+    local form (16-bit columns, FP32 products with FP64 sums for FP32
+    values); dx matches the oracle and the plain adjoint within the FP32
+    bar."""
     from merbit_amd.autograd import MerbitOperator
     n, offs = 1_200_000, np.arange(-3, 4)
     say("default operator adjoint banded")
@@ -250,7 +259,7 @@ def test_operator_default_adjoint_rule_banded(orc, device):
     op = MerbitOperator(d_rp, d_col, n, value_dtype=torch.float32)
     dx = op.adjoint_x(d_val, d_g)
     c = op.adjoint_choice
-    assert c["rule"] == "local" and c["compress_cols"] == 1 and c["accumulation"] == 1, c
+    assert c["rule"] == "local" and c["compress_cols"] == 1 and c["accumulation"] == 2, c
     cp, ri, pm = orc.csr_transpose(rp, cols, n)
     da, dabs = orc.spmv_accurate(cp, ri, np.ascontiguousarray(vals[pm]).astype(np.float64),
                                  g.astype(np.float64))
