@@ -399,18 +399,36 @@ def bench_mixed(name, A, x, dev, args, checks, f64_ms=None, copies=1):
         if c:
             rec["compressed_cols"] = c
         if kind != "f64":  # the same plans with FP64 sums: exact or FP32-rounded products
+            from merbit_amd.spmv import launch_rule
+            vdt = torch.float32 if kind == "f32" else torch.bfloat16
             for acc, key in (("f64", "acc_f64"), ("f32prod", "acc_f32prod")):
-                for p in plans:
+                aplans, ahot = plans, hot
+                if acc == "f32prod" and hot:
+                    # 4-byte product slots leave LDS for a larger hot set at the
+                    # same tiles per CU: the launch rule's count for this form
+                    ahot = launch_rule(cps[0]["rp"], cps[0]["col"], ncol, vdt)["hot_columns"]
+                    if ahot and ahot != hot:
+                        aplans = make_plans(cps, ncol, nnz, ipt, ahot, kind)
+                        for p in aplans:
+                            p.set_stream(stream)
+                            p.set_residency(resid)
+                    else:
+                        ahot = hot
+                for p in aplans:
                     p.set_accumulation(acc)
-                ms_a, kern_a = timed(plans, cps)
+                ms_a, kern_a = timed(aplans, cps)
                 checks.append((f"{name}_{kind}_{'fp64_sums' if acc == 'f64' else 'f32_products'}",
                                A, stored, xs, cps[0]["y"].cpu().numpy(), tol))
                 rec[key] = {"us_per_step": round(ms_a * 1e3, 2),
                             "tile_kernel_us": None if kern_a is None else round(kern_a * 1e3, 2),
-                            "gbs": gbs(b, ms_a), "speedup_vs_f64": round(f64_ms / ms_a, 3)}
-                c = compressed(plans, cps, f64_ms)
+                            "gbs": gbs(b, ms_a), "speedup_vs_f64": round(f64_ms / ms_a, 3),
+                            "hot_columns": ahot}
+                c = compressed(aplans, cps, f64_ms)
                 if c:
                     rec[key]["compressed_cols"] = c
+                if aplans is not plans:
+                    for p in aplans:
+                        p.close()
         for p in plans:
             p.close()
         # the launch rule's plans: what MerbitPlan does by default
@@ -431,6 +449,9 @@ def bench_mixed(name, A, x, dev, args, checks, f64_ms=None, copies=1):
         log(f"{name} {kind}: tuned {ms * 1e3:.1f} us/step (ipt {ipt}, {stream}, "
             f"{resid or 'all'}/CU, hot {hot})"
             + ("" if kind == "f64" else f" x{f64_ms / ms:.2f} vs fp64")
+            + ("" if kind == "f64" else
+               f"; FP32 products {rec['acc_f32prod']['us_per_step']:.1f} us "
+               f"(hot {rec['acc_f32prod']['hot_columns']})")
             + f"; rule {ch['rule']} {ms_r * 1e3:.1f} us (x{f64_ms / ms_r:.2f})")
         del cps
     return out
@@ -920,6 +941,8 @@ def north_star_summary(extras):
             if "compressed_cols" in acc:
                 m[f"{kind}_tuned_f32_products_16bit_cols"] = \
                     acc["compressed_cols"]["speedup_vs_f64"]
+            elif acc:
+                m[f"{kind}_tuned_f32_products"] = acc["speedup_vs_f64"]
         e = [v["cond_max"] for k, v in errs.items()
              if k.startswith(f"{name}_f32") or k.startswith(f"{name}_bf16")]
         if e:
