@@ -135,15 +135,18 @@ HOT_COLUMNS = {"f64": 4096, "f32": 5800, "bf16": 5800}
 HOT_COLUMNS_IPT12 = {"f64": 6900}
 
 
-def make_plans(copies, ncol, nnz, ipt, hot=0, kind="f64"):
-    """One plan per rotated copy (each keeps its own hot-column copy of the
-    columns when hot > 0, so no array is shared across the rotation)."""
+def make_plans(copies, ncol, nnz, ipt, hot=0, kind="f64", srt=False):
+    """One plan per rotated copy (each keeps its own hot-column or sorted
+    copy of the columns when hot > 0 or srt, so no array is shared across
+    the rotation)."""
     from merbit_amd.spmv import MerbitPlan
     plans = []
     for c in copies:
         p = MerbitPlan(c["rp"], ncol, nnz, items_per_thread=ipt)
         if hot:
             p.split_hot(c["col"], hot)
+        if srt and not p.sort_cols(c["col"]):
+            raise RuntimeError("tile-sorted columns refused for this plan")
         plans.append(p)
     return plans
 
@@ -171,18 +174,20 @@ def spmv_calls(plans, copies):
             for p, c in zip(plans, copies)]
 
 
-def tune_shape(args, copies, ncol, nnz, kind, hot_ok):
+def tune_shape(args, copies, ncol, nnz, kind, hot_ok, sort_ok=True):
     """Launch shape: --ipt / --stream / --residency when given, else the
-    (items per thread, stream policy, tiles per CU, hot columns) whose plans
-    run the timed workload (the rotated copies) fastest over a short timing
-    of each (the first, in candidate order, within 1 % of the fastest).
+    (items per thread, stream policy, tiles per CU, hot columns, tile-sorted
+    columns) whose plans run the timed workload (the rotated copies) fastest
+    over a short timing of each (the first, in candidate order, within 1 %
+    of the fastest).
     Tuning on the rotated copies matters: on one copy resident in the 256 MB
     cache the shapes time alike on the R-MAT matrix, while rotated copies
     separate them; a matrix read once per product gains from non-temporal
     stream loads that cost a cache-resident one 6 %; fewer tiles per CU cut
     the L1 misses of gather-bound products while HBM-bound ones want every
     tile resident; the hot-column split (items per thread 8 and 12) pays on
-    power-law patterns."""
+    power-law patterns, tile-sorted columns (8, 12 and 16, without the
+    split) where a tile's gathers share x lines."""
     from merbit_amd.spmv import ITEMS_PER_THREAD
     ipts = [args.ipt] if args.ipt else [i for i in ITEMS_PER_THREAD if i in (4, 8, 12, 16)]
     modes = [args.stream] if args.stream else ["cached", "nontemporal"]
@@ -194,16 +199,21 @@ def tune_shape(args, copies, ncol, nnz, kind, hot_ok):
     def hot_fits(i, h):  # hot kernels exist for 8 and 12 items per thread
         return not h or (i in (8, 12) and (h != HOT_COLUMNS_IPT12.get(kind) or i == 12
                                            or args.hot is not None))
-    cands = [(i, m, w, h) for i in ipts for h in hots if hot_fits(i, h)
+    sorts = [False] + ([True] if sort_ok and not args.no_sort else [])
+    forms = [(h, False) for h in hots] + [(0, True) for s_ in sorts if s_]
+
+    def form_fits(i, h, srt):
+        return (i in (8, 12, 16) and not h) if srt else hot_fits(i, h)
+    cands = [(i, m, w, h, srt) for i in ipts for h, srt in forms if form_fits(i, h, srt)
              for m in modes for w in (wgs if not h else [0])]
     if len(cands) == 1:
         return cands[0]
     times = {}
     for ipt in ipts:
-        for hot in hots:
-            if not hot_fits(ipt, hot):
+        for hot, srt in forms:
+            if not form_fits(ipt, hot, srt):
                 continue
-            plans = make_plans(copies, ncol, nnz, ipt, hot, kind)
+            plans = make_plans(copies, ncol, nnz, ipt, hot, kind, srt)
             calls = spmv_calls(plans, copies)
             for mode in modes:
                 for p in plans:
@@ -211,7 +221,7 @@ def tune_shape(args, copies, ncol, nnz, kind, hot_ok):
                 for wg in (wgs if not hot else [0]):
                     for p in plans:
                         p.set_residency(wg)
-                    times[(ipt, mode, wg, hot)] = min(
+                    times[(ipt, mode, wg, hot, srt)] = min(
                         time_steps(calls, 10 * len(calls), 2 * len(calls))[0] for _ in range(3))
             for p in plans:
                 p.close()
@@ -230,14 +240,14 @@ def bench_rmat_f64(A, x, dev, args, checks):
                    y=torch.empty(nrow, dtype=torch.float64, device=dev))
               for _ in range(max(1, args.copies))]
     c0 = copies[0]
-    ipt, stream, resid, hot = tune_shape(args, copies, ncol, nnz, "f64", not args.no_hot)
+    ipt, stream, resid, hot, srt = tune_shape(args, copies, ncol, nnz, "f64", not args.no_hot)
     plan_ms = []
     for _ in range(5):  # plan construction (the reference's tim1), host clock
         t0 = time.perf_counter()
         plan = MerbitPlan(c0["rp"], ncol, nnz, items_per_thread=ipt)
         plan_ms.append((time.perf_counter() - t0) * 1e3)
         plan.close()
-    plans = make_plans(copies, ncol, nnz, ipt, hot)
+    plans = make_plans(copies, ncol, nnz, ipt, hot, srt=srt)
     for p in plans:
         p.set_stream(stream)
         p.set_residency(resid)
@@ -252,8 +262,8 @@ def bench_rmat_f64(A, x, dev, args, checks):
         plans[0].set_stream("cached")
         warm_ms, warm_kern = time_steps(calls[:1], args.steps, args.warmup, plans[0].set_events)
     ceilings = None
-    if not args.no_probes and ipt in (8, 12):
-        ceilings = bench_ceilings(A, copies, ncol, nnz, ipt, stream, resid, args)
+    if not args.no_probes and ipt in (8, 12, 16):
+        ceilings = bench_ceilings(A, copies, ncol, nnz, ipt, stream, resid, args, srt)
     # the launch rule's plans (the library default), on the same rotation,
     # and on one copy over and over (the rule's own use: a matrix reused)
     auto_ms = []
@@ -272,14 +282,16 @@ def bench_rmat_f64(A, x, dev, args, checks):
     rule = dict(rp_rule[0].choice)
     for p in rp_rule:
         p.close()
-    log(f"rmat fp64 (ipt {ipt}, {stream} stream, {resid or 'all'} tiles/CU, hot {hot}): "
+    log(f"rmat fp64 (ipt {ipt}, {stream} stream, {resid or 'all'} tiles/CU, hot {hot}"
+        f"{', sorted columns' if srt else ''}): "
         f"{ms * 1e3:.1f} us/step rotated ({gbs(b_alg, ms)} GB/s), tile kernel "
         f"{kern * 1e3:.1f} us; rule ({rule['rule']}) {rule_ms * 1e3:.1f} us"
         + (f"; warm {warm_ms * 1e3:.1f} us, rule's warm {rule_warm_ms * 1e3:.1f} us"
            if warm_ms else "") + f"; rule's plan creation {np.median(auto_ms):.2f} ms")
     res = dict(b_alg=b_alg, ms=ms, kern=kern, warm_ms=warm_ms, warm_kern=warm_kern,
                ntiles=plans[0].ntiles, tile_items=plans[0].tile_items, ipt=ipt, stream=stream,
-               resid=resid, hot=hot, plan_ms=float(np.median(plan_ms)), ceilings=ceilings,
+               resid=resid, hot=hot, srt=srt, plan_ms=float(np.median(plan_ms)),
+               ceilings=ceilings,
                rule={"us_per_step": round(rule_ms * 1e3, 2), "gbs": gbs(b_alg, rule_ms),
                      "warm_us_per_step": (round(rule_warm_ms * 1e3, 2) if rule_warm_ms
                                           else None),
@@ -292,15 +304,16 @@ def bench_rmat_f64(A, x, dev, args, checks):
     return res
 
 
-def bench_ceilings(A, copies, ncol, nnz, ipt, stream, resid, args):
+def bench_ceilings(A, copies, ncol, nnz, ipt, stream, resid, args, srt=False):
     """The headline tile kernel's two halves on their own (probes.hip, at
     the same tile shape, residency and stream policy, rotated copies): the
     stream probe reads row_ptr, col and the values as the tile kernel stages
-    them and writes y; the gather probe reads col and gathers x.  Beside
-    them the plain product (no hot split) of the same plans."""
+    them and writes y; the gather probe reads col (the plan's tile-sorted
+    columns when srt) and gathers x.  Beside them the product of the same
+    plans (no hot split)."""
     import torch
     from merbit_amd.compare import probe
-    plans = make_plans(copies, ncol, nnz, ipt, 0)
+    plans = make_plans(copies, ncol, nnz, ipt, 0, srt=srt)
     for p in plans:
         p.set_stream(stream)
         p.set_residency(resid)
@@ -323,7 +336,8 @@ def bench_ceilings(A, copies, ncol, nnz, ipt, stream, resid, args):
            "gather_only_gathers_per_s": round(nnz / (res["gather"] * 1e-3) / 1e9, 1),
            "plain_product_us": round(res["plain"] * 1e3, 2),
            "stream_plus_gather_us": round((res["stream"] + res["gather"]) * 1e3, 2),
-           "shape": {"items_per_thread": ipt, "stream": stream, "tiles_per_cu": resid},
+           "shape": {"items_per_thread": ipt, "stream": stream, "tiles_per_cu": resid,
+                     "sorted_columns": srt},
            "what": "probes.hip: the tile kernel's staging loads alone (stream) and its column "
                    "loads plus x gathers alone (gather), G gathers/s"}
     log(f"ceilings: stream only {out['stream_only_us']} us ({out['stream_only_gbs']} GB/s), "
@@ -376,9 +390,10 @@ def bench_mixed(name, A, x, dev, args, checks, f64_ms=None, copies=1):
             xs, s_val, s_x, ydt = x32, (4 if kind == "f32" else 2), 4, torch.float32
         cps = make_copies(stored, xs, ydt)
         b = workloads.algorithmic_bytes(nrow, ncol, nnz, s_val, s_x, s_x)
-        ipt, stream, resid, hot = tune_shape(args, cps, ncol, nnz, kind,
-                                             name == "rmat" and not args.no_hot)
-        plans = make_plans(cps, ncol, nnz, ipt, hot, kind)
+        ipt, stream, resid, hot, srt = tune_shape(args, cps, ncol, nnz, kind,
+                                                  name == "rmat" and not args.no_hot,
+                                                  name == "rmat")
+        plans = make_plans(cps, ncol, nnz, ipt, hot, kind, srt)
         for p in plans:
             p.set_stream(stream)
             p.set_residency(resid)
@@ -391,7 +406,7 @@ def bench_mixed(name, A, x, dev, args, checks, f64_ms=None, copies=1):
                "us_per_step": round(ms * 1e3, 2),
                "tile_kernel_us": None if kern is None else round(kern * 1e3, 2),
                "items_per_thread": ipt, "stream": stream, "tiles_per_cu": resid,
-               "hot_columns": hot,
+               "hot_columns": hot, "sorted_columns": srt,
                "pct_hbm_peak": round(100 * b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 1)}
         if kind != "f64":
             rec["speedup_vs_f64"] = round(f64_ms / ms, 3)
@@ -447,7 +462,7 @@ def bench_mixed(name, A, x, dev, args, checks, f64_ms=None, copies=1):
             p.close()
         out[kind] = rec
         log(f"{name} {kind}: tuned {ms * 1e3:.1f} us/step (ipt {ipt}, {stream}, "
-            f"{resid or 'all'}/CU, hot {hot})"
+            f"{resid or 'all'}/CU, hot {hot}{', sorted' if srt else ''})"
             + ("" if kind == "f64" else f" x{f64_ms / ms:.2f} vs fp64")
             + ("" if kind == "f64" else
                f"; FP32 products {rec['acc_f32prod']['us_per_step']:.1f} us "
@@ -540,14 +555,26 @@ def bench_batch(dev, args):
                       torch.from_numpy(A.val).to(dev), torch.from_numpy(x).to(dev),
                       torch.empty(A.nrow, dtype=torch.float32, device=dev), A.ncol))
     ipt, tuned_us = autotune_batch(items, reps=3)
-    b = MerbitBatch(items, items_per_thread=ipt)
-    # tiles per CU, timed like the single-matrix shapes (first within 1 %)
+    # the autotuned shape, and the members' tile-sorted columns at each shape
+    # that has them; then tiles per CU, timed like the single-matrix shapes
+    # (first within 1 %)
+    forms = [(ipt, False)] + ([] if args.no_sort else [(i, True) for i in (8, 12, 16)])
     wgs = [args.residency] if args.residency is not None else [0, 3, 4, 5]
-    t_wg = {}
-    for wg in wgs:
-        b.set_residency(wg)
-        t_wg[wg] = min(time_steps([b.spmv], 5, 2)[0] for _ in range(3))
-    resid = next(wg for wg in wgs if t_wg[wg] <= 1.01 * min(t_wg.values()))
+    t_form = {}
+    for fi, fs in forms:
+        bb = MerbitBatch(items, items_per_thread=fi)
+        if fs and not bb.sort_cols(True):
+            bb.close()
+            continue
+        for wg in wgs:
+            bb.set_residency(wg)
+            t_form[(fi, fs, wg)] = min(time_steps([bb.spmv], 5, 2)[0] for _ in range(3))
+        bb.close()
+    best = min(t_form.values())
+    ipt, srt, resid = next(k for k in t_form if t_form[k] <= 1.01 * best)
+    b = MerbitBatch(items, items_per_thread=ipt)
+    if srt:
+        b.sort_cols(True)
     b.set_residency(resid)
     reps = 10
     g = b.graph(repeats=reps)
@@ -558,19 +585,23 @@ def bench_batch(dev, args):
     # per-matrix launches for comparison (same arithmetic, one launch pair each)
     from merbit_amd.spmv import MerbitPlan
     plans = [MerbitPlan(it[0], it[5], it[1].numel(), items_per_thread=ipt) for it in items]
-    for p in plans:
+    for p, it in zip(plans, items):
         p.set_residency(resid)
+        if srt:
+            p.sort_cols(it[1])
     calls = [lambda: [p.spmv(it[0], it[1], it[2], it[3], it[4]) for p, it in zip(plans, items)]]
     ms_loop, _ = time_steps(calls, max(5, args.steps // 20), 2)
     out = {"matrices": len(mats), "rows": int(sum(A.nrow for A in mats)), "nnz": int(nnz),
-           "autotuned_items_per_thread": ipt, "tiles_per_cu": resid or "all",
+           "items_per_thread": ipt, "autotuned_items_per_thread": forms[0][0],
+           "sorted_columns": srt, "tiles_per_cu": resid or "all",
            "tile_items": b.tile_items,
            "persistent_grid": b.grid, "gbs": gbs(nbytes, ms),
            "gflops": round(2 * nnz / (ms * 1e-3) / 1e9, 1), "us_per_batch": round(ms * 1e3, 1),
            "us_per_batch_separate_launches": round(ms_loop * 1e3, 1),
            "speedup_vs_separate_launches": round(ms_loop / ms, 2),
            "launch": f"one hipGraph of {reps} persistent batch products, per-product time"}
-    log(f"batch: {ms * 1e3:.1f} us per batch product ({out['gbs']} GB/s), separate launches "
+    log(f"batch: {ms * 1e3:.1f} us per batch product ({out['gbs']} GB/s; ipt {ipt}"
+        f"{', sorted columns' if srt else ''}, {resid or 'all'}/CU), separate launches "
         f"{ms_loop * 1e3:.1f} us")
     for p in plans:
         p.close()
@@ -1001,6 +1032,8 @@ def main():
                     help="skip the stream-only and gather-only probes of the headline kernel")
     ap.add_argument("--no-hot", action="store_true",
                     help="leave the hot-column split out of the tuned candidates")
+    ap.add_argument("--no-sort", action="store_true",
+                    help="leave tile-sorted columns out of the tuned candidates")
     ap.add_argument("--hot", type=int, default=None,
                     help="hot columns of the power-law products (0 = none); default: tuned")
     args = ap.parse_args()
@@ -1035,6 +1068,7 @@ def main():
         "plan_create_ms": round(h["plan_ms"], 3),
         "headline_rule": h["rule"],
         "hot_columns": h["hot"],
+        "sorted_columns": h["srt"],
         "ceilings": h["ceilings"],
     }
     lap = (None, None, None, 0)
@@ -1082,7 +1116,7 @@ def main():
                                "(a=.57,b=c=.19), 3 rotated copies",
                    "nrow": A.nrow, "nnz": A.nnz, "items_per_thread": h["ipt"],
                    "stream_loads": h["stream"], "tiles_per_cu": h["resid"] or "all",
-                   "hot_columns": h["hot"],
+                   "hot_columns": h["hot"], "sorted_columns": h["srt"],
                    "wg_size": 256, "copies_rotated": max(1, args.copies),
                    "timed_as": "eager launches" if args.no_graph else "one HIP graph of the steps"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

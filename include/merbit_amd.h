@@ -160,6 +160,26 @@ int mbx_plan_compress_cols(mbx_plan* plan, const int32_t* d_col, void* stream);
 int mbx_plan_split_hot(mbx_plan* plan, const int32_t* d_col, int32_t max_hot, void* stream,
                        double* covered);
 
+/* Tile-sorted gathers (no reference counterpart; the reference gathers
+ * x[col] in the order of the nonzeros, include/tile.cuh:293-305).  Keeps a
+ * copy of the column array in which every tile's columns are sorted and
+ * laid out in the order the tile kernel gathers them, each word carrying the
+ * element's position in its tile: each gather instruction then reads 64
+ * consecutive sorted columns, which share x lines, so the vector L1 sees
+ * fewer requests (uniform random and power-law columns, whose tiles touch
+ * some x lines many times).  The gathered values go back to their elements'
+ * product slots in LDS, so the products and results are bit-identical to the
+ * plain product's.  Applies to plans with items_per_thread 8, 12 or 16 and
+ * to products on the vector staging path; other products read d_col as
+ * before, so d_col must stay the pattern's column array.  MBX_EUNSUPPORTED
+ * (nothing kept) for other shapes, ncol above 2^21 (items_per_thread 8) or
+ * 2^20 (12, 16), or a plan with 16-bit columns or a hot split (which in turn
+ * refuse a sorted plan); MBX_EINVAL when a column lies outside [0, ncol).
+ * d_col NULL drops the copy.  Allocates 4*nnz bytes and synchronises
+ * `stream` (and the device, when it replaces a copy); clones do not inherit
+ * it. */
+int mbx_plan_sort_cols(mbx_plan* plan, const int32_t* d_col, void* stream);
+
 /* Optional timing hook: when ev_begin and ev_end (hipEvent_t) are both
  * non-NULL, the next spmv or spmm call on this plan attaches them to its tile
  * kernel's dispatch (hipExtLaunchKernel): their interval is that kernel's
@@ -257,6 +277,14 @@ int mbx_batch_set_stream(mbx_batch* batch, int32_t mode);
 /* Tiles resident per CU for the batch's launches (mbx_plan_set_residency);
  * the persistent grid shrinks to wg_per_cu per CU. */
 int mbx_batch_set_residency(mbx_batch* batch, int32_t wg_per_cu);
+/* Tile-sorted gathers for every member (as mbx_plan_sort_cols; no reference
+ * counterpart): enable 1 builds the members' sorted column copies from the
+ * items' col arrays, 0 drops them.  Batches with items_per_thread 8, 12 or
+ * 16 whose members all suit the vector staging path; MBX_EUNSUPPORTED
+ * otherwise or when a member's ncol exceeds the limit of mbx_plan_sort_cols,
+ * MBX_EINVAL when a column lies outside [0, ncol).  Results stay
+ * bit-identical.  Allocates 4 bytes per nonzero and synchronises `stream`. */
+int mbx_batch_sort_cols(mbx_batch* batch, int32_t enable, void* stream);
 
 /* HIP graphs of repeated products (the reference's timed loop of
  * maxiter*perrnd launches, include/tile.cuh:596-603): `repeats` back-to-back
@@ -315,17 +343,22 @@ int mbx_plan_stats(const mbx_plan* plan, const int32_t* d_row_ptr, mbx_pattern_s
  * but compiles one SIGMA and BLOCK (CMakeLists.txt:4-6) for every matrix.
  * Features: tiles at 8 items per thread, the widest column range of one such
  * tile, x bytes, and the share of the entries in the 4096 (FP64 x) or 7900
- * (FP32 x) most frequent columns.  Rules, first match:
- *   MBX_RULE_FEW_TILES       fewer than 16 tiles per CU: 4 items per thread
- *                            (2 when that gives fewer tiles than CUs);
+ * (FP32 x) most frequent columns.  "Sorted" below: 16 items per thread and
+ * tile-sorted gathers (sort_cols, mbx_plan_sort_cols), taken when the
+ * pattern is not local and ncol is at most 2^20.  Rules, first match:
+ *   MBX_RULE_FEW_TILES       fewer than 16 tiles per CU: sorted when that
+ *                            still gives 2 tiles per CU, else 4 items per
+ *                            thread (2 when that gives fewer tiles than CUs);
  *   MBX_RULE_LOCAL           every tile spans < 65536 columns (banded):
  *                            8 (FP64) or 12 (FP32, BF16) items per thread, all tiles
  *                            resident, 16-bit columns, FP32 products and
  *                            FP64 sums for FP32/BF16 values;
- *   MBX_RULE_X_FITS          x of at most 2 MiB: 8, at most 4 tiles per CU;
+ *   MBX_RULE_X_FITS          x of at most 2 MiB: sorted, else 8 items per
+ *                            thread and at most 4 tiles per CU;
  *   MBX_RULE_HOT_COLUMNS     the most frequent columns hold >= 1/4 of the
- *                            entries: 8, the hot-column split (FP32/BF16
- *                            values: FP32 products, FP64 sums);
+ *                            entries: sorted, else 8 and the hot-column
+ *                            split; FP32/BF16 values with FP32 products and
+ *                            FP64 sums;
  *   MBX_RULE_SPREAD_COLUMNS  otherwise: 8, 3 tiles per CU, and the
  *                            column-blocked form recommended (column_blocks
  *                            for mbx_colblock_create).
@@ -334,7 +367,8 @@ int mbx_plan_stats(const mbx_plan* plan, const int32_t* d_row_ptr, mbx_pattern_s
  * 1/16 of the chosen tile and items per thread at most 8.  The stream policy stays MBX_STREAM_AUTO.
  * Synchronises `stream` (reads the features back).  mbx_plan_create_auto
  * makes the plan with the choice applied (row-aligned tiles, items per
- * thread, residency, accumulation, 16-bit columns, hot split);
+ * thread, residency, accumulation, 16-bit columns, sorted gathers, hot
+ * split; a pattern whose columns the sort refuses keeps the plain gathers);
  * column_blocks is a recommendation for the caller (a plan is a CSR
  * product). */
 enum {
@@ -354,6 +388,7 @@ typedef struct mbx_launch_choice {
   int32_t hot_columns;       /* mbx_plan_split_hot max_hot (0: none) */
   int32_t column_blocks;     /* > 0: the column-blocked form with this many blocks */
   int32_t row_tiles;         /* 1: row-aligned tiles (mbx_plan_create_rows) */
+  int32_t sort_cols;         /* 1: tile-sorted gathers (mbx_plan_sort_cols) */
   /* the features */
   int32_t cus, ntiles8, max_tile_span, max_row;
   double x_bytes, hot_share;

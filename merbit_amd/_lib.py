@@ -18,6 +18,7 @@ ABI_SYMBOLS = (
     "mbx_plan_info",
     "mbx_plan_partition", "mbx_plan_set_stream", "mbx_batch_set_stream",
     "mbx_plan_set_residency", "mbx_batch_set_residency", "mbx_plan_compress_cols",
+    "mbx_plan_sort_cols", "mbx_batch_sort_cols",
     "mbx_plan_set_accumulation",
     "mbx_plan_set_events", "mbx_spmv_f64", "mbx_spmv_f32", "mbx_spmv_bf16",
     "mbx_spmm_workspace_bytes", "mbx_spmm_f32",
@@ -75,7 +76,7 @@ class LaunchChoice(ctypes.Structure):
                 ("stream_mode", ctypes.c_int32), ("wg_per_cu", ctypes.c_int32),
                 ("accumulation", ctypes.c_int32), ("compress_cols", ctypes.c_int32),
                 ("hot_columns", ctypes.c_int32), ("column_blocks", ctypes.c_int32),
-                ("row_tiles", ctypes.c_int32),
+                ("row_tiles", ctypes.c_int32), ("sort_cols", ctypes.c_int32),
                 ("cus", ctypes.c_int32), ("ntiles8", ctypes.c_int32),
                 ("max_tile_span", ctypes.c_int32), ("max_row", ctypes.c_int32),
                 ("x_bytes", ctypes.c_double), ("hot_share", ctypes.c_double)]
@@ -154,6 +155,8 @@ def lib() -> ctypes.CDLL:
     L.mbx_autotune_batch.argtypes = [vp, i32, i32, i32, vp, ctypes.POINTER(i32),
                                      ctypes.POINTER(f32)]
     L.mbx_plan_compress_cols.argtypes = [vp, vp, vp]
+    L.mbx_plan_sort_cols.argtypes = [vp, vp, vp]
+    L.mbx_batch_sort_cols.argtypes = [vp, i32, vp]
     L.mbx_plan_set_accumulation.argtypes = [vp, i32]
     L.mbx_colblock_create.argtypes = [vp, vp, i32, i32, i32, i32, i32, vp, ctypes.POINTER(vp)]
     L.mbx_colblock_destroy.argtypes = [vp]

@@ -94,6 +94,16 @@ class MerbitBatch:
         (mbx_batch_set_residency); 0 = as many as fit."""
         check(lib().mbx_batch_set_residency(self._h, int(wg_per_cu)), "mbx_batch_set_residency")
 
+    def sort_cols(self, enable: bool = True, stream=None) -> bool:
+        """Tile-sorted gathers for every member (mbx_batch_sort_cols); False
+        when the batch's shape or a member rules them out.  Results are
+        bit-identical either way."""
+        st = lib().mbx_batch_sort_cols(self._h, int(bool(enable)), _stream_handle(stream))
+        if st == -3:  # MBX_EUNSUPPORTED
+            return False
+        check(st, "mbx_batch_sort_cols")
+        return bool(enable)
+
     def spmv(self, alpha: float = 1.0, beta: float = 0.0, stream=None) -> None:
         check(lib().mbx_batch_spmv(self._h, alpha, beta, _stream_handle(stream)),
               "mbx_batch_spmv")

@@ -9,7 +9,9 @@
 //                 rows ending in the tile are written (the y stream);
 //   gather probe  the tile's column groups and the x gathers of its
 //                 nonzeros through the same buffer resource, summed per
-//                 workgroup and written once per tile.
+//                 workgroup and written once per tile; on a plan with
+//                 tile-sorted columns (mbx_plan_sort_cols) the plan's sorted
+//                 words, gathered in their order as the product does.
 // Each writes what it reads into a result so no load is dead; the results
 // themselves are not meaningful.
 
@@ -54,15 +56,15 @@ __global__ __launch_bounds__(256) void stream_probe_kernel(const int32_t* __rest
   for (int i = threadIdx.x; i < R.rows(); i += 256) y[R.r0() + i] = t;
 }
 
-template <int IPT>
+template <int IPT, class Q>
 __global__ __launch_bounds__(256) void gather_probe_kernel(const int32_t* __restrict__ col,
                                                            const double* __restrict__ x, int ncol,
                                                            const int4* __restrict__ coord,
                                                            int ntiles, double* __restrict__ y) {
   __shared__ double s_w[4];
   const int tile = mbx_detail::xcd_tile(blockIdx.x, ntiles);
-  constexpr int NG = TileRegs<256, IPT, P>::NG;
-  TileRegs<256, IPT, P> R;
+  constexpr int NG = TileRegs<256, IPT, Q>::NG;
+  TileRegs<256, IPT, Q> R;
   mbx_detail::tile_coords(R, coord, tile);
   const int k0 = R.k0(), nnz_t = R.nnz();
   double s = 0.0;
@@ -71,7 +73,7 @@ __global__ __launch_bounds__(256) void gather_probe_kernel(const int32_t* __rest
 #pragma unroll
     for (int j = 0; j < NG; ++j)
       if (j == 0 || j * 256 < nkg)
-        mbx_detail::load_group<P::kIndexNT>(col, min(kb + (int)threadIdx.x + j * 256, klast),
+        mbx_detail::load_group<Q::kIndexNT>(col, min(kb + (int)threadIdx.x + j * 256, klast),
                                             R.cg[j]);
     mbx_detail::tile_issue_gathers(R, x, ncol);
 #pragma unroll
@@ -91,10 +93,15 @@ int launch(const mbx_plan* p, int which, const int32_t* rp, const int32_t* col,
   if (which == 0) {
     mbx_detail::allow_lds(stream_probe_kernel<IPT>, lds);
     stream_probe_kernel<IPT><<<p->ntiles, 256, lds, s>>>(rp, col, val, p->d_coord, p->ntiles, y);
+  } else if (p->d_scol) {
+    using Q = mbx_detail::SortX<P>;
+    mbx_detail::allow_lds(gather_probe_kernel<IPT, Q>, lds);
+    gather_probe_kernel<IPT, Q><<<p->ntiles, 256, lds, s>>>(p->d_scol, x, p->ncol, p->d_coord,
+                                                            p->ntiles, y);
   } else {
-    mbx_detail::allow_lds(gather_probe_kernel<IPT>, lds);
-    gather_probe_kernel<IPT><<<p->ntiles, 256, lds, s>>>(col, x, p->ncol, p->d_coord,
-                                                         p->ntiles, y);
+    mbx_detail::allow_lds(gather_probe_kernel<IPT, P>, lds);
+    gather_probe_kernel<IPT, P><<<p->ntiles, 256, lds, s>>>(col, x, p->ncol, p->d_coord,
+                                                            p->ntiles, y);
   }
   return hipGetLastError() == hipSuccess ? MBX_OK : MBX_EHIP;
 }
@@ -104,8 +111,8 @@ int launch(const mbx_plan* p, int which, const int32_t* rp, const int32_t* col,
 extern "C" {
 
 // which: 0 = stream probe (y: nrow doubles), 1 = gather probe (y: ntiles
-// doubles).  The plan must have 8 or 12 items per thread; its residency cap
-// applies.  row_ptr, col and val 16-byte aligned.
+// doubles).  The plan must have 8, 12 or 16 items per thread; its residency
+// cap applies.  row_ptr, col and val 16-byte aligned.
 int mbp_probe(const mbx_plan* p, int32_t which, const int32_t* rp, const int32_t* col,
               const double* val, const double* x, double* y, void* stream) {
   if (!p || !y || (which != 0 && which != 1)) return MBX_EINVAL;
@@ -118,6 +125,7 @@ int mbp_probe(const mbx_plan* p, int32_t which, const int32_t* rp, const int32_t
   hipStream_t s = (hipStream_t)stream;
   if (p->ipt == 8) return launch<8>(p, which, rp, col, val, x, y, s);
   if (p->ipt == 12) return launch<12>(p, which, rp, col, val, x, y, s);
+  if (p->ipt == 16) return launch<16>(p, which, rp, col, val, x, y, s);
   return MBX_EUNSUPPORTED;
 }
 

@@ -109,6 +109,24 @@ __global__ __launch_bounds__(WG) void batch_spmv_kernel(
   }
 }
 
+// Tile-sorted column words of every member (mbx_batch_sort_cols): one
+// workgroup per global tile reads its member's columns from `mats` and writes
+// the words to the member's array in `sorted` (see sort_tile_cols).
+template <int IPT>
+__global__ __launch_bounds__(256) void batch_sort_cols_kernel(
+    const BatchMat* __restrict__ mats, const BatchMat* __restrict__ sorted,
+    const int* __restrict__ tile_base, int count, const int4* __restrict__ coord,
+    int* __restrict__ bad) {
+  __shared__ uint32_t s_key[1 << mbx_detail::sort_slot_bits<256, IPT>()];
+  __shared__ int s_wsum[4];
+  const int g = blockIdx.x;
+  const int m = batch_matrix_of(tile_base, count, g);
+  const BatchMat M = mats[m];
+  const int4* c = coord + M.coord_base + (g - M.tile_base);
+  mbx_detail::sort_tile_cols<256, IPT>(M.col, M.ncol, c[0].y, c[1].y,
+                                       const_cast<int32_t*>(sorted[m].col), bad, s_key, s_wsum);
+}
+
 template <class P>
 __global__ __launch_bounds__(256) void batch_fixup_kernel(
     const BatchMat* __restrict__ mats, const int* __restrict__ tile_base, int count,
@@ -142,20 +160,33 @@ struct mbx_batch {
   int32_t stream_mode;  // MBX_STREAM_*
   int32_t wg_per_cu;    // resident tiles per CU; 0 = as many as fit
   int32_t cus;          // compute units of the device
+  // mbx_batch_sort_cols: the member table with every column array replaced
+  // by the member's tile-sorted words (in d_scol), and the resident grid of
+  // the kernel that reads them
+  BatchMat* d_mats_sorted;
+  int32_t* d_scol;
+  int32_t grid_sorted;
 };
 
 namespace {
 
 // Resident workgroups of the kernel variant the batch will launch.
 template <class P, int IPT>
-int batch_grid(int device, bool vec) {
+int batch_grid(int device, bool vec, bool sorted) {
+  if constexpr (!P::kSorted && (IPT == 8 || IPT == 12 || IPT == 16)) {
+    if (sorted) return batch_grid<mbx_detail::SortX<P>, IPT>(device, true, false);
+  }
   int per_cu = 0, cus = 0;
   const size_t lds = TileSmem<256, IPT, P>::bytes();
-  const hipError_t e =
-      vec ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per_cu, batch_spmv_kernel<256, IPT, true, P>, 256, lds)
-          : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per_cu, batch_spmv_kernel<256, IPT, false, P>, 256, lds);
+  hipError_t e;
+  if constexpr (P::kSorted)  // vector staging only
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, batch_spmv_kernel<256, IPT, true, P>, 256, lds);
+  else
+    e = vec ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                  &per_cu, batch_spmv_kernel<256, IPT, true, P>, 256, lds)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                  &per_cu, batch_spmv_kernel<256, IPT, false, P>, 256, lds);
   if (e != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
     return 0;
@@ -170,7 +201,29 @@ void batch_launch(const mbx_batch* b, hipStream_t s, typename P::scal_t alpha,
   if (b->wg_per_cu > 0) grid = std::min(grid, b->wg_per_cu * b->cus);  // persistent
   const size_t lds = mbx_detail::residency_lds(TileSmem<256, IPT, P>::bytes(), b->wg_per_cu);
   const bool nt = mbx_detail::stream_nontemporal(b->stream_mode, b->footprint);
-  if (b->vec && nt) {
+  bool launched = false;
+  if constexpr (IPT == 8 || IPT == 12 || IPT == 16) {  // the members' tile-sorted columns
+    if (b->vec && b->d_mats_sorted) {
+      int sgrid = std::min(b->grid_sorted, b->total_tiles);
+      if (b->wg_per_cu > 0) sgrid = std::min(sgrid, b->wg_per_cu * b->cus);
+      if (nt) {
+        using Q = mbx_detail::SortX<mbx_detail::Streamed<P>>;
+        mbx_detail::allow_lds(batch_spmv_kernel<256, IPT, true, Q>, lds);
+        batch_spmv_kernel<256, IPT, true, Q><<<sgrid, 256, lds, s>>>(
+            b->d_mats_sorted, b->d_tile_base, b->count, b->total_tiles, b->d_coord, b->d_carry,
+            alpha, beta, bz);
+      } else {
+        using Q = mbx_detail::SortX<P>;
+        mbx_detail::allow_lds(batch_spmv_kernel<256, IPT, true, Q>, lds);
+        batch_spmv_kernel<256, IPT, true, Q><<<sgrid, 256, lds, s>>>(
+            b->d_mats_sorted, b->d_tile_base, b->count, b->total_tiles, b->d_coord, b->d_carry,
+            alpha, beta, bz);
+      }
+      launched = true;
+    }
+  }
+  if (launched) {
+  } else if (b->vec && nt) {
     mbx_detail::allow_lds(batch_spmv_kernel<256, IPT, true, mbx_detail::Streamed<P>>, lds);
     batch_spmv_kernel<256, IPT, true, mbx_detail::Streamed<P>><<<grid, 256, lds, s>>>(
         b->d_mats, b->d_tile_base, b->count, b->total_tiles, b->d_coord, b->d_carry, alpha, beta,
@@ -194,13 +247,13 @@ void batch_launch(const mbx_batch* b, hipStream_t s, typename P::scal_t alpha,
 
 template <class P>
 int batch_dispatch(const mbx_batch* b, hipStream_t s, double alpha, double beta, bool grid_only,
-                   int device) {
+                   int device, bool sorted) {
   using S = typename P::scal_t;
   switch (b->ipt) {
-#define MBX_CASE(I)                                                  \
-  case I:                                                           \
-    if (grid_only) return batch_grid<P, I>(device, b->vec);         \
-    batch_launch<P, I>(b, s, (S)alpha, (S)beta);                    \
+#define MBX_CASE(I)                                                    \
+  case I:                                                             \
+    if (grid_only) return batch_grid<P, I>(device, b->vec, sorted);   \
+    batch_launch<P, I>(b, s, (S)alpha, (S)beta);                      \
     return 0;
     MBX_CASE(2) MBX_CASE(4) MBX_CASE(6) MBX_CASE(8) MBX_CASE(10) MBX_CASE(12) MBX_CASE(14)
     MBX_CASE(16)
@@ -210,11 +263,13 @@ int batch_dispatch(const mbx_batch* b, hipStream_t s, double alpha, double beta,
 }
 
 int batch_run(const mbx_batch* b, hipStream_t s, double alpha, double beta, bool grid_only,
-              int device) {
+              int device, bool sorted = false) {
   switch (b->kind) {
-    case MBX_VALUE_F64: return batch_dispatch<F64>(b, s, alpha, beta, grid_only, device);
-    case MBX_VALUE_F32: return batch_dispatch<F32C>(b, s, alpha, beta, grid_only, device);
-    case MBX_VALUE_BF16: return batch_dispatch<BF16C>(b, s, alpha, beta, grid_only, device);
+    case MBX_VALUE_F64: return batch_dispatch<F64>(b, s, alpha, beta, grid_only, device, sorted);
+    case MBX_VALUE_F32:
+      return batch_dispatch<F32C>(b, s, alpha, beta, grid_only, device, sorted);
+    case MBX_VALUE_BF16:
+      return batch_dispatch<BF16C>(b, s, alpha, beta, grid_only, device, sorted);
     default: return -1;
   }
 }
@@ -290,6 +345,9 @@ int mbx_batch_create(const mbx_batch_item* items, int32_t count, int32_t value_k
   b->d_tile_base = nullptr;
   b->d_coord = nullptr;
   b->d_carry = nullptr;
+  b->d_mats_sorted = nullptr;
+  b->d_scol = nullptr;
+  b->grid_sorted = 0;
   hipStream_t s = (hipStream_t)stream;
   if (hipMalloc((void**)&b->d_mats, sizeof(BatchMat) * (size_t)count) != hipSuccess ||
       hipMalloc((void**)&b->d_tile_base, sizeof(int) * (size_t)count) != hipSuccess ||
@@ -347,7 +405,93 @@ void mbx_batch_destroy(mbx_batch* b) {
   if (b->d_tile_base) (void)hipFree(b->d_tile_base);
   if (b->d_coord) (void)hipFree(b->d_coord);
   if (b->d_carry) (void)hipFree(b->d_carry);
+  if (b->d_mats_sorted) (void)hipFree(b->d_mats_sorted);
+  if (b->d_scol) (void)hipFree(b->d_scol);
   delete b;
+}
+
+int mbx_batch_sort_cols(mbx_batch* b, int32_t enable, void* stream) {
+  if (!b) return MBX_EINVAL;
+  hipStream_t s = (hipStream_t)stream;
+  if (b->d_mats_sorted || b->d_scol) {  // products on any stream may still read them
+    (void)hipDeviceSynchronize();
+    if (b->d_mats_sorted) (void)hipFree(b->d_mats_sorted);
+    if (b->d_scol) (void)hipFree(b->d_scol);
+    b->d_mats_sorted = nullptr;
+    b->d_scol = nullptr;
+  }
+  if (!enable) return MBX_OK;
+  if (b->ipt != 8 && b->ipt != 12 && b->ipt != 16) return MBX_EUNSUPPORTED;
+  if (!b->vec) return MBX_EUNSUPPORTED;  // the scalar staging path reads the columns as given
+  const int sb = b->ipt == 8 ? mbx_detail::sort_slot_bits<256, 8>()
+                             : mbx_detail::sort_slot_bits<256, 16>();  // 12 for IPT 12 too
+  std::vector<BatchMat> h((size_t)b->count);
+  if (hipMemcpy(h.data(), b->d_mats, sizeof(BatchMat) * (size_t)b->count,
+                hipMemcpyDeviceToHost) != hipSuccess)
+    return MBX_EHIP;
+  size_t total = 0;
+  std::vector<size_t> off((size_t)b->count);
+  for (int m = 0; m < b->count; ++m) {
+    if ((long long)h[(size_t)m].ncol > (1ll << (32 - sb))) return MBX_EUNSUPPORTED;
+    off[(size_t)m] = total;
+    total += ((size_t)h[(size_t)m].nnz + 3) / 4 * 4;  // whole 16-byte groups
+  }
+  if (b->total_tiles == 0 || total == 0) return MBX_OK;
+  int device = 0;
+  if (hipGetDevice(&device) != hipSuccess) return MBX_EHIP;
+  const int grid = batch_run(b, s, 0.0, 0.0, true, device, true);
+  if (grid <= 0) return MBX_EHIP;
+  BatchMat* d_sorted = nullptr;
+  int32_t* sc = nullptr;
+  int* d_bad = nullptr;
+  if (hipMalloc((void**)&d_sorted, sizeof(BatchMat) * (size_t)b->count) != hipSuccess ||
+      hipMalloc((void**)&sc, sizeof(int32_t) * total) != hipSuccess ||
+      hipMalloc((void**)&d_bad, sizeof(int)) != hipSuccess) {
+    (void)hipGetLastError();
+    if (d_sorted) (void)hipFree(d_sorted);
+    if (sc) (void)hipFree(sc);
+    return MBX_ENOMEM;
+  }
+  std::vector<BatchMat> hs = h;
+  for (int m = 0; m < b->count; ++m) hs[(size_t)m].col = sc + off[(size_t)m];
+  int st = MBX_OK, bad = 0;
+  if (hipMemcpyAsync(d_sorted, hs.data(), sizeof(BatchMat) * (size_t)b->count,
+                     hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemsetAsync(sc, 0, sizeof(int32_t) * total, s) != hipSuccess ||
+      hipMemsetAsync(d_bad, 0, sizeof(int), s) != hipSuccess)
+    st = MBX_EHIP;
+  if (st == MBX_OK) {
+    switch (b->ipt) {
+      case 8:
+        batch_sort_cols_kernel<8><<<b->total_tiles, 256, 0, s>>>(
+            b->d_mats, d_sorted, b->d_tile_base, b->count, b->d_coord, d_bad);
+        break;
+      case 12:
+        batch_sort_cols_kernel<12><<<b->total_tiles, 256, 0, s>>>(
+            b->d_mats, d_sorted, b->d_tile_base, b->count, b->d_coord, d_bad);
+        break;
+      default:
+        batch_sort_cols_kernel<16><<<b->total_tiles, 256, 0, s>>>(
+            b->d_mats, d_sorted, b->d_tile_base, b->count, b->d_coord, d_bad);
+        break;
+    }
+    // the host table dies here: wait for the copy that reads it
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      st = MBX_EHIP;
+  }
+  (void)hipFree(d_bad);
+  if (st == MBX_OK && bad) st = MBX_EINVAL;  // a column outside [0, ncol)
+  if (st != MBX_OK) {
+    (void)hipFree(d_sorted);
+    (void)hipFree(sc);
+    return st;
+  }
+  b->d_mats_sorted = d_sorted;
+  b->d_scol = sc;
+  b->grid_sorted = grid;
+  return MBX_OK;
 }
 
 int mbx_batch_info(const mbx_batch* b, int32_t* total_tiles, int32_t* tile_items,

@@ -162,6 +162,7 @@ int split_hot_counted(mbx_plan* p, const int32_t* d_col, int32_t max_hot,
   drop_hot(p);
   if (!d_col || max_hot == 0 || p->nnz == 0 || p->ncol == 0) return MBX_OK;
   if (max_hot > kMaxHotColumns) return MBX_EUNSUPPORTED;
+  if (p->d_scol) return MBX_EUNSUPPORTED;  // the sorted columns exclude the split
   const int ncol = p->ncol, nnz = p->nnz;
   const int nhot = max_hot < ncol ? max_hot : ncol;
   const int padded = (nnz + 3) / 4 * 4;

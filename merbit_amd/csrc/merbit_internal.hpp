@@ -40,6 +40,9 @@ struct mbx_plan {
   void* d_xh;
   int32_t nhot;
   int32_t cus;  // compute units of the plan's device (persistent grids)
+  // mbx_plan_sort_cols: every tile's columns sorted, in gather order, with
+  // the element's tile position in the low bits (nnz padded to whole groups)
+  int32_t* d_scol;
 };
 
 // Most hot columns a plan keeps (mbx_plan_split_hot): 128 KB of FP32 x.

@@ -8,7 +8,9 @@
 // the product (measured on MI355X, warm and rotated copies):
 //   * few tiles (under 16 per CU at 8 items per thread): the launch is
 //     short and unevenly filled, 4 items per thread give twice the tiles
-//     (2 when even that leaves CUs without a tile);
+//     (2 when even that leaves CUs without a tile), unless tile-sorted
+//     gathers (below) still have 2 tiles per CU (a uniform random batch
+//     member of 95160 rows: 14.9 against 16.0 us);
 //   * local columns (every tile of 2048 merge items spans < 65536
 //     columns, e.g. banded): the gathers hit L2 and the stream is the bound:
 //     every tile resident, 16-bit column offsets, and for FP32 / bfloat16
@@ -18,14 +20,20 @@
 //     8 items per thread for FP64 values, 12 for FP32 and bfloat16 values
 //     (their shorter values leave a tile's loads short; with 4-byte products
 //     the larger tile still fits: FP32 kernel 155.5 -> 151.9 us);
-//   * x of at most 2 MiB: the gathers stay in L2; 8 items per thread, at
-//     most 4 tiles per CU;
+//   * x of at most 2 MiB: the gathers stay in L2; tile-sorted gathers
+//     (below), else 8 items per thread, at most 4 tiles per CU;
 //   * larger x with a hot column set (the kHot* most frequent columns hold
 //     at least a quarter of the entries, power-law patterns): the product is
-//     bound by the vector L1's miss slots; the hot values go to LDS
-//     (mbx_plan_split_hot) and the tiles per CU follow from that LDS; FP32 /
-//     bfloat16 values with FP32 products, whose smaller tile LDS leaves room
-//     for more hot values at 3 tiles per CU;
+//     bound by the vector L1's miss slots; tile-sorted gathers, else the hot
+//     values in LDS (mbx_plan_split_hot) with the tiles per CU following
+//     from that LDS; FP32 / bfloat16 values with FP32 products, whose
+//     smaller tile LDS leaves room for more hot values at 3 tiles per CU;
+//   where a pattern is not local and ncol fits the sorted words (2^20 at 16
+//   items per thread), tile-sorted gathers (mbx_plan_sort_cols) at 16 items
+//   per thread: each gather instruction reads 64 consecutive sorted columns
+//   of its tile, so lines shared within the tile are requested once (warm
+//   R-MAT FP64 85.8 -> 76.6 us against the hot split, FP32 67.4 -> 62.1;
+//   a uniform random batch member of 194898 rows 57.9 -> 50.0);
 //   * larger x without one (uniform random graphs): 3 tiles per CU, and the
 //     column-blocked form (mbx_colblock_*) is recommended with slices of at
 //     most 3.5 MiB of x.
@@ -56,6 +64,11 @@ constexpr double kHotShare = 0.25;
 constexpr long long kXFitsBytes = 2ll << 20;
 constexpr long long kSliceBytes = (7ll << 20) / 2;  // column-blocked form's x slice
 constexpr int kFewTilesPerCu = 16;
+// Tile-sorted gathers: 16 items per thread (the slot bits leave 20 for the
+// column), and the few-tiles rule takes them from 2 tiles per CU on.
+constexpr int kSortIpt = 16;
+constexpr long long kSortMaxCols = 1ll << 20;
+constexpr int kSortMinTilesPerCu = 2;
 // Row-aligned tiles at 12 items per thread measured slower (banded BF16 /
 // FP32 tile kernel +8 %, against +-1 % at 8): the rule keeps them to <= 8.
 constexpr int kRowTilesMaxIpt = 8;
@@ -211,9 +224,17 @@ int launch_rule(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow, in
     c.max_row = max_row;
   }
   const bool local = nnz > 0 && c.max_tile_span < 65536;
+  const bool sortable = nnz > 0 && !local && ncol <= kSortMaxCols;
+  auto sorted = [&c]() {
+    c.items_per_thread = kSortIpt;
+    c.sort_cols = 1;
+  };
   if (c.ntiles8 < kFewTilesPerCu * cus) {
-    // twice the tiles at 4 items per thread; 2 when even those leave CUs idle
-    c.items_per_thread = 2ll * c.ntiles8 < cus ? 2 : 4;
+    // tiles of 16 items per thread are half those of 8
+    if (sortable && (long long)c.ntiles8 >= 2ll * kSortMinTilesPerCu * cus)
+      sorted();
+    else  // twice the tiles at 4 items per thread; 2 when even those leave CUs idle
+      c.items_per_thread = 2ll * c.ntiles8 < cus ? 2 : 4;
     c.rule = MBX_RULE_FEW_TILES;
   } else if (local) {
     c.items_per_thread = f64 ? 8 : 12;
@@ -221,12 +242,20 @@ int launch_rule(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow, in
     if (!f64) c.accumulation = MBX_ACC_F32_PRODUCTS;
     c.rule = MBX_RULE_LOCAL;
   } else if ((long long)ncol * sx <= kXFitsBytes) {
-    c.items_per_thread = 8;
-    c.wg_per_cu = 4;
+    if (sortable) {
+      sorted();
+    } else {
+      c.items_per_thread = 8;
+      c.wg_per_cu = 4;
+    }
     c.rule = MBX_RULE_X_FITS;
   } else if (c.hot_share >= kHotShare) {
-    c.items_per_thread = 8;
-    c.hot_columns = hot;
+    if (sortable) {
+      sorted();
+    } else {
+      c.items_per_thread = 8;
+      c.hot_columns = hot;
+    }
     if (!f64) c.accumulation = MBX_ACC_F32_PRODUCTS;
     c.rule = MBX_RULE_HOT_COLUMNS;
   } else {
@@ -286,6 +315,13 @@ int mbx_plan_create_auto(const int32_t* d_row_ptr, const int32_t* d_col, int32_t
     st = mbx_plan_compress_cols(p, d_col, stream);
     if (st == MBX_EUNSUPPORTED) {  // cannot happen for a local pattern; keep 32-bit columns
       c.compress_cols = 0;
+      st = MBX_OK;
+    }
+  }
+  if (st == MBX_OK && c.sort_cols) {
+    st = mbx_plan_sort_cols(p, d_col, stream);
+    if (st == MBX_EUNSUPPORTED || st == MBX_EINVAL) {  // columns out of range: plain gathers
+      c.sort_cols = 0;
       st = MBX_OK;
     }
   }

@@ -188,6 +188,21 @@ __global__ __launch_bounds__(WG) void tile_spmv_hot_kernel(
   }
 }
 
+// Tile-sorted column words of a plan (mbx_plan_sort_cols), one workgroup
+// per tile (see sort_tile_cols).
+template <int IPT>
+__global__ __launch_bounds__(256) void sort_cols_kernel(const int32_t* __restrict__ col,
+                                                        int ncol,
+                                                        const int4* __restrict__ coord,
+                                                        int32_t* __restrict__ scol,
+                                                        int* __restrict__ bad) {
+  __shared__ uint32_t s_key[1 << mbx_detail::sort_slot_bits<256, IPT>()];
+  __shared__ int s_wsum[4];
+  const int t = blockIdx.x;
+  mbx_detail::sort_tile_cols<256, IPT>(col, ncol, coord[t].y, coord[t + 1].y, scol, bad, s_key,
+                                       s_wsum);
+}
+
 // Carry fixup, one thread per tile (see fixup_tile).
 template <class P>
 __global__ __launch_bounds__(256) void carry_fixup_kernel(
@@ -278,6 +293,26 @@ hipError_t launch_tile(const mbx_plan* p, hipStream_t s, const int32_t* rp,
         mbx_detail::allow_lds(tile_spmv_kernel<256, IPT, true, Q>, lds);
         timed_launch(p, tile_spmv_kernel<256, IPT, true, Q>, p->ntiles, 256, lds, s, rp, c16, val,
                      x, y, p->d_coord, p->d_carry, p->ntiles, p->ncol, alpha, beta, bz);
+      }
+      launched = true;
+    }
+  }
+  // the plan's tile-sorted columns (read in place of the caller's)
+  constexpr bool kSortShape = IPT == 8 || IPT == 12 || IPT == 16;
+  if constexpr (kSortShape) {
+    if (!launched && p->d_scol && mbx_detail::vec_path_ok(rp, p->d_scol, val, p->ncol)) {
+      if (nt) {
+        using Q = mbx_detail::SortX<mbx_detail::Streamed<P>>;
+        mbx_detail::allow_lds(tile_spmv_kernel<256, IPT, true, Q>, lds);
+        timed_launch(p, tile_spmv_kernel<256, IPT, true, Q>, p->ntiles, 256, lds, s, rp,
+                     p->d_scol, val, x, y, p->d_coord, p->d_carry, p->ntiles, p->ncol, alpha,
+                     beta, bz);
+      } else {
+        using Q = mbx_detail::SortX<P>;
+        mbx_detail::allow_lds(tile_spmv_kernel<256, IPT, true, Q>, lds);
+        timed_launch(p, tile_spmv_kernel<256, IPT, true, Q>, p->ntiles, 256, lds, s, rp,
+                     p->d_scol, val, x, y, p->d_coord, p->d_carry, p->ntiles, p->ncol, alpha,
+                     beta, bz);
       }
       launched = true;
     }
@@ -523,6 +558,7 @@ void mbx_plan_destroy(mbx_plan* p) {
   }
   if (p->d_carry) (void)hipFree(p->d_carry);
   if (p->d_col16) (void)hipFree(p->d_col16);
+  if (p->d_scol) (void)hipFree(p->d_scol);
   if (p->d_colh || p->d_hot || p->d_xh) mbx_detail::plan_drop_hot(p);
   delete p;
 }
@@ -533,6 +569,7 @@ int mbx_plan_clone(const mbx_plan* p, mbx_plan** out) {
   mbx_plan* c = new (std::nothrow) mbx_plan(*p);
   if (!c) return MBX_ENOMEM;
   c->d_col16 = nullptr;  // the clone reads the caller's columns
+  c->d_scol = nullptr;
   c->d_colh = nullptr;
   c->d_hot = nullptr;
   c->d_xh = nullptr;
@@ -606,6 +643,7 @@ int mbx_plan_compress_cols(mbx_plan* p, const int32_t* d_col, void* stream) {
   }
   if (!d_col) return MBX_OK;  // back to the caller's 32-bit columns
   if (p->ipt != 8 && p->ipt != 12) return MBX_EUNSUPPORTED;
+  if (p->d_scol) return MBX_EUNSUPPORTED;  // the sorted columns exclude them
   if (p->ntiles == 0 || p->nnz == 0) return MBX_OK;
   int* d_wide = nullptr;
   uint16_t* c16 = nullptr;
@@ -634,6 +672,59 @@ int mbx_plan_compress_cols(mbx_plan* p, const int32_t* d_col, void* stream) {
     return st;
   }
   p->d_col16 = c16;
+  return MBX_OK;
+}
+
+int mbx_plan_sort_cols(mbx_plan* p, const int32_t* d_col, void* stream) {
+  if (!p) return MBX_EINVAL;
+  hipStream_t s = (hipStream_t)stream;
+  if (p->d_scol) {  // products on any stream may still read them
+    (void)hipDeviceSynchronize();
+    (void)hipFree(p->d_scol);
+    p->d_scol = nullptr;
+  }
+  if (!d_col) return MBX_OK;  // back to the caller's columns
+  if (p->ipt != 8 && p->ipt != 12 && p->ipt != 16) return MBX_EUNSUPPORTED;
+  if (p->d_col16 || (p->d_colh && p->nhot > 0)) return MBX_EUNSUPPORTED;
+  const int sb = p->ipt == 8 ? mbx_detail::sort_slot_bits<256, 8>()
+                             : mbx_detail::sort_slot_bits<256, 16>();  // 12 for IPT 12 too
+  if ((long long)p->ncol > (1ll << (32 - sb))) return MBX_EUNSUPPORTED;
+  if (p->ntiles == 0 || p->nnz == 0) return MBX_OK;
+  const size_t padded = ((size_t)p->nnz + 3) / 4 * 4;  // whole 16-byte groups
+  int* d_bad = nullptr;
+  int32_t* sc = nullptr;
+  if (hipMalloc((void**)&d_bad, sizeof(int)) != hipSuccess) return MBX_ENOMEM;
+  if (hipMalloc((void**)&sc, sizeof(int32_t) * padded) != hipSuccess) {
+    (void)hipFree(d_bad);
+    return MBX_ENOMEM;
+  }
+  int st = MBX_OK, bad = 0;
+  if (hipMemsetAsync(d_bad, 0, sizeof(int), s) != hipSuccess ||
+      hipMemsetAsync(sc, 0, sizeof(int32_t) * padded, s) != hipSuccess)
+    st = MBX_EHIP;
+  if (st == MBX_OK) {
+    switch (p->ipt) {
+      case 8: sort_cols_kernel<8><<<p->ntiles, 256, 0, s>>>(d_col, p->ncol, p->d_coord, sc, d_bad);
+        break;
+      case 12:
+        sort_cols_kernel<12><<<p->ntiles, 256, 0, s>>>(d_col, p->ncol, p->d_coord, sc, d_bad);
+        break;
+      default:
+        sort_cols_kernel<16><<<p->ntiles, 256, 0, s>>>(d_col, p->ncol, p->d_coord, sc, d_bad);
+        break;
+    }
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      st = MBX_EHIP;
+  }
+  (void)hipFree(d_bad);
+  if (st == MBX_OK && bad) st = MBX_EINVAL;  // a column outside [0, ncol)
+  if (st != MBX_OK) {
+    (void)hipFree(sc);
+    return st;
+  }
+  p->d_scol = sc;
   return MBX_OK;
 }
 

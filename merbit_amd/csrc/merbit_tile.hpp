@@ -32,6 +32,7 @@ struct F64 {
   static constexpr bool kStreamedValueNT = false;
   static constexpr bool kCol16 = false;
   static constexpr bool kHotX = false;
+  static constexpr bool kSorted = false;
   using val_t = double;
   using x_t = double;
   using y_t = double;
@@ -73,6 +74,7 @@ struct CompF32 {
   static constexpr bool kStreamedValueNT = true;
   static constexpr bool kCol16 = false;
   static constexpr bool kHotX = false;
+  static constexpr bool kSorted = false;
   using val_t = V;
   using x_t = float;
   using y_t = float;
@@ -139,6 +141,7 @@ struct Acc64 {
   static constexpr bool kStreamedValueNT = true;
   static constexpr bool kCol16 = false;
   static constexpr bool kHotX = false;
+  static constexpr bool kSorted = false;
   using val_t = V;
   using x_t = float;
   using y_t = float;
@@ -223,6 +226,29 @@ template <class P>
 struct HotX : P {
   static constexpr bool kHotX = true;
 };
+
+// P reading the plan's tile-sorted columns (mbx_plan_sort_cols) instead of
+// the caller's: within each tile the nonzeros' columns are sorted and laid
+// out so that each gather instruction's 64 lanes read 64 consecutive sorted
+// columns, which share x lines and so need fewer vector-L1 requests.  Each
+// 32-bit word holds the column in its high 32 - kSortSlotBits bits and the
+// element's position in the tile (its product slot) in the low bits; the
+// gathered x values are written to their slots in LDS and read back there by
+// the threads that hold the matching values, so every product, and the
+// result, is the same as the plain product's.  Vector staging path only.
+template <class P>
+struct SortX : P {
+  static constexpr bool kSorted = true;
+};
+
+// Bits of the tile position in a tile-sorted column word: enough for the
+// WG * IPT merge items of a tile.
+template <int WG, int IPT>
+constexpr int sort_slot_bits() {
+  int b = 0;
+  while ((1 << b) < WG * IPT) ++b;
+  return b;
+}
 
 // out[0] = most rows ending in one tile, out[1] = tiles whose first row
 // began in an earlier tile (they need the carry fixup kernel).
@@ -599,7 +625,44 @@ __device__ __forceinline__ void tile_issue_gathers(TileRegs<WG, IPT, P>& R,
     for (int j = 0; j < NG; ++j) {
       if (j == 0 || j * WG < nkg) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) buffer_gather(rx, R.cg[j][v], R.xv[j][v]);
+        for (int v = 0; v < 4; ++v) {
+          if constexpr (P::kSorted)
+            buffer_gather(rx, (int)((uint32_t)R.cg[j][v] >> sort_slot_bits<WG, IPT>()),
+                          R.xv[j][v]);
+          else
+            buffer_gather(rx, R.cg[j][v], R.xv[j][v]);
+        }
+      }
+    }
+  }
+}
+
+// Tile-sorted columns (P::kSorted): each gathered x value goes to the LDS
+// slot of its element (the low bits of its column word, offset as the
+// products are), as an x_t in the first bytes of that product slot.  Only
+// elements of the tile are written; the words of the clamped groups at the
+// tile's ends belong to other tiles.  Needs a barrier before the slots are
+// read (tile_write_stage).
+template <int WG, int IPT, class P>
+__device__ __forceinline__ void sorted_scatter_x(const TileRegs<WG, IPT, P>& R,
+                                                 typename P::lds_t* s_v) {
+  constexpr int NG = TileRegs<WG, IPT, P>::NG;
+  constexpr uint32_t kMask = (1u << sort_slot_bits<WG, IPT>()) - 1u;
+  using x_t = typename P::x_t;
+  const int tid = threadIdx.x;
+  const int k0 = R.k0(), nnz_t = R.nnz();
+  if (nnz_t > 0) {
+    const int kb = k0 >> 2, nkg = ((k0 + nnz_t - 1) >> 2) - kb + 1;
+    const int off = k0 & 3;
+#pragma unroll
+    for (int j = 0; j < NG; ++j) {
+      if (j == 0 || j * WG < nkg) {
+        const int q = tid + j * WG;  // group kb + q
+        const int lo = k0 - 4 * (kb + q), hi = lo + nnz_t;
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          if (v >= lo && v < hi)
+            *reinterpret_cast<x_t*>(s_v + ((uint32_t)R.cg[j][v] & kMask) + off) = R.xv[j][v];
       }
     }
   }
@@ -634,9 +697,20 @@ __device__ __forceinline__ void tile_write_stage(const TileRegs<WG, IPT, P>& R, 
       if (j == 0 || j * WG < nkg) {
         const int q = tid + j * WG;  // group kb + q
         const int lo = k0 - 4 * (kb + q), hi = lo + nnz_t;
+        typename P::x_t xr[4];
+        if constexpr (P::kSorted) {  // x of these elements, scattered to their slots
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            xr[v] = (v >= lo && v < hi)
+                        ? *reinterpret_cast<const typename P::x_t*>(s_v + 4 * q + v)
+                        : typename P::x_t(0);
+        } else {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) xr[v] = R.xv[j][v];
+        }
         lds_t pr[4];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) pr[v] = P::product(R.vg[j][v], R.xv[j][v]);
+        for (int v = 0; v < 4; ++v) pr[v] = P::product(R.vg[j][v], xr[v]);
         if (lo <= 0 && hi >= 4) {
           lds_store4(s_v + 4 * q, pr);
         } else {
@@ -833,6 +907,9 @@ __device__ __forceinline__ void spmv_tile(
   static_assert(WG % 64 == 0 && WG / 64 <= 4, "scan scratch holds up to 4 waves");
   static_assert(VEC || !P::kCol16, "16-bit columns are read by the vector staging only");
   static_assert(VEC || !P::kHotX, "hot columns are read by the vector staging only");
+  static_assert(VEC || !P::kSorted, "sorted columns are read by the vector staging only");
+  static_assert(!(P::kSorted && (P::kHotX || P::kCol16)),
+                "sorted columns exclude the hot split and 16-bit columns");
   typename P::lds_t* s_p = sm.v;
   int4 c0, c1;
   // Stage: products to LDS and the row-end marks, every global load of the
@@ -843,6 +920,10 @@ __device__ __forceinline__ void spmv_tile(
     tile_issue_loads(R, row_ptr, col, val);
     tile_issue_gathers(R, x, ncol);
     clear_marks(sm);
+    // the previous tile of a persistent workgroup is done with the LDS (its
+    // loop's barrier), so the x values can go to their slots before this
+    // barrier, which then makes them visible
+    if constexpr (P::kSorted) sorted_scatter_x(R, sm.v);
     __syncthreads();
     if constexpr (P::kHotX) hot_substitute(R, sm.xh);
     tile_write_stage(R, sm.mask, sm.wcnt, sm.v);
@@ -858,6 +939,80 @@ __device__ __forceinline__ void spmv_tile(
   }
   __syncthreads();
   finish_tile(sm, s_p, tile, c0, c1, y, carry, alpha, beta, beta_zero);
+}
+
+// Plan-time build of one tile's sorted column words (mbx_plan_sort_cols,
+// mbx_batch_sort_cols; read through SortX): the words (column << SB) | i of
+// the tile's nonzeros i = 0..n-1 are sorted in LDS (bitonic; the slot bits
+// make every key distinct), then written to the tile's positions in the
+// order the vector staging gathers them -- round j, wave w, element v, lane
+// l of the 16-byte groups -- so each gather instruction's 64 lanes take 64
+// consecutive sorted columns.  *bad is set when a column lies outside
+// [0, ncol).  One workgroup of WG threads per tile; s_key holds 2^SB words,
+// s_wsum one int per wave.
+template <int WG, int IPT>
+__device__ __forceinline__ void sort_tile_cols(const int32_t* __restrict__ col, int ncol, int k0,
+                                               int k1, int32_t* __restrict__ scol,
+                                               int* __restrict__ bad, uint32_t* s_key,
+                                               int* s_wsum) {
+  constexpr int SB = sort_slot_bits<WG, IPT>();
+  constexpr int N2 = 1 << SB;
+  constexpr int NW = WG / 64;
+  constexpr int NG = (WG * IPT / 4 + 2 + WG - 1) / WG;  // as TileRegs
+  constexpr int C = 4 * NG;  // gather-order positions per thread
+  const int tid = threadIdx.x;
+  const int n = k1 - k0;
+  int out_of_range = 0;
+  for (int i = tid; i < N2; i += WG) {
+    uint32_t key = 0xFFFFFFFFu;
+    if (i < n) {
+      const int c = col[k0 + i];
+      out_of_range |= (c < 0 || c >= ncol) ? 1 : 0;
+      key = ((uint32_t)c << SB) | (uint32_t)i;
+    }
+    s_key[i] = key;
+  }
+  if (out_of_range) atomicOr(bad, 1);
+  __syncthreads();
+  for (int k = 2; k <= N2; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = tid; i < N2; i += WG) {
+        const int ixj = i ^ j;
+        if (ixj > i) {
+          const uint32_t a = s_key[i], b = s_key[ixj];
+          if ((a > b) == ((i & k) == 0)) {
+            s_key[i] = b;
+            s_key[ixj] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // gather-order position kk = ((j * NW + w) * 4 + v) * 64 + l holds element
+  // v of group kb + j * WG + w * 64 + l; thread tid ranks kk in [tid*C, tid*C+C)
+  const int kb = k0 >> 2;
+  int cnt = 0;
+#pragma unroll
+  for (int u = 0; u < C; ++u) {
+    const int kk = tid * C + u;
+    const int q = ((kk >> 8) / NW) * WG + ((kk >> 8) % NW) * 64 + (kk & 63);
+    const int p = 4 * (kb + q) + ((kk >> 6) & 3);
+    cnt += (p >= k0 && p < k1) ? 1 : 0;
+  }
+  const int incl = wave_prefix_sum(cnt);
+  if ((tid & 63) == 63) s_wsum[tid >> 6] = incl;
+  __syncthreads();
+  int rank = incl - cnt;
+  for (int w = 0; w < NW; ++w)
+    if (w < (tid >> 6)) rank += s_wsum[w];
+#pragma unroll
+  for (int u = 0; u < C; ++u) {
+    const int kk = tid * C + u;
+    const int q = ((kk >> 8) / NW) * WG + ((kk >> 8) % NW) * 64 + (kk & 63);
+    const int p = 4 * (kb + q) + ((kk >> 6) & 3);
+    if (p >= k0 && p < k1) scol[p] = (int32_t)s_key[rank++];
+  }
 }
 
 // Carry fixup for tile t >= 1 whose first row began in an earlier tile ts and

@@ -186,6 +186,18 @@ class MerbitPlan:
         check(st, "mbx_plan_compress_cols")
         return col is not None
 
+    def sort_cols(self, col: torch.Tensor | None, stream=None) -> bool:
+        """Keep every tile's columns sorted in gather order
+        (mbx_plan_sort_cols), so the x gathers share lines; False when the
+        plan's shape, ncol or another column option rules it out.  None
+        drops the copy.  Results are bit-identical either way."""
+        ptr = None if col is None else _dptr(col, torch.int32, "col")
+        st = lib().mbx_plan_sort_cols(self.handle, ptr, _stream_handle(stream))
+        if st == -3:  # MBX_EUNSUPPORTED
+            return False
+        check(st, "mbx_plan_sort_cols")
+        return col is not None
+
     def split_hot(self, col: torch.Tensor | None, max_hot: int, stream=None) -> float:
         """Keep the max_hot most frequent columns' x values in LDS during
         products (mbx_plan_split_hot); returns the share of the nonzeros in

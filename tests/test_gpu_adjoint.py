@@ -199,11 +199,12 @@ def test_adjoint_permutation_boundaries(orc, device, dtype):
 def test_operator_default_adjoint_rule_rmat(orc, device, dtype):
     """A default MerbitOperator runs the launch rule on the transposed
     pattern (mbx_adjoint_auto).  The R-MAT's rows are skewed like its
-    columns, so A^T gets the hot-column split; for FP64 dx is bit-identical
-    to the plain single-product adjoint of the same shape (8 items per
-    thread), for FP32 the rule also picks FP32 products with FP64 sums, so dx
-    matches the plain adjoint within the FP32 bar; both match the oracle's
-    accurate A^T product."""
+    columns, so A^T meets the hot-columns rule, which with 2^20 columns
+    takes tile-sorted gathers at 16 items per thread; for FP64 dx is
+    bit-identical to the plain single-product adjoint of the same shape, for
+    FP32 the rule also picks FP32 products with FP64 sums, so dx matches the
+    plain adjoint within the FP32 bar; both match the oracle's accurate A^T
+    product."""
     from merbit_amd import workloads
     from merbit_amd.autograd import MerbitOperator
     A, _ = workloads.rmat()
@@ -217,9 +218,9 @@ def test_operator_default_adjoint_rule_rmat(orc, device, dtype):
     assert op.adjoint_blocks is None
     dx = op.adjoint_x(d_val, d_g)
     c = op.adjoint_choice
-    assert c["rule"] == "hot_columns" and c["hot_columns"] > 0, c
-    ref_op = MerbitOperator(d_rp, d_col, A.ncol, items_per_thread=8, blocks=0,
-                            adjoint_blocks=0, value_dtype=dtype)
+    assert c["rule"] == "hot_columns" and c["sort_cols"] == 1 and c["hot_columns"] == 0, c
+    ref_op = MerbitOperator(d_rp, d_col, A.ncol, items_per_thread=c["items_per_thread"],
+                            blocks=0, adjoint_blocks=0, value_dtype=dtype)
     dx_ref = ref_op.adjoint_x(d_val, d_g)
     cp, ri, pm = orc.csr_transpose(A.row_ptr, A.col, A.ncol)
     gh = d_g.double().cpu().numpy()

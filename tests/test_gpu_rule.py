@@ -111,7 +111,8 @@ def test_rule_within_5pct_of_timed_best(device, name):
     t_tuned = time_us(lambda: tuned.spmv(d_rp, d_col, d_val, d_x, y), reps)
     t_rule = time_us(lambda: rule.spmv(d_rp, d_col, d_val, d_x, y), reps)
     say(f"{name}: rule {c['rule']} ipt {c['items_per_thread']} wg {c['wg_per_cu']} "
-        f"hot {c['hot_columns']} col16 {c['compress_cols']} acc {c['accumulation']}: "
+        f"hot {c['hot_columns']} sort {c['sort_cols']} col16 {c['compress_cols']} "
+        f"acc {c['accumulation']}: "
         f"{t_rule:.2f} us; tuned ipt {ipt.value} mode {mode.value} wg {wg.value}: "
         f"{t_tuned:.2f} us (autotuner {us.value:.2f}); ratio {t_rule / t_tuned:.3f}")
     assert t_rule <= 1.05 * t_tuned, (t_rule, t_tuned)
