@@ -201,6 +201,8 @@ def tune_shape(args, copies, ncol, nnz, kind, hot_ok, sort_ok=True):
                                            or args.hot is not None))
     sorts = [False] + ([True] if sort_ok and not args.no_sort else [])
     forms = [(h, False) for h in hots] + [(0, True) for s_ in sorts if s_]
+    if args.sort and sort_ok:  # only the tile-sorted columns
+        forms = [(0, True)]
 
     def form_fits(i, h, srt):
         return (i in (8, 12, 16) and not h) if srt else hot_fits(i, h)
@@ -1034,6 +1036,8 @@ def main():
                     help="leave the hot-column split out of the tuned candidates")
     ap.add_argument("--no-sort", action="store_true",
                     help="leave tile-sorted columns out of the tuned candidates")
+    ap.add_argument("--sort", action="store_true",
+                    help="only tile-sorted columns for the power-law products (no hot split)")
     ap.add_argument("--hot", type=int, default=None,
                     help="hot columns of the power-law products (0 = none); default: tuned")
     args = ap.parse_args()
