@@ -695,13 +695,17 @@ def bench_adjoint(dev, args, checks):
     vb = cb.values(val)
     yb = torch.empty(n, dtype=torch.float32, device=dev)
     tuned = {}
-    for mode in ("cached", "nontemporal"):
-        cb.set_stream(mode)
-        for wg in (0, 3, 4):
-            cb.set_residency(wg)
-            tuned[(mode, wg)] = min(time_steps([lambda: cb.spmv(vb, xx, yb)], 10, 2)[0]
-                                    for _ in range(2))
-    cb_mode, cb_wg = min(tuned, key=tuned.get)
+    for srt in ((False,) if args.no_sort else (False, True)):
+        if srt and not cb.sort_cols(True):
+            break
+        for mode in ("cached", "nontemporal"):
+            cb.set_stream(mode)
+            for wg in (0, 3, 4):
+                cb.set_residency(wg)
+                tuned[(mode, wg, srt)] = min(time_steps([lambda: cb.spmv(vb, xx, yb)], 10, 2)[0]
+                                             for _ in range(2))
+    cb_mode, cb_wg, cb_srt = min(tuned, key=tuned.get)
+    cb.sort_cols(cb_srt)
     cb.set_stream(cb_mode)
     cb.set_residency(cb_wg)
     vals_ms, _ = time_steps([lambda: cb.values(val, out=vb)], steps, args.warmup)
@@ -751,7 +755,7 @@ def bench_adjoint(dev, args, checks):
     out["forward_column_blocked"] = {
         "us": round(fwdb_ms * 1e3, 2), "gbs": gbs(b_fwd, fwdb_ms),
         "gflops": round(2 * nnz / (fwdb_ms * 1e-3) / 1e9, 1), "blocks": cb.nblocks,
-        "stream_loads": cb_mode, "tiles_per_cu": cb_wg,
+        "stream_loads": cb_mode, "tiles_per_cu": cb_wg, "sorted_columns": cb_srt,
         "speedup_vs_csr": round(fwd_ms / fwdb_ms, 2),
         "dval_blocked_us": round(dvb_ms * 1e3, 2), "dval_speedup": round(dv_ms / dvb_ms, 2),
         "dx_blocked_us": round(dxb_ms * 1e3, 2), "dx_speedup": round(dx_ms / dxb_ms, 2),
@@ -846,7 +850,9 @@ def bench_pagerank(A, dev, args):
     ingest_ms = (time.perf_counter() - t0) * 1e3
     del row, col
     iters = 20
-    plan = MerbitPlan(rp, n, vt.numel(), items_per_thread=args.ipt or 12)
+    # the launch rule's plan (the library default) unless a shape is pinned
+    plan = MerbitPlan(rp, n, vt.numel(), items_per_thread=args.ipt) if args.ipt else \
+        MerbitPlan(rp, n, vt.numel(), col=ct, value_dtype=torch.float64)
     x0 = torch.full((n,), 1.0 / n, dtype=torch.float64, device=dev)
     y = torch.empty(n, dtype=torch.float64, device=dev)
     nbytes = int(lib().mbx_pagerank_workspace_bytes(n))
@@ -862,8 +868,13 @@ def bench_pagerank(A, dev, args):
     lib().mbx_graph_destroy(g)
     spmv_ms, _ = time_steps([lambda: plan.spmv(rp, ct, vt, x0, y)], args.steps, args.warmup)
     mass = float(plan.pagerank(rp, ct, vt, damp=0.85, iters=iters, x0=x0).sum())
+    ch = getattr(plan, "choice", None)
+    shape = {"items_per_thread": plan.items_per_thread}
+    if ch:
+        shape.update(rule=ch["rule"], sorted_columns=bool(ch["sort_cols"]),
+                     hot_columns=ch["hot_columns"])
     plan.close()
-    out = {"nrow": n, "nnz": int(vt.numel()), "iterations": iters, "damp": 0.85,
+    out = {"nrow": n, "nnz": int(vt.numel()), "iterations": iters, "damp": 0.85, "plan": shape,
            "ingest_ms": round(ingest_ms, 2), "us_per_iteration": round(pr_ms * 1e3 / iters, 2),
            "us_per_plain_spmv": round(spmv_ms * 1e3, 2), "mass_after": mass}
     log(f"pagerank: ingest {ingest_ms:.1f} ms, {out['us_per_iteration']} us/iteration "

@@ -486,6 +486,13 @@ int mbx_colblock_sddmm(const mbx_colblock* cb, int32_t value_kind, const void* d
                        const void* d_x, void* d_dval_blocked, void* stream);
 int mbx_colblock_set_stream(mbx_colblock* cb, int32_t mode);
 int mbx_colblock_set_residency(mbx_colblock* cb, int32_t wg_per_cu);
+/* Tile-sorted gathers in every block product (mbx_plan_sort_cols on each
+ * block's plan, and on the A^T form's once prepared; no reference
+ * counterpart): enable 1 sorts, 0 drops.  MBX_EUNSUPPORTED (nothing kept)
+ * when the blocks' items_per_thread is not 8, 12 or 16 or a block spans
+ * more columns than the sorted words hold; results stay bit-identical.
+ * Synchronises `stream`. */
+int mbx_colblock_sort_cols(mbx_colblock* cb, int32_t enable, void* stream);
 int mbx_colblock_adjoint_prepare(mbx_colblock* cb, const int32_t* d_row_ptr,
                                  const int32_t* d_col, int32_t nblocks,
                                  int32_t items_per_thread, void* stream, int32_t* nblocks_out);

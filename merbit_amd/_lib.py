@@ -18,7 +18,7 @@ ABI_SYMBOLS = (
     "mbx_plan_info",
     "mbx_plan_partition", "mbx_plan_set_stream", "mbx_batch_set_stream",
     "mbx_plan_set_residency", "mbx_batch_set_residency", "mbx_plan_compress_cols",
-    "mbx_plan_sort_cols", "mbx_batch_sort_cols",
+    "mbx_plan_sort_cols", "mbx_batch_sort_cols", "mbx_colblock_sort_cols",
     "mbx_plan_set_accumulation",
     "mbx_plan_set_events", "mbx_spmv_f64", "mbx_spmv_f32", "mbx_spmv_bf16",
     "mbx_spmm_workspace_bytes", "mbx_spmm_f32",
@@ -157,6 +157,7 @@ def lib() -> ctypes.CDLL:
     L.mbx_plan_compress_cols.argtypes = [vp, vp, vp]
     L.mbx_plan_sort_cols.argtypes = [vp, vp, vp]
     L.mbx_batch_sort_cols.argtypes = [vp, i32, vp]
+    L.mbx_colblock_sort_cols.argtypes = [vp, i32, vp]
     L.mbx_plan_set_accumulation.argtypes = [vp, i32]
     L.mbx_colblock_create.argtypes = [vp, vp, i32, i32, i32, i32, i32, vp, ctypes.POINTER(vp)]
     L.mbx_colblock_destroy.argtypes = [vp]

@@ -62,6 +62,7 @@ struct mbx_colblock {
   mbx_colblock* t = nullptr;
   mbx_adjoint* tp = nullptr;  // bucketed permutation: this form's slots -> the A^T form's
   int32_t nnz_b[kMaxBlocks] = {};
+  bool sorted = false;  // mbx_colblock_sort_cols: the block plans gather in sorted order
 };
 
 namespace {
@@ -425,6 +426,27 @@ int mbx_colblock_set_residency(mbx_colblock* cb, int32_t wg_per_cu) {
   return MBX_OK;
 }
 
+int mbx_colblock_sort_cols(mbx_colblock* cb, int32_t enable, void* stream) {
+  if (!cb) return MBX_EINVAL;
+  for (int j = 0; j < cb->nb; ++j) (void)mbx_plan_sort_cols(cb->plan[j], nullptr, stream);
+  if (cb->t) (void)mbx_colblock_sort_cols(cb->t, 0, stream);
+  cb->sorted = false;
+  if (!enable) return MBX_OK;
+  int st = MBX_OK;
+  for (int j = 0; j < cb->nb && st == MBX_OK; ++j)
+    st = mbx_plan_sort_cols(cb->plan[j], cb->col + cb->base[j], stream);
+  if (st == MBX_OK && cb->t) {
+    st = mbx_colblock_sort_cols(cb->t, 1, stream);
+    if (st == MBX_EUNSUPPORTED) st = MBX_OK;  // the A^T form keeps its plain gathers
+  }
+  if (st != MBX_OK) {
+    (void)mbx_colblock_sort_cols(cb, 0, stream);
+    return st;
+  }
+  cb->sorted = true;
+  return MBX_OK;
+}
+
 int mbx_colblock_adjoint_prepare(mbx_colblock* cb, const int32_t* d_row_ptr,
                                  const int32_t* d_col, int32_t nblocks,
                                  int32_t items_per_thread, void* stream, int32_t* nblocks_out) {
@@ -441,6 +463,10 @@ int mbx_colblock_adjoint_prepare(mbx_colblock* cb, const int32_t* d_row_ptr,
   if (st == MBX_OK)
     st = mbx_colblock_create(adj->col_ptr, adj->row_idx, cb->ncol, cb->nrow, cb->nnz, nblocks,
                              items_per_thread, stream, &cb->t);
+  if (st == MBX_OK && cb->sorted) {  // the A^T form gathers as this one does, where it can
+    const int ss = mbx_colblock_sort_cols(cb->t, 1, stream);
+    if (ss != MBX_EUNSUPPORTED) st = ss;
+  }
   // The permutation over [0, slots): destination q (a slot of the A^T form,
   // or one past its last block) reads source slot perm[q] of this form.
   // Real entries pair through the CSC position; the A^T form's padding and

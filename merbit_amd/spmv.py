@@ -478,6 +478,16 @@ class ColumnBlockedCsr:
         check(lib().mbx_colblock_set_residency(self.handle, wg_per_cu),
               "mbx_colblock_set_residency")
 
+    def sort_cols(self, enable: bool = True, stream=None) -> bool:
+        """Tile-sorted gathers in every block product
+        (mbx_colblock_sort_cols); False when the blocks' shape rules them
+        out.  Results are bit-identical either way."""
+        st = lib().mbx_colblock_sort_cols(self.handle, int(bool(enable)), _stream_handle(stream))
+        if st == -3:  # MBX_EUNSUPPORTED
+            return False
+        check(st, "mbx_colblock_sort_cols")
+        return bool(enable)
+
     def prepare_adjoint(self, row_ptr: torch.Tensor, col: torch.Tensor, nblocks: int = 0,
                         items_per_thread: int = 8, stream=None) -> int:
         """Build the column-blocked form of A^T (blocks over A's rows) from the
