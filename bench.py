@@ -810,9 +810,9 @@ def bench_adjoint_rmat(A, x, dev, args, checks):
                         "dval_us": round(v_ms * 1e3, 2),
                         "step_us": round((f_ms + x_ms + v_ms) * 1e3, 2)}
         d = ops["defaults"]
-        res["defaults"]["forward_rule"] = {k: d.plan.choice[k] for k in ("rule", "hot_columns")}
-        res["defaults"]["adjoint_rule"] = {k: d.adjoint_choice[k]
-                                           for k in ("rule", "hot_columns")}
+        keys = ("rule", "items_per_thread", "sort_cols", "hot_columns")
+        res["defaults"]["forward_rule"] = {k: d.plan.choice[k] for k in keys}
+        res["defaults"]["adjoint_rule"] = {k: d.adjoint_choice[k] for k in keys}
         res["step_speedup"] = round(res["pinned"]["step_us"] / res["defaults"]["step_us"], 3)
         res["dx_speedup"] = round(res["pinned"]["dx_us"] / res["defaults"]["dx_us"], 3)
         cp, ri, pm = d.transpose()  # A^T from the device's own transpose
