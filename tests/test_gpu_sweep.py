@@ -1,12 +1,14 @@
 """Randomised sweep over the product's options in combination (seeded, so a
 failure names a reproducible case): pattern shape and row-length
 distribution, items per thread, value kind, stream policy, residency cap,
-hot-column split, 16-bit columns, FP64 sums, alpha/beta and value arrays
-that are not 16-byte aligned (the scalar staging path).
+hot-column split, 16-bit columns, tile-sorted gathers, FP64 sums,
+alpha/beta and value arrays that are not 16-byte aligned (the scalar
+staging path).
 
 Every plain result is checked against the oracle's accurate product (FP64
 1e-12, FP32/BF16 1e-6, condition-scaled), the options that keep the
-arithmetic (stream policy, residency, hot split, 16-bit columns) must give
+arithmetic (stream policy, residency, hot split, 16-bit columns, sorted
+gathers) must give
 the plan's plain product bit for bit, and the rest (FP64 sums, alpha/beta,
 the scalar path) must stay within the bars."""
 from __future__ import annotations
@@ -110,6 +112,11 @@ def test_random_option_sweep(orc, device):
             opts.append(f"hot {nh}")
         if ipt in (8, 12) and rng.random() < 0.4 and plan.compress_cols(d_col):
             opts.append("col16")
+        # tile-sorted gathers where neither of those took (an own draw, so
+        # the cases above stay as they were)
+        if ipt in (8, 12, 16) and len(opts) == 2 and \
+                np.random.default_rng(7000 + case).random() < 0.7 and plan.sort_cols(d_col):
+            opts.append("sorted")
         y = plan.spmv(d_rp, d_col, d_val, d_x)
         assert torch.equal(y.view(torch.uint8), base.view(torch.uint8)), (case, opts)
         # alpha / beta against the host's combination of the oracle's product
@@ -185,6 +192,9 @@ def test_random_row_tiles_sweep(orc, device):
             opts.append(f"hot {nh}")
         if ipt in (8, 12) and rng.random() < 0.5 and plan.compress_cols(d_col):
             opts.append("col16")
+        if ipt in (8, 12, 16) and len(opts) == 2 and \
+                np.random.default_rng(9000 + case).random() < 0.7 and plan.sort_cols(d_col):
+            opts.append("sorted")
         y = plan.spmv(d_rp, d_col, d_val, d_x)
         assert torch.equal(y.view(torch.uint8), base.view(torch.uint8)), (case, opts)
         if nnz > 0 and rng.random() < 0.3:
@@ -261,6 +271,8 @@ def test_random_batch_sweep(orc, device):
         b = MerbitBatch(mats, items_per_thread=ipt)
         b.set_stream(["auto", "cached", "nontemporal"][int(rng.integers(0, 3))])
         b.set_residency(int(rng.integers(0, 5)))
+        if ipt in (8, 12, 16) and np.random.default_rng(8000 + case).random() < 0.6:
+            b.sort_cols(True)  # tile-sorted gathers (an own draw: the cases stay)
         b.spmv(alpha, beta)
         torch.cuda.synchronize()
         for i, ((rp, col, vals, x, y0), m) in enumerate(zip(host, mats)):
