@@ -413,6 +413,15 @@ int mbx_autotune_spmv_launch(const int32_t* d_row_ptr, int32_t nrow, int32_t nco
                              const void* d_x, void* d_y, int32_t reps, void* stream,
                              int32_t* best_ipt, int32_t* best_stream, int32_t* best_wg_per_cu,
                              float* best_us);
+/* As mbx_autotune_spmv_launch, with tile-sorted gathers (mbx_plan_sort_cols)
+ * as a further candidate at every items_per_thread that takes them;
+ * *best_sorted is 1 when the fastest launch sorts (apply it with
+ * mbx_plan_sort_cols on the plan of that shape).  No reference counterpart. */
+int mbx_autotune_spmv_form(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol, int32_t nnz,
+                           const int32_t* d_col, const void* d_val, int32_t value_kind,
+                           const void* d_x, void* d_y, int32_t reps, void* stream,
+                           int32_t* best_ipt, int32_t* best_stream, int32_t* best_wg_per_cu,
+                           int32_t* best_sorted, float* best_us);
 int mbx_autotune_spmv(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol, int32_t nnz,
                       const int32_t* d_col, const void* d_val, int32_t value_kind,
                       const void* d_x, void* d_y, int32_t reps, void* stream, int32_t* best_ipt,

@@ -24,7 +24,8 @@ ABI_SYMBOLS = (
     "mbx_spmm_workspace_bytes", "mbx_spmm_f32",
     "mbx_batch_create", "mbx_batch_destroy", "mbx_batch_info", "mbx_batch_spmv",
     "mbx_batch_graph_create", "mbx_spmv_graph_create", "mbx_graph_launch", "mbx_graph_destroy",
-    "mbx_autotune_spmv", "mbx_autotune_spmv_launch", "mbx_autotune_spmm", "mbx_autotune_batch",
+    "mbx_autotune_spmv", "mbx_autotune_spmv_launch", "mbx_autotune_spmv_form",
+    "mbx_autotune_spmm", "mbx_autotune_batch",
     "mbx_adjoint_create", "mbx_adjoint_destroy", "mbx_adjoint_transpose", "mbx_adjoint_dx",
     "mbx_adjoint_set_blocks", "mbx_adjoint_auto",
     "mbx_sddmm", "mbx_ingest_ref", "mbx_plan_stats",
@@ -149,6 +150,10 @@ def lib() -> ctypes.CDLL:
     L.mbx_autotune_spmv_launch.argtypes = [vp, i32, i32, i32, vp, vp, i32, vp, vp, i32, vp,
                                            ctypes.POINTER(i32), ctypes.POINTER(i32),
                                            ctypes.POINTER(i32), ctypes.POINTER(f32)]
+    L.mbx_autotune_spmv_form.argtypes = [vp, i32, i32, i32, vp, vp, i32, vp, vp, i32, vp,
+                                         ctypes.POINTER(i32), ctypes.POINTER(i32),
+                                         ctypes.POINTER(i32), ctypes.POINTER(i32),
+                                         ctypes.POINTER(f32)]
     L.mbx_autotune_spmm.argtypes = [vp, i32, i32, i32, vp, vp, i32, vp, i64, i32, vp, i64, i32,
                                     vp, ctypes.c_size_t, i32, vp, ctypes.POINTER(i32),
                                     ctypes.POINTER(f32)]

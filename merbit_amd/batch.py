@@ -172,6 +172,26 @@ def autotune_spmv_launch(row_ptr, col, val, x, y, ncol=None, reps: int = 10, str
     return ipt.value, name, wg.value, us.value
 
 
+def autotune_spmv_form(row_ptr, col, val, x, y, ncol=None, reps: int = 10, stream=None):
+    """As autotune_spmv_launch with tile-sorted gathers as a further
+    candidate (mbx_autotune_spmv_form): (items_per_thread, stream mode
+    name, tiles per CU, sorted, microseconds)."""
+    kind = value_kind(val)
+    v = val.view(torch.uint16) if val.dtype == torch.bfloat16 else val
+    vdt = _vec_dtype(kind)
+    ncol = x.numel() if ncol is None else ncol
+    ipt, mode, wg, srt = (ctypes.c_int32() for _ in range(4))
+    us = ctypes.c_float()
+    check(lib().mbx_autotune_spmv_form(
+        _dptr(row_ptr, torch.int32, "row_ptr"), row_ptr.numel() - 1, ncol, col.numel(),
+        _dptr(col, torch.int32, "col"), _dptr(v, v.dtype, "val"), kind, _dptr(x, vdt, "x"),
+        _dptr(y, vdt, "y"), reps, _stream_handle(stream), ctypes.byref(ipt), ctypes.byref(mode),
+        ctypes.byref(wg), ctypes.byref(srt), ctypes.byref(us)), "mbx_autotune_spmv_form")
+    from .spmv import STREAM_MODES
+    name = {v: k for k, v in STREAM_MODES.items()}[mode.value]
+    return ipt.value, name, wg.value, bool(srt.value), us.value
+
+
 def autotune_spmm(row_ptr, col, val, X, Y, reps: int = 5, stream=None):
     """(best items_per_thread, microseconds per product) for this SpMM."""
     from .spmv import MerbitPlan

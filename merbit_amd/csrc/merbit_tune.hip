@@ -61,6 +61,64 @@ int spmv_call(const mbx_plan* p, int kind, const int32_t* rp, const int32_t* col
 
 }  // namespace
 
+namespace {
+
+// Items per thread x stream policy x tiles per CU, and with `sorted` the
+// same shapes with tile-sorted gathers where a plan takes them.
+int autotune_launch(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol, int32_t nnz,
+                    const int32_t* d_col, const void* d_val, int32_t value_kind,
+                    const void* d_x, void* d_y, int32_t reps, hipStream_t s, bool sorted,
+                    int32_t* best_ipt, int32_t* best_stream, int32_t* best_wg_per_cu,
+                    int32_t* best_sorted, float* best_us) {
+  const int kModes[] = {MBX_STREAM_CACHED, MBX_STREAM_NONTEMPORAL};
+  const int kResidency[] = {0, 3, 4, 5};
+  *best_ipt = 0;
+  *best_us = 0.f;
+  if (best_sorted) *best_sorted = 0;
+  for (int ipt : kSpmvCandidates) {
+    mbx_plan* p = nullptr;
+    int st = mbx_plan_create(d_row_ptr, nrow, ncol, nnz, ipt, 256, s, &p);
+    if (st == MBX_EOVERFLOW) continue;
+    if (st != MBX_OK) return st;
+    for (int form = 0; form < (sorted ? 2 : 1); ++form) {
+      if (form == 1) {
+        st = mbx_plan_sort_cols(p, d_col, s);
+        if (st == MBX_EUNSUPPORTED || st == MBX_EINVAL || nnz == 0) break;  // plain only
+        if (st != MBX_OK) {
+          mbx_plan_destroy(p);
+          return st;
+        }
+      }
+      for (int mode : kModes) {
+        for (int wg : kResidency) {
+          p->stream_mode = mode;
+          p->wg_per_cu = wg;
+          float us = 0.f;
+          st = time_calls(
+              s, reps,
+              [&] { return spmv_call(p, value_kind, d_row_ptr, d_col, d_val, d_x, d_y, s); },
+              &us);
+          if (st != MBX_OK) {
+            mbx_plan_destroy(p);
+            return st;
+          }
+          if (*best_ipt == 0 || us < *best_us) {
+            *best_ipt = ipt;
+            *best_stream = mode;
+            *best_wg_per_cu = wg;
+            if (best_sorted) *best_sorted = form;
+            *best_us = us;
+          }
+        }
+      }
+    }
+    mbx_plan_destroy(p);
+  }
+  return *best_ipt ? MBX_OK : MBX_EOVERFLOW;
+}
+
+}  // namespace
+
 extern "C" {
 
 int mbx_plan_stats(const mbx_plan* p, const int32_t* d_row_ptr, mbx_pattern_stats* out) {
@@ -146,39 +204,21 @@ int mbx_autotune_spmv_launch(const int32_t* d_row_ptr, int32_t nrow, int32_t nco
                              int32_t* best_ipt, int32_t* best_stream, int32_t* best_wg_per_cu,
                              float* best_us) {
   if (!best_ipt || !best_stream || !best_wg_per_cu || !best_us || reps < 1) return MBX_EINVAL;
-  hipStream_t s = (hipStream_t)stream;
-  const int kModes[] = {MBX_STREAM_CACHED, MBX_STREAM_NONTEMPORAL};
-  const int kResidency[] = {0, 3, 4, 5};
-  *best_ipt = 0;
-  *best_us = 0.f;
-  for (int ipt : kSpmvCandidates) {
-    mbx_plan* p = nullptr;
-    int st = mbx_plan_create(d_row_ptr, nrow, ncol, nnz, ipt, 256, stream, &p);
-    if (st == MBX_EOVERFLOW) continue;
-    if (st != MBX_OK) return st;
-    for (int mode : kModes) {
-      for (int wg : kResidency) {
-        p->stream_mode = mode;
-        p->wg_per_cu = wg;
-        float us = 0.f;
-        st = time_calls(
-            s, reps,
-            [&] { return spmv_call(p, value_kind, d_row_ptr, d_col, d_val, d_x, d_y, s); }, &us);
-        if (st != MBX_OK) {
-          mbx_plan_destroy(p);
-          return st;
-        }
-        if (*best_ipt == 0 || us < *best_us) {
-          *best_ipt = ipt;
-          *best_stream = mode;
-          *best_wg_per_cu = wg;
-          *best_us = us;
-        }
-      }
-    }
-    mbx_plan_destroy(p);
-  }
-  return *best_ipt ? MBX_OK : MBX_EOVERFLOW;
+  return autotune_launch(d_row_ptr, nrow, ncol, nnz, d_col, d_val, value_kind, d_x, d_y, reps,
+                         (hipStream_t)stream, false, best_ipt, best_stream, best_wg_per_cu,
+                         nullptr, best_us);
+}
+
+int mbx_autotune_spmv_form(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol, int32_t nnz,
+                           const int32_t* d_col, const void* d_val, int32_t value_kind,
+                           const void* d_x, void* d_y, int32_t reps, void* stream,
+                           int32_t* best_ipt, int32_t* best_stream, int32_t* best_wg_per_cu,
+                           int32_t* best_sorted, float* best_us) {
+  if (!best_ipt || !best_stream || !best_wg_per_cu || !best_sorted || !best_us || reps < 1)
+    return MBX_EINVAL;
+  return autotune_launch(d_row_ptr, nrow, ncol, nnz, d_col, d_val, value_kind, d_x, d_y, reps,
+                         (hipStream_t)stream, true, best_ipt, best_stream, best_wg_per_cu,
+                         best_sorted, best_us);
 }
 
 int mbx_autotune_spmm(const int32_t* d_row_ptr, int32_t nrow, int32_t ncol, int32_t nnz,

@@ -1,8 +1,9 @@
 """The launch rule (mbx_launch_rule / mbx_plan_create_auto): which rule each
 workload class meets, that the plan it makes gives the same results as a
 plain plan within the bars, and that its launch is within 5 % of the
-timing autotuner's best (mbx_autotune_spmv_launch, which sweeps items per
-thread x stream policy x tiles per CU) on the benchmark workloads.  The
+timing autotuner's best (mbx_autotune_spmv_form, which sweeps items per
+thread x stream policy x tiles per CU x tile-sorted gathers) on the
+benchmark workloads.  The
 autotuner's pick is re-timed with the same method as the rule's plan (the
 minimum of its 64 single timings is biased low), both as graph replays."""
 from __future__ import annotations
@@ -91,16 +92,18 @@ def test_rule_within_5pct_of_timed_best(device, name):
     y = torch.empty(A.nrow, dtype=torch.float64 if kind == 0 else torch.float32, device=device)
     reps = max(5, min(200, int(4e7 / max(A.nnz, 1))))
     s = torch.cuda.current_stream().cuda_stream
-    ipt, mode, wg = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    ipt, mode, wg, srt = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
     us = ctypes.c_float()
-    assert lib().mbx_autotune_spmv_launch(d_rp.data_ptr(), A.nrow, A.ncol, A.nnz,
-                                          d_col.data_ptr(), d_val.data_ptr(), kind,
-                                          d_x.data_ptr(), y.data_ptr(), reps, s,
-                                          ctypes.byref(ipt), ctypes.byref(mode), ctypes.byref(wg),
-                                          ctypes.byref(us)) == 0
+    assert lib().mbx_autotune_spmv_form(d_rp.data_ptr(), A.nrow, A.ncol, A.nnz,
+                                        d_col.data_ptr(), d_val.data_ptr(), kind,
+                                        d_x.data_ptr(), y.data_ptr(), reps, s,
+                                        ctypes.byref(ipt), ctypes.byref(mode), ctypes.byref(wg),
+                                        ctypes.byref(srt), ctypes.byref(us)) == 0
     tuned = MerbitPlan(d_rp, A.ncol, A.nnz, items_per_thread=ipt.value)
     tuned.set_stream({1: "cached", 2: "nontemporal"}[mode.value])
     tuned.set_residency(wg.value)
+    if srt.value:
+        assert tuned.sort_cols(d_col)
     rule = MerbitPlan(d_rp, A.ncol, A.nnz, col=d_col, value_dtype=vdt)
     c = rule.choice
     assert c["rule"] == EXPECTED[name], c
@@ -113,7 +116,8 @@ def test_rule_within_5pct_of_timed_best(device, name):
     say(f"{name}: rule {c['rule']} ipt {c['items_per_thread']} wg {c['wg_per_cu']} "
         f"hot {c['hot_columns']} sort {c['sort_cols']} col16 {c['compress_cols']} "
         f"acc {c['accumulation']}: "
-        f"{t_rule:.2f} us; tuned ipt {ipt.value} mode {mode.value} wg {wg.value}: "
+        f"{t_rule:.2f} us; tuned ipt {ipt.value} mode {mode.value} wg {wg.value} "
+        f"sort {srt.value}: "
         f"{t_tuned:.2f} us (autotuner {us.value:.2f}); ratio {t_rule / t_tuned:.3f}")
     assert t_rule <= 1.05 * t_tuned, (t_rule, t_tuned)
     tuned.close()

@@ -375,6 +375,34 @@ def test_rule_takes_sorted_gathers(orc, device):
     assert c["rule"] == "hot_columns" and c["sort_cols"] == 1 and c["hot_columns"] == 0, c
 
 
+def test_autotune_form(device):
+    """The timing autotuner over launch shapes and sorted gathers returns a
+    shape the plan takes, whose product matches the plain product of that
+    shape bit for bit."""
+    from merbit_amd.batch import autotune_spmv_form
+    from merbit_amd.spmv import MerbitPlan
+    rng = np.random.default_rng(23)
+    rp, col = uniform_csr(rng, 60000, 60000, 20)
+    nnz = int(rp[-1])
+    v, x = values(rng, "f64", nnz, 60000)
+    d_rp, d_col, d_val, d_x = dev(rp, device), dev(col, device), dev(v, device), dev(x, device)
+    y = torch.empty(60000, dtype=torch.float64, device=device)
+    say("autotune form")
+    ipt, mode, wg, srt, us = autotune_spmv_form(d_rp, d_col, d_val, d_x, y, reps=3)
+    assert ipt in (2, 4, 6, 8, 10, 12, 14, 16) and mode in ("cached", "nontemporal")
+    assert wg in (0, 3, 4, 5) and us > 0 and (not srt or ipt in (8, 12, 16))
+    ref = MerbitPlan(d_rp, 60000, nnz, items_per_thread=ipt)
+    plan = MerbitPlan(d_rp, 60000, nnz, items_per_thread=ipt)
+    plan.set_stream(mode)
+    plan.set_residency(wg)
+    if srt:
+        assert plan.sort_cols(d_col)
+    assert np.array_equal(bits(plan.spmv(d_rp, d_col, d_val, d_x)),
+                          bits(ref.spmv(d_rp, d_col, d_val, d_x)))
+    plan.close()
+    ref.close()
+
+
 @pytest.mark.slow
 @pytest.mark.parametrize("kind", ["f64", "f32"])
 def test_sorted_rmat_full(orc, device, kind):
