@@ -560,7 +560,7 @@ def bench_batch(dev, args):
     # the autotuned shape, and the members' tile-sorted columns at each shape
     # that has them; then tiles per CU, timed like the single-matrix shapes
     # (first within 1 %)
-    forms = [(ipt, False)] + ([] if args.no_sort else [(i, True) for i in (8, 12, 16)])
+    forms = [(ipt, False)] + ([] if args.no_sort else [(i, True) for i in (8, 12, 16, 32)])
     wgs = [args.residency] if args.residency is not None else [0, 3, 4, 5]
     t_form = {}
     for fi, fs in forms:
@@ -586,7 +586,10 @@ def bench_batch(dev, args):
     nnz = sum(A.nnz for A in mats)
     # per-matrix launches for comparison (same arithmetic, one launch pair each)
     from merbit_amd.spmv import MerbitPlan
-    plans = [MerbitPlan(it[0], it[5], it[1].numel(), items_per_thread=ipt) for it in items]
+    # (single-matrix plans stop at 16 items per thread: the batch's 32 is its
+    # 512-thread tile)
+    plans = [MerbitPlan(it[0], it[5], it[1].numel(), items_per_thread=min(ipt, 16))
+             for it in items]
     for p, it in zip(plans, items):
         p.set_residency(resid)
         if srt:

@@ -254,7 +254,11 @@ enum { MBX_VALUE_F64 = 0, MBX_VALUE_F32 = 1, MBX_VALUE_BF16 = 2 };
  * (double, float or bfloat16 bits), x and y double for MBX_VALUE_F64 and
  * float otherwise.  mbx_batch_spmv computes y_m = alpha*A_m*x_m + beta*y_m
  * for every item with the same per-tile arithmetic as mbx_spmv_*, so the
- * results are bit-identical to one call per matrix.  mbx_batch_create
+ * results are bit-identical to one call per matrix.  items_per_thread takes
+ * the plan's values and also 32: tiles of 8192 merge items run by 512
+ * threads of 16 items (for mbx_batch_sort_cols, whose sorted gathers share
+ * more x lines in a larger tile; members up to 2^19 columns), whose results
+ * no single-matrix plan reproduces bit for bit.  mbx_batch_create
  * synchronises `stream` once (it uploads the descriptors). */
 typedef struct {
   const int32_t* row_ptr;
@@ -279,8 +283,8 @@ int mbx_batch_set_stream(mbx_batch* batch, int32_t mode);
 int mbx_batch_set_residency(mbx_batch* batch, int32_t wg_per_cu);
 /* Tile-sorted gathers for every member (as mbx_plan_sort_cols; no reference
  * counterpart): enable 1 builds the members' sorted column copies from the
- * items' col arrays, 0 drops them.  Batches with items_per_thread 8, 12 or
- * 16 whose members all suit the vector staging path; MBX_EUNSUPPORTED
+ * items' col arrays, 0 drops them.  Batches with items_per_thread 8, 12, 16
+ * or 32 whose members all suit the vector staging path; MBX_EUNSUPPORTED
  * otherwise or when a member's ncol exceeds the limit of mbx_plan_sort_cols,
  * MBX_EINVAL when a column lies outside [0, ncol).  Results stay
  * bit-identical.  Allocates 4 bytes per nonzero and synchronises `stream`. */

@@ -109,22 +109,32 @@ __global__ __launch_bounds__(WG) void batch_spmv_kernel(
   }
 }
 
+// Kernel shape of a batch's items_per_thread: 32 is a tile of 8192 merge
+// items run by 512 threads of 16 items (the larger tile shares more x lines
+// between its sorted gathers), every other value 256 threads.
+template <int IPT>
+struct BatchShape {
+  static constexpr int WG = IPT == 32 ? 512 : 256;
+  static constexpr int KI = IPT == 32 ? 16 : IPT;
+  static constexpr bool kSort = KI == 8 || KI == 12 || KI == 16;
+};
+
 // Tile-sorted column words of every member (mbx_batch_sort_cols): one
 // workgroup per global tile reads its member's columns from `mats` and writes
 // the words to the member's array in `sorted` (see sort_tile_cols).
-template <int IPT>
-__global__ __launch_bounds__(256) void batch_sort_cols_kernel(
+template <int WG, int IPT>
+__global__ __launch_bounds__(WG) void batch_sort_cols_kernel(
     const BatchMat* __restrict__ mats, const BatchMat* __restrict__ sorted,
     const int* __restrict__ tile_base, int count, const int4* __restrict__ coord,
     int* __restrict__ bad) {
-  __shared__ uint32_t s_key[1 << mbx_detail::sort_slot_bits<256, IPT>()];
-  __shared__ int s_wsum[4];
+  __shared__ uint32_t s_key[1 << mbx_detail::sort_slot_bits<WG, IPT>()];
+  __shared__ int s_wsum[WG / 64];
   const int g = blockIdx.x;
   const int m = batch_matrix_of(tile_base, count, g);
   const BatchMat M = mats[m];
   const int4* c = coord + M.coord_base + (g - M.tile_base);
-  mbx_detail::sort_tile_cols<256, IPT>(M.col, M.ncol, c[0].y, c[1].y,
-                                       const_cast<int32_t*>(sorted[m].col), bad, s_key, s_wsum);
+  mbx_detail::sort_tile_cols<WG, IPT>(M.col, M.ncol, c[0].y, c[1].y,
+                                      const_cast<int32_t*>(sorted[m].col), bad, s_key, s_wsum);
 }
 
 template <class P>
@@ -173,20 +183,21 @@ namespace {
 // Resident workgroups of the kernel variant the batch will launch.
 template <class P, int IPT>
 int batch_grid(int device, bool vec, bool sorted) {
-  if constexpr (!P::kSorted && (IPT == 8 || IPT == 12 || IPT == 16)) {
+  using S = BatchShape<IPT>;
+  if constexpr (!P::kSorted && S::kSort) {
     if (sorted) return batch_grid<mbx_detail::SortX<P>, IPT>(device, true, false);
   }
   int per_cu = 0, cus = 0;
-  const size_t lds = TileSmem<256, IPT, P>::bytes();
+  const size_t lds = TileSmem<S::WG, S::KI, P>::bytes();
   hipError_t e;
   if constexpr (P::kSorted)  // vector staging only
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, batch_spmv_kernel<256, IPT, true, P>, 256, lds);
+        &per_cu, batch_spmv_kernel<S::WG, S::KI, true, P>, S::WG, lds);
   else
     e = vec ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                  &per_cu, batch_spmv_kernel<256, IPT, true, P>, 256, lds)
+                  &per_cu, batch_spmv_kernel<S::WG, S::KI, true, P>, S::WG, lds)
             : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                  &per_cu, batch_spmv_kernel<256, IPT, false, P>, 256, lds);
+                  &per_cu, batch_spmv_kernel<S::WG, S::KI, false, P>, S::WG, lds);
   if (e != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
     return 0;
@@ -196,26 +207,28 @@ int batch_grid(int device, bool vec, bool sorted) {
 template <class P, int IPT>
 void batch_launch(const mbx_batch* b, hipStream_t s, typename P::scal_t alpha,
                   typename P::scal_t beta) {
+  using S = BatchShape<IPT>;
+  constexpr int WG = S::WG, KI = S::KI;
   const int bz = beta == typename P::scal_t(0);
   int grid = std::min(b->grid, b->total_tiles);
   if (b->wg_per_cu > 0) grid = std::min(grid, b->wg_per_cu * b->cus);  // persistent
-  const size_t lds = mbx_detail::residency_lds(TileSmem<256, IPT, P>::bytes(), b->wg_per_cu);
+  const size_t lds = mbx_detail::residency_lds(TileSmem<WG, KI, P>::bytes(), b->wg_per_cu);
   const bool nt = mbx_detail::stream_nontemporal(b->stream_mode, b->footprint);
   bool launched = false;
-  if constexpr (IPT == 8 || IPT == 12 || IPT == 16) {  // the members' tile-sorted columns
+  if constexpr (S::kSort) {  // the members' tile-sorted columns
     if (b->vec && b->d_mats_sorted) {
       int sgrid = std::min(b->grid_sorted, b->total_tiles);
       if (b->wg_per_cu > 0) sgrid = std::min(sgrid, b->wg_per_cu * b->cus);
       if (nt) {
         using Q = mbx_detail::SortX<mbx_detail::Streamed<P>>;
-        mbx_detail::allow_lds(batch_spmv_kernel<256, IPT, true, Q>, lds);
-        batch_spmv_kernel<256, IPT, true, Q><<<sgrid, 256, lds, s>>>(
+        mbx_detail::allow_lds(batch_spmv_kernel<WG, KI, true, Q>, lds);
+        batch_spmv_kernel<WG, KI, true, Q><<<sgrid, WG, lds, s>>>(
             b->d_mats_sorted, b->d_tile_base, b->count, b->total_tiles, b->d_coord, b->d_carry,
             alpha, beta, bz);
       } else {
         using Q = mbx_detail::SortX<P>;
-        mbx_detail::allow_lds(batch_spmv_kernel<256, IPT, true, Q>, lds);
-        batch_spmv_kernel<256, IPT, true, Q><<<sgrid, 256, lds, s>>>(
+        mbx_detail::allow_lds(batch_spmv_kernel<WG, KI, true, Q>, lds);
+        batch_spmv_kernel<WG, KI, true, Q><<<sgrid, WG, lds, s>>>(
             b->d_mats_sorted, b->d_tile_base, b->count, b->total_tiles, b->d_coord, b->d_carry,
             alpha, beta, bz);
       }
@@ -224,18 +237,18 @@ void batch_launch(const mbx_batch* b, hipStream_t s, typename P::scal_t alpha,
   }
   if (launched) {
   } else if (b->vec && nt) {
-    mbx_detail::allow_lds(batch_spmv_kernel<256, IPT, true, mbx_detail::Streamed<P>>, lds);
-    batch_spmv_kernel<256, IPT, true, mbx_detail::Streamed<P>><<<grid, 256, lds, s>>>(
+    mbx_detail::allow_lds(batch_spmv_kernel<WG, KI, true, mbx_detail::Streamed<P>>, lds);
+    batch_spmv_kernel<WG, KI, true, mbx_detail::Streamed<P>><<<grid, WG, lds, s>>>(
         b->d_mats, b->d_tile_base, b->count, b->total_tiles, b->d_coord, b->d_carry, alpha, beta,
         bz);
   } else if (b->vec) {
-    mbx_detail::allow_lds(batch_spmv_kernel<256, IPT, true, P>, lds);
-    batch_spmv_kernel<256, IPT, true, P><<<grid, 256, lds, s>>>(
+    mbx_detail::allow_lds(batch_spmv_kernel<WG, KI, true, P>, lds);
+    batch_spmv_kernel<WG, KI, true, P><<<grid, WG, lds, s>>>(
         b->d_mats, b->d_tile_base, b->count, b->total_tiles, b->d_coord, b->d_carry, alpha, beta,
         bz);
   } else {
-    mbx_detail::allow_lds(batch_spmv_kernel<256, IPT, false, P>, lds);
-    batch_spmv_kernel<256, IPT, false, P><<<grid, 256, lds, s>>>(
+    mbx_detail::allow_lds(batch_spmv_kernel<WG, KI, false, P>, lds);
+    batch_spmv_kernel<WG, KI, false, P><<<grid, WG, lds, s>>>(
         b->d_mats, b->d_tile_base, b->count, b->total_tiles, b->d_coord, b->d_carry, alpha, beta,
         bz);
   }
@@ -256,7 +269,7 @@ int batch_dispatch(const mbx_batch* b, hipStream_t s, double alpha, double beta,
     batch_launch<P, I>(b, s, (S)alpha, (S)beta);                      \
     return 0;
     MBX_CASE(2) MBX_CASE(4) MBX_CASE(6) MBX_CASE(8) MBX_CASE(10) MBX_CASE(12) MBX_CASE(14)
-    MBX_CASE(16)
+    MBX_CASE(16) MBX_CASE(32)
 #undef MBX_CASE
     default: return -1;
   }
@@ -286,7 +299,7 @@ bool vec_item_ok(const mbx_batch_item& it, int kind) {
 
 bool ipt_ok(int ipt) {
   return ipt == 2 || ipt == 4 || ipt == 6 || ipt == 8 || ipt == 10 || ipt == 12 || ipt == 14 ||
-         ipt == 16;
+         ipt == 16 || ipt == 32;
 }
 
 }  // namespace
@@ -421,10 +434,11 @@ int mbx_batch_sort_cols(mbx_batch* b, int32_t enable, void* stream) {
     b->d_scol = nullptr;
   }
   if (!enable) return MBX_OK;
-  if (b->ipt != 8 && b->ipt != 12 && b->ipt != 16) return MBX_EUNSUPPORTED;
+  if (b->ipt != 8 && b->ipt != 12 && b->ipt != 16 && b->ipt != 32) return MBX_EUNSUPPORTED;
   if (!b->vec) return MBX_EUNSUPPORTED;  // the scalar staging path reads the columns as given
-  const int sb = b->ipt == 8 ? mbx_detail::sort_slot_bits<256, 8>()
-                             : mbx_detail::sort_slot_bits<256, 16>();  // 12 for IPT 12 too
+  const int sb = b->ipt == 8    ? mbx_detail::sort_slot_bits<256, 8>()
+                 : b->ipt == 32 ? mbx_detail::sort_slot_bits<512, 16>()
+                                : mbx_detail::sort_slot_bits<256, 16>();  // 12 for IPT 12 too
   std::vector<BatchMat> h((size_t)b->count);
   if (hipMemcpy(h.data(), b->d_mats, sizeof(BatchMat) * (size_t)b->count,
                 hipMemcpyDeviceToHost) != hipSuccess)
@@ -463,15 +477,19 @@ int mbx_batch_sort_cols(mbx_batch* b, int32_t enable, void* stream) {
   if (st == MBX_OK) {
     switch (b->ipt) {
       case 8:
-        batch_sort_cols_kernel<8><<<b->total_tiles, 256, 0, s>>>(
+        batch_sort_cols_kernel<256, 8><<<b->total_tiles, 256, 0, s>>>(
             b->d_mats, d_sorted, b->d_tile_base, b->count, b->d_coord, d_bad);
         break;
       case 12:
-        batch_sort_cols_kernel<12><<<b->total_tiles, 256, 0, s>>>(
+        batch_sort_cols_kernel<256, 12><<<b->total_tiles, 256, 0, s>>>(
             b->d_mats, d_sorted, b->d_tile_base, b->count, b->d_coord, d_bad);
         break;
-      default:
-        batch_sort_cols_kernel<16><<<b->total_tiles, 256, 0, s>>>(
+      case 16:
+        batch_sort_cols_kernel<256, 16><<<b->total_tiles, 256, 0, s>>>(
+            b->d_mats, d_sorted, b->d_tile_base, b->count, b->d_coord, d_bad);
+        break;
+      default:  // 32: tiles of 512 threads x 16 items
+        batch_sort_cols_kernel<512, 16><<<b->total_tiles, 512, 0, s>>>(
             b->d_mats, d_sorted, b->d_tile_base, b->count, b->d_coord, d_bad);
         break;
     }

@@ -758,6 +758,7 @@ struct TileSmem {
   static constexpr int NW = WG / 64;
   static constexpr size_t kV = sizeof(typename P::lds_t) * (WG * IPT + 4);
   static constexpr size_t kW = (sizeof(double) * NW + 15) / 16 * 16;
+  static constexpr size_t kF = (sizeof(int) * NW + 15) / 16 * 16;  // one int per wave
   static constexpr size_t kM = sizeof(unsigned) * WG;
   typename P::lds_t* v;
   double* wtail;
@@ -770,10 +771,10 @@ struct TileSmem {
       : v(reinterpret_cast<typename P::lds_t*>(base)),
         wtail(reinterpret_cast<double*>(base + kV)),
         wflag(reinterpret_cast<int*>(base + kV + kW)),
-        tail(reinterpret_cast<double*>(base + kV + kW + 16)),
-        mask(reinterpret_cast<unsigned*>(base + kV + kW + 32)),
-        wcnt(reinterpret_cast<int*>(base + kV + kW + 32 + kM)) {}
-  static constexpr size_t bytes() { return kV + kW + 32 + kM + 16; }
+        tail(reinterpret_cast<double*>(base + kV + kW + kF)),
+        mask(reinterpret_cast<unsigned*>(base + kV + kW + kF + 16)),
+        wcnt(reinterpret_cast<int*>(base + kV + kW + kF + 16 + kM)) {}
+  static constexpr size_t bytes() { return kV + kW + kF + 16 + kM + kF; }
 };
 
 // The walk of one thread over `items` merge items from local coordinate
@@ -904,7 +905,7 @@ __device__ __forceinline__ void spmv_tile(
     const typename P::x_t* __restrict__ x, typename P::y_t* __restrict__ y,
     const int4* __restrict__ coord, double* __restrict__ carry, typename P::scal_t alpha,
     typename P::scal_t beta, int beta_zero) {
-  static_assert(WG % 64 == 0 && WG / 64 <= 4, "scan scratch holds up to 4 waves");
+  static_assert(WG % 64 == 0 && WG / 64 <= 8, "tiles of up to 8 waves");
   static_assert(VEC || !P::kCol16, "16-bit columns are read by the vector staging only");
   static_assert(VEC || !P::kHotX, "hot columns are read by the vector staging only");
   static_assert(VEC || !P::kSorted, "sorted columns are read by the vector staging only");

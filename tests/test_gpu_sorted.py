@@ -258,6 +258,55 @@ def test_batch_sorted_bit_identical(device, kind, ipt):
     b.close()
 
 
+@pytest.mark.parametrize("kind", ["f64", "f32", "bf16"])
+def test_batch_wide_tiles(orc, device, kind):
+    """Batches of 8192-item tiles (items_per_thread 32: 512 threads of 16
+    items), plain and with sorted gathers: every member within its bar of
+    the oracle (empty members included), the sorted product bit-identical
+    to the plain one, with and without a residency cap; members beyond 2^19
+    columns refuse the sort."""
+    from merbit_amd.batch import MerbitBatch
+    rng = np.random.default_rng(131)
+    sizes = [(0, 5, 1), (1, 50, 3), (900, 900, 9), (60000, 60000, 25), (777, 5000, 40),
+             (25000, 600, 14), (3000, 3000, 0), (120000, 150000, 6)]
+    items, host = [], []
+    ydt = torch.float64 if kind == "f64" else torch.float32
+    for nrow, ncol, lam in sizes:
+        rp, col = uniform_csr(rng, nrow, ncol, lam) if nrow else (np.zeros(1, np.int32),
+                                                                  np.zeros(0, np.int32))
+        v, x = values(rng, kind, int(rp[-1]), ncol)
+        y = torch.full((nrow,), float("nan"), dtype=ydt, device=device)
+        items.append((dev(rp, device), dev(col, device), dev(v, device), dev(x, device), y, ncol))
+        host.append((rp, col, v, x))
+    tol = 1e-12 if kind == "f64" else 1e-6
+    say(f"batch wide tiles {kind}")
+    b = MerbitBatch(items, items_per_thread=32)
+    assert b.tile_items == 8192
+    b.spmv()
+    torch.cuda.synchronize()
+    ref = []
+    for (rp, col, v, x), it in zip(host, items):
+        ya, absr = orc.spmv_accurate(rp, col, v, x)
+        got = it[4].double().cpu().numpy()
+        assert cond_err(got, ya, absr) <= tol
+        ref.append(bits(it[4]))
+    assert b.sort_cols(True)
+    for wg in (0, 1):
+        b.set_residency(wg)
+        b.spmv()
+        torch.cuda.synchronize()
+        for i, it in enumerate(items):
+            assert np.array_equal(bits(it[4]), ref[i]), (i, wg)
+    b.close()
+    # a member of more than 2^19 columns: no sorted form at this shape
+    big = (items[3][0], items[3][1], items[3][2],
+           torch.zeros((1 << 19) + 1, dtype=items[3][3].dtype, device=device), items[3][4],
+           (1 << 19) + 1)
+    b = MerbitBatch([items[2], big], items_per_thread=32)
+    assert not b.sort_cols(True)
+    b.close()
+
+
 def test_batch_sorted_refusals(device):
     from merbit_amd.batch import MerbitBatch
     rng = np.random.default_rng(3)
