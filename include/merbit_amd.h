@@ -347,11 +347,13 @@ int mbx_plan_stats(const mbx_plan* plan, const int32_t* d_row_ptr, mbx_pattern_s
  * but compiles one SIGMA and BLOCK (CMakeLists.txt:4-6) for every matrix.
  * Features: tiles at 8 items per thread, the widest column range of one such
  * tile, x bytes, and the share of the entries in the 4096 (FP64 x) or 7900
- * (FP32 x) most frequent columns.  "Sorted" below: 16 items per thread and
- * tile-sorted gathers (sort_cols, mbx_plan_sort_cols), taken when the
- * pattern is not local and ncol is at most 2^20.  Rules, first match:
+ * (FP32 x) most frequent columns.  "Sorted" below: tile-sorted gathers
+ * (sort_cols, mbx_plan_sort_cols) at 16 items per thread, taken when the
+ * pattern is not local and ncol is at most 2^20; for the few-tiles and
+ * x-fits rules at 32 (tiles of 8192 items, 512 threads) when ncol is at
+ * most 2^19 and that leaves a tile per CU.  Rules, first match:
  *   MBX_RULE_FEW_TILES       fewer than 16 tiles per CU: sorted when that
- *                            still gives 2 tiles per CU, else 4 items per
+ *                            still gives a tile per CU, else 4 items per
  *                            thread (2 when that gives fewer tiles than CUs);
  *   MBX_RULE_LOCAL           every tile spans < 65536 columns (banded):
  *                            8 (FP64) or 12 (FP32, BF16) items per thread, all tiles

@@ -65,10 +65,17 @@ constexpr long long kXFitsBytes = 2ll << 20;
 constexpr long long kSliceBytes = (7ll << 20) / 2;  // column-blocked form's x slice
 constexpr int kFewTilesPerCu = 16;
 // Tile-sorted gathers: 16 items per thread (the slot bits leave 20 for the
-// column), and the few-tiles rule takes them from 2 tiles per CU on.
+// column), or for uniform patterns of at most 2^19 columns 32, the tiles of
+// 8192 items run by 512 threads (13 slot bits): twice the entries per x line
+// (batch members of 194898 rows 52.9 -> 46.8 us, of 100399 rows 18.2 ->
+// 16.2, of 95160 rows 15.7 either way).  The few-tiles rule takes them from
+// 1 tile per CU on (a member of 33513 rows: 10.7 us at 4 items per thread
+// unsorted, 10.2 sorted at 16).
 constexpr int kSortIpt = 16;
+constexpr int kSortWideIpt = 32;
 constexpr long long kSortMaxCols = 1ll << 20;
-constexpr int kSortMinTilesPerCu = 2;
+constexpr long long kSortWideMaxCols = 1ll << 19;
+constexpr int kSortMinTilesPerCu = 1;
 // Row-aligned tiles at 12 items per thread measured slower (banded BF16 /
 // FP32 tile kernel +8 %, against +-1 % at 8): the rule keeps them to <= 8.
 constexpr int kRowTilesMaxIpt = 8;
@@ -225,14 +232,20 @@ int launch_rule(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow, in
   }
   const bool local = nnz > 0 && c.max_tile_span < 65536;
   const bool sortable = nnz > 0 && !local && ncol <= kSortMaxCols;
-  auto sorted = [&c]() {
-    c.items_per_thread = kSortIpt;
+  const bool wide_ok = ncol <= kSortWideMaxCols;
+  // sorted gathers at 16 items per thread, or (uniform patterns) the wide
+  // tiles when there is a tile of them per CU (their tiles are 1/4 of those
+  // at 8 items per thread)
+  auto sorted = [&](bool uniform) {
+    const bool wide = uniform && wide_ok &&
+                      (long long)c.ntiles8 >= 4ll * kSortMinTilesPerCu * cus;
+    c.items_per_thread = wide ? kSortWideIpt : kSortIpt;
     c.sort_cols = 1;
   };
   if (c.ntiles8 < kFewTilesPerCu * cus) {
     // tiles of 16 items per thread are half those of 8
     if (sortable && (long long)c.ntiles8 >= 2ll * kSortMinTilesPerCu * cus)
-      sorted();
+      sorted(true);
     else  // twice the tiles at 4 items per thread; 2 when even those leave CUs idle
       c.items_per_thread = 2ll * c.ntiles8 < cus ? 2 : 4;
     c.rule = MBX_RULE_FEW_TILES;
@@ -243,7 +256,7 @@ int launch_rule(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow, in
     c.rule = MBX_RULE_LOCAL;
   } else if ((long long)ncol * sx <= kXFitsBytes) {
     if (sortable) {
-      sorted();
+      sorted(true);
     } else {
       c.items_per_thread = 8;
       c.wg_per_cu = 4;
@@ -251,7 +264,7 @@ int launch_rule(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow, in
     c.rule = MBX_RULE_X_FITS;
   } else if (c.hot_share >= kHotShare) {
     if (sortable) {
-      sorted();
+      sorted(false);
     } else {
       c.items_per_thread = 8;
       c.hot_columns = hot;

@@ -190,17 +190,17 @@ __global__ __launch_bounds__(WG) void tile_spmv_hot_kernel(
 
 // Tile-sorted column words of a plan (mbx_plan_sort_cols), one workgroup
 // per tile (see sort_tile_cols).
-template <int IPT>
-__global__ __launch_bounds__(256) void sort_cols_kernel(const int32_t* __restrict__ col,
-                                                        int ncol,
-                                                        const int4* __restrict__ coord,
-                                                        int32_t* __restrict__ scol,
-                                                        int* __restrict__ bad) {
-  __shared__ uint32_t s_key[1 << mbx_detail::sort_slot_bits<256, IPT>()];
-  __shared__ int s_wsum[4];
+template <int WG, int IPT>
+__global__ __launch_bounds__(WG) void sort_cols_kernel(const int32_t* __restrict__ col,
+                                                       int ncol,
+                                                       const int4* __restrict__ coord,
+                                                       int32_t* __restrict__ scol,
+                                                       int* __restrict__ bad) {
+  __shared__ uint32_t s_key[1 << mbx_detail::sort_slot_bits<WG, IPT>()];
+  __shared__ int s_wsum[WG / 64];
   const int t = blockIdx.x;
-  mbx_detail::sort_tile_cols<256, IPT>(col, ncol, coord[t].y, coord[t + 1].y, scol, bad, s_key,
-                                       s_wsum);
+  mbx_detail::sort_tile_cols<WG, IPT>(col, ncol, coord[t].y, coord[t + 1].y, scol, bad, s_key,
+                                      s_wsum);
 }
 
 // Carry fixup, one thread per tile (see fixup_tile).
@@ -225,7 +225,7 @@ namespace {
 bool supported_shape(int wg, int ipt) {
   if (wg != 256) return false;
   switch (ipt) {
-    case 2: case 4: case 6: case 8: case 10: case 12: case 14: case 16: return true;
+    case 2: case 4: case 6: case 8: case 10: case 12: case 14: case 16: case 32: return true;
     default: return false;
   }
 }
@@ -268,6 +268,8 @@ hipError_t launch_tile(const mbx_plan* p, hipStream_t s, const int32_t* rp,
                        const int32_t* col, const typename P::val_t* val,
                        const typename P::x_t* x, typename P::y_t* y,
                        typename P::scal_t alpha, typename P::scal_t beta) {
+  // 32 items per thread: tiles of 8192 merge items run by 512 threads of 16
+  constexpr int WG = IPT == 32 ? 512 : 256, KI = IPT == 32 ? 16 : IPT;
   const int bz = beta == typename P::scal_t(0);
   const bool vec = mbx_detail::vec_path_ok(rp, col, val, p->ncol);
   const long long footprint = 4ll * (p->nrow + 1) +
@@ -275,7 +277,7 @@ hipError_t launch_tile(const mbx_plan* p, hipStream_t s, const int32_t* rp,
                               (long long)p->ncol * sizeof(typename P::x_t) +
                               (long long)p->nrow * sizeof(typename P::y_t);
   const bool nt = mbx_detail::stream_nontemporal(p->stream_mode, footprint);
-  const size_t lds = mbx_detail::residency_lds(TileSmem<256, IPT, P>::bytes(), p->wg_per_cu);
+  const size_t lds = mbx_detail::residency_lds(TileSmem<WG, KI, P>::bytes(), p->wg_per_cu);
   // the plan's 16-bit columns, instantiated for the tile shapes the banded
   // and graph products use
   constexpr bool kC16 = IPT == 8 || IPT == 12;
@@ -285,32 +287,32 @@ hipError_t launch_tile(const mbx_plan* p, hipStream_t s, const int32_t* rp,
       const int32_t* c16 = reinterpret_cast<const int32_t*>(p->d_col16);
       if (nt) {
         using Q = mbx_detail::Col16<mbx_detail::Streamed<P>>;
-        mbx_detail::allow_lds(tile_spmv_kernel<256, IPT, true, Q>, lds);
-        timed_launch(p, tile_spmv_kernel<256, IPT, true, Q>, p->ntiles, 256, lds, s, rp, c16, val,
+        mbx_detail::allow_lds(tile_spmv_kernel<WG, KI, true, Q>, lds);
+        timed_launch(p, tile_spmv_kernel<WG, KI, true, Q>, p->ntiles, WG, lds, s, rp, c16, val,
                      x, y, p->d_coord, p->d_carry, p->ntiles, p->ncol, alpha, beta, bz);
       } else {
         using Q = mbx_detail::Col16<P>;
-        mbx_detail::allow_lds(tile_spmv_kernel<256, IPT, true, Q>, lds);
-        timed_launch(p, tile_spmv_kernel<256, IPT, true, Q>, p->ntiles, 256, lds, s, rp, c16, val,
+        mbx_detail::allow_lds(tile_spmv_kernel<WG, KI, true, Q>, lds);
+        timed_launch(p, tile_spmv_kernel<WG, KI, true, Q>, p->ntiles, WG, lds, s, rp, c16, val,
                      x, y, p->d_coord, p->d_carry, p->ntiles, p->ncol, alpha, beta, bz);
       }
       launched = true;
     }
   }
   // the plan's tile-sorted columns (read in place of the caller's)
-  constexpr bool kSortShape = IPT == 8 || IPT == 12 || IPT == 16;
+  constexpr bool kSortShape = KI == 8 || KI == 12 || KI == 16;
   if constexpr (kSortShape) {
     if (!launched && p->d_scol && mbx_detail::vec_path_ok(rp, p->d_scol, val, p->ncol)) {
       if (nt) {
         using Q = mbx_detail::SortX<mbx_detail::Streamed<P>>;
-        mbx_detail::allow_lds(tile_spmv_kernel<256, IPT, true, Q>, lds);
-        timed_launch(p, tile_spmv_kernel<256, IPT, true, Q>, p->ntiles, 256, lds, s, rp,
+        mbx_detail::allow_lds(tile_spmv_kernel<WG, KI, true, Q>, lds);
+        timed_launch(p, tile_spmv_kernel<WG, KI, true, Q>, p->ntiles, WG, lds, s, rp,
                      p->d_scol, val, x, y, p->d_coord, p->d_carry, p->ntiles, p->ncol, alpha,
                      beta, bz);
       } else {
         using Q = mbx_detail::SortX<P>;
-        mbx_detail::allow_lds(tile_spmv_kernel<256, IPT, true, Q>, lds);
-        timed_launch(p, tile_spmv_kernel<256, IPT, true, Q>, p->ntiles, 256, lds, s, rp,
+        mbx_detail::allow_lds(tile_spmv_kernel<WG, KI, true, Q>, lds);
+        timed_launch(p, tile_spmv_kernel<WG, KI, true, Q>, p->ntiles, WG, lds, s, rp,
                      p->d_scol, val, x, y, p->d_coord, p->d_carry, p->ntiles, p->ncol, alpha,
                      beta, bz);
       }
@@ -326,17 +328,17 @@ hipError_t launch_tile(const mbx_plan* p, hipStream_t s, const int32_t* rp,
   }
   if (launched) {
   } else if (vec && nt) {
-    mbx_detail::allow_lds(tile_spmv_kernel<256, IPT, true, mbx_detail::Streamed<P>>, lds);
-    timed_launch(p, tile_spmv_kernel<256, IPT, true, mbx_detail::Streamed<P>>, p->ntiles, 256,
+    mbx_detail::allow_lds(tile_spmv_kernel<WG, KI, true, mbx_detail::Streamed<P>>, lds);
+    timed_launch(p, tile_spmv_kernel<WG, KI, true, mbx_detail::Streamed<P>>, p->ntiles, WG,
                  lds, s, rp, col, val, x, y, p->d_coord, p->d_carry, p->ntiles, p->ncol, alpha,
                  beta, bz);
   } else if (vec) {
-    mbx_detail::allow_lds(tile_spmv_kernel<256, IPT, true, P>, lds);
-    timed_launch(p, tile_spmv_kernel<256, IPT, true, P>, p->ntiles, 256, lds, s, rp, col, val, x,
+    mbx_detail::allow_lds(tile_spmv_kernel<WG, KI, true, P>, lds);
+    timed_launch(p, tile_spmv_kernel<WG, KI, true, P>, p->ntiles, WG, lds, s, rp, col, val, x,
                  y, p->d_coord, p->d_carry, p->ntiles, p->ncol, alpha, beta, bz);
   } else {
-    mbx_detail::allow_lds(tile_spmv_kernel<256, IPT, false, P>, lds);
-    timed_launch(p, tile_spmv_kernel<256, IPT, false, P>, p->ntiles, 256, lds, s, rp, col, val,
+    mbx_detail::allow_lds(tile_spmv_kernel<WG, KI, false, P>, lds);
+    timed_launch(p, tile_spmv_kernel<WG, KI, false, P>, p->ntiles, WG, lds, s, rp, col, val,
                  x, y, p->d_coord, p->d_carry, p->ntiles, p->ncol, alpha, beta, bz);
   }
   p->ev_begin = p->ev_end = nullptr;
@@ -368,6 +370,7 @@ int run_spmv(const mbx_plan* p, const int32_t* rp, const int32_t* col,
     case 12: e = launch_tile<P, 12>(p, s, rp, col, val, x, y, alpha, beta); break;
     case 14: e = launch_tile<P, 14>(p, s, rp, col, val, x, y, alpha, beta); break;
     case 16: e = launch_tile<P, 16>(p, s, rp, col, val, x, y, alpha, beta); break;
+    case 32: e = launch_tile<P, 32>(p, s, rp, col, val, x, y, alpha, beta); break;
     default: return MBX_EUNSUPPORTED;
   }
   return e == hipSuccess ? MBX_OK : MBX_EHIP;
@@ -684,10 +687,11 @@ int mbx_plan_sort_cols(mbx_plan* p, const int32_t* d_col, void* stream) {
     p->d_scol = nullptr;
   }
   if (!d_col) return MBX_OK;  // back to the caller's columns
-  if (p->ipt != 8 && p->ipt != 12 && p->ipt != 16) return MBX_EUNSUPPORTED;
+  if (p->ipt != 8 && p->ipt != 12 && p->ipt != 16 && p->ipt != 32) return MBX_EUNSUPPORTED;
   if (p->d_col16 || (p->d_colh && p->nhot > 0)) return MBX_EUNSUPPORTED;
-  const int sb = p->ipt == 8 ? mbx_detail::sort_slot_bits<256, 8>()
-                             : mbx_detail::sort_slot_bits<256, 16>();  // 12 for IPT 12 too
+  const int sb = p->ipt == 8    ? mbx_detail::sort_slot_bits<256, 8>()
+                 : p->ipt == 32 ? mbx_detail::sort_slot_bits<512, 16>()
+                                : mbx_detail::sort_slot_bits<256, 16>();  // 12 for IPT 12 too
   if ((long long)p->ncol > (1ll << (32 - sb))) return MBX_EUNSUPPORTED;
   if (p->ntiles == 0 || p->nnz == 0) return MBX_OK;
   const size_t padded = ((size_t)p->nnz + 3) / 4 * 4;  // whole 16-byte groups
@@ -704,13 +708,20 @@ int mbx_plan_sort_cols(mbx_plan* p, const int32_t* d_col, void* stream) {
     st = MBX_EHIP;
   if (st == MBX_OK) {
     switch (p->ipt) {
-      case 8: sort_cols_kernel<8><<<p->ntiles, 256, 0, s>>>(d_col, p->ncol, p->d_coord, sc, d_bad);
+      case 8:
+        sort_cols_kernel<256, 8><<<p->ntiles, 256, 0, s>>>(d_col, p->ncol, p->d_coord, sc, d_bad);
         break;
       case 12:
-        sort_cols_kernel<12><<<p->ntiles, 256, 0, s>>>(d_col, p->ncol, p->d_coord, sc, d_bad);
+        sort_cols_kernel<256, 12><<<p->ntiles, 256, 0, s>>>(d_col, p->ncol, p->d_coord, sc,
+                                                             d_bad);
         break;
-      default:
-        sort_cols_kernel<16><<<p->ntiles, 256, 0, s>>>(d_col, p->ncol, p->d_coord, sc, d_bad);
+      case 16:
+        sort_cols_kernel<256, 16><<<p->ntiles, 256, 0, s>>>(d_col, p->ncol, p->d_coord, sc,
+                                                             d_bad);
+        break;
+      default:  // 32: tiles of 512 threads x 16 items
+        sort_cols_kernel<512, 16><<<p->ntiles, 512, 0, s>>>(d_col, p->ncol, p->d_coord, sc,
+                                                             d_bad);
         break;
     }
     if (hipGetLastError() != hipSuccess ||

@@ -43,8 +43,10 @@ def uniform_csr(rng, nrow, ncol, lam):
 
 
 @pytest.mark.parametrize("kind", ["f64", "f32", "bf16"])
-@pytest.mark.parametrize("ipt", [8, 12, 16])
-def test_sorted_bit_identical(device, kind, ipt):
+@pytest.mark.parametrize("ipt", [8, 12, 16, 32])
+def test_sorted_bit_identical(orc, device, kind, ipt):
+    """32 items per thread is the 8192-item tile of 512 threads x 16 items,
+    whose plain product is checked against the oracle here too."""
     from merbit_amd.spmv import MerbitPlan
     rng = np.random.default_rng(71 + ipt)
     T = 256 * ipt
@@ -69,6 +71,10 @@ def test_sorted_bit_identical(device, kind, ipt):
             ref = plan.spmv(d_rp, d_col, d_val, d_x)
             ref_ab = plan.spmv(d_rp, d_col, d_val, d_x, dev(y0, device), -1.5, 0.5)
             say(f"sorted {kind} ipt {ipt} {name} acc {acc}")
+            if ipt == 32:
+                ya, absr = orc.spmv_accurate(rp, col, vals, x)
+                assert cond_err(ref.double().cpu().numpy(), ya, absr) <= \
+                    (1e-12 if kind == "f64" else 2.5e-7), (name, acc)
             assert plan.sort_cols(d_col)
             for mode in ("cached", "nontemporal"):
                 plan.set_stream(mode)
@@ -372,10 +378,10 @@ def test_colblock_sorted_bit_identical(device, kind):
 
 
 def test_rule_takes_sorted_gathers(orc, device):
-    """The launch rule takes tile-sorted gathers at 16 items per thread for
-    non-local patterns whose columns fit the sorted words: uniform random
-    patterns whose x fits L2 (with few tiles or more), and one with a hot
-    column set; not for a
+    """The launch rule takes tile-sorted gathers for non-local patterns whose
+    columns fit the sorted words: at 32 items per thread (8192-item tiles)
+    for uniform random patterns whose x fits L2 (with few tiles or more), at
+    16 for one with a hot column set; not for a
     local pattern, a pattern too small for 2 tiles per CU at 16 items per
     thread, or one with more than 2^20 columns (which keeps the hot split).
     The rule's plans match the oracle."""
@@ -384,12 +390,12 @@ def test_rule_takes_sorted_gathers(orc, device):
     say("rule sorted: uniform, x fits")
     rp, col = uniform_csr(rng, 300000, 300000, 40)
     c = launch_rule(dev(rp, device), dev(col, device), 300000, value_dtype=torch.float32)
-    assert c["rule"] == "x_fits" and c["sort_cols"] == 1 and c["items_per_thread"] == 16, c
+    assert c["rule"] == "x_fits" and c["sort_cols"] == 1 and c["items_per_thread"] == 32, c
     say("rule sorted: uniform, few tiles")
     rp, col = uniform_csr(rng, 200000, 200000, 30)
     d_rp, d_col = dev(rp, device), dev(col, device)
     c = launch_rule(d_rp, d_col, 200000)
-    assert c["rule"] == "few_tiles" and c["sort_cols"] == 1 and c["items_per_thread"] == 16, c
+    assert c["rule"] == "few_tiles" and c["sort_cols"] == 1 and c["items_per_thread"] == 32, c
     v, x = values(rng, "f64", int(rp[-1]), 200000)
     plan = MerbitPlan(d_rp, 200000, int(rp[-1]), col=d_col)
     assert plan.choice["sort_cols"] == 1
