@@ -45,7 +45,8 @@ typedef struct mbx_plan mbx_plan;
  * Runs the merge-path partition search over (row_ptr, nnz index) on `stream`
  * and keeps one (row, nnz) coordinate per tile of wg_size*items_per_thread
  * merge items plus the per-tile carry slots.  items_per_thread plays the
- * reference's SIGMA (instantiated: 2, 4, 6, 8, 10, 12, 14, 16) and wg_size
+ * reference's SIGMA (instantiated: 2, 4, 6, 8, 10, 12, 14, 16 and 32, whose
+ * tiles of 256*32 merge items run as 512 threads of 16 items) and wg_size
  * its BLOCK (256); 0 selects the defaults (8, 256).
  * Synchronises `stream` once: it reads back whether any row crosses a tile
  * boundary (the carry fixup launch is skipped when none does) and the ends

@@ -22,8 +22,9 @@ STREAM_MODES = {"auto": 0, "cached": 1, "nontemporal": 2}
 ACCUMULATION_MODES = {"compensated": 0, "f64": 1, "f32prod": 2}
 ACCUMULATION_NAMES = {v: k for k, v in ACCUMULATION_MODES.items()}
 
-# Items per thread with an instantiated tile kernel (the reference's SIGMA).
-ITEMS_PER_THREAD = (2, 4, 6, 8, 10, 12, 14, 16)
+# Items per thread with an instantiated tile kernel (the reference's SIGMA;
+# the tiles of 32 hold 8192 merge items and run as 512 threads of 16).
+ITEMS_PER_THREAD = (2, 4, 6, 8, 10, 12, 14, 16, 32)
 
 _VALUE_KINDS = {torch.float64: "f64", torch.float32: "f32", torch.uint16: "bf16",
                 torch.bfloat16: "bf16"}
