@@ -354,7 +354,9 @@ int mbx_plan_stats(const mbx_plan* plan, const int32_t* d_row_ptr, mbx_pattern_s
  * x-fits rules at 32 (tiles of 8192 items, 512 threads) when ncol is at
  * most 2^19 and that leaves a tile per CU.  Rules, first match:
  *   MBX_RULE_FEW_TILES       fewer than 16 tiles per CU: sorted when that
- *                            still gives a tile per CU, else 4 items per
+ *                            still gives a tile per CU (at 12 items per
+ *                            thread when the hot-column share is at least
+ *                            1/4, a power law), else 4 items per
  *                            thread (2 when that gives fewer tiles than CUs);
  *   MBX_RULE_LOCAL           every tile spans < 65536 columns (banded):
  *                            8 (FP64) or 12 (FP32, BF16) items per thread, all tiles

@@ -73,6 +73,7 @@ constexpr int kFewTilesPerCu = 16;
 // unsorted, 10.2 sorted at 16).
 constexpr int kSortIpt = 16;
 constexpr int kSortWideIpt = 32;
+constexpr int kSortFewHotIpt = 12;  // few tiles, power law
 constexpr long long kSortMaxCols = 1ll << 20;
 constexpr long long kSortWideMaxCols = 1ll << 19;
 constexpr int kSortMinTilesPerCu = 1;
@@ -243,10 +244,18 @@ int launch_rule(const int32_t* d_row_ptr, const int32_t* d_col, int32_t nrow, in
     c.sort_cols = 1;
   };
   if (c.ntiles8 < kFewTilesPerCu * cus) {
-    // tiles of 16 items per thread are half those of 8
-    if (sortable && (long long)c.ntiles8 >= 2ll * kSortMinTilesPerCu * cus)
-      sorted(true);
-    else  // twice the tiles at 4 items per thread; 2 when even those leave CUs idle
+    // tiles of 16 items per thread are half those of 8.  A power-law pattern
+    // takes 12 items per thread: the hot lines are shared within smaller
+    // tiles too, and more tiles fill the chip (R-MAT of 2^18 rows and 2^22
+    // entries: 23.6 us at 12 against 26.2 at 16 and 29.4 at 32)
+    if (sortable && (long long)c.ntiles8 >= 2ll * kSortMinTilesPerCu * cus) {
+      if (c.hot_share >= kHotShare) {
+        c.items_per_thread = kSortFewHotIpt;
+        c.sort_cols = 1;
+      } else {
+        sorted(true);
+      }
+    } else  // twice the tiles at 4 items per thread; 2 when even those leave CUs idle
       c.items_per_thread = 2ll * c.ntiles8 < cus ? 2 : 4;
     c.rule = MBX_RULE_FEW_TILES;
   } else if (local) {

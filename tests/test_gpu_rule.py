@@ -51,8 +51,9 @@ def time_us(fn, reps, rounds=5):
 def operands(name):
     from merbit_amd import gen, workloads
     if name.startswith("rmat"):
-        A, x = workloads.rmat()
-        if name == "rmat_f64":
+        # rmat18_*: 2^18 rows and 2^22 entries, a power law in the few-tiles class
+        A, x = workloads.rmat(18) if name.startswith("rmat18") else workloads.rmat()
+        if name in ("rmat_f64", "rmat18_f64"):
             return A, A.val, x
         return A, A.val.astype(np.float32), x.astype(np.float32)
     if name.startswith("banded"):
@@ -76,7 +77,7 @@ def operands(name):
 EXPECTED = {"rmat_f64": "hot_columns", "rmat_f32": "hot_columns", "banded_f64": "local",
             "banded_f32": "local", "banded_bf16": "local", "graph_f32": "spread_columns",
             "laplacian_f64": "few_tiles", "batch_21": "few_tiles", "batch_15": "few_tiles",
-            "batch_38": "x_fits"}
+            "batch_38": "x_fits", "rmat18_f64": "few_tiles"}
 
 
 @pytest.mark.slow
